@@ -1,0 +1,163 @@
+#!/usr/bin/env python
+"""Throughput of the OpenSBLI RHS + RK3 time advance on one MI355X.
+
+Workload: 3-D Taylor-Green vortex, 256^3 periodic, FP64, Re = 1600, M = 0.1,
+WENO5-Z characteristic LLF flux (Roe average) with 4th-order central viscous
+terms and the 3-register RK3 (the TGV script's RungeKutta(3)).  A step is one
+RK3 iteration (three stages).  The state is resident in HBM before timing.
+
+Prints one JSON line:
+  value = Mcell-updates/s per RK stage = cells * stages * steps / t / 1e6
+  roofline = the dominant kernel's algorithmic bytes per launch / its mean
+             launch time (HIP events on the library's stream) vs 8 TB/s
+  cpu_baseline = the oracle (C restatement of the generated OPS kernel
+             sequence) on the host's cores, one step of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, conv, steps=1):
+    """Time the oracle on this host, the same problem, `steps` RK3 steps."""
+    from opensbli_amd.problems import taylor_green
+    from oracle.ref import RefSolver
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    problem, q0 = taylor_green(n, conv=conv)
+    s = RefSolver(problem)
+    s.set_state(q0)
+    t0 = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n ** 3 * 3 * steps / dt / 1e6,
+        "unit": "Mcell-updates/s per RK stage",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} RK3 step(s) of the same {n}^3 TGV {conv} problem "
+                  f"({dt:.1f} s, OMP_NUM_THREADS={threads})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--conv", default="weno_z", choices=["weno_z", "weno_js", "teno6", "teno5", "central_skew"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import taylor_green
+
+    n = args.n
+    problem, q0 = taylor_green(n, conv=args.conv)
+    blk = Block(problem)
+    blk.set_state(q0)
+    stages = blk.num_stages()
+    _log(f"bench: TGV {n}^3 {args.conv} RK3, warmup {args.warmup} steps")
+    blk.run(args.warmup)
+    blk.sync()
+
+    blk.profile(True)
+    torch.cuda.synchronize()
+    blk.sync()
+    t0 = time.perf_counter()
+    blk.run(args.steps)
+    blk.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = blk.kernel_stats()
+    blk.profile(False)
+    work = blk.kernel_work()
+    elapsed = t1 - t0
+    cells = n ** 3
+    value = cells * stages * args.steps / elapsed / 1e6
+    ms_per_step = elapsed * 1e3 / args.steps
+    stage_bytes, stage_flops = blk.work_per_stage()
+    stage_evals = stages * args.steps
+
+    # dominant kernel family by total time
+    fam = max(stats, key=lambda k: stats[k][0])
+    tot_ms, launches = stats[fam]
+    avg_s = tot_ms / 1e3 / max(launches, 1)
+    kb, kf = work[fam]
+    roofline = {
+        "kernel": fam,
+        "bound": "hbm",
+        "achieved": kb / avg_s / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": kb / avg_s / 1e9 / HBM_PEAK_GBS,
+        "traffic": None,
+        "avg_launch_ms": avg_s * 1e3,
+        "launches": launches,
+    }
+    roofline_fp64 = {
+        "kernel": fam,
+        "achieved": kf / avg_s / 1e12,
+        "peak": FP64_PEAK_TFS,
+        "unit": "TFLOP/s",
+        "frac": kf / avg_s / 1e12 / FP64_PEAK_TFS,
+    }
+    stage_roofline = {
+        "hbm_frac": stage_bytes * stage_evals / elapsed / 1e9 / HBM_PEAK_GBS,
+        "fp64_frac": stage_flops * stage_evals / elapsed / 1e12 / FP64_PEAK_TFS,
+        "bytes_per_cell_stage": stage_bytes / cells,
+        "flops_per_cell_stage": stage_flops / cells,
+    }
+    kernels = {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items() if v[1]}
+    blk.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        _log(f"bench: cpu baseline ({args.cpu_steps} step of {n}^3 on the host)")
+        cpu = cpu_baseline(n, args.conv, args.cpu_steps)
+
+    out = {
+        "metric": "Mcell-updates/s per RK stage (FP64, 256^3 TGV)",
+        "value": value,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic Taylor-Green initial state)",
+        "config": {"workload": f"taylor_green_vortex_{n}^3_{args.conv}_roe_viscous_rk3", "grid": [n, n, n],
+                   "scheme": args.conv, "rk_stages": stages, "Re": 1600.0, "Minf": 0.1},
+        "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
+        "stage_roofline": stage_roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * opensbli_amd.h -- C-ABI of libosbl_hip.so, the MI355X (gfx950) implementation
+ * of the right-hand-side and Runge-Kutta time-advance kernels that OpenSBLI
+ * otherwise emits as OPS C.
+ *
+ * The reference has no FFI for this path: its generated program calls the OPS
+ * runtime directly.  Each entry point below names the OPS calls / generated
+ * constructs it replaces (paths relative to the OpenSBLI source tree).
+ *
+ * Conventions
+ *  - Plain C types only; no ownership crosses the boundary.  Host buffers are
+ *    borrowed for the duration of a call.  Device buffers passed to the *_dev
+ *    entry points must live on the context's device.
+ *  - Every int-returning call returns 0 on success and a negative OSBL_E_* code
+ *    on failure; osbl_last_error() then returns a thread-local message.
+ *  - A context is used by one host thread at a time.  All device work of a
+ *    context is ordered on its own HIP stream.
+ *  - Field layout across the boundary: interior points only, x fastest,
+ *    n[0]*n[1]*n[2] doubles per conservative variable, variable order
+ *    rho, rhou0..rhou(ndim-1), rhoE (the time_advance_arrays order,
+ *    opensbliequations.py:426-438).
+ */
+#ifndef OPENSBLI_AMD_H
+#define OPENSBLI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSBL_ABI_VERSION 1
+
+/* error codes */
+#define OSBL_OK 0
+#define OSBL_E_ARG (-1)      /* invalid argument / unsupported configuration */
+#define OSBL_E_HIP (-2)      /* HIP runtime failure */
+#define OSBL_E_NOMEM (-3)    /* device allocation failed */
+#define OSBL_E_STATE (-4)    /* call not valid in the context's state */
+
+/* convective discretisation: the spatial scheme objects of the problem script */
+#define OSBL_CONV_CENTRAL_SKEW 0 /* Central(4) on Skew(...) terms (parsing.py:75-111, scheme.py:187-252) */
+#define OSBL_CONV_CENTRAL_CONS 1 /* Central(4) on Conservative(...) fluxes (parsing.py:189-216) */
+#define OSBL_CONV_WENO_JS 2      /* LLFWeno(5, formulation='JS') (weno.py:341-369,468-521) */
+#define OSBL_CONV_WENO_Z 3       /* LLFWeno(5, formulation='Z') (weno.py:283-338) */
+#define OSBL_CONV_TENO5 4        /* LLFTeno(5) (teno.py:195-213,468-530) */
+#define OSBL_CONV_TENO6 5        /* LLFTeno(6) (teno.py:216-234) */
+
+#define OSBL_AVG_SIMPLE 0 /* SimpleAverage([0,1]) (averaging.py:31-59) */
+#define OSBL_AVG_ROE 1    /* RoeAverage([0,1])    (averaging.py:62-114) */
+
+#define OSBL_VISC_NONE 0
+#define OSBL_VISC_CONST 1 /* tau_ij=(1/Re)(...), q_j = D_j T/((gama-1)Minf^2 Pr Re) (taylor_green_vortex.py:18-19) */
+
+#define OSBL_RK3 0        /* RungeKutta(3), 3-register (rk_sbli.py:16-146) */
+#define OSBL_RK3_LS 1     /* RungeKuttaLS(3) (rk_LS.py:91-94) */
+#define OSBL_RK3_LS_SSP 2 /* RungeKuttaLS(3, formulation='SSP') (rk_LS.py:74-89) */
+#define OSBL_RK4_LS 3     /* RungeKuttaLS(4), five stages (rk_LS.py:96-97) */
+
+#define OSBL_BC_PERIODIC 0 /* PeriodicBC (periodic.py:7-56, exchange.py:9-57) */
+#define OSBL_BC_EXTRAP0 1  /* ExtrapolationBC(order=0) (extrapolation.py:29-58) */
+
+/*
+ * The problem definition, a flat mirror of the constants OpenSBLI declares
+ * (ConstantsToDeclare, kernel.py:20-35; substitute_simulation_parameters,
+ * helperfunctions.py:130-149) and of the scheme/BC objects on the block.
+ */
+typedef struct osbl_config {
+  int32_t ndim;        /* 1, 2 or 3 */
+  int32_t n[3];        /* block0np0..2; 1 for absent dimensions */
+  double delta[3];     /* Delta0block0..Delta2block0 */
+  int32_t conv;        /* OSBL_CONV_* */
+  int32_t averaging;   /* OSBL_AVG_* (characteristic schemes only) */
+  int32_t viscous;     /* OSBL_VISC_* */
+  int32_t rk;          /* OSBL_RK* */
+  int32_t bc[3][2];    /* OSBL_BC_* per direction and side */
+  double gama, Minf, Re, Pr, dt, eps, teno_ct;
+} osbl_config;
+
+typedef struct osbl_ctx osbl_ctx;
+
+/* ABI version of the loaded library (OSBL_ABI_VERSION). */
+int osbl_abi_version(void);
+
+/* Thread-local message describing the last failure. */
+const char* osbl_last_error(void);
+
+/*
+ * Create a context: device, stream and every dataset of the block.
+ * Replaces ops_init / ops_decl_block / ops_decl_dat / ops_decl_const /
+ * ops_partition of the generated main() (opsc.py:540-553, 617-662, 693-722).
+ */
+int osbl_create(const osbl_config* cfg, osbl_ctx** out);
+
+/* Release every device resource of the context (replaces ops_exit, opsc.py:435-443). */
+void osbl_destroy(osbl_ctx* ctx);
+
+/*
+ * Upload / download one conservative variable (interior points, x fastest).
+ * set_field replaces the initial-condition kernel of GridBasedInitialisation
+ * (gridbasedinit.py:46-57) or ops_decl_dat_hdf5 (opsc.py:702-705);
+ * get_field replaces ops_fetch_dat_hdf5_file (io_hdf5.py:99-127).
+ */
+int osbl_set_field(osbl_ctx* ctx, int var, const double* host);
+int osbl_get_field(osbl_ctx* ctx, int var, double* host);
+
+/* Same, with a device pointer (e.g. a torch tensor's data_ptr()); enqueued on the context's stream. */
+int osbl_set_field_dev(osbl_ctx* ctx, int var, const double* dev);
+int osbl_get_field_dev(osbl_ctx* ctx, int var, double* dev);
+
+/*
+ * Run niter iterations of the time loop: boundary conditions, save of the
+ * RK register, then for every stage the constituent relations, the spatial
+ * scheme kernels, the RK update and the boundary conditions
+ * (the Timers/DoLoop tree of algorithm.py:423-477).  Stream-ordered; returns
+ * once the work is enqueued.
+ */
+int osbl_run(osbl_ctx* ctx, int64_t niter);
+
+/*
+ * Apply the boundary conditions to the current state and evaluate one RHS;
+ * copies the residual (nvar blocks of interior points) to host memory.
+ * Corresponds to one pass of the spatial kernels of a stage (Residual arrays,
+ * opensbliequations.py:238-246).
+ */
+int osbl_rhs(osbl_ctx* ctx, double* residual_host);
+
+/* Wait for all work enqueued on the context's stream. */
+int osbl_sync(osbl_ctx* ctx);
+
+/* The context's hipStream_t, as an opaque pointer (for event timing by the caller). */
+void* osbl_stream(osbl_ctx* ctx);
+
+/*
+ * Per-kernel timing: when enabled, every launch is bracketed with HIP events
+ * on the context's stream and accumulated per kernel family.
+ * osbl_kernel_stats fills name (NUL-terminated, up to name_len bytes),
+ * total milliseconds and launch count of family `index`; returns the number
+ * of families, or a negative error.  Replaces OPS's per-kernel timing output
+ * (ops_timing_output, opsc.py:435-443).
+ */
+int osbl_profile(osbl_ctx* ctx, int enable);
+int osbl_kernel_stats(osbl_ctx* ctx, int index, char* name, int name_len, double* total_ms, int64_t* launches);
+
+/*
+ * Algorithmic work of one RK stage of this configuration: HBM bytes
+ * 8*(4*nvar) per interior cell (read Q and the RK register, write both) and
+ * FP64 operations per interior cell counted from the kernel formulas.
+ */
+int osbl_work_per_stage(osbl_ctx* ctx, double* bytes, double* flops);
+
+/* Algorithmic HBM bytes and FP64 operations of one launch of kernel family
+   `index` (the families of osbl_kernel_stats). */
+int osbl_kernel_work(osbl_ctx* ctx, int index, double* bytes, double* flops);
+
+/* Number of RK stages of the configured temporal scheme. */
+int osbl_num_stages(osbl_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPENSBLI_AMD_H */

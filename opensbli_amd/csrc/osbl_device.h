@@ -1,0 +1,492 @@
+// osbl_device.h -- device-side building blocks of the OpenSBLI RHS on gfx950:
+// grid indexing, constituent relations, Roe averaging, the Euler
+// eigensystems, WENO5-JS/Z and TENO5/6 reconstructions.
+//
+// All arithmetic is FP64.  Formulas follow the reference's symbolic
+// definitions; where an expression is rearranged to save divisions the
+// rearrangement is algebraically exact and noted at the function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
+
+#include "../../include/opensbli_amd.h"
+
+namespace osbl {
+
+// ---------------------------------------------------------------------------
+// Storage: one array per field, x fastest.  The interior x range starts at a
+// 128-byte boundary (offset OX doubles into each row), y and z carry `h`
+// halo points.  Absent dimensions have n = 1 and no halo.
+// ---------------------------------------------------------------------------
+constexpr int OX = 16;
+
+struct Grid {
+  int nd;
+  int n[3];
+  int h[3];
+  int pitch;  // doubles per row
+  int rows;   // rows per plane
+  int planes;
+  int plane;  // doubles per plane
+  long long fs;  // doubles per field (padded)
+  __host__ __device__ __forceinline__ int idx(int i, int j, int k) const {
+    return (i + OX) + pitch * ((j + h[1]) + rows * (k + h[2]));
+  }
+  __host__ __device__ __forceinline__ int stride(int d) const { return d == 0 ? 1 : (d == 1 ? pitch : plane); }
+};
+
+struct Phys {
+  double gama, gm1, Minf, Re, Pr, dt, eps, teno_ct;
+  double inv_delta[3];
+  double inv_re, cq;  // 1/Re and 1/((gama-1) Minf^2 Pr Re)
+};
+
+// ---------------------------------------------------------------------------
+// compile-time loop
+// ---------------------------------------------------------------------------
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor<B + 1, E>(f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Point state: conservative variables and the constituent relations
+// u_i = rhou_i/rho, p = (gama-1)(rhoE - rho/2 u.u), a = (gama p/rho)^0.5
+// (taylor_green_vortex.py:29-31, Sod_shock_tube.py:26-28).
+// ---------------------------------------------------------------------------
+template <int ND>
+struct Pt {
+  double q[ND + 2];
+  double u[ND];
+  double p, a;
+};
+
+template <int ND>
+__device__ __forceinline__ void load_pt(Pt<ND>& s, const double* __restrict__ Q, long long fs, int off,
+                                         const Phys& ph) {
+#pragma unroll
+  for (int m = 0; m < ND + 2; ++m) s.q[m] = Q[m * fs + off];
+  const double ir = 1.0 / s.q[0];
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    s.u[m] = s.q[1 + m] * ir;
+    ke += s.u[m] * s.u[m];
+  }
+  s.p = ph.gm1 * (s.q[ND + 1] - 0.5 * s.q[0] * ke);
+  s.a = sqrt(ph.gama * s.p * ir);
+}
+
+// ---------------------------------------------------------------------------
+// Interface state between points i and i+1 (the averaged symbols of the
+// eigensystem).  Roe: averaging.py:70-114.  Simple: averaging.py:38-59.
+// ---------------------------------------------------------------------------
+template <int ND>
+struct Avg {
+  double rho, a, u[ND];
+  double ia, ir;           // 1/a, 1/rho (inv_AVG_a, inv_AVG_rho, shock_capturing.py:319-331)
+  double phi;              // (gama-1)|u|^2/2
+  double alpha, bta;       // rho/(a sqrt2), 1/(rho a sqrt2)
+  double Hh;               // a^2/(gama-1) + |u|^2/2  (1-D eigensystem)
+};
+
+template <int ND, int AVG>
+__device__ __forceinline__ void interface_state(Avg<ND>& v, const Pt<ND>& L, const Pt<ND>& R, const Phys& ph) {
+  if constexpr (AVG == OSBL_AVG_ROE) {
+    const double sL = sqrt(L.q[0]), sR = sqrt(R.q[0]);
+    const double inv = 1.0 / (sR + sL);
+    v.rho = sL * sR;  // sqrt(rho_L rho_R)
+    double ke = 0.0;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      v.u[m] = (sL * L.u[m] + sR * R.u[m]) * inv;
+      ke += v.u[m] * v.u[m];
+    }
+    // sqrt(rho) H = (rhoE + p)/sqrt(rho)
+    const double hh = ((L.q[ND + 1] + L.p) / sL + (R.q[ND + 1] + R.p) / sR) * inv;
+    v.a = sqrt(ph.gm1 * (hh - 0.5 * ke));
+  } else {
+    v.rho = 0.5 * (L.q[0] + R.q[0]);
+#pragma unroll
+    for (int m = 0; m < ND; ++m) v.u[m] = 0.5 * (L.u[m] + R.u[m]);
+    v.a = 0.5 * (L.a + R.a);
+  }
+  double uu = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) uu += v.u[m] * v.u[m];
+  v.ia = 1.0 / v.a;
+  v.ir = 1.0 / v.rho;
+  v.phi = ph.gm1 * 0.5 * uu;
+  constexpr double RS2 = 0.70710678118654752440;  // 1/sqrt(2.0)
+  v.alpha = v.rho * v.ia * RS2;
+  v.bta = v.ir * v.ia * RS2;
+  v.Hh = v.a * v.a / ph.gm1 + 0.5 * uu;
+}
+
+// ---------------------------------------------------------------------------
+// Left/right eigenvectors of the Euler flux Jacobian for a Cartesian
+// direction D (unit normal k = e_D), euler_eigensystem.py:57-135.  Field
+// order follows the reference: 3-D (U,U,U,U+a,U-a), 2-D (U,U,U+a,U-a),
+// 1-D (u-a,u,u+a).  In 3-D the rows k != D are the two shear waves whose
+// non-zero entries sit at component c = 3-D-k with sign eps(D,k,c); this is
+// the reference's k_t algebra with k_t = e_D substituted.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int lc3(int D, int k) { return 3 - D - k; }
+__host__ __device__ constexpr int eps3(int a, int b, int c) {
+  return (a == 0 && b == 1 && c == 2) || (a == 1 && b == 2 && c == 0) || (a == 2 && b == 0 && c == 1) ? 1 : -1;
+}
+
+// bit m set when L[k][m] is structurally non-zero
+template <int ND, int D, int K>
+__host__ __device__ constexpr unsigned lmask() {
+  if constexpr (ND == 3) {
+    if constexpr (K < 3 && K != D) return 1u | (1u << (1 + lc3(D, K)));
+    else return 0x1Fu;
+  } else if constexpr (ND == 2) {
+    if constexpr (K == 1) return 1u | (1u << (1 + (1 - D)));
+    else return 0xFu;
+  } else {
+    return 0x7u;
+  }
+}
+template <int ND, int D, int K>
+__host__ __device__ constexpr unsigned rmask() {
+  if constexpr (ND == 3) {
+    if constexpr (K < 3 && K != D) return (1u << (1 + lc3(D, K))) | (1u << 4);
+    else return 0x1Fu;
+  } else if constexpr (ND == 2) {
+    if constexpr (K == 1) return (1u << (1 + (1 - D))) | (1u << 3);
+    else return 0xFu;
+  } else {
+    return 0x7u;
+  }
+}
+
+// row K of the left eigenvector matrix
+template <int ND, int D, int K>
+__device__ __forceinline__ void lrow(const Avg<ND>& v, const Phys& ph, double l[ND + 2]) {
+  const double gm1 = ph.gm1;
+  if constexpr (ND == 3) {
+    const double ia2 = v.ia * v.ia;
+    if constexpr (K == D) {
+      l[0] = 1.0 - v.phi * ia2;
+      l[1] = gm1 * v.u[0] * ia2;
+      l[2] = gm1 * v.u[1] * ia2;
+      l[3] = gm1 * v.u[2] * ia2;
+      l[4] = -gm1 * ia2;
+    } else if constexpr (K < 3) {
+      constexpr int c = lc3(D, K);
+      constexpr double s = (double)eps3(D, K, c);
+      l[0] = -s * v.u[c] * v.ir;
+      l[1 + c] = s * v.ir;
+    } else {
+      constexpr double sg = (K == 3) ? 1.0 : -1.0;
+      const double th = v.u[D];
+      l[0] = v.bta * (v.phi - sg * th * v.a);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) l[1 + m] = -v.bta * (gm1 * v.u[m] - (m == D ? sg * v.a : 0.0));
+      l[4] = v.bta * gm1;
+    }
+  } else if constexpr (ND == 2) {
+    const double ia2 = v.ia * v.ia;
+    if constexpr (K == 0) {
+      l[0] = 1.0 - v.phi * ia2;
+      l[1] = gm1 * v.u[0] * ia2;
+      l[2] = gm1 * v.u[1] * ia2;
+      l[3] = -gm1 * ia2;
+    } else if constexpr (K == 1) {
+      constexpr int c = 1 - D;
+      constexpr double s = (D == 0) ? -1.0 : 1.0;
+      l[0] = -s * v.u[c] * v.ir;
+      l[1 + c] = s * v.ir;
+    } else {
+      constexpr double sg = (K == 2) ? 1.0 : -1.0;
+      const double th = v.u[D];
+      l[0] = v.bta * (v.phi - sg * v.a * th);
+#pragma unroll
+      for (int m = 0; m < 2; ++m) l[1 + m] = -v.bta * (gm1 * v.u[m] - (m == D ? sg * v.a : 0.0));
+      l[3] = v.bta * gm1;
+    }
+  } else {
+    const double u = v.u[0], a = v.a, Hh = v.Hh;
+    const double den = 2.0 * Hh - u * u;
+    const double iden = 1.0 / den;
+    if constexpr (K == 0) {
+      l[0] = u * (2.0 * Hh + a * u - u * u) * v.ia * 0.5 * iden;
+      l[1] = (-Hh - a * u + 0.5 * u * u) * v.ia * iden;
+      l[2] = iden;
+    } else if constexpr (K == 1) {
+      l[0] = 2.0 * (Hh - u * u) * iden;
+      l[1] = 2.0 * u * iden;
+      l[2] = -2.0 * iden;
+    } else {
+      l[0] = u * (-2.0 * Hh + a * u + u * u) * v.ia * 0.5 * iden;
+      l[1] = (Hh - a * u - 0.5 * u * u) * v.ia * iden;
+      l[2] = iden;
+    }
+  }
+}
+
+// column K of the right eigenvector matrix
+template <int ND, int D, int K>
+__device__ __forceinline__ void rcol(const Avg<ND>& v, const Phys& ph, double r[ND + 2]) {
+  const double igm1 = 1.0 / ph.gm1;
+  if constexpr (ND == 3) {
+    if constexpr (K == D) {
+      r[0] = 1.0;
+      r[1] = v.u[0];
+      r[2] = v.u[1];
+      r[3] = v.u[2];
+      r[4] = v.phi * igm1;
+    } else if constexpr (K < 3) {
+      constexpr int c = lc3(D, K);
+      constexpr double s = (double)eps3(D, K, c);
+      r[1 + c] = s * v.rho;
+      r[4] = s * v.rho * v.u[c];
+    } else {
+      constexpr double sg = (K == 3) ? 1.0 : -1.0;
+      r[0] = v.alpha;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) r[1 + m] = v.alpha * (v.u[m] + (m == D ? sg * v.a : 0.0));
+      r[4] = v.alpha * ((v.phi + v.a * v.a) * igm1 + sg * v.u[D] * v.a);
+    }
+  } else if constexpr (ND == 2) {
+    if constexpr (K == 0) {
+      r[0] = 1.0;
+      r[1] = v.u[0];
+      r[2] = v.u[1];
+      r[3] = v.phi * igm1;
+    } else if constexpr (K == 1) {
+      constexpr int c = 1 - D;
+      constexpr double s = (D == 0) ? -1.0 : 1.0;
+      r[1 + c] = s * v.rho;
+      r[3] = s * v.rho * v.u[c];
+    } else {
+      constexpr double sg = (K == 2) ? 1.0 : -1.0;
+      r[0] = v.alpha;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) r[1 + m] = v.alpha * (v.u[m] + (m == D ? sg * v.a : 0.0));
+      r[3] = v.alpha * ((v.phi + v.a * v.a) * igm1 + sg * v.a * v.u[D]);
+    }
+  } else {
+    const double u = v.u[0], a = v.a, Hh = v.Hh;
+    if constexpr (K == 0) {
+      r[0] = 1.0; r[1] = u - a; r[2] = Hh - u * a;
+    } else if constexpr (K == 1) {
+      r[0] = 1.0; r[1] = u; r[2] = 0.5 * u * u;
+    } else {
+      r[0] = 1.0; r[1] = u + a; r[2] = Hh + u * a;
+    }
+  }
+}
+
+// |lambda_K| at a stencil point from its own u_D and a (shock_capturing.py:512-536)
+template <int ND, int D, int K>
+__device__ __forceinline__ double abs_lambda(const Pt<ND>& s) {
+  const double ud = s.u[D];
+  if constexpr (ND == 1) {
+    if constexpr (K == 0) return fabs(ud - s.a);
+    else if constexpr (K == 1) return fabs(ud);
+    else return fabs(ud + s.a);
+  } else {
+    constexpr int NV = ND + 2;
+    if constexpr (K == NV - 2) return fabs(ud + s.a);
+    else if constexpr (K == NV - 1) return fabs(ud - s.a);
+    else return fabs(ud);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructions at i+1/2.  Arguments are the 6 points f[-2..3] in the
+// orientation of the "right" reconstruction (weno.py side +1, applied to
+// f+).  The left reconstruction (side -1, applied to f-) is the right one
+// applied to the mirrored points f[3-p] -- except for TENO6, whose
+// right-side fourth indicator has the unsquared 781/720 term
+// (teno.py:166-167), hence the QUIRK flag.
+// ---------------------------------------------------------------------------
+
+// WENO5-Z (weno.py:283-338) with eps = 1e-14, p = 2.  The weights are
+// alpha_r = d_r (1 + tau^2/b_r^2), b_r = eps + beta_r; multiplying every
+// alpha_r by b0^2 b1^2 b2^2 leaves omega_r = alpha_r / sum(alpha) unchanged
+// and needs one division instead of four.
+__device__ __forceinline__ double weno5z(double fm2, double fm1, double f0, double f1, double f2) {
+  const double t0 = f0 - 2.0 * f1 + f2, s0 = 3.0 * f0 - 4.0 * f1 + f2;
+  const double t1 = fm1 - 2.0 * f0 + f1, s1 = fm1 - f1;
+  const double t2 = fm2 - 2.0 * fm1 + f0, s2 = fm2 - 4.0 * fm1 + 3.0 * f0;
+  constexpr double C13 = 13.0 / 12.0;
+  const double b0 = C13 * t0 * t0 + 0.25 * s0 * s0;
+  const double b1 = C13 * t1 * t1 + 0.25 * s1 * s1;
+  const double b2 = C13 * t2 * t2 + 0.25 * s2 * s2;
+  const double tau = fabs(b0 - b2);
+  const double tt = tau * tau;
+  const double e0 = 1.0e-14 + b0, e1 = 1.0e-14 + b1, e2 = 1.0e-14 + b2;
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double A0 = 0.3 * (q0 + tt) * (q1 * q2);
+  const double A1 = 0.6 * (q1 + tt) * (q0 * q2);
+  const double A2 = 0.1 * (q2 + tt) * (q0 * q1);
+  constexpr double S6 = 1.0 / 6.0;
+  const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
+  const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
+  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  return (A0 * r0 + A1 * r1 + A2 * r2) / (A0 + A1 + A2);
+}
+
+// WENO5-JS (weno.py:341-369) with eps = 1e-6: alpha_r = d_r/(eps+beta_r)^2,
+// rewritten as d_r * prod_{s!=r} (eps+beta_s)^2 with one division.
+__device__ __forceinline__ double weno5js(double fm2, double fm1, double f0, double f1, double f2) {
+  const double t0 = f0 - 2.0 * f1 + f2, s0 = 3.0 * f0 - 4.0 * f1 + f2;
+  const double t1 = fm1 - 2.0 * f0 + f1, s1 = fm1 - f1;
+  const double t2 = fm2 - 2.0 * fm1 + f0, s2 = fm2 - 4.0 * fm1 + 3.0 * f0;
+  constexpr double C13 = 13.0 / 12.0;
+  const double e0 = 1.0e-6 + (C13 * t0 * t0 + 0.25 * s0 * s0);
+  const double e1 = 1.0e-6 + (C13 * t1 * t1 + 0.25 * s1 * s1);
+  const double e2 = 1.0e-6 + (C13 * t2 * t2 + 0.25 * s2 * s2);
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double A0 = 0.3 * (q1 * q2), A1 = 0.6 * (q0 * q2), A2 = 0.1 * (q0 * q1);
+  constexpr double S6 = 1.0 / 6.0;
+  const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
+  const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
+  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  return (A0 * r0 + A1 * r1 + A2 * r2) / (A0 + A1 + A2);
+}
+
+__device__ __forceinline__ double pow6(double x) {
+  const double x2 = x * x;
+  return x2 * x2 * x2;
+}
+
+// TENO5 / TENO6 (teno.py:57-234, 407-465), literal weights:
+// alpha_r = (1 + tau/(eps + beta_r))^6, delta_r = 0 if alpha_r*(1/sum alpha) < C_T,
+// omega_r = d_r delta_r / sum(d delta).
+template <bool SIX, bool QUIRK>
+__device__ __forceinline__ double teno(double fm2, double fm1, double f0, double f1, double f2, double f3,
+                                       double eps, double ct) {
+  constexpr double C13 = 13.0 / 12.0;
+  double q;
+  q = fm1 - f1;
+  double b0 = 0.25 * q * q;
+  q = fm1 - 2.0 * f0 + f1;
+  b0 += C13 * q * q;
+  q = 3.0 * f0 - 4.0 * f1 + f2;
+  double b1 = 0.25 * q * q;
+  q = f0 - 2.0 * f1 + f2;
+  b1 += C13 * q * q;
+  q = fm2 - 4.0 * fm1 + 3.0 * f0;
+  double b2 = 0.25 * q * q;
+  q = fm2 - 2.0 * fm1 + f0;
+  b2 += C13 * q * q;
+  constexpr double S6 = 1.0 / 6.0;
+  const double r0 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
+  const double r1 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
+  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  if constexpr (SIX) {
+    q = -11.0 * f0 + 18.0 * f1 - 9.0 * f2 + 2.0 * f3;
+    double b3 = (1.0 / 36.0) * q * q;
+    q = 2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3;
+    b3 += C13 * q * q;
+    q = -f0 + 3.0 * f1 - 3.0 * f2 + f3;
+    if constexpr (QUIRK) b3 += (781.0 / 720.0) * q;
+    else b3 += (781.0 / 720.0) * q * q;
+    const double r3 = (3.0 * f0 + 13.0 * f1 - 5.0 * f2 + f3) * (1.0 / 12.0);
+    const double tau = fabs(b3 - (b0 + b2 - 4.0 * b1) * S6);
+    const double a0 = pow6(1.0 + tau / (eps + b0));
+    const double a1 = pow6(1.0 + tau / (eps + b1));
+    const double a2 = pow6(1.0 + tau / (eps + b2));
+    const double a3 = pow6(1.0 + tau / (eps + b3));
+    const double ias = 1.0 / (a0 + a1 + a2 + a3);
+    const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
+    const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
+    const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
+    const double d3 = (a3 * ias < ct) ? 0.0 : 1.0;
+    constexpr double D0 = 231.0 / 500.0, D1 = 3.0 / 10.0, D2 = 27.0 / 500.0, D3 = 23.0 / 125.0;
+    const double w0 = D0 * d0, w1 = D1 * d1, w2 = D2 * d2, w3 = D3 * d3;
+    const double iws = 1.0 / (w0 + w1 + w2 + w3);
+    return (w0 * r0 + w1 * r1 + w2 * r2 + w3 * r3) * iws;
+  } else {
+    const double tau = fabs(b0 - b2);
+    const double a0 = pow6(1.0 + tau / (eps + b0));
+    const double a1 = pow6(1.0 + tau / (eps + b1));
+    const double a2 = pow6(1.0 + tau / (eps + b2));
+    const double ias = 1.0 / (a0 + a1 + a2);
+    const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
+    const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
+    const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
+    constexpr double D0 = 11.0 / 20.0, D1 = 4.0 / 10.0, D2 = 1.0 / 20.0;
+    const double w0 = D0 * d0, w1 = D1 * d1, w2 = D2 * d2;
+    const double iws = 1.0 / (w0 + w1 + w2);
+    return (w0 * r0 + w1 * r1 + w2 * r2) * iws;
+  }
+}
+
+// Sum of the right reconstruction of f+ and the left reconstruction of f-
+// (shock_capturing.py:257-265) for one characteristic field.
+template <int CONV>
+__device__ __forceinline__ double recon_pair(const double fp[6], const double fm[6], double eps, double ct) {
+  if constexpr (CONV == OSBL_CONV_WENO_Z) {
+    return weno5z(fp[0], fp[1], fp[2], fp[3], fp[4]) + weno5z(fm[5], fm[4], fm[3], fm[2], fm[1]);
+  } else if constexpr (CONV == OSBL_CONV_WENO_JS) {
+    return weno5js(fp[0], fp[1], fp[2], fp[3], fp[4]) + weno5js(fm[5], fm[4], fm[3], fm[2], fm[1]);
+  } else if constexpr (CONV == OSBL_CONV_TENO6) {
+    return teno<true, true>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct) +
+           teno<true, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct);
+  } else {
+    return teno<false, false>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct) +
+           teno<false, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LLF characteristic flux at the interface between window points 2 and 3
+// (shock_capturing.py:357-448).  CF_k = L_k . F is formed from CS_k = L_k . Q
+// through F = u_D Q + p (e_{1+D} + u_D e_last), which holds exactly for the
+// Euler flux vector.
+// ---------------------------------------------------------------------------
+template <int ND, int D, int CONV, int AVG>
+__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2]) {
+  constexpr int NV = ND + 2;
+  Avg<ND> v;
+  interface_state<ND, AVG>(v, w[2], w[3], ph);
+#pragma unroll
+  for (int m = 0; m < NV; ++m) fh[m] = 0.0;
+  sfor<0, NV>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    constexpr unsigned LM = lmask<ND, D, K>();
+    constexpr unsigned RM = rmask<ND, D, K>();
+    double l[NV];
+    lrow<ND, D, K>(v, ph, l);
+    double lam = 0.0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) lam = fmax(lam, abs_lambda<ND, D, K>(w[p]));
+    // coefficient of p in L_k . F - u_D L_k . Q
+    double fp[6], fm[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      double cs = 0.0;
+      sfor<0, NV>([&](auto mc) {
+        constexpr int M = decltype(mc)::value;
+        if constexpr ((LM >> M) & 1u) cs += l[M] * w[p].q[M];
+      });
+      double lp = 0.0;
+      if constexpr ((LM >> (1 + D)) & 1u) lp = l[1 + D];
+      if constexpr ((LM >> (NV - 1)) & 1u) lp += w[p].u[D] * l[NV - 1];
+      const double cf = w[p].u[D] * cs + w[p].p * lp;
+      fp[p] = 0.5 * (cf + lam * cs);
+      fm[p] = 0.5 * (cf - lam * cs);
+    }
+    const double rc = recon_pair<CONV>(fp, fm, ph.eps, ph.teno_ct);
+    double r[NV];
+    rcol<ND, D, K>(v, ph, r);
+    sfor<0, NV>([&](auto mc) {
+      constexpr int M = decltype(mc)::value;
+      if constexpr ((RM >> M) & 1u) fh[M] += r[M] * rc;
+    });
+  });
+}
+
+}  // namespace osbl

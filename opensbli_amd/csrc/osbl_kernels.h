@@ -1,0 +1,19 @@
+// osbl_kernels.h -- launch interface of the gfx950 kernels (host side).
+#pragma once
+
+#include "osbl_device.h"
+
+namespace osbl {
+
+void launch_halo_periodic(double* Q, int nv, const Grid& g, hipStream_t s);
+void launch_extrap0(double* Q, int nv, const Grid& g, int d, int side, hipStream_t s);
+void launch_prims(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
+void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                  bool acc, hipStream_t s);
+void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
+                   double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s);
+void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s);
+void launch_unpack(const double* in, double* F, const Grid& g, hipStream_t s);
+long long halo_points(const Grid& g);
+
+}  // namespace osbl

@@ -1,0 +1,493 @@
+// osbl_kernels.hip -- gfx950 kernels of one RK stage of the OpenSBLI RHS:
+//   * boundary fills (periodic wrap, zeroth-order extrapolation),
+//   * constituent relations for the central/viscous stencils,
+//   * one characteristic LLF sweep per direction (WENO5-JS/Z, TENO5/6),
+//   * the fused central (skew or conservative) + viscous + RK update.
+#include "osbl_kernels.h"
+
+namespace osbl {
+
+// ---------------------------------------------------------------------------
+// Periodic boundary (periodic.py:42-56): every halo point of the shell takes
+// the value of its periodic image, corners and edges included, in one pass.
+// The shell is enumerated as z-slabs (full x,y extent), then y-slabs of the
+// interior z range, then x-slabs of the interior y,z range.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_halo_periodic(double* __restrict__ Q, int nv, Grid g, long long total) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int hx = g.h[0], hy = g.h[1], hz = g.h[2];
+  const int PX = g.n[0] + 2 * hx, PY = g.n[1] + 2 * hy;
+  const long long nzs = 2LL * hz * PY * PX;
+  const long long nys = 2LL * hy * g.n[2] * PX;
+  int i, j, k;
+  if (t < nzs) {
+    const int ii = (int)(t % PX);
+    const long long r = t / PX;
+    const int jj = (int)(r % PY);
+    const int kk = (int)(r / PY);
+    i = ii - hx;
+    j = jj - hy;
+    k = kk < hz ? kk - hz : g.n[2] + (kk - hz);
+  } else if (t < nzs + nys) {
+    const long long u = t - nzs;
+    const int ii = (int)(u % PX);
+    const long long r = u / PX;
+    const int jj = (int)(r % (2 * hy));
+    k = (int)(r / (2 * hy));
+    i = ii - hx;
+    j = jj < hy ? jj - hy : g.n[1] + (jj - hy);
+  } else {
+    const long long u = t - nzs - nys;
+    const int ii = (int)(u % (2 * hx));
+    const long long r = u / (2 * hx);
+    j = (int)(r % g.n[1]);
+    k = (int)(r / g.n[1]);
+    i = ii < hx ? ii - hx : g.n[0] + (ii - hx);
+  }
+  auto wrap = [](int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+  const int dst = g.idx(i, j, k);
+  const int src = g.idx(wrap(i, g.n[0]), wrap(j, g.n[1]), wrap(k, g.n[2]));
+  for (int m = 0; m < nv; ++m) Q[m * g.fs + dst] = Q[m * g.fs + src];
+}
+
+// Zeroth-order extrapolation (extrapolation.py:29-58): on the boundary plane
+// of direction d and side s (the other directions' halos included), the
+// boundary point and its halo points take the value of the first point inside.
+__global__ void __launch_bounds__(256) k_extrap0(double* __restrict__ Q, int nv, Grid g, int d, int side) {
+  const int e1 = (d == 0) ? 1 : 0, e2 = (d == 2) ? 1 : 2;
+  const int n1 = g.n[e1] + 2 * g.h[e1], n2 = g.n[e2] + 2 * g.h[e2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n1 * n2) return;
+  int c[3];
+  c[e1] = t % n1 - g.h[e1];
+  c[e2] = t / n1 - g.h[e2];
+  const int N = g.n[d];
+  const int b = side == 0 ? 0 : N - 1;
+  const int from = side == 0 ? 1 : N - 2;
+  const int dir = side == 0 ? -1 : 1;
+  c[d] = from;
+  const int src = g.idx(c[0], c[1], c[2]);
+  for (int m = 0; m < nv; ++m) {
+    const double v = Q[m * g.fs + src];
+    for (int hh = 0; hh <= g.h[d]; ++hh) {
+      c[d] = b + dir * hh;
+      Q[m * g.fs + g.idx(c[0], c[1], c[2])] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Constituent relations over the padded domain: u_m, p, T
+// (taylor_green_vortex.py:29-31).  Read by the central and viscous stencils.
+// ---------------------------------------------------------------------------
+template <int ND>
+__global__ void __launch_bounds__(256) k_prims(const double* __restrict__ Q, double* __restrict__ P, Grid g, Phys ph) {
+  const int PX = g.n[0] + 2 * g.h[0];
+  const int PY = g.n[1] + 2 * g.h[1];
+  const int PZ = g.n[2] + 2 * g.h[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - g.h[0];
+  const int j = blockIdx.y - g.h[1];
+  const int k = blockIdx.z - g.h[2];
+  if (i >= g.n[0] + g.h[0]) return;
+  (void)PX; (void)PY; (void)PZ;
+  const int x = g.idx(i, j, k);
+  const double rho = Q[x];
+  const double ir = 1.0 / rho;
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    const double u = Q[(1 + m) * g.fs + x] * ir;
+    P[m * g.fs + x] = u;
+    ke += u * u;
+  }
+  const double p = ph.gm1 * (Q[(ND + 1) * g.fs + x] - 0.5 * rho * ke);
+  P[ND * g.fs + x] = p;
+  P[(ND + 1) * g.fs + x] = p * ph.gama * ph.Minf * ph.Minf * ir;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic sweep along direction D.  Each thread owns a segment of
+// `seg` cells of one grid line and slides a 6-point window along it: every
+// point's constituent relations are evaluated once, every interface flux
+// once, and the residual of a cell is the difference of its two interface
+// fluxes (shock_capturing.py:21-34).  Threads of a wave are adjacent in x,
+// except for D = 0, where they are adjacent lines.
+// ---------------------------------------------------------------------------
+template <int ND, int D, int CONV, int AVG, bool ACC>
+__global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q, double* __restrict__ R, Grid g,
+                                                     Phys ph, int seg, long long total) {
+  constexpr int NV = ND + 2;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int c[3];
+  int s;
+  if constexpr (D == 0) {
+    c[1] = (int)(t % g.n[1]);
+    const long long r = t / g.n[1];
+    c[2] = (int)(r % g.n[2]);
+    s = (int)(r / g.n[2]);
+  } else {
+    constexpr int O = (D == 1) ? 2 : 1;
+    c[0] = (int)(t % g.n[0]);
+    const long long r = t / g.n[0];
+    c[O] = (int)(r % g.n[O]);
+    s = (int)(r / g.n[O]);
+  }
+  const int c0 = s * seg;
+  const int c1 = min(c0 + seg, g.n[D]);
+  c[D] = c0 - 3;
+  const int st = g.stride(D);
+  int off = g.idx(c[0], c[1], c[2]);
+  const double inv = ph.inv_delta[D];
+
+  Pt<ND> w[6];
+#pragma unroll
+  for (int p = 0; p < 5; ++p) load_pt<ND>(w[p], Q, g.fs, off + p * st, ph);
+  double fprev[NV];
+  for (int f = c0 - 1; f < c1; ++f) {
+    load_pt<ND>(w[5], Q, g.fs, off + 5 * st, ph);
+    double fh[NV];
+    char_flux<ND, D, CONV, AVG>(w, ph, fh);
+    if (f >= c0) {
+      const int x = off + 2 * st;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double v = -(fh[m] - fprev[m]) * inv;
+        if constexpr (ACC) R[m * g.fs + x] += v;
+        else R[m * g.fs + x] = v;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; ++m) fprev[m] = fh[m];
+#pragma unroll
+    for (int p = 0; p < 5; ++p) w[p] = w[p + 1];
+    off += st;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Central fourth-order stencils (scheme.py:81-85): D1 weights
+// [1/12, -2/3, 0, 2/3, -1/12], D2 weights [-1/12, 4/3, -5/2, 4/3, -1/12].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double d1(const double* __restrict__ f, int x, int st) {
+  return (1.0 / 12.0) * (f[x - 2 * st] - f[x + 2 * st]) + (2.0 / 3.0) * (f[x + st] - f[x - st]);
+}
+__device__ __forceinline__ double d2(const double* __restrict__ f, int x, int st) {
+  return (-1.0 / 12.0) * (f[x - 2 * st] + f[x + 2 * st]) + (4.0 / 3.0) * (f[x - st] + f[x + st]) - 2.5 * f[x];
+}
+
+// Viscous terms of the constant-viscosity stress tensor and heat flux
+// (taylor_green_vortex.py:14-19), expanded by the product rule as the
+// reference's Der.differentiate does (parsing.py:141-186): second derivatives
+// with the D2 stencil, mixed derivatives as D_b(D_a u) with a < b
+// (opensblifunctions.py:341-353, scheme.py:360-387).  P holds u_0..u_{ND-1}, p, T.
+template <int ND>
+__device__ __forceinline__ void viscous_terms(const double* __restrict__ P, long long fs, int x, const int st[3],
+                                              const Phys& ph, double vr[ND + 2]) {
+  const double* U[3] = {P, P + fs, P + 2 * fs};
+  const double* T = P + (ND + 1) * fs;
+  double du[ND][ND], dd[ND][ND][ND];  // du[m][j] = D_j u_m; dd[m][a][b] = D_a D_b u_m (symmetric use)
+#pragma unroll
+  for (int m = 0; m < ND; ++m)
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      du[m][j] = ph.inv_delta[j] * d1(U[m], x, st[j]);
+      dd[m][j][j] = ph.inv_delta[j] * ph.inv_delta[j] * d2(U[m], x, st[j]);
+    }
+  // mixed derivatives needed: u_a and u_b for every pair a < b
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int b = a + 1; b < ND; ++b)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int m = q == 0 ? a : b;
+        const double* f = U[m];
+        const int sa = st[a], sb = st[b];
+        const double im2 = ph.inv_delta[a] * d1(f, x - 2 * sb, sa);
+        const double im1 = ph.inv_delta[a] * d1(f, x - sb, sa);
+        const double ip1 = ph.inv_delta[a] * d1(f, x + sb, sa);
+        const double ip2 = ph.inv_delta[a] * d1(f, x + 2 * sb, sa);
+        const double v = ph.inv_delta[b] * ((1.0 / 12.0) * (im2 - ip2) + (2.0 / 3.0) * (ip1 - im1));
+        dd[m][a][b] = v;
+        dd[m][b][a] = v;
+      }
+  double divu = 0.0;
+#pragma unroll
+  for (int k = 0; k < ND; ++k) divu += du[k][k];
+  double energy = 0.0;
+#pragma unroll
+  for (int j = 0; j < ND; ++j) energy += ph.cq * ph.inv_delta[j] * ph.inv_delta[j] * d2(T, x, st[j]);
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    const double um = U[m][x];
+    double sk = 0.0;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) sk += dd[k][m][k];
+    double visc = 0.0;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      double tt = dd[m][j][j] + dd[j][m][j];
+      if (j == m) tt -= (2.0 / 3.0) * sk;
+      const double dtau = ph.inv_re * tt;
+      const double tau = ph.inv_re * (du[m][j] + du[j][m] - (j == m ? (2.0 / 3.0) * divu : 0.0));
+      visc += dtau;
+      energy += du[m][j] * tau + um * dtau;
+    }
+    vr[1 + m] += visc;
+  }
+  vr[ND + 1] += energy;
+}
+
+// Central convective terms.  Skew/Blaisdell form (parsing.py:75-111,
+// scheme.py:187-252) or the conservative flux form (parsing.py:189-216).
+template <int ND, int CONV>
+__device__ __forceinline__ void central_convective(const double* __restrict__ Q, const double* __restrict__ P,
+                                                   long long fs, int x, const int st[3], const Phys& ph,
+                                                   double r[ND + 2]) {
+  constexpr int NV = ND + 2;
+  const double* U[3] = {P, P + fs, P + 2 * fs};
+  const double* Pp = P + ND * fs;
+#pragma unroll
+  for (int m = 0; m < NV; ++m) r[m] = 0.0;
+#pragma unroll
+  for (int j = 0; j < ND; ++j) {
+    const int sj = st[j];
+    const double inv = ph.inv_delta[j];
+    constexpr double W1A = 1.0 / 12.0, W1B = 2.0 / 3.0;
+    // points -2, -1, +1, +2 with weights  W1A, -W1B, W1B, -W1A
+    const int offs[4] = {x - 2 * sj, x - sj, x + sj, x + 2 * sj};
+    const double wts[4] = {W1A, -W1B, W1B, -W1A};
+    if constexpr (CONV == OSBL_CONV_CENTRAL_SKEW) {
+      double d_ru[NV], d_q[NV], d_uj = 0.0, d_p = 0.0, d_puj = 0.0;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) { d_ru[m] = 0.0; d_q[m] = 0.0; }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int y = offs[q];
+        const double w = wts[q];
+        const double uj = U[j][y];
+        const double pp = Pp[y];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          const double qm = Q[m * fs + y];
+          d_ru[m] += w * (qm * uj);
+          d_q[m] += w * qm;
+        }
+        d_uj += w * uj;
+        d_p += w * pp;
+        d_puj += w * (pp * uj);
+      }
+      const double uj = U[j][x];
+      const double duj = inv * d_uj;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double qm = Q[m * fs + x];
+        double t = -0.5 * (inv * d_ru[m] + uj * (inv * d_q[m]) + qm * duj);
+        if (m == 1 + j) t -= inv * d_p;
+        if (m == NV - 1) t -= inv * d_puj;
+        r[m] += t;
+      }
+    } else {
+      double dF[NV];
+#pragma unroll
+      for (int m = 0; m < NV; ++m) dF[m] = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int y = offs[q];
+        const double w = wts[q];
+        const double uj = U[j][y];
+        const double pp = Pp[y];
+        dF[0] += w * Q[(1 + j) * fs + y];
+#pragma unroll
+        for (int m = 0; m < ND; ++m) dF[1 + m] += w * (Q[(1 + m) * fs + y] * uj + (m == j ? pp : 0.0));
+        dF[NV - 1] += w * ((pp + Q[(NV - 1) * fs + y]) * uj);
+      }
+#pragma unroll
+      for (int m = 0; m < NV; ++m) r[m] -= inv * dF[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Residual assembly + RK update for one interior cell.
+//   MODE 0: 3-register RK3 stage (rk_sbli.py:117-124): Qn = Qo + dt*cA*R,
+//           Qo += dt*cB*R; at the first stage Qo is read from Q (the save of
+//           rk_sbli.py:126-133 fused into the stage).
+//   MODE 1: low-storage stage (rk_LS.py:155-166): Qo = cA*Qo + dt*R,
+//           Qn = Q + cB*Qo (Qo not read when cA == 0).
+//   MODE 2: residual only (written to R).
+// For characteristic schemes the convective residual is already in R.
+// ---------------------------------------------------------------------------
+template <int ND, int CONV, bool VISC, int MODE>
+__global__ void __launch_bounds__(256) k_update(const double* __restrict__ Q, const double* __restrict__ P,
+                                                double* __restrict__ R, double* __restrict__ Qo,
+                                                double* __restrict__ Qn, Grid g, Phys ph, double cA, double cB,
+                                                int first) {
+  constexpr int NV = ND + 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n[0]) return;
+  const int j = blockIdx.y, k = blockIdx.z;
+  const int x = g.idx(i, j, k);
+  const int st[3] = {1, g.pitch, g.plane};
+  double r[NV];
+  if constexpr (CONV == OSBL_CONV_CENTRAL_SKEW || CONV == OSBL_CONV_CENTRAL_CONS) {
+    central_convective<ND, CONV>(Q, P, g.fs, x, st, ph, r);
+  } else {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) r[m] = R[m * g.fs + x];
+  }
+  if constexpr (VISC) viscous_terms<ND>(P, g.fs, x, st, ph, r);
+  if constexpr (MODE == 2) {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) R[m * g.fs + x] = r[m];
+  } else if constexpr (MODE == 0) {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      const double qo = first ? Q[m * g.fs + x] : Qo[m * g.fs + x];
+      Qn[m * g.fs + x] = qo + ph.dt * cA * r[m];
+      Qo[m * g.fs + x] = qo + ph.dt * cB * r[m];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      const double tmp = (cA == 0.0 ? 0.0 : cA * Qo[m * g.fs + x]) + ph.dt * r[m];
+      Qo[m * g.fs + x] = tmp;
+      Qn[m * g.fs + x] = Q[m * g.fs + x] + cB * tmp;
+    }
+  }
+}
+
+// interior <-> contiguous copies
+__global__ void __launch_bounds__(256) k_pack(const double* __restrict__ F, double* __restrict__ out, Grid g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n[0]) return;
+  const int j = blockIdx.y, k = blockIdx.z;
+  out[(long long)i + (long long)g.n[0] * (j + (long long)g.n[1] * k)] = F[g.idx(i, j, k)];
+}
+__global__ void __launch_bounds__(256) k_unpack(const double* __restrict__ in, double* __restrict__ F, Grid g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n[0]) return;
+  const int j = blockIdx.y, k = blockIdx.z;
+  F[g.idx(i, j, k)] = in[(long long)i + (long long)g.n[0] * (j + (long long)g.n[1] * k)];
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers (instantiation tables)
+// ---------------------------------------------------------------------------
+template <int ND, int D, int CONV, int AVG>
+static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
+                           hipStream_t s) {
+  long long lines = 1;
+  for (int e = 0; e < 3; ++e)
+    if (e != D) lines *= g.n[e];
+  const long long nseg = (g.n[D] + seg - 1) / seg;
+  const long long total = lines * nseg;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total);
+  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total);
+}
+
+template <int ND, int D, int CONV>
+static void launch_sweep_avg(int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
+                             hipStream_t s) {
+  if (avg == OSBL_AVG_ROE) launch_sweep_t<ND, D, CONV, OSBL_AVG_ROE>(Q, R, g, ph, seg, acc, s);
+  else launch_sweep_t<ND, D, CONV, OSBL_AVG_SIMPLE>(Q, R, g, ph, seg, acc, s);
+}
+
+template <int ND, int D>
+static void launch_sweep_conv(int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                              bool acc, hipStream_t s) {
+  switch (conv) {
+    case OSBL_CONV_WENO_JS: launch_sweep_avg<ND, D, OSBL_CONV_WENO_JS>(avg, Q, R, g, ph, seg, acc, s); break;
+    case OSBL_CONV_WENO_Z: launch_sweep_avg<ND, D, OSBL_CONV_WENO_Z>(avg, Q, R, g, ph, seg, acc, s); break;
+    case OSBL_CONV_TENO5: launch_sweep_avg<ND, D, OSBL_CONV_TENO5>(avg, Q, R, g, ph, seg, acc, s); break;
+    default: launch_sweep_avg<ND, D, OSBL_CONV_TENO6>(avg, Q, R, g, ph, seg, acc, s); break;
+  }
+}
+
+void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                  bool acc, hipStream_t s) {
+  if (nd == 3) {
+    if (d == 0) launch_sweep_conv<3, 0>(conv, avg, Q, R, g, ph, seg, acc, s);
+    else if (d == 1) launch_sweep_conv<3, 1>(conv, avg, Q, R, g, ph, seg, acc, s);
+    else launch_sweep_conv<3, 2>(conv, avg, Q, R, g, ph, seg, acc, s);
+  } else if (nd == 2) {
+    if (d == 0) launch_sweep_conv<2, 0>(conv, avg, Q, R, g, ph, seg, acc, s);
+    else launch_sweep_conv<2, 1>(conv, avg, Q, R, g, ph, seg, acc, s);
+  } else {
+    launch_sweep_conv<1, 0>(conv, avg, Q, R, g, ph, seg, acc, s);
+  }
+}
+
+template <int ND, int CONV, bool VISC>
+static void launch_update_m(int mode, const double* Q, const double* P, double* R, double* Qo, double* Qn,
+                            const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s) {
+  dim3 grid((g.n[0] + 127) / 128, g.n[1], g.n[2]);
+  if (mode == 0) k_update<ND, CONV, VISC, 0><<<grid, 128, 0, s>>>(Q, P, R, Qo, Qn, g, ph, cA, cB, first);
+  else if (mode == 1) k_update<ND, CONV, VISC, 1><<<grid, 128, 0, s>>>(Q, P, R, Qo, Qn, g, ph, cA, cB, first);
+  else k_update<ND, CONV, VISC, 2><<<grid, 128, 0, s>>>(Q, P, R, Qo, Qn, g, ph, cA, cB, first);
+}
+
+template <int ND>
+static void launch_update_nd(int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
+                             double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first,
+                             hipStream_t s) {
+  const bool central = conv == OSBL_CONV_CENTRAL_SKEW || conv == OSBL_CONV_CENTRAL_CONS;
+  if (central) {
+    if (conv == OSBL_CONV_CENTRAL_SKEW) {
+      if (visc) launch_update_m<ND, OSBL_CONV_CENTRAL_SKEW, true>(mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+      else launch_update_m<ND, OSBL_CONV_CENTRAL_SKEW, false>(mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+    } else {
+      if (visc) launch_update_m<ND, OSBL_CONV_CENTRAL_CONS, true>(mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+      else launch_update_m<ND, OSBL_CONV_CENTRAL_CONS, false>(mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+    }
+  } else {
+    if (visc) launch_update_m<ND, OSBL_CONV_WENO_Z, true>(mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+    else launch_update_m<ND, OSBL_CONV_WENO_Z, false>(mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+  }
+}
+
+void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
+                   double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s) {
+  if (nd == 3) launch_update_nd<3>(conv, visc, mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+  else if (nd == 2) launch_update_nd<2>(conv, visc, mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+  else launch_update_nd<1>(conv, visc, mode, Q, P, R, Qo, Qn, g, ph, cA, cB, first, s);
+}
+
+void launch_prims(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s) {
+  const int PX = g.n[0] + 2 * g.h[0];
+  dim3 grid((PX + 255) / 256, g.n[1] + 2 * g.h[1], g.n[2] + 2 * g.h[2]);
+  if (nd == 3) k_prims<3><<<grid, 256, 0, s>>>(Q, P, g, ph);
+  else if (nd == 2) k_prims<2><<<grid, 256, 0, s>>>(Q, P, g, ph);
+  else k_prims<1><<<grid, 256, 0, s>>>(Q, P, g, ph);
+}
+
+long long halo_points(const Grid& g) {
+  const long long PX = g.n[0] + 2 * g.h[0], PY = g.n[1] + 2 * g.h[1];
+  return 2LL * g.h[2] * PY * PX + 2LL * g.h[1] * g.n[2] * PX + 2LL * g.h[0] * g.n[1] * (long long)g.n[2];
+}
+
+void launch_halo_periodic(double* Q, int nv, const Grid& g, hipStream_t s) {
+  const long long total = halo_points(g);
+  if (total == 0) return;
+  k_halo_periodic<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(Q, nv, g, total);
+}
+
+void launch_extrap0(double* Q, int nv, const Grid& g, int d, int side, hipStream_t s) {
+  const int e1 = (d == 0) ? 1 : 0, e2 = (d == 2) ? 1 : 2;
+  const int n = (g.n[e1] + 2 * g.h[e1]) * (g.n[e2] + 2 * g.h[e2]);
+  k_extrap0<<<(n + 255) / 256, 256, 0, s>>>(Q, nv, g, d, side);
+}
+
+void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s) {
+  dim3 grid((g.n[0] + 255) / 256, g.n[1], g.n[2]);
+  k_pack<<<grid, 256, 0, s>>>(F, out, g);
+}
+void launch_unpack(const double* in, double* F, const Grid& g, hipStream_t s) {
+  dim3 grid((g.n[0] + 255) / 256, g.n[1], g.n[2]);
+  k_unpack<<<grid, 256, 0, s>>>(in, F, g);
+}
+
+}  // namespace osbl

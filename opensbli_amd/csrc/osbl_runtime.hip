@@ -1,0 +1,480 @@
+// osbl_runtime.hip -- host runtime and C-ABI of libosbl_hip.so.
+//
+// A context owns the block's datasets on one device and a HIP stream; the
+// time loop of the generated program (algorithm.py:423-477) is replayed as a
+// sequence of stream-ordered launches per RK stage:
+//   boundary conditions -> [constituent relations] -> characteristic sweeps
+//   -> fused central/viscous residual + RK update -> boundary conditions.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "osbl_kernels.h"
+
+using namespace osbl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return fail(OSBL_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+enum Family { F_BC = 0, F_PRIMS, F_SWEEP_X, F_SWEEP_Y, F_SWEEP_Z, F_UPDATE, F_COUNT };
+const char* kFamilyNames[F_COUNT] = {"boundary", "constituent_relations", "char_sweep_x", "char_sweep_y",
+                                     "char_sweep_z", "residual_rk_update"};
+
+int rk_coefficients(int rk, double* a, double* b) {
+  switch (rk) {
+    case OSBL_RK3:  // rknew / rkold, rk_sbli.py:58-60
+      a[0] = 2.0 / 3.0; a[1] = 5.0 / 12.0; a[2] = 3.0 / 5.0;
+      b[0] = 1.0 / 4.0; b[1] = 3.0 / 20.0; b[2] = 3.0 / 5.0;
+      return 3;
+    case OSBL_RK3_LS:  // rk_LS.py:91-94
+      a[0] = 0.0; a[1] = -5.0 / 9.0; a[2] = -153.0 / 128.0;
+      b[0] = 1.0 / 3.0; b[1] = 15.0 / 16.0; b[2] = 8.0 / 15.0;
+      return 3;
+    case OSBL_RK3_LS_SSP: {  // rk_LS.py:74-89 (Gottlieb-Shu-Tadmor c = 0.924574)
+      const double c = 0.924574;
+      const double z1 = sqrt(36 * c * c * c * c + 36 * c * c * c - 135 * c * c + 84 * c - 12);
+      const double z2 = 2 * c * c + c - 2;
+      const double z3 = 12 * c * c * c * c - 18 * c * c * c + 18 * c * c - 11 * c + 2;
+      const double z4 = 36 * c * c * c * c - 36 * c * c * c + 13 * c * c - 8 * c + 4;
+      const double z5 = 69 * c * c * c - 62 * c * c + 28 * c - 8;
+      const double z6 = 34 * c * c * c * c - 46 * c * c * c + 34 * c * c - 13 * c + 2;
+      const double y = 3 * z2 - z1;
+      b[0] = c;
+      b[1] = (12 * c * (c - 1) * y - y * y) / (144 * c * (3 * c - 2) * (c - 1) * (c - 1));
+      b[2] = (-24 * (3 * c - 2) * (c - 1) * (c - 1)) / (y * y - 12 * c * (c - 1) * y);
+      a[0] = 0.0;
+      a[1] = (-z1 * (6 * c * c - 4 * c + 1) + 3 * z3) / ((2 * c + 1) * z1 - 3 * (c + 2) * (2 * c - 1) * (2 * c - 1));
+      a[2] = (-z1 * z4 + 108 * (2 * c - 1) * pow(c, 5) - 3 * (2 * c - 1) * z5) /
+             (24 * z1 * c * pow(c - 1, 4) + 72 * c * z6 + 72 * pow(c, 6) * (2 * c - 13));
+      return 3;
+    }
+    case OSBL_RK4_LS: {  // rk_LS.py:96-97
+      const double A[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+      const double B[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+      for (int i = 0; i < 5; ++i) { a[i] = A[i]; b[i] = B[i]; }
+      return 5;
+    }
+    default:
+      return 0;
+  }
+}
+
+bool is_central(int conv) { return conv == OSBL_CONV_CENTRAL_SKEW || conv == OSBL_CONV_CENTRAL_CONS; }
+
+}  // namespace
+
+struct osbl_ctx {
+  osbl_config cfg{};
+  int nd = 0, nv = 0;
+  Grid g{};
+  Phys ph{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* Qbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* Qo = nullptr;
+  double* R = nullptr;
+  double* P = nullptr;
+  double* stage_buf = nullptr;  // contiguous interior staging for host copies
+  int nstages = 0;
+  double A[5]{}, B[5]{};
+  int seg = 32;
+  bool need_prims = false;
+  // profiling
+  bool prof = false;
+  struct Rec { int fam; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double fam_ms[F_COUNT]{};
+  long long fam_n[F_COUNT]{};
+};
+
+namespace {
+
+hipEvent_t get_event(osbl_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// bracket a launch with events when profiling
+template <class F>
+void timed(osbl_ctx* c, int fam, F&& launch) {
+  if (!c->prof) {
+    launch();
+    return;
+  }
+  hipEvent_t a = get_event(c), b = get_event(c);
+  (void)hipEventRecord(a, c->stream);
+  launch();
+  (void)hipEventRecord(b, c->stream);
+  c->recs.push_back({fam, a, b});
+}
+
+void drain_records(osbl_ctx* c) {
+  if (c->recs.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->fam_ms[r.fam] += ms;
+      c->fam_n[r.fam] += 1;
+    }
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+}
+
+double* Qcur(osbl_ctx* c) { return c->Qbuf[c->cur]; }
+
+void apply_bcs(osbl_ctx* c, double* Q) {
+  // Periodic directions: one pass that gives every halo point its periodic
+  // image.  Non-periodic faces are then filled by their own kernels, which
+  // overwrite whatever the wrap put there, in the order of the reference's
+  // boundary list (direction by direction, side 0 then side 1).
+  bool any_periodic = false;
+  for (int d = 0; d < c->nd; ++d) any_periodic |= c->cfg.bc[d][0] == OSBL_BC_PERIODIC;
+  if (any_periodic) timed(c, F_BC, [&] { launch_halo_periodic(Q, c->nv, c->g, c->stream); });
+  for (int d = 0; d < c->nd; ++d)
+    for (int s = 0; s < 2; ++s)
+      if (c->cfg.bc[d][s] == OSBL_BC_EXTRAP0) timed(c, F_BC, [&] { launch_extrap0(Q, c->nv, c->g, d, s, c->stream); });
+}
+
+// spatial residual of the current state into R (characteristic part only),
+// followed by the fused central/viscous + update kernel with the given mode
+void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
+  double* Q = Qcur(c);
+  double* Qn = c->Qbuf[1 - c->cur];
+  const int conv = c->cfg.conv;
+  if (c->need_prims) timed(c, F_PRIMS, [&] { launch_prims(c->nd, Q, c->P, c->g, c->ph, c->stream); });
+  if (!is_central(conv)) {
+    for (int d = 0; d < c->nd; ++d) {
+      timed(c, F_SWEEP_X + d, [&] {
+        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, d > 0, c->stream);
+      });
+    }
+  }
+  timed(c, F_UPDATE, [&] {
+    launch_update(c->nd, conv, c->cfg.viscous != OSBL_VISC_NONE, mode, Q, c->P, c->R, c->Qo, Qn, c->g, c->ph, cA,
+                  cB, first, c->stream);
+  });
+}
+
+int check_config(const osbl_config* cfg) {
+  if (cfg->ndim < 1 || cfg->ndim > 3) return fail(OSBL_E_ARG, "ndim must be 1, 2 or 3 (got %d)", cfg->ndim);
+  for (int d = 0; d < 3; ++d) {
+    const int n = cfg->n[d];
+    if (d < cfg->ndim) {
+      if (n < 8) return fail(OSBL_E_ARG, "block0np%d = %d: need at least 8 points", d, n);
+      if (!(cfg->delta[d] > 0.0)) return fail(OSBL_E_ARG, "Delta%dblock0 must be positive", d);
+    } else if (n != 1) {
+      return fail(OSBL_E_ARG, "block0np%d must be 1 for a %d-D block", d, cfg->ndim);
+    }
+  }
+  if (cfg->conv < OSBL_CONV_CENTRAL_SKEW || cfg->conv > OSBL_CONV_TENO6)
+    return fail(OSBL_E_ARG, "unknown convective scheme %d", cfg->conv);
+  if (cfg->averaging != OSBL_AVG_SIMPLE && cfg->averaging != OSBL_AVG_ROE)
+    return fail(OSBL_E_ARG, "unknown averaging %d", cfg->averaging);
+  if (cfg->viscous != OSBL_VISC_NONE && cfg->viscous != OSBL_VISC_CONST)
+    return fail(OSBL_E_ARG, "unknown viscous model %d", cfg->viscous);
+  if (cfg->rk < OSBL_RK3 || cfg->rk > OSBL_RK4_LS) return fail(OSBL_E_ARG, "unknown RK scheme %d", cfg->rk);
+  for (int d = 0; d < cfg->ndim; ++d) {
+    for (int s = 0; s < 2; ++s)
+      if (cfg->bc[d][s] != OSBL_BC_PERIODIC && cfg->bc[d][s] != OSBL_BC_EXTRAP0)
+        return fail(OSBL_E_ARG, "unknown boundary condition %d on direction %d side %d", cfg->bc[d][s], d, s);
+    if ((cfg->bc[d][0] == OSBL_BC_PERIODIC) != (cfg->bc[d][1] == OSBL_BC_PERIODIC))
+      return fail(OSBL_E_ARG, "direction %d: a periodic boundary needs both sides periodic", d);
+  }
+  if (cfg->gama <= 1.0) return fail(OSBL_E_ARG, "gama must exceed 1");
+  if (cfg->viscous != OSBL_VISC_NONE && (cfg->Re <= 0.0 || cfg->Pr <= 0.0 || cfg->Minf <= 0.0))
+    return fail(OSBL_E_ARG, "viscous terms need positive Re, Pr and Minf");
+  return OSBL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int osbl_abi_version(void) { return OSBL_ABI_VERSION; }
+
+const char* osbl_last_error(void) { return g_err.c_str(); }
+
+int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
+  if (!cfg || !out) return fail(OSBL_E_ARG, "null argument");
+  *out = nullptr;
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(OSBL_E_HIP, "no HIP device");
+  auto* c = new osbl_ctx();
+  c->cfg = *cfg;
+  c->nd = cfg->ndim;
+  c->nv = cfg->ndim + 2;
+  HIPCHK(hipGetDevice(&c->device));
+  const int halo = is_central(cfg->conv) && cfg->viscous == OSBL_VISC_NONE ? 2 : 4;
+  Grid& g = c->g;
+  g.nd = c->nd;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = d < c->nd ? cfg->n[d] : 1;
+    g.h[d] = d < c->nd ? halo : 0;
+  }
+  g.pitch = ((OX + g.n[0] + g.h[0]) + 15) / 16 * 16;
+  g.rows = g.n[1] + 2 * g.h[1];
+  g.planes = g.n[2] + 2 * g.h[2];
+  g.plane = g.pitch * g.rows;
+  long long fs = (long long)g.plane * g.planes;
+  fs = (fs + 63) / 64 * 64;
+  g.fs = fs;
+  if (fs * (long long)c->nv >= (1LL << 31)) {
+    delete c;
+    return fail(OSBL_E_ARG, "block too large for 32-bit in-field offsets");
+  }
+  Phys& ph = c->ph;
+  ph.gama = cfg->gama;
+  ph.gm1 = cfg->gama - 1.0;
+  ph.Minf = cfg->Minf;
+  ph.Re = cfg->Re;
+  ph.Pr = cfg->Pr;
+  ph.dt = cfg->dt;
+  ph.eps = cfg->eps;
+  ph.teno_ct = cfg->teno_ct;
+  for (int d = 0; d < 3; ++d) ph.inv_delta[d] = d < c->nd ? 1.0 / cfg->delta[d] : 0.0;
+  ph.inv_re = cfg->viscous != OSBL_VISC_NONE ? 1.0 / cfg->Re : 0.0;
+  ph.cq = cfg->viscous != OSBL_VISC_NONE ? 1.0 / ((cfg->gama - 1.0) * cfg->Minf * cfg->Minf * cfg->Pr * cfg->Re) : 0.0;
+  c->nstages = rk_coefficients(cfg->rk, c->A, c->B);
+  c->need_prims = is_central(cfg->conv) || cfg->viscous != OSBL_VISC_NONE;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t bundle = (size_t)fs * c->nv * sizeof(double);
+  auto alloc = [&](double** p, size_t bytes) -> int {
+    if (hipMalloc((void**)p, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "hipMalloc of %zu bytes failed", bytes);
+    if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess) return fail(OSBL_E_HIP, "hipMemsetAsync failed");
+    return OSBL_OK;
+  };
+  if ((rc = alloc(&c->Qbuf[0], bundle)) || (rc = alloc(&c->Qbuf[1], bundle)) || (rc = alloc(&c->Qo, bundle)) ||
+      (rc = alloc(&c->R, bundle)) ||
+      (rc = alloc(&c->stage_buf, (size_t)g.n[0] * g.n[1] * g.n[2] * sizeof(double) * c->nv))) {
+    osbl_destroy(c);
+    return rc;
+  }
+  if (c->need_prims && (rc = alloc(&c->P, (size_t)fs * (c->nd + 2) * sizeof(double)))) {
+    osbl_destroy(c);
+    return rc;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    osbl_destroy(c);
+    return fail(OSBL_E_HIP, "stream synchronize failed");
+  }
+  *out = c;
+  return OSBL_OK;
+}
+
+void osbl_destroy(osbl_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  (void)hipFree(c->Qbuf[0]);
+  (void)hipFree(c->Qbuf[1]);
+  (void)hipFree(c->Qo);
+  (void)hipFree(c->R);
+  (void)hipFree(c->P);
+  (void)hipFree(c->stage_buf);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static long long cells(const osbl_ctx* c) { return (long long)c->g.n[0] * c->g.n[1] * c->g.n[2]; }
+
+int osbl_set_field_dev(osbl_ctx* c, int var, const double* dev) {
+  if (!c || !dev) return fail(OSBL_E_ARG, "null argument");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable index %d out of range [0,%d)", var, c->nv);
+  launch_unpack(dev, Qcur(c) + var * c->g.fs, c->g, c->stream);
+  HIPCHK(hipGetLastError());
+  return OSBL_OK;
+}
+
+int osbl_get_field_dev(osbl_ctx* c, int var, double* dev) {
+  if (!c || !dev) return fail(OSBL_E_ARG, "null argument");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable index %d out of range [0,%d)", var, c->nv);
+  launch_pack(Qcur(c) + var * c->g.fs, dev, c->g, c->stream);
+  HIPCHK(hipGetLastError());
+  return OSBL_OK;
+}
+
+int osbl_set_field(osbl_ctx* c, int var, const double* host) {
+  if (!c || !host) return fail(OSBL_E_ARG, "null argument");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable index %d out of range [0,%d)", var, c->nv);
+  HIPCHK(hipMemcpyAsync(c->stage_buf, host, cells(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc = osbl_set_field_dev(c, var, c->stage_buf);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+int osbl_get_field(osbl_ctx* c, int var, double* host) {
+  if (!c || !host) return fail(OSBL_E_ARG, "null argument");
+  int rc = osbl_get_field_dev(c, var, c->stage_buf);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(host, c->stage_buf, cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+int osbl_run(osbl_ctx* c, int64_t niter) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  if (niter < 0) return fail(OSBL_E_ARG, "niter must be non-negative");
+  const int mode = c->cfg.rk == OSBL_RK3 ? 0 : 1;
+  if (niter > 0) apply_bcs(c, Qcur(c));
+  for (int64_t it = 0; it < niter; ++it) {
+    for (int s = 0; s < c->nstages; ++s) {
+      stage(c, mode, c->A[s], c->B[s], s == 0);
+      c->cur ^= 1;
+      apply_bcs(c, Qcur(c));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return OSBL_OK;
+}
+
+int osbl_rhs(osbl_ctx* c, double* residual_host) {
+  if (!c || !residual_host) return fail(OSBL_E_ARG, "null argument");
+  apply_bcs(c, Qcur(c));
+  stage(c, 2, 0.0, 0.0, 0);
+  HIPCHK(hipGetLastError());
+  const long long n = cells(c);
+  for (int m = 0; m < c->nv; ++m) launch_pack(c->R + m * c->g.fs, c->stage_buf + m * n, c->g, c->stream);
+  HIPCHK(hipMemcpyAsync(residual_host, c->stage_buf, n * c->nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+int osbl_sync(osbl_ctx* c) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+void* osbl_stream(osbl_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int osbl_profile(osbl_ctx* c, int enable) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  drain_records(c);
+  if (enable && !c->prof) {
+    for (int f = 0; f < F_COUNT; ++f) { c->fam_ms[f] = 0.0; c->fam_n[f] = 0; }
+  }
+  c->prof = enable != 0;
+  return OSBL_OK;
+}
+
+int osbl_kernel_stats(osbl_ctx* c, int index, char* name, int name_len, double* total_ms, int64_t* launches) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  drain_records(c);
+  if (index < 0 || index >= F_COUNT) return F_COUNT;
+  if (name && name_len > 0) {
+    std::strncpy(name, kFamilyNames[index], name_len - 1);
+    name[name_len - 1] = '\0';
+  }
+  if (total_ms) *total_ms = c->fam_ms[index];
+  if (launches) *launches = c->fam_n[index];
+  return F_COUNT;
+}
+
+// ---------------------------------------------------------------------------
+// Algorithmic work model.  FP64 operations are counted from the reference's
+// formulas (FMA = 2, divide, sqrt, abs and max = 1 each), per interior cell:
+//  * one characteristic interface (shock_capturing.py:357-536): Roe average,
+//    eigenvectors, 6-point projection of F and Q, wave speeds, LLF split and
+//    R.recon -- 850 (3-D), 620 (2-D), 300 (1-D) -- plus two reconstructions
+//    per characteristic field: WENO5-Z 76, WENO5-JS 70, TENO5 88, TENO6 100;
+//    plus 2 per variable for the flux difference;
+//  * central skew convective terms per direction: 13 five-point derivatives
+//    and the residual assembly, 140 (3-D), 110 (2-D), 80 (1-D);
+//  * constant-viscosity terms: 530 (3-D), 200 (2-D), 40 (1-D);
+//  * constituent relations 6 per variable; RK update 6 per variable.
+// Bytes: 8*(4*nvar) per cell and stage (read Q and the RK register, write
+// both); per kernel family, the datasets the kernel must read and write once.
+// ---------------------------------------------------------------------------
+static double recon_side_flops(int conv) {
+  switch (conv) {
+    case OSBL_CONV_WENO_Z: return 76.0;
+    case OSBL_CONV_WENO_JS: return 70.0;
+    case OSBL_CONV_TENO5: return 88.0;
+    default: return 100.0;
+  }
+}
+static double interface_flops(const osbl_ctx* c) {
+  const double base = c->nd == 3 ? 850.0 : (c->nd == 2 ? 620.0 : 300.0);
+  return base + 2.0 * c->nv * recon_side_flops(c->cfg.conv) + 2.0 * c->nv;
+}
+static double viscous_flops(const osbl_ctx* c) {
+  if (c->cfg.viscous == OSBL_VISC_NONE) return 0.0;
+  return c->nd == 3 ? 530.0 : (c->nd == 2 ? 200.0 : 40.0);
+}
+static double central_flops(const osbl_ctx* c) { return c->nd == 3 ? 140.0 : (c->nd == 2 ? 110.0 : 80.0); }
+
+int osbl_work_per_stage(osbl_ctx* c, double* bytes, double* flops) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  const double n = (double)cells(c);
+  if (bytes) *bytes = 8.0 * 4.0 * c->nv * n;
+  if (flops) {
+    double f = 6.0 * c->nv + 6.0 * c->nv + viscous_flops(c);
+    f += is_central(c->cfg.conv) ? c->nd * central_flops(c) : c->nd * interface_flops(c);
+    *flops = f * n;
+  }
+  return OSBL_OK;
+}
+
+int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  if (family < 0 || family >= F_COUNT) return fail(OSBL_E_ARG, "kernel family %d out of range", family);
+  const double n = (double)cells(c);
+  const double nv = c->nv;
+  double b = 0.0, f = 0.0;
+  switch (family) {
+    case F_BC: b = 16.0 * nv * (double)halo_points(c->g); break;
+    case F_PRIMS: b = 8.0 * (nv + c->nd + 2) * n; f = 6.0 * nv * n; break;
+    case F_SWEEP_X: b = 16.0 * nv * n; f = interface_flops(c) * n; break;
+    case F_SWEEP_Y:
+    case F_SWEEP_Z: b = 24.0 * nv * n; f = interface_flops(c) * n; break;
+    case F_UPDATE: {
+      // read Q (central) or R, the constituent relations when viscous, the RK register; write Q and register
+      double rd = (is_central(c->cfg.conv) ? nv : nv) + (c->need_prims ? c->nd + 2 : 0) + nv;
+      b = 8.0 * (rd + 2.0 * nv) * n;
+      f = (viscous_flops(c) + 6.0 * nv + (is_central(c->cfg.conv) ? c->nd * central_flops(c) : 0.0)) * n;
+      break;
+    }
+  }
+  if (bytes) *bytes = b;
+  if (flops) *flops = f;
+  return OSBL_OK;
+}
+
+int osbl_num_stages(osbl_ctx* c) { return c ? c->nstages : fail(OSBL_E_ARG, "null context"); }
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+"""ctypes binding of libosbl_hip.so (the C-ABI declared in include/opensbli_amd.h).
+
+The HIP library is the only compute path: if it is missing or cannot be
+loaded, every entry point raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libosbl_hip.so")
+
+CONV = {"central_skew": 0, "central_cons": 1, "weno_js": 2, "weno_z": 3, "teno5": 4, "teno6": 5}
+AVG = {"simple": 0, "roe": 1}
+VISC = {"none": 0, "const": 1}
+RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
+BC = {"periodic": 0, "extrap0": 1}
+
+# symbols the header declares (checked by the ABI test)
+EXPORTS = [
+    "osbl_abi_version", "osbl_last_error", "osbl_create", "osbl_destroy", "osbl_set_field", "osbl_get_field",
+    "osbl_set_field_dev", "osbl_get_field_dev", "osbl_run", "osbl_rhs", "osbl_sync", "osbl_stream",
+    "osbl_profile", "osbl_kernel_stats", "osbl_work_per_stage", "osbl_kernel_work", "osbl_num_stages",
+]
+
+
+class OsblConfig(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int32),
+        ("n", ctypes.c_int32 * 3),
+        ("delta", ctypes.c_double * 3),
+        ("conv", ctypes.c_int32),
+        ("averaging", ctypes.c_int32),
+        ("viscous", ctypes.c_int32),
+        ("rk", ctypes.c_int32),
+        ("bc", (ctypes.c_int32 * 2) * 3),
+        ("gama", ctypes.c_double),
+        ("Minf", ctypes.c_double),
+        ("Re", ctypes.c_double),
+        ("Pr", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("teno_ct", ctypes.c_double),
+    ]
+
+
+class OsblError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libosbl_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OsblError(f"{LIB_PATH} is missing: build it with `python -m opensbli_amd.build` "
+                        "or __graft_entry__.build(); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
+    L.osbl_abi_version.restype = ctypes.c_int
+    L.osbl_last_error.restype = ctypes.c_char_p
+    L.osbl_create.argtypes = [ctypes.POINTER(OsblConfig), ctypes.POINTER(vp)]
+    L.osbl_destroy.argtypes = [vp]
+    L.osbl_destroy.restype = None
+    L.osbl_set_field.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_get_field.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_set_field_dev.argtypes = [vp, ctypes.c_int, vp]
+    L.osbl_get_field_dev.argtypes = [vp, ctypes.c_int, vp]
+    L.osbl_run.argtypes = [vp, i64]
+    L.osbl_rhs.argtypes = [vp, dp]
+    L.osbl_sync.argtypes = [vp]
+    L.osbl_stream.argtypes = [vp]
+    L.osbl_stream.restype = vp
+    L.osbl_profile.argtypes = [vp, ctypes.c_int]
+    L.osbl_kernel_stats.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, dp,
+                                    ctypes.POINTER(ctypes.c_int64)]
+    L.osbl_work_per_stage.argtypes = [vp, dp, dp]
+    L.osbl_kernel_work.argtypes = [vp, ctypes.c_int, dp, dp]
+    L.osbl_num_stages.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc < 0:
+        raise OsblError(load().osbl_last_error().decode() or f"osbl error {rc}")
+    return rc
+
+
+def make_config(p):
+    """osbl_config from a problem description dict (see opensbli_amd.problems)."""
+    c = OsblConfig()
+    nd = p["ndim"]
+    c.ndim = nd
+    for d in range(3):
+        c.n[d] = p["n"][d] if d < nd else 1
+        c.delta[d] = p["delta"][d] if d < nd else 1.0
+    c.conv = CONV[p["conv"]]
+    c.averaging = AVG[p.get("averaging", "roe")]
+    c.viscous = VISC[p.get("viscous", "none")]
+    c.rk = RK[p.get("rk", "rk3")]
+    bcs = p.get("bc", [["periodic", "periodic"]] * 3)
+    for d in range(3):
+        for s in range(2):
+            c.bc[d][s] = BC[bcs[d][s]] if d < nd else 0
+    c.gama = p.get("gama", 1.4)
+    c.Minf = p.get("Minf", 1.0)
+    c.Re = p.get("Re", 1.0)
+    c.Pr = p.get("Pr", 1.0)
+    c.dt = p["dt"]
+    c.eps = p.get("eps", 1e-16)
+    c.teno_ct = p.get("teno_ct", 1e-6)
+    return c
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Block:
+    """One block on the GPU: the state lives in HBM between calls.
+
+    Fields are numpy arrays of interior points in (z, y, x) order (x fastest),
+    variable order rho, rhou0.., rhoE.
+    """
+
+    def __init__(self, problem):
+        L = load()
+        self.p = dict(problem)
+        self.nd = problem["ndim"]
+        self.nvar = self.nd + 2
+        self.shape = tuple(reversed(list(problem["n"][: self.nd])))
+        self.cfg = make_config(problem)
+        h = ctypes.c_void_p()
+        check(L.osbl_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().osbl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, q):
+        for m in range(self.nvar):
+            a = np.ascontiguousarray(q[m], dtype=np.float64)
+            if a.shape != self.shape:
+                raise ValueError(f"field {m} has shape {a.shape}, expected {self.shape}")
+            check(load().osbl_set_field(self.h, m, _dptr(a)))
+
+    def get_state(self):
+        out = np.empty((self.nvar,) + self.shape)
+        for m in range(self.nvar):
+            a = np.empty(self.shape)
+            check(load().osbl_get_field(self.h, m, _dptr(a)))
+            out[m] = a
+        return out
+
+    def set_state_dev(self, ptrs):
+        for m, ptr in enumerate(ptrs):
+            check(load().osbl_set_field_dev(self.h, m, ctypes.c_void_p(ptr)))
+
+    def get_state_dev(self, ptrs):
+        for m, ptr in enumerate(ptrs):
+            check(load().osbl_get_field_dev(self.h, m, ctypes.c_void_p(ptr)))
+
+    def rhs(self):
+        out = np.empty((self.nvar,) + self.shape)
+        check(load().osbl_rhs(self.h, _dptr(out)))
+        return out
+
+    def run(self, niter):
+        check(load().osbl_run(self.h, int(niter)))
+
+    def sync(self):
+        check(load().osbl_sync(self.h))
+
+    def stream(self):
+        return load().osbl_stream(self.h)
+
+    def profile(self, enable):
+        check(load().osbl_profile(self.h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        L = load()
+        out = {}
+        n = check(L.osbl_kernel_stats(self.h, -1, None, 0, None, None))
+        for i in range(n):
+            name = ctypes.create_string_buffer(64)
+            ms = ctypes.c_double()
+            cnt = ctypes.c_int64()
+            check(L.osbl_kernel_stats(self.h, i, name, 64, ctypes.byref(ms), ctypes.byref(cnt)))
+            out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def work_per_stage(self):
+        b, f = ctypes.c_double(), ctypes.c_double()
+        check(load().osbl_work_per_stage(self.h, ctypes.byref(b), ctypes.byref(f)))
+        return b.value, f.value
+
+    def kernel_work(self):
+        """Per-launch algorithmic (bytes, flops) of every kernel family, by name."""
+        L = load()
+        out = {}
+        n = check(L.osbl_kernel_stats(self.h, -1, None, 0, None, None))
+        for i in range(n):
+            name = ctypes.create_string_buffer(64)
+            check(L.osbl_kernel_stats(self.h, i, name, 64, None, None))
+            b, f = ctypes.c_double(), ctypes.c_double()
+            check(L.osbl_kernel_work(self.h, i, ctypes.byref(b), ctypes.byref(f)))
+            out[name.value.decode()] = (b.value, f.value)
+        return out
+
+    def num_stages(self):
+        return check(load().osbl_num_stages(self.h))

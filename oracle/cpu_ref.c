@@ -1,0 +1,889 @@
+/*
+ * cpu_ref.c -- TEST INFRASTRUCTURE ONLY (the oracle).
+ *
+ * A CPU restatement of the kernel sequence that OpenSBLI's OPSC target emits
+ * for the right-hand side and Runge-Kutta time advance.  It keeps the
+ * structure of the generated code: OPS-style storage (one array per field,
+ * x fastest, 5 halo points on every side, opsc.py:693-722), constituent
+ * relations evaluated into arrays over their halo-extended ranges
+ * (scheme.py:159-185, shock_capturing.py:119-131), one work array per
+ * derivative or reconstructed flux, a residual kernel that assigns
+ * (scheme.py:284-297, shock_capturing.py:21-34) followed by a viscous kernel
+ * that accumulates, then the RK kernels (rk_sbli.py:102-133, rk_LS.py:139-166)
+ * and the boundary exchanges (periodic.py:42-56, extrapolation.py:29-58).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library.  It is the checker, never the thing measured or shipped.
+ *
+ * Parity notes.  SymPy 1.1 rewrites expressions (factor, horner) before OPSC
+ * prints them; this restatement evaluates the same formulas in a plain order,
+ * which changes results only at the rounding level.  The sqrt(2.0) literal of
+ * the eigenvector matrices (euler_eigensystem.py:109) is printed by SymPy to
+ * 15 significant digits; here it is the exact double.  Both are "parity
+ * unpinned" at the last-ulp level and irrelevant at the 1e-10 bar.
+ */
+#include "cpu_ref.h"
+
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define H 5           /* OPS halo depth for every dataset, scheme.py:47-60 */
+#define MAXV 5
+#define NWK 64
+
+struct ref_state {
+  ref_config c;
+  int nd, nvar;
+  int n[3], h[3];
+  ptrdiff_t sx, sy, sz, ntot, st[3];
+  double* q[MAXV];
+  double* qold[MAXV];  /* Q_RKold (3-register) or tempRK (low storage) */
+  double* res[MAXV];
+  double *u[3], *p, *T, *a;
+  double* wk[NWK];
+};
+
+static inline ptrdiff_t IX(const ref_state* s, int i, int j, int k) {
+  return (ptrdiff_t)(i + s->h[0]) + s->sx * ((ptrdiff_t)(j + s->h[1]) + s->sy * (ptrdiff_t)(k + s->h[2]));
+}
+
+typedef struct { int lo[3], hi[3]; } range3;
+
+/* interior range, extended by ext[d] points on both sides of every active direction */
+static range3 rng(const ref_state* s, int e0, int e1, int e2) {
+  range3 r;
+  int e[3] = {e0, e1, e2};
+  for (int d = 0; d < 3; ++d) {
+    if (d < s->nd) { r.lo[d] = -e[d]; r.hi[d] = s->n[d] + e[d]; }
+    else { r.lo[d] = 0; r.hi[d] = 1; }
+  }
+  return r;
+}
+static range3 rng_all(const ref_state* s, int e) { return rng(s, e, e, e); }
+
+#define LOOP(R)                                                                        \
+  _Pragma("omp parallel for collapse(2) schedule(static) if(s->ntot > 200000)")        \
+  for (int k = (R).lo[2]; k < (R).hi[2]; ++k)                                          \
+    for (int j = (R).lo[1]; j < (R).hi[1]; ++j)                                        \
+      for (int i = (R).lo[0]; i < (R).hi[0]; ++i)
+
+/* ------------------------------------------------------------------------- */
+/* Finite-difference weights (scheme.py:81-85 via sympy finite_diff_weights   */
+/* on points -2..2, opensblifunctions.py:485-521).                            */
+/* ------------------------------------------------------------------------- */
+static const double W1[5] = {1.0 / 12.0, -2.0 / 3.0, 0.0, 2.0 / 3.0, -1.0 / 12.0};
+static const double W2[5] = {-1.0 / 12.0, 4.0 / 3.0, -5.0 / 2.0, 4.0 / 3.0, -1.0 / 12.0};
+
+void ref_central_weights(double* w1, double* w2) {
+  for (int i = 0; i < 5; ++i) { w1[i] = W1[i]; w2[i] = W2[i]; }
+}
+
+/* out = D_d f (first derivative) over range R */
+static void deriv1(ref_state* s, const double* f, double* out, int d, range3 R) {
+  const ptrdiff_t st = s->st[d];
+  const double inv = 1.0 / s->c.delta[d];
+  LOOP(R) {
+    ptrdiff_t x = IX(s, i, j, k);
+    out[x] = inv * (W1[0] * f[x - 2 * st] + W1[1] * f[x - st] + W1[3] * f[x + st] + W1[4] * f[x + 2 * st]);
+  }
+}
+/* out = D_dd f (homogeneous second derivative) over range R */
+static void deriv2(ref_state* s, const double* f, double* out, int d, range3 R) {
+  const ptrdiff_t st = s->st[d];
+  const double inv = 1.0 / s->c.delta[d];
+  const double inv2 = inv * inv;
+  LOOP(R) {
+    ptrdiff_t x = IX(s, i, j, k);
+    out[x] = inv2 * (W2[0] * f[x - 2 * st] + W2[1] * f[x - st] + W2[2] * f[x] + W2[3] * f[x + st] +
+                     W2[4] * f[x + 2 * st]);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Constituent relations (taylor_green_vortex.py:29-31, Sod_shock_tube.py:26-28) */
+/* Evaluated over the whole allocated range minus the outermost points; the   */
+/* generated code evaluates them over interior + halotype, which is a subset, */
+/* and values outside that subset are never read.                             */
+/* ------------------------------------------------------------------------- */
+static void constituent_relations(ref_state* s) {
+  const int nd = s->nd;
+  const double g = s->c.gama, M2 = s->c.Minf * s->c.Minf;
+  range3 R = rng_all(s, H);
+  double* rho = s->q[0];
+  double* rhoE = s->q[nd + 1];
+  LOOP(R) {
+    ptrdiff_t x = IX(s, i, j, k);
+    double ke = 0.0;
+    for (int m = 0; m < nd; ++m) {
+      double um = s->q[1 + m][x] / rho[x];
+      s->u[m][x] = um;
+      ke += um * um;
+    }
+    double pp = (g - 1.0) * (rhoE[x] - 0.5 * rho[x] * ke);
+    s->p[x] = pp;
+    s->T[x] = pp * g * M2 / rho[x];
+    s->a[x] = sqrt(g * pp / rho[x]);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Central convective terms                                                  */
+/* ------------------------------------------------------------------------- */
+/* Skew (Blaisdell) form, parsing.py:75-111 and scheme.py:187-252:            */
+/*   R_rho  = -1/2 sum_j [D_j(rho u_j) + u_j D_j rho + rho D_j u_j]            */
+/*   R_rhoui= -1/2 sum_j [D_j(rhou_i u_j) + u_j D_j rhou_i + rhou_i D_j u_j]   */
+/*            - D_i p                                                          */
+/*   R_rhoE = -1/2 sum_j [D_j(rhoE u_j) + u_j D_j rhoE + rhoE D_j u_j]         */
+/*            - sum_j D_j(p u_j)                                               */
+/* Product fields are evaluated into work arrays over interior +-2 in j       */
+/* (scheme.py:218-225) and every derivative into its own work array.          */
+static void central_skew(ref_state* s) {
+  const int nd = s->nd, nv = s->nvar;
+  range3 I = rng_all(s, 0);
+  for (int m = 0; m < nv; ++m) {
+    double* r = s->res[m];
+    LOOP(I) r[IX(s, i, j, k)] = 0.0;
+  }
+  for (int d = 0; d < nd; ++d) {
+    int e[3] = {0, 0, 0};
+    e[d] = 2;
+    range3 P = rng(s, e[0], e[1], e[2]);
+    /* products: wk0 = rho*u_d, wk1+m = rhou_m*u_d, wk(1+nd) = rhoE*u_d, wk(2+nd) = p*u_d */
+    double** w = s->wk;
+    const double* ud = s->u[d];
+    LOOP(P) {
+      ptrdiff_t x = IX(s, i, j, k);
+      w[0][x] = s->q[0][x] * ud[x];
+      for (int m = 0; m < nd; ++m) w[1 + m][x] = s->q[1 + m][x] * ud[x];
+      w[1 + nd][x] = s->q[nd + 1][x] * ud[x];
+      w[2 + nd][x] = s->p[x] * ud[x];
+    }
+    /* derivative work arrays: 16.. */
+    const int D0 = 16;
+    double* dprod[8];
+    double* dq[MAXV];
+    for (int t = 0; t < nd + 3; ++t) {
+      dprod[t] = w[D0 + t];
+      deriv1(s, w[t], dprod[t], d, I);
+    }
+    for (int m = 0; m < nv; ++m) {
+      dq[m] = w[D0 + 8 + m];
+      deriv1(s, s->q[m], dq[m], d, I);
+    }
+    double* dud = w[D0 + 14];
+    double* dp = w[D0 + 15];
+    deriv1(s, ud, dud, d, I);
+    deriv1(s, s->p, dp, d, I);
+    /* residual contributions of direction d (the generated residual kernel sums over j) */
+    LOOP(I) {
+      ptrdiff_t x = IX(s, i, j, k);
+      double u_ = ud[x], du_ = dud[x];
+      s->res[0][x] += -0.5 * (dprod[0][x] + u_ * dq[0][x] + s->q[0][x] * du_);
+      for (int m = 0; m < nd; ++m) {
+        double t = -0.5 * (dprod[1 + m][x] + u_ * dq[1 + m][x] + s->q[1 + m][x] * du_);
+        if (m == d) t -= dp[x];
+        s->res[1 + m][x] += t;
+      }
+      s->res[nd + 1][x] += -0.5 * (dprod[1 + nd][x] + u_ * dq[nd + 1][x] + s->q[nd + 1][x] * du_) - dprod[2 + nd][x];
+    }
+  }
+}
+
+/* Conservative central form: R = -sum_j D_j F_j with F_j the Euler flux       */
+/* (Conservative(rhou_j,x_j), Conservative(rhou_i*u_j + KD(_i,_j)*p, x_j),     */
+/* Conservative((p+rhoE)*u_j, x_j); parsing.py:189-216, scheme.py:218-225).   */
+static void central_conservative(ref_state* s) {
+  const int nd = s->nd, nv = s->nvar;
+  range3 I = rng_all(s, 0);
+  for (int m = 0; m < nv; ++m) {
+    double* r = s->res[m];
+    LOOP(I) r[IX(s, i, j, k)] = 0.0;
+  }
+  for (int d = 0; d < nd; ++d) {
+    int e[3] = {0, 0, 0};
+    e[d] = 2;
+    range3 P = rng(s, e[0], e[1], e[2]);
+    double** w = s->wk;
+    const double* ud = s->u[d];
+    LOOP(P) {
+      ptrdiff_t x = IX(s, i, j, k);
+      for (int m = 0; m < nd; ++m) w[1 + m][x] = s->q[1 + m][x] * ud[x] + (m == d ? s->p[x] : 0.0);
+      w[1 + nd][x] = (s->p[x] + s->q[nd + 1][x]) * ud[x];
+    }
+    deriv1(s, s->q[1 + d], w[20], d, I);
+    for (int m = 0; m < nd; ++m) deriv1(s, w[1 + m], w[21 + m], d, I);
+    deriv1(s, w[1 + nd], w[21 + nd], d, I);
+    LOOP(I) {
+      ptrdiff_t x = IX(s, i, j, k);
+      s->res[0][x] -= w[20][x];
+      for (int m = 0; m < nd + 1; ++m) s->res[1 + m][x] -= w[21 + m][x];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Viscous central terms with constant viscosity (TGV stress/heat-flux form,  */
+/* taylor_green_vortex.py:14-19, scheme.py:299-327,360-387).  Every term      */
+/* containing Re is accumulated into the residual (scheme.py:263-267,291-294). */
+/* Mixed derivatives are D_b(D_a u) with a < b (opensblifunctions.py:341-353), */
+/* the inner one evaluated over +-2 halo points in b.                          */
+/* ------------------------------------------------------------------------- */
+static void viscous_constant(ref_state* s) {
+  const int nd = s->nd;
+  const double Re = s->c.Re, g = s->c.gama, M = s->c.Minf, Pr = s->c.Pr;
+  const double inv_re = 1.0 / Re;
+  const double cq = 1.0 / ((g - 1.0) * M * M * Pr * Re);
+  double** w = s->wk;
+  /* work layout: du[m][j] at 0+3m+j, d2u[m][j] at 9+3m+j, d2T[j] at 18+j, mixed at 21.. */
+  double* du[3][3];
+  double* d2u[3][3];
+  double* d2T[3];
+  double* mix[3][3][3]; /* mix[m][a][b], a<b: D_b(D_a u_m) */
+  for (int m = 0; m < 3; ++m)
+    for (int jj = 0; jj < 3; ++jj) { du[m][jj] = w[3 * m + jj]; d2u[m][jj] = w[9 + 3 * m + jj]; }
+  for (int jj = 0; jj < 3; ++jj) d2T[jj] = w[18 + jj];
+  int nm = 21;
+  for (int m = 0; m < 3; ++m)
+    for (int a_ = 0; a_ < 3; ++a_)
+      for (int b = 0; b < 3; ++b) mix[m][a_][b] = (a_ < b && a_ < nd && b < nd) ? w[nm++] : NULL;
+
+  range3 I = rng_all(s, 0);
+  for (int m = 0; m < nd; ++m)
+    for (int a_ = 0; a_ < nd; ++a_) {
+      int e[3] = {2, 2, 2};
+      e[a_] = 0;
+      deriv1(s, s->u[m], du[m][a_], a_, rng(s, e[0], e[1], e[2]));
+      deriv2(s, s->u[m], d2u[m][a_], a_, I);
+    }
+  for (int a_ = 0; a_ < nd; ++a_) deriv2(s, s->T, d2T[a_], a_, I);
+  for (int m = 0; m < nd; ++m)
+    for (int a_ = 0; a_ < nd; ++a_)
+      for (int b = a_ + 1; b < nd; ++b) deriv1(s, du[m][a_], mix[m][a_][b], b, I);
+
+  LOOP(I) {
+    ptrdiff_t x = IX(s, i, j, k);
+#define DD(mm, aa, bb) ((aa) == (bb) ? d2u[mm][aa][x] : ((aa) < (bb) ? mix[mm][aa][bb][x] : mix[mm][bb][aa][x]))
+    double divu = 0.0;
+    for (int kk = 0; kk < nd; ++kk) divu += du[kk][kk][x];
+    double energy = 0.0;
+    for (int jj = 0; jj < nd; ++jj) energy += cq * d2T[jj][x];
+    for (int m = 0; m < nd; ++m) {
+      double visc = 0.0;
+      for (int jj = 0; jj < nd; ++jj) {
+        /* D_j tau_mj = (1/Re)(D_jj u_m + D_j D_m u_j - 2/3 delta_mj sum_k D_m D_k u_k) */
+        double t = DD(m, jj, jj) + DD(jj, m, jj);
+        if (jj == m) {
+          double sk = 0.0;
+          for (int kk = 0; kk < nd; ++kk) sk += DD(kk, m, kk);
+          t -= (2.0 / 3.0) * sk;
+        }
+        double dtau = inv_re * t;
+        double tau = inv_re * (du[m][jj][x] + du[jj][m][x] - (jj == m ? (2.0 / 3.0) * divu : 0.0));
+        visc += dtau;
+        energy += du[m][jj][x] * tau + s->u[m][x] * dtau;
+      }
+      s->res[1 + m][x] += visc;
+    }
+    s->res[nd + 1][x] += energy;
+#undef DD
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* WENO / TENO reconstruction (weno.py, teno.py)                              */
+/* f points at f[0] of a window f[-2..3].                                     */
+/* ------------------------------------------------------------------------- */
+/* Jiang-Shu smoothness indicators of WENO5 (weno.py:138-205): beta_r is the
+ * sum over l = 1, 2 of h^(2l-1) * integral over the cell of (d^l p_r/dx^l)^2
+ * for the quadratic p_r through the stencil's nodes, applied with shift 0
+ * (right) or 1 (left).  The reference prints these as the expanded quadratic
+ * form in Horner order; that form cancels to beta from terms of size f^2 and
+ * carries rounding noise of order 1e-16*f^2, which WENO-Z's eps = 1e-14
+ * amplifies in near-uniform regions.  The same quadratic form is evaluated
+ * here as 13/12 (a - 2b + c)^2 + 1/4 (first-difference combination)^2, which
+ * is algebraically identical and carries rounding relative to beta itself.
+ * Agreement with the reference's own floating-point noise at that level is
+ * unpinned. */
+static void js_beta(const double* f, int shift, double* beta) {
+  /* beta_r on nodes f[-r+shift], f[-r+1+shift], f[-r+2+shift] */
+  for (int r = 0; r < 3; ++r) {
+    const double* g = f - r + shift;
+    const double a = g[0], b = g[1], c = g[2];
+    const double t = a - 2.0 * b + c;
+    double u;
+    if (r == 0) u = 3.0 * a - 4.0 * b + c;
+    else if (r == 1) u = a - c;
+    else u = a - 4.0 * b + 3.0 * c;
+    beta[r] = 13.0 / 12.0 * t * t + 0.25 * u * u;
+  }
+}
+
+/* ENO coefficients c_rj and optimal weights d_r of WENO5 (weno.py:70-120),   */
+/* func_points = -r+j+shift (weno.py:122-135).                                */
+static const double WENO_C_R[3][3] = {{1.0 / 3.0, 5.0 / 6.0, -1.0 / 6.0}, {-1.0 / 6.0, 5.0 / 6.0, 1.0 / 3.0},
+                                      {1.0 / 3.0, -7.0 / 6.0, 11.0 / 6.0}};
+static const double WENO_D_R[3] = {3.0 / 10.0, 3.0 / 5.0, 1.0 / 10.0};
+static const double WENO_C_L[3][3] = {{11.0 / 6.0, -7.0 / 6.0, 1.0 / 3.0}, {1.0 / 3.0, 5.0 / 6.0, -1.0 / 6.0},
+                                      {-1.0 / 6.0, 5.0 / 6.0, 1.0 / 3.0}};
+static const double WENO_D_L[3] = {1.0 / 10.0, 3.0 / 5.0, 3.0 / 10.0};
+
+static double weno5(int conv, int side, const double* f) {
+  int shift = side > 0 ? 0 : 1;
+  double beta[3], alpha[3];
+  js_beta(f, shift, beta);
+  const double(*c)[3] = side > 0 ? WENO_C_R : WENO_C_L;
+  const double* dr = side > 0 ? WENO_D_R : WENO_D_L;
+  if (conv == REF_CONV_WENO_Z) {
+    const double eps = 1.0e-14;          /* weno.py:286 */
+    double tau = fabs(beta[0] - beta[2]); /* weno.py:300-301 */
+    for (int r = 0; r < 3; ++r) {
+      double t = tau / (eps + beta[r]);
+      alpha[r] = dr[r] * (1.0 + t * t);  /* weno.py:320-323, p = 2 */
+    }
+  } else {
+    const double eps = 1e-6;             /* weno.py:344 */
+    for (int r = 0; r < 3; ++r) alpha[r] = dr[r] / ((eps + beta[r]) * (eps + beta[r]));
+  }
+  double inv_alpha_sum = 1.0 / (alpha[0] + alpha[1] + alpha[2]);
+  double recon = 0.0;
+  for (int r = 0; r < 3; ++r) {
+    double omega = alpha[r] * inv_alpha_sum;
+    double eno = 0.0;
+    for (int jj = 0; jj < 3; ++jj) eno += c[r][jj] * f[-r + jj + shift];
+    recon += omega * eno;
+  }
+  return recon;
+}
+
+/* TENO candidate stencils, coefficients and optimal weights, literal in      */
+/* teno.py:95-141; smoothness indicators literal in teno.py:159-177 including */
+/* the unsquared 781/720 term of the right-side fourth stencil.               */
+static const int TENO_PTS_R[4][4] = {{-1, 0, 1, 0}, {0, 1, 2, 0}, {-2, -1, 0, 0}, {0, 1, 2, 3}};
+static const int TENO_PTS_L[4][4] = {{0, 1, 2, 0}, {-1, 0, 1, 0}, {1, 2, 3, 0}, {-2, -1, 0, 1}};
+static const double TENO_C_R[4][4] = {{-1.0 / 6.0, 5.0 / 6.0, 2.0 / 6.0, 0.0},
+                                      {2.0 / 6.0, 5.0 / 6.0, -1.0 / 6.0, 0.0},
+                                      {2.0 / 6.0, -7.0 / 6.0, 11.0 / 6.0, 0.0},
+                                      {3.0 / 12.0, 13.0 / 12.0, -5.0 / 12.0, 1.0 / 12.0}};
+static const double TENO_C_L[4][4] = {{2.0 / 6.0, 5.0 / 6.0, -1.0 / 6.0, 0.0},
+                                      {-1.0 / 6.0, 5.0 / 6.0, 2.0 / 6.0, 0.0},
+                                      {11.0 / 6.0, -7.0 / 6.0, 1.0 / 3.0, 0.0},
+                                      {1.0 / 12.0, -5.0 / 12.0, 13.0 / 12.0, 3.0 / 12.0}};
+static const double TENO6_D[4] = {231.0 / 500.0, 3.0 / 10.0, 27.0 / 500.0, 23.0 / 125.0};
+static const double TENO5_D[3] = {11.0 / 20.0, 4.0 / 10.0, 1.0 / 20.0};
+
+static void teno_beta(int side, const double* f, double* b) {
+  double q;
+  if (side > 0) {
+    q = f[-1] - f[1];
+    b[0] = 0.25 * q * q;
+    q = f[-1] - 2.0 * f[0] + f[1];
+    b[0] += 13.0 / 12.0 * q * q;
+    q = 3.0 * f[0] - 4.0 * f[1] + f[2];
+    b[1] = 0.25 * q * q;
+    q = f[0] - 2.0 * f[1] + f[2];
+    b[1] += 13.0 / 12.0 * q * q;
+    q = f[-2] - 4.0 * f[-1] + 3.0 * f[0];
+    b[2] = 0.25 * q * q;
+    q = f[-2] - 2.0 * f[-1] + f[0];
+    b[2] += 13.0 / 12.0 * q * q;
+    q = -11.0 * f[0] + 18.0 * f[1] - 9.0 * f[2] + 2.0 * f[3];
+    b[3] = 1.0 / 36.0 * q * q;
+    q = 2.0 * f[0] - 5.0 * f[1] + 4.0 * f[2] - f[3];
+    b[3] += 13.0 / 12.0 * q * q;
+    q = -f[0] + 3.0 * f[1] - 3.0 * f[2] + f[3];
+    b[3] += 781.0 / 720.0 * q; /* teno.py:166-167: not squared */
+  } else {
+    q = f[0] - f[2];
+    b[0] = 0.25 * q * q;
+    q = f[0] - 2.0 * f[1] + f[2];
+    b[0] += 13.0 / 12.0 * q * q;
+    q = f[-1] - 4.0 * f[0] + 3.0 * f[1];
+    b[1] = 0.25 * q * q;
+    q = f[-1] - 2.0 * f[0] + f[1];
+    b[1] += 13.0 / 12.0 * q * q;
+    q = 3.0 * f[1] - 4.0 * f[2] + f[3];
+    b[2] = 0.25 * q * q;
+    q = f[1] - 2.0 * f[2] + f[3];
+    b[2] += 13.0 / 12.0 * q * q;
+    q = -2.0 * f[-2] + 9.0 * f[-1] - 18.0 * f[0] + 11.0 * f[1];
+    b[3] = 1.0 / 36.0 * q * q;
+    q = -f[-2] + 4.0 * f[-1] - 5.0 * f[0] + 2.0 * f[1];
+    b[3] += 13.0 / 12.0 * q * q;
+    q = -f[-2] + 3.0 * f[-1] - 3.0 * f[0] + f[1];
+    b[3] += 781.0 / 720.0 * q * q;
+  }
+}
+
+static double teno(int conv, int side, const double* f, double eps, double ct) {
+  const int ns = conv == REF_CONV_TENO6 ? 4 : 3;
+  const double* d = conv == REF_CONV_TENO6 ? TENO6_D : TENO5_D;
+  const int(*pts)[4] = side > 0 ? TENO_PTS_R : TENO_PTS_L;
+  const double(*c)[4] = side > 0 ? TENO_C_R : TENO_C_L;
+  double b[4], alpha[4], delta[4];
+  teno_beta(side, f, b);
+  double tau = ns == 4 ? fabs(b[3] - (b[0] + b[2] - 4.0 * b[1]) / 6.0) /* teno.py:230 */
+                       : fabs(b[0] - b[2]);                           /* teno.py:209 */
+  double asum = 0.0;
+  for (int r = 0; r < ns; ++r) {
+    alpha[r] = pow(1.0 + tau / (eps + b[r]), 6.0); /* C = 1, q = 6 */
+    asum += alpha[r];
+  }
+  double inv_alpha_sum = 1.0 / asum;
+  double dsum = 0.0;
+  for (int r = 0; r < ns; ++r) {
+    delta[r] = (alpha[r] * inv_alpha_sum < ct) ? 0.0 : 1.0; /* teno.py:445-465 */
+    dsum += d[r] * delta[r];
+  }
+  double inv_omega_sum = 1.0 / dsum;
+  double recon = 0.0;
+  for (int r = 0; r < ns; ++r) {
+    int w = (r == 3) ? 4 : 3;
+    double eno = 0.0;
+    for (int jj = 0; jj < w; ++jj) eno += f[pts[r][jj]] * c[r][jj];
+    recon += d[r] * delta[r] * inv_omega_sum * eno;
+  }
+  return recon;
+}
+
+double ref_recon_side(int conv, int side, const double* f6, double eps, double ct) {
+  const double* f = f6 + 2;
+  if (conv == REF_CONV_WENO_JS || conv == REF_CONV_WENO_Z) return weno5(conv, side, f);
+  return teno(conv, side, f, eps, ct);
+}
+
+int ref_smoothness(int conv, int side, const double* f6, double* beta) {
+  const double* f = f6 + 2;
+  if (conv == REF_CONV_WENO_JS || conv == REF_CONV_WENO_Z) {
+    js_beta(f, side > 0 ? 0 : 1, beta);
+    return 3;
+  }
+  teno_beta(side, f, beta);
+  return conv == REF_CONV_TENO6 ? 4 : 3;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Euler eigensystems (euler_eigensystem.py:42-135), unit normal k = e_d for  */
+/* Cartesian grids (metrics diagonal, euler_eigensystem.py:49-54).           */
+/* L and R row-major nvar x nvar; 1/a and 1/rho used as inv_AVG_a/inv_AVG_rho */
+/* (shock_capturing.py:319-331,428-437).                                     */
+/* ------------------------------------------------------------------------- */
+void ref_eigensystem(int nd, int d, double g, double rho, const double* u, double a, double* L, double* R,
+                     double* lam) {
+  const double ia = 1.0 / a, ir = 1.0 / rho, gm1 = g - 1.0;
+  const double s2 = sqrt(2.0);
+  const double alpha = rho / (a * s2), bta = ir * ia / s2;
+  if (nd == 1) {
+    double u0 = u[0];
+    double Hh = a * a / gm1 + u0 * u0 / 2.0;
+    double den = 2.0 * Hh - u0 * u0;
+    lam[0] = u0 - a; lam[1] = u0; lam[2] = u0 + a;
+    double Rm[9] = {1, 1, 1, u0 - a, u0, u0 + a, Hh - u0 * a, u0 * u0 / 2.0, Hh + u0 * a};
+    double Lm[9] = {u0 * (2.0 * Hh + a * u0 - u0 * u0) * ia / (2.0 * den),
+                    (-Hh - a * u0 + u0 * u0 / 2.0) * ia / den,
+                    1.0 / den,
+                    2.0 * (Hh - u0 * u0) / den,
+                    2.0 * u0 / den,
+                    2.0 / (-2.0 * Hh + u0 * u0),
+                    u0 * (-2.0 * Hh + a * u0 + u0 * u0) * ia / (2.0 * den),
+                    (Hh - a * u0 - u0 * u0 / 2.0) * ia / den,
+                    1.0 / den};
+    memcpy(R, Rm, sizeof(Rm));
+    memcpy(L, Lm, sizeof(Lm));
+    return;
+  }
+  double kt[3] = {0, 0, 0};
+  kt[d] = 1.0;
+  if (nd == 2) {
+    double u0 = u[0], u1 = u[1];
+    double th = kt[0] * u0 + kt[1] * u1;
+    double phi = gm1 * ((u0 * u0 + u1 * u1) / 2.0);
+    lam[0] = th; lam[1] = th; lam[2] = th + a; lam[3] = th - a;
+    double Rm[16] = {1, 0, alpha, alpha,
+                     u0, kt[1] * rho, alpha * (u0 + kt[0] * a), alpha * (u0 - kt[0] * a),
+                     u1, -kt[0] * rho, alpha * (u1 + kt[1] * a), alpha * (u1 - kt[1] * a),
+                     phi / gm1, rho * (kt[1] * u0 - kt[0] * u1), alpha * ((phi + a * a) / gm1 + a * th),
+                     alpha * ((phi + a * a) / gm1 - a * th)};
+    double Lm[16] = {1 - phi * ia * ia, gm1 * u0 * ia * ia, gm1 * u1 * ia * ia, -gm1 * ia * ia,
+                     -(kt[1] * u0 - kt[0] * u1) * ir, kt[1] * ir, -kt[0] * ir, 0,
+                     bta * (phi - a * th), bta * (kt[0] * a - gm1 * u0), bta * (kt[1] * a - gm1 * u1), bta * gm1,
+                     bta * (phi + a * th), -bta * (kt[0] * a + gm1 * u0), -bta * (kt[1] * a + gm1 * u1), bta * gm1};
+    memcpy(R, Rm, sizeof(Rm));
+    memcpy(L, Lm, sizeof(Lm));
+    return;
+  }
+  double u0 = u[0], u1 = u[1], u2 = u[2];
+  double th = kt[0] * u0 + kt[1] * u1 + kt[2] * u2;
+  double phi = gm1 * ((u0 * u0 + u1 * u1 + u2 * u2) / 2.0);
+  double ia2 = ia * ia;
+  lam[0] = th; lam[1] = th; lam[2] = th; lam[3] = th + a; lam[4] = th - a;
+  double Rm[25] = {
+      kt[0], kt[1], kt[2], alpha, alpha,
+      kt[0] * u0, kt[1] * u0 - kt[2] * rho, kt[2] * u0 + kt[1] * rho, alpha * (u0 + kt[0] * a), alpha * (u0 - kt[0] * a),
+      kt[0] * u1 + kt[2] * rho, kt[1] * u1, kt[2] * u1 - kt[0] * rho, alpha * (u1 + kt[1] * a), alpha * (u1 - kt[1] * a),
+      kt[0] * u2 - kt[1] * rho, kt[1] * u2 + kt[0] * rho, kt[2] * u2, alpha * (u2 + kt[2] * a), alpha * (u2 - kt[2] * a),
+      kt[0] * phi / gm1 + rho * (kt[2] * u1 - kt[1] * u2), kt[1] * phi / gm1 + rho * (kt[0] * u2 - kt[2] * u0),
+      kt[2] * phi / gm1 + rho * (kt[1] * u0 - kt[0] * u1), alpha * ((phi + a * a) / gm1 + th * a),
+      alpha * ((phi + a * a) / gm1 - th * a)};
+  double Lm[25] = {
+      kt[0] * (1 - phi * ia2) - (kt[2] * u1 - kt[1] * u2) * ir, kt[0] * gm1 * u0 * ia2, kt[0] * gm1 * u1 * ia2 + kt[2] * ir,
+      kt[0] * gm1 * u2 * ia2 - kt[1] * ir, -kt[0] * gm1 * ia2,
+      kt[1] * (1 - phi * ia2) - (kt[0] * u2 - kt[2] * u0) * ir, kt[1] * gm1 * u0 * ia2 - kt[2] * ir, kt[1] * gm1 * u1 * ia2,
+      kt[1] * gm1 * u2 * ia2 + kt[0] * ir, -kt[1] * gm1 * ia2,
+      kt[2] * (1 - phi * ia2) - (kt[1] * u0 - kt[0] * u1) * ir, kt[2] * gm1 * u0 * ia2 + kt[1] * ir,
+      kt[2] * gm1 * u1 * ia2 - kt[0] * ir, kt[2] * gm1 * u2 * ia2, -kt[2] * gm1 * ia2,
+      bta * (phi - th * a), -bta * (gm1 * u0 - kt[0] * a), -bta * (gm1 * u1 - kt[1] * a), -bta * (gm1 * u2 - kt[2] * a),
+      bta * gm1,
+      bta * (phi + th * a), -bta * (gm1 * u0 + kt[0] * a), -bta * (gm1 * u1 + kt[1] * a), -bta * (gm1 * u2 + kt[2] * a),
+      bta * gm1};
+  memcpy(R, Rm, sizeof(Rm));
+  memcpy(L, Lm, sizeof(Lm));
+}
+
+/* Roe average (averaging.py:70-114); q = [rho, rhou.., rhoE, p] */
+void ref_roe_average(int nd, double g, const double* qL, const double* qR, double* out) {
+  double rL = qL[0], rR = qR[0];
+  double sL = sqrt(rL), sR = sqrt(rR);
+  double inv = 1.0 / (sR + sL);
+  out[0] = sqrt(rL * rR);
+  double ke = 0.0;
+  for (int m = 0; m < nd; ++m) {
+    double um = (sL * (qL[1 + m] / rL) + sR * (qR[1 + m] / rR)) * inv;
+    out[1 + m] = um;
+    ke += um * um;
+  }
+  ke = ke / 2.0;
+  double HL = (qL[nd + 1] + qL[nd + 2]) / rL, HR = (qR[nd + 1] + qR[nd + 2]) / rR;
+  double Hh = (sL * HL + sR * HR) * inv;
+  out[nd + 1] = sqrt((g - 1.0) * (Hh - ke));
+}
+
+/* Characteristic LLF reconstruction kernel of one direction                  */
+/* (shock_capturing.py:357-448, weno.py:486-521, teno.py:489-530), evaluated  */
+/* at i+1/2 for i in [-1, N) along d; the flux is written into wk[5d+m].      */
+static void reconstruction_direction(ref_state* s, int d) {
+  const int nd = s->nd, nv = s->nvar;
+  const ptrdiff_t st = s->st[d];
+  const double g = s->c.gama;
+  const int conv = s->c.conv;
+  range3 R = rng_all(s, 0);
+  R.lo[d] = -1;
+  double** out = s->wk + 5 * d;
+  LOOP(R) {
+    ptrdiff_t x = IX(s, i, j, k);
+    double avg[5], ua[3];
+    if (s->c.averaging == REF_AVG_ROE) {
+      double qL[7], qR[7];
+      for (int m = 0; m < nv; ++m) { qL[m] = s->q[m][x]; qR[m] = s->q[m][x + st]; }
+      /* the Roe average uses the velocity datasets u_m = rhou_m/rho */
+      qL[nv] = s->p[x];
+      qR[nv] = s->p[x + st];
+      double rL = qL[0], rR = qR[0], sL = sqrt(rL), sR = sqrt(rR);
+      double inv = 1.0 / (sR + sL);
+      avg[0] = sqrt(rL * rR);
+      double ke = 0.0;
+      for (int m = 0; m < nd; ++m) {
+        ua[m] = (sL * s->u[m][x] + sR * s->u[m][x + st]) * inv;
+        ke += ua[m] * ua[m];
+      }
+      ke = ke / 2.0;
+      double HL = (qL[nd + 1] + qL[nv]) / rL, HR = (qR[nd + 1] + qR[nv]) / rR;
+      double Hh = (sL * HL + sR * HR) * inv;
+      avg[1] = sqrt((g - 1.0) * (Hh - ke));
+    } else {
+      /* SimpleAverage([0,1]) of the symbols the eigensystem uses (averaging.py:38-59) */
+      avg[0] = (s->q[0][x] + s->q[0][x + st]) / 2.0;
+      for (int m = 0; m < nd; ++m) ua[m] = (s->u[m][x] + s->u[m][x + st]) / 2.0;
+      avg[1] = (s->a[x] + s->a[x + st]) / 2.0;
+    }
+    double L[25], Rv[25], lam[5];
+    ref_eigensystem(nd, d, g, avg[0], ua, avg[1], L, Rv, lam);
+    /* window p = -2..3 */
+    double CF[5][6], CS[5][6];
+    double lmax[5] = {0, 0, 0, 0, 0};
+    for (int pp = 0; pp < 6; ++pp) {
+      ptrdiff_t y = x + (pp - 2) * st;
+      double Fv[5], Qv[5];
+      double udp = s->u[d][y], pr = s->p[y];
+      Fv[0] = s->q[1 + d][y];
+      for (int m = 0; m < nd; ++m) Fv[1 + m] = s->q[1 + m][y] * udp + (m == d ? pr : 0.0);
+      Fv[nd + 1] = (pr + s->q[nd + 1][y]) * udp;
+      for (int m = 0; m < nv; ++m) Qv[m] = s->q[m][y];
+      for (int kk = 0; kk < nv; ++kk) {
+        double cf = 0.0, cs = 0.0;
+        for (int m = 0; m < nv; ++m) { cf += L[kk * nv + m] * Fv[m]; cs += L[kk * nv + m] * Qv[m]; }
+        CF[kk][pp] = cf;
+        CS[kk][pp] = cs;
+      }
+      /* pointwise eigenvalues with the stencil point's u_d and a (shock_capturing.py:512-536) */
+      double ap = s->a[y];
+      double lp[5];
+      if (nd == 1) { lp[0] = udp - ap; lp[1] = udp; lp[2] = udp + ap; }
+      else if (nd == 2) { lp[0] = udp; lp[1] = udp; lp[2] = udp + ap; lp[3] = udp - ap; }
+      else { lp[0] = udp; lp[1] = udp; lp[2] = udp; lp[3] = udp + ap; lp[4] = udp - ap; }
+      for (int kk = 0; kk < nv; ++kk) lmax[kk] = fmax(lmax[kk], fabs(lp[kk]));
+    }
+    double recon[5];
+    for (int kk = 0; kk < nv; ++kk) {
+      double fp[6], fm[6];
+      for (int pp = 0; pp < 6; ++pp) {
+        fp[pp] = 0.5 * (CF[kk][pp] + lmax[kk] * CS[kk][pp]);
+        fm[pp] = 0.5 * (CF[kk][pp] - lmax[kk] * CS[kk][pp]);
+      }
+      recon[kk] = ref_recon_side(conv, 1, fp, s->c.eps, s->c.teno_ct) + ref_recon_side(conv, -1, fm, s->c.eps, s->c.teno_ct);
+    }
+    for (int m = 0; m < nv; ++m) {
+      double fl = 0.0;
+      for (int kk = 0; kk < nv; ++kk) fl += Rv[m * nv + kk] * recon[kk];
+      out[m][x] = fl;
+    }
+  }
+}
+
+/* Shock-capturing residual (shock_capturing.py:21-34, opensblifunctions.py:575-593): */
+/* R_m = -sum_d (wk_dm[i] - wk_dm[i-1]) / Delta_d, assigned.                   */
+static void shock_capturing(ref_state* s) {
+  const int nd = s->nd, nv = s->nvar;
+  for (int d = 0; d < nd; ++d) reconstruction_direction(s, d);
+  range3 I = rng_all(s, 0);
+  LOOP(I) {
+    ptrdiff_t x = IX(s, i, j, k);
+    for (int m = 0; m < nv; ++m) {
+      double r = 0.0;
+      for (int d = 0; d < nd; ++d) {
+        const double* w = s->wk[5 * d + m];
+        r += (w[x] - w[x - s->st[d]]) / s->c.delta[d];
+      }
+      s->res[m][x] = -r;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Boundary conditions                                                       */
+/* ------------------------------------------------------------------------- */
+/* Periodic exchange (periodic.py:42-56): side 0 copies interior planes        */
+/* [0,H) into [N,N+H); side 1 copies [N-H,N) into [-H,0); extents in the other */
+/* directions include their halos so edges fill through later directions.    */
+static void periodic_exchange(ref_state* s, double** arrs, int narr, int d, int side) {
+  int N = s->n[d];
+  range3 R;
+  for (int e = 0; e < 3; ++e) {
+    R.lo[e] = -s->h[e];
+    R.hi[e] = s->n[e] + s->h[e];
+  }
+  if (side == 0) { R.lo[d] = 0; R.hi[d] = H; }
+  else { R.lo[d] = N - H; R.hi[d] = N; }
+  ptrdiff_t shift = (side == 0 ? N : -N) * s->st[d];
+  for (int v = 0; v < narr; ++v) {
+    double* f = arrs[v];
+    LOOP(R) {
+      ptrdiff_t x = IX(s, i, j, k);
+      f[x + shift] = f[x];
+    }
+  }
+}
+
+/* Zeroth-order extrapolation (extrapolation.py:29-58, bc_core.py:144-197):   */
+/* side 0: q[0..-nh] = q[1]; side 1: q[N-1..N-1+nh] = q[N-2]; the kernel runs   */
+/* on the boundary plane with the other directions' halos included.           */
+static void extrapolation0(ref_state* s, double** arrs, int narr, int d, int side) {
+  int N = s->n[d];
+  range3 R;
+  for (int e = 0; e < 3; ++e) {
+    R.lo[e] = -s->h[e];
+    R.hi[e] = s->n[e] + s->h[e];
+  }
+  int b = side == 0 ? 0 : N - 1;
+  R.lo[d] = b;
+  R.hi[d] = b + 1;
+  const ptrdiff_t st = s->st[d];
+  const int sgn = side == 0 ? -1 : 1;      /* from_side_factor */
+  const int to = side == 0 ? 1 : -1;       /* to_side_factor */
+  /* n_halos is |halo| of the boundary (3 below, 4 above for WENO/TENO halos,
+     weno.py:17-29); the deeper halo points are never read by any kernel, so
+     filling all H of them changes no result and keeps them finite */
+  const int nh = H;
+  for (int v = 0; v < narr; ++v) {
+    double* f = arrs[v];
+    LOOP(R) {
+      ptrdiff_t x = IX(s, i, j, k);
+      for (int hh = 0; hh <= nh; ++hh) f[x + sgn * hh * st] = f[x + to * st];
+    }
+  }
+}
+
+static void apply_bcs(ref_state* s) {
+  for (int d = 0; d < s->nd; ++d)
+    for (int side = 0; side < 2; ++side) {
+      if (s->c.bc[d][side] == REF_BC_PERIODIC) periodic_exchange(s, s->q, s->nvar, d, side);
+      else extrapolation0(s, s->q, s->nvar, d, side);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+static void evaluate_rhs(ref_state* s) {
+  constituent_relations(s);
+  switch (s->c.conv) {
+    case REF_CONV_CENTRAL_SKEW: central_skew(s); break;
+    case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
+    default: shock_capturing(s); break;
+  }
+  if (s->c.viscous == REF_VISC_CONST) viscous_constant(s);
+}
+
+int ref_rk_coefficients(int rk, double* a, double* b) {
+  if (rk == REF_RK3_SBLI) { /* rknew (stage) and rkold (solution), rk_sbli.py:58-60 */
+    a[0] = 2.0 / 3.0; a[1] = 5.0 / 12.0; a[2] = 3.0 / 5.0;
+    b[0] = 1.0 / 4.0; b[1] = 3.0 / 20.0; b[2] = 3.0 / 5.0;
+    return 3;
+  }
+  if (rk == REF_RK3_LS) { /* rk_LS.py:91-94 */
+    a[0] = 0.0; a[1] = -5.0 / 9.0; a[2] = -153.0 / 128.0;
+    b[0] = 1.0 / 3.0; b[1] = 15.0 / 16.0; b[2] = 8.0 / 15.0;
+    return 3;
+  }
+  if (rk == REF_RK3_LS_SSP) { /* rk_LS.py:74-89, evaluated in double */
+    double c = 0.924574;
+    double z1 = sqrt(36 * c * c * c * c + 36 * c * c * c - 135 * c * c + 84 * c - 12);
+    double z2 = 2 * c * c + c - 2;
+    double z3 = 12 * c * c * c * c - 18 * c * c * c + 18 * c * c - 11 * c + 2;
+    double z4 = 36 * c * c * c * c - 36 * c * c * c + 13 * c * c - 8 * c + 4;
+    double z5 = 69 * c * c * c - 62 * c * c + 28 * c - 8;
+    double z6 = 34 * c * c * c * c - 46 * c * c * c + 34 * c * c - 13 * c + 2;
+    b[0] = 0.924574;
+    b[1] = (12 * c * (c - 1) * (3 * z2 - z1) - (3 * z2 - z1) * (3 * z2 - z1)) / (144 * c * (3 * c - 2) * (c - 1) * (c - 1));
+    b[2] = (-24 * (3 * c - 2) * (c - 1) * (c - 1)) / ((3 * z2 - z1) * (3 * z2 - z1) - 12 * c * (c - 1) * (3 * z2 - z1));
+    a[0] = 0.0;
+    a[1] = (-z1 * (6 * c * c - 4 * c + 1) + 3 * z3) / ((2 * c + 1) * z1 - 3 * (c + 2) * (2 * c - 1) * (2 * c - 1));
+    a[2] = (-z1 * z4 + 108 * (2 * c - 1) * c * c * c * c * c - 3 * (2 * c - 1) * z5) /
+           (24 * z1 * c * (c - 1) * (c - 1) * (c - 1) * (c - 1) + 72 * c * z6 + 72 * c * c * c * c * c * c * (2 * c - 13));
+    return 3;
+  }
+  /* rk_LS.py:96-97, five stages */
+  const double A[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+  const double B[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+  for (int i = 0; i < 5; ++i) { a[i] = A[i]; b[i] = B[i]; }
+  return 5;
+}
+
+static void time_step(ref_state* s) {
+  const int nv = s->nvar;
+  const double dt = s->c.dt;
+  double A[5], B[5];
+  int ns = ref_rk_coefficients(s->c.rk, A, B);
+  range3 I = rng_all(s, 0);
+  apply_bcs(s); /* temporal_start = bc_kernels + start kernels (algorithm.py:423-425) */
+  if (s->c.rk == REF_RK3_SBLI) {
+    for (int m = 0; m < nv; ++m) { /* "Save equations" Q_old = Q (rk_sbli.py:126-133) */
+      double *o = s->qold[m], *q = s->q[m];
+      LOOP(I) { ptrdiff_t x = IX(s, i, j, k); o[x] = q[x]; }
+    }
+  }
+  for (int st = 0; st < ns; ++st) {
+    evaluate_rhs(s);
+    for (int m = 0; m < nv; ++m) {
+      double *o = s->qold[m], *q = s->q[m], *r = s->res[m];
+      if (s->c.rk == REF_RK3_SBLI) {
+        double an = A[st], bo = B[st];
+        LOOP(I) { ptrdiff_t x = IX(s, i, j, k); q[x] = o[x] + dt * an * r[x]; }   /* Sub stage advancement */
+        LOOP(I) { ptrdiff_t x = IX(s, i, j, k); o[x] = o[x] + dt * bo * r[x]; }   /* Temporal solution advancement */
+      } else {
+        double a_ = A[st], b_ = B[st];
+        LOOP(I) {
+          ptrdiff_t x = IX(s, i, j, k);
+          o[x] = a_ * o[x] + dt * r[x];   /* tempRK = rkA*tempRK + dt*R */
+          q[x] = q[x] + b_ * o[x];        /* Q = Q + rkB*tempRK */
+        }
+      }
+    }
+    apply_bcs(s);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+int ref_create(const ref_config* cfg, ref_state** out) {
+  if (cfg->ndim < 1 || cfg->ndim > 3) return -1;
+  ref_state* s = (ref_state*)calloc(1, sizeof(ref_state));
+  s->c = *cfg;
+  s->nd = cfg->ndim;
+  s->nvar = cfg->ndim + 2;
+  for (int d = 0; d < 3; ++d) {
+    s->n[d] = d < s->nd ? cfg->n[d] : 1;
+    s->h[d] = d < s->nd ? H : 0;
+  }
+  s->sx = s->n[0] + 2 * s->h[0];
+  s->sy = s->n[1] + 2 * s->h[1];
+  s->sz = s->n[2] + 2 * s->h[2];
+  s->ntot = s->sx * s->sy * s->sz;
+  s->st[0] = 1; s->st[1] = s->sx; s->st[2] = s->sx * s->sy;
+  const size_t bytes = (size_t)s->ntot * sizeof(double);
+  for (int m = 0; m < s->nvar; ++m) {
+    s->q[m] = (double*)calloc(1, bytes);
+    s->qold[m] = (double*)calloc(1, bytes);
+    s->res[m] = (double*)calloc(1, bytes);
+  }
+  for (int m = 0; m < s->nd; ++m) s->u[m] = (double*)calloc(1, bytes);
+  s->p = (double*)calloc(1, bytes);
+  s->T = (double*)calloc(1, bytes);
+  s->a = (double*)calloc(1, bytes);
+  /* the residual kernels consume every convective work array before the
+     viscous kernels run, so the two groups share one pool */
+  int nwk = (cfg->conv == REF_CONV_CENTRAL_SKEW || cfg->conv == REF_CONV_CENTRAL_CONS) ? 32 : 15;
+  if (cfg->viscous != REF_VISC_NONE && nwk < 30) nwk = 30;
+  for (int w = 0; w < nwk; ++w) s->wk[w] = (double*)calloc(1, bytes);
+  *out = s;
+  return 0;
+}
+
+void ref_destroy(ref_state* s) {
+  if (!s) return;
+  for (int m = 0; m < MAXV; ++m) { free(s->q[m]); free(s->qold[m]); free(s->res[m]); }
+  for (int m = 0; m < 3; ++m) free(s->u[m]);
+  free(s->p); free(s->T); free(s->a);
+  for (int w = 0; w < NWK; ++w) free(s->wk[w]);
+  free(s);
+}
+
+int ref_set_field(ref_state* s, int var, const double* v) {
+  if (var < 0 || var >= s->nvar) return -1;
+  double* f = s->q[var];
+  ptrdiff_t c = 0;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) f[IX(s, i, j, k)] = v[c++];
+  return 0;
+}
+
+int ref_get_field(const ref_state* s, int var, double* v) {
+  if (var < 0 || var >= s->nvar) return -1;
+  const double* f = s->q[var];
+  ptrdiff_t c = 0;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) v[c++] = f[IX(s, i, j, k)];
+  return 0;
+}
+
+int ref_rhs(ref_state* s, double* residual) {
+  apply_bcs(s);
+  constituent_relations(s);
+  switch (s->c.conv) {
+    case REF_CONV_CENTRAL_SKEW: central_skew(s); break;
+    case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
+    default: shock_capturing(s); break;
+  }
+  if (s->c.viscous == REF_VISC_CONST) viscous_constant(s);
+  ptrdiff_t c = 0;
+  for (int m = 0; m < s->nvar; ++m)
+    for (int k = 0; k < s->n[2]; ++k)
+      for (int j = 0; j < s->n[1]; ++j)
+        for (int i = 0; i < s->n[0]; ++i) residual[c++] = s->res[m][IX(s, i, j, k)];
+  return 0;
+}
+
+int ref_run(ref_state* s, int64_t niter) {
+  for (int64_t it = 0; it < niter; ++it) time_step(s);
+  return 0;
+}

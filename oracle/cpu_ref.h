@@ -1,0 +1,72 @@
+/*
+ * cpu_ref.h -- TEST INFRASTRUCTURE ONLY (the oracle).
+ *
+ * CPU restatement of the kernel sequence that OpenSBLI's OPSC target
+ * (opensbli/code_generation/opsc.py) emits for the RHS and Runge-Kutta time
+ * advance.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load this library, and only as the checker.  Nothing in the product
+ * (opensbli_amd/) links, loads or calls it.
+ */
+#ifndef OSBL_CPU_REF_H
+#define OSBL_CPU_REF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Convective discretisations (the scheme objects the problem script installs). */
+enum {
+  REF_CONV_CENTRAL_SKEW = 0,  /* Central(4) on the Skew/Blaisdell form, taylor_green_vortex.py:13-15 */
+  REF_CONV_CENTRAL_CONS = 1,  /* Central(4) on Conservative(...) fluxes, parsing.py:189-216 */
+  REF_CONV_WENO_JS = 2,       /* LLFWeno(5, formulation='JS'), weno.py:341-369 */
+  REF_CONV_WENO_Z = 3,        /* LLFWeno(5, formulation='Z'),  weno.py:283-338 */
+  REF_CONV_TENO5 = 4,         /* LLFTeno(5), teno.py:195-213 */
+  REF_CONV_TENO6 = 5          /* LLFTeno(6), teno.py:216-234 */
+};
+enum { REF_AVG_SIMPLE = 0, REF_AVG_ROE = 1 };           /* averaging.py:31,62 */
+enum { REF_VISC_NONE = 0, REF_VISC_CONST = 1 };         /* taylor_green_vortex.py:18-19 */
+enum { REF_RK3_SBLI = 0, REF_RK3_LS = 1, REF_RK3_LS_SSP = 2, REF_RK4_LS = 3 };  /* rk_sbli.py, rk_LS.py */
+enum { REF_BC_PERIODIC = 0, REF_BC_EXTRAP0 = 1 };       /* periodic.py, extrapolation.py */
+
+typedef struct ref_config {
+  int32_t ndim;
+  int32_t n[3];          /* block0np0..2 (1 for absent dimensions) */
+  double delta[3];       /* Delta{d}block0 */
+  int32_t conv, averaging, viscous, rk;
+  int32_t bc[3][2];
+  double gama, Minf, Re, Pr, dt, eps, teno_ct;
+} ref_config;
+
+typedef struct ref_state ref_state;
+
+int ref_create(const ref_config* cfg, ref_state** out);
+void ref_destroy(ref_state* s);
+/* interior values, x fastest, n0*n1*n2 doubles */
+int ref_set_field(ref_state* s, int var, const double* interior);
+int ref_get_field(const ref_state* s, int var, double* interior);
+/* apply the boundary conditions, then evaluate one RHS into residual[nvar][cells] */
+int ref_rhs(ref_state* s, double* residual);
+/* niter full iterations of the generated time loop (algorithm.py:466-470) */
+int ref_run(ref_state* s, int64_t niter);
+
+/* ---- per-function entry points used by the pinning tests ---- */
+/* f points at the 6-point window f[-2..3] (array of 6); side +1 or -1 */
+double ref_recon_side(int conv, int side, const double* f6, double eps, double teno_ct);
+/* smoothness indicators (WENO: 3, TENO5: 3, TENO6: 4) written to beta */
+int ref_smoothness(int conv, int side, const double* f6, double* beta);
+/* eigensystem at an averaged state (rho, u[ndim], a) for direction d:
+   L (nvar*nvar row-major), R (nvar*nvar), lambda (nvar) */
+void ref_eigensystem(int ndim, int d, double gama, double rho, const double* u, double a,
+                     double* L, double* R, double* lam);
+/* Roe average between two states; out = [rho, u0..u(ndim-1), a] */
+void ref_roe_average(int ndim, double gama, const double* qL, const double* qR, double* out);
+/* central weights on points -2..2 of the first and second derivative */
+void ref_central_weights(double* w1, double* w2);
+/* RK coefficients of a temporal scheme: returns the number of stages */
+int ref_rk_coefficients(int rk, double* a, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

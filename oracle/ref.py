@@ -1,0 +1,144 @@
+"""ctypes binding of the oracle library (oracle/cpu_ref.c).
+
+Test infrastructure only: the checker for the HIP path, never the product.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {"plain": os.path.join(_HERE, "libosbl_oracle.so"), "fma": os.path.join(_HERE, "libosbl_oracle_fma.so")}
+
+CONV = {"central_skew": 0, "central_cons": 1, "weno_js": 2, "weno_z": 3, "teno5": 4, "teno6": 5}
+AVG = {"simple": 0, "roe": 1}
+VISC = {"none": 0, "const": 1}
+RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
+BC = {"periodic": 0, "extrap0": 1}
+
+
+class RefConfig(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int32),
+        ("n", ctypes.c_int32 * 3),
+        ("delta", ctypes.c_double * 3),
+        ("conv", ctypes.c_int32),
+        ("averaging", ctypes.c_int32),
+        ("viscous", ctypes.c_int32),
+        ("rk", ctypes.c_int32),
+        ("bc", (ctypes.c_int32 * 2) * 3),
+        ("gama", ctypes.c_double),
+        ("Minf", ctypes.c_double),
+        ("Re", ctypes.c_double),
+        ("Pr", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("teno_ct", ctypes.c_double),
+    ]
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not all(os.path.exists(p) for p in _LIBS.values()):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "all"], check=True, stdout=subprocess.DEVNULL)
+    return _LIBS["plain"]
+
+
+_libs = {}
+
+
+def lib(variant="plain"):
+    """The oracle library; variant "fma" is the same source built with FMA contraction."""
+    if variant not in _libs:
+        if not os.path.exists(_LIBS[variant]):
+            build()
+        L = ctypes.CDLL(_LIBS[variant])
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.ref_create.argtypes = [ctypes.POINTER(RefConfig), ctypes.POINTER(ctypes.c_void_p)]
+        L.ref_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.ref_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.ref_rhs.argtypes = [ctypes.c_void_p, dp]
+        L.ref_run.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.ref_recon_side.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]
+        L.ref_recon_side.restype = ctypes.c_double
+        L.ref_smoothness.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        L.ref_eigensystem.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp,
+                                      ctypes.c_double, dp, dp, dp]
+        L.ref_roe_average.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, dp]
+        L.ref_central_weights.argtypes = [dp, dp]
+        L.ref_rk_coefficients.argtypes = [ctypes.c_int, dp, dp]
+        _libs[variant] = L
+    return _libs[variant]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_config(p):
+    """Build a RefConfig from a problem dict (the same dict the HIP path takes)."""
+    c = RefConfig()
+    c.ndim = p["ndim"]
+    for d in range(3):
+        c.n[d] = p["n"][d] if d < p["ndim"] else 1
+        c.delta[d] = p["delta"][d] if d < p["ndim"] else 1.0
+    c.conv = CONV[p["conv"]]
+    c.averaging = AVG[p.get("averaging", "roe")]
+    c.viscous = VISC[p.get("viscous", "none")]
+    c.rk = RK[p.get("rk", "rk3")]
+    for d in range(3):
+        for s in range(2):
+            c.bc[d][s] = BC[p.get("bc", [["periodic"] * 2] * 3)[d][s]] if d < p["ndim"] else 0
+    c.gama = p.get("gama", 1.4)
+    c.Minf = p.get("Minf", 1.0)
+    c.Re = p.get("Re", 1.0)
+    c.Pr = p.get("Pr", 1.0)
+    c.dt = p["dt"]
+    c.eps = p.get("eps", 1e-16)
+    c.teno_ct = p.get("teno_ct", 1e-6)
+    return c
+
+
+class RefSolver:
+    """One block solved by the oracle; fields are interior arrays in (z, y, x) order."""
+
+    def __init__(self, problem, variant="plain"):
+        self.p = problem
+        self.L = lib(variant)
+        self.nd = problem["ndim"]
+        self.nvar = self.nd + 2
+        self.shape = tuple(reversed(problem["n"][: self.nd]))
+        self.cfg = make_config(problem)
+        h = ctypes.c_void_p()
+        if self.L.ref_create(ctypes.byref(self.cfg), ctypes.byref(h)) != 0:
+            raise RuntimeError("ref_create failed")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+    def set_state(self, q):
+        for m in range(self.nvar):
+            a = np.ascontiguousarray(q[m], dtype=np.float64)
+            assert a.shape == self.shape
+            self.L.ref_set_field(self.h, m, _ptr(a))
+
+    def get_state(self):
+        out = np.empty((self.nvar,) + self.shape)
+        for m in range(self.nvar):
+            a = np.empty(self.shape)
+            self.L.ref_get_field(self.h, m, _ptr(a))
+            out[m] = a
+        return out
+
+    def rhs(self):
+        out = np.empty((self.nvar,) + self.shape)
+        self.L.ref_rhs(self.h, _ptr(out))
+        return out
+
+    def run(self, niter):
+        self.L.ref_run(self.h, int(niter))

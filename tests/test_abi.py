@@ -1,0 +1,72 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(osbl_\w+)\s*\(", src, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "osbl_create" in names and "osbl_run" in names and len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    from opensbli_amd import native
+    lib = native.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.osbl_abi_version() == 1
+    assert sorted(native.EXPORTS) == declared_functions()
+
+
+def test_config_struct_layout_matches_header(tmp_path):
+    """osbl_config as ctypes lays it out equals the C compiler's layout of the header."""
+    import subprocess
+    from opensbli_amd.native import OsblConfig
+    fields = [f[0] for f in OsblConfig._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("{f} %zu\\n", offsetof(osbl_config, {f}));\n' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "opensbli_amd.h"\nint main(void) {\n'
+                   + body + '  printf("sizeof %zu\\n", sizeof(osbl_config));\n  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(OsblConfig, f).offset, f
+    assert int(out["sizeof"]) == ctypes.sizeof(OsblConfig)
+
+
+def test_create_without_device_fails_cleanly():
+    """Without a GPU the library reports an error instead of falling back."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from opensbli_amd import native
+    from opensbli_amd.problems import taylor_green
+    problem, _ = taylor_green(16)
+    with pytest.raises(native.OsblError):
+        native.Block(problem)
+
+
+def test_invalid_config_rejected():
+    from opensbli_amd import native
+    from opensbli_amd.problems import taylor_green
+    problem, _ = taylor_green(16)
+    problem["n"] = [4, 16, 16]
+    with pytest.raises(native.OsblError, match="at least 8"):
+        native.Block(problem)

@@ -1,0 +1,163 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle on identical inputs.
+
+Bar: relative L2 <= 1e-10 per conservative variable (FP64), after the step
+counts of the benchmark configurations (100 steps) at sizes the oracle runs
+in seconds.  WENO5-Z's hard-coded eps = 1e-14 (weno.py:286) makes some
+components rounding-sensitive: two builds of the oracle itself (with and
+without FMA contraction) differ there.  For those schemes the bar of a
+variable is max(1e-10, 4 x that oracle self-spread), measured in the test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(problem, q0, oracle):
+    from opensbli_amd.native import Block
+    g = Block(problem)
+    g.set_state(q0)
+    r = oracle.RefSolver(problem)
+    r.set_state(q0)
+    return g, r
+
+
+def _bar(problem, q0, oracle, steps, ref_state):
+    """Per-variable bar: max(TOL, 4 x spread between the two oracle builds)."""
+    f = oracle.RefSolver(problem, variant="fma")
+    f.set_state(q0)
+    f.run(steps)
+    spread = rel_l2(f.get_state(), ref_state)
+    return np.maximum(TOL, 4.0 * spread), spread
+
+
+@pytest.mark.parametrize("conv", ["central_skew", "central_cons", "weno_z", "weno_js", "teno6", "teno5"])
+def test_rhs_tgv3d(conv, oracle):
+    from opensbli_amd.problems import taylor_green
+    _say(f"case rhs tgv3d conv={conv} n=24")
+    problem, q0 = taylor_green(24, conv=conv)
+    g, r = _pair(problem, q0, oracle)
+    rr = r.rhs()
+    err = rel_l2(g.rhs(), rr)
+    f = oracle.RefSolver(problem, variant="fma")
+    f.set_state(q0)
+    spread = rel_l2(f.rhs(), rr)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+
+
+@pytest.mark.parametrize("conv", ["central_skew", "weno_z", "teno6", "weno_js", "teno5"])
+def test_run_tgv3d_100_steps(conv, oracle):
+    from opensbli_amd.problems import taylor_green
+    n, steps = 24, 100
+    _say(f"case run tgv3d conv={conv} n={n} steps={steps}")
+    problem, q0 = taylor_green(n, conv=conv)
+    g, r = _pair(problem, q0, oracle)
+    g.run(steps)
+    r.run(steps)
+    qr = r.get_state()
+    err = rel_l2(g.get_state(), qr)
+    bar, spread = _bar(problem, q0, oracle, steps, qr)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < bar), (err, bar)
+
+
+@pytest.mark.parametrize("rk", ["ls3", "ls3_ssp", "ls4"])
+def test_run_tgv3d_lowstorage(rk, oracle):
+    from opensbli_amd.problems import taylor_green
+    n, steps = 16, 20
+    _say(f"case run tgv3d rk={rk} n={n} steps={steps}")
+    problem, q0 = taylor_green(n, conv="teno6", rk=rk)
+    g, r = _pair(problem, q0, oracle)
+    g.run(steps)
+    r.run(steps)
+    qr = r.get_state()
+    err = rel_l2(g.get_state(), qr)
+    bar, spread = _bar(problem, q0, oracle, steps, qr)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < bar), (err, bar)
+
+
+def test_c1_tgv64_central_100_steps(oracle):
+    """Configuration 1 at full size: 64^3, central skew, RK3, 100 steps."""
+    from opensbli_amd.problems import taylor_green
+    _say("case C1 tgv64 central 100 steps")
+    problem, q0 = taylor_green(64, conv="central_skew")
+    g, r = _pair(problem, q0, oracle)
+    g.run(100)
+    r.run(100)
+    err = rel_l2(g.get_state(), r.get_state())
+    _say("rel L2", err)
+    assert np.all(err < TOL), err
+
+
+@pytest.mark.parametrize("conv", ["weno_z", "teno6", "central_skew", "central_cons"])
+def test_2d_kh(conv, oracle):
+    from opensbli_amd.problems import kelvin_helmholtz
+    _say(f"case 2d kh conv={conv} n=64 steps=100")
+    amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
+    problem, q0 = kelvin_helmholtz(64, amps, conv=conv)
+    g, r = _pair(problem, q0, oracle)
+    err = rel_l2(g.rhs(), r.rhs())
+    _say("rhs rel L2", err)
+    assert np.all(err < TOL), err
+    g.run(100)
+    r.run(100)
+    qr = r.get_state()
+    err = rel_l2(g.get_state(), qr)
+    bar, spread = _bar(problem, q0, oracle, 100, qr)
+    _say("state rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < bar), (err, bar)
+
+
+@pytest.mark.parametrize("conv", ["weno_z", "teno5", "teno6", "weno_js"])
+@pytest.mark.parametrize("n", [200, 1037])
+def test_1d_shock_tube(conv, n, oracle):
+    from opensbli_amd.problems import shock_tube
+    _say(f"case 1d sod conv={conv} n={n}")
+    problem, q0 = shock_tube(n, conv=conv)
+    g, r = _pair(problem, q0, oracle)
+    nsteps = 100
+    g.run(nsteps)
+    r.run(nsteps)
+    qr = r.get_state()
+    err = rel_l2(g.get_state(), qr)
+    bar, spread = _bar(problem, q0, oracle, nsteps, qr)
+    _say("state rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < bar), (err, bar)
+
+
+def test_tgv_symmetry_256_weno_z():
+    """Full size (256^3) WENO5-Z: properties that hold at any size.
+    The TGV initial state and the scheme are invariant under x -> -x with
+    (u, v, w) -> (-u, v, w) about x = pi, so rho stays mirror-symmetric; mass
+    is conserved to rounding on the periodic grid."""
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import taylor_green
+    _say("case tgv256 weno_z 2 steps symmetry/conservation")
+    n = 256
+    problem, q0 = taylor_green(n, conv="weno_z")
+    g = Block(problem)
+    g.set_state(q0)
+    g.run(2)
+    q = g.get_state()
+    assert np.all(np.isfinite(q))
+    m0, m1 = q0[0].sum(), q[0].sum()
+    assert abs(m1 - m0) / abs(m0) < 1e-13
+    # mirror about x = pi: index i -> (n - i) mod n
+    idx = (-np.arange(n)) % n
+    rho_m = q[0][:, :, idx]
+    assert np.max(np.abs(rho_m - q[0])) < 1e-12 * np.max(np.abs(q[0]))
+    ru_m = -q[1][:, :, idx]
+    assert np.max(np.abs(ru_m - q[1])) < 1e-11 * np.max(np.abs(q[1]))
