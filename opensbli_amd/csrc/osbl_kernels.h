@@ -12,6 +12,8 @@ void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, 
                   bool acc, hipStream_t s);
 void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
                    double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s);
+void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g, const Phys& ph,
+                          double cA, double cB, int first, int sz, hipStream_t s);
 void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s);
 void launch_unpack(const double* in, double* F, const Grid& g, hipStream_t s);
 long long halo_points(const Grid& g);

@@ -167,6 +167,98 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
 }
 
 // ---------------------------------------------------------------------------
+// Characteristic sweep along x for ND >= 2.  A wave owns 64 consecutive grid
+// lines (y,z rows) and one x segment of each; every lane slides the 6-point
+// window along its own row.  Rows are strided in memory, so the conservative
+// variables are staged through LDS in chunks of XC points: one chunk is a
+// 64-row x XC-point tile per variable, read from HBM in 64-byte row pieces
+// and stored column-major (lane = row) so that each lane's reads from LDS are
+// bank-conflict free.
+// ---------------------------------------------------------------------------
+constexpr int XC = 8;         // points per chunk
+constexpr int XLD = 65;       // padded column stride (doubles) of the LDS tile
+
+template <int ND, int CONV, int AVG, bool ACC>
+__global__ void __launch_bounds__(64, 2) k_char_sweep_x(const double* __restrict__ Q, double* __restrict__ R, Grid g,
+                                                      Phys ph, int seg, int nrows) {
+  constexpr int NV = ND + 2;
+  __shared__ double tile[NV][XC * XLD];
+  const int lane = threadIdx.x;
+  const int row0 = blockIdx.x * 64;
+  const int row = row0 + lane;
+  const bool active = row < nrows;
+  const int c0 = blockIdx.y * seg;
+  const int c1 = min(c0 + seg, g.n[0]);
+  const int cb = c0 - 3;           // first point of the window sequence
+  const int last = c1 + 2;         // last point needed
+  // row r -> (j, k)
+  auto row_base = [&](int r) { return g.idx(0, r % g.n[1], r / g.n[1]); };
+  const int my_base = active ? row_base(row) : 0;
+
+  auto load_chunk = [&](int first) {
+    // 64 rows x XC points x NV variables; lane -> (row slot = lane/XC + 8*t, col = lane%XC)
+    const int col = lane % XC;
+    const int p = first + col;
+#pragma unroll
+    for (int t = 0; t < 64 / (64 / XC); ++t) {
+      const int rs = lane / XC + (64 / XC) * t;
+      const int r = row0 + rs;
+      if (r < nrows && p <= last) {
+        const int off = row_base(r) + p;
+#pragma unroll
+        for (int m = 0; m < NV; ++m) tile[m][col * XLD + rs] = Q[m * g.fs + off];
+      }
+    }
+  };
+  auto pt_from_tile = [&](Pt<ND>& s, int col) {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) s.q[m] = tile[m][col * XLD + lane];
+    const double ir = 1.0 / s.q[0];
+    double ke = 0.0;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      s.u[m] = s.q[1 + m] * ir;
+      ke += s.u[m] * s.u[m];
+    }
+    s.p = ph.gm1 * (s.q[ND + 1] - 0.5 * s.q[0] * ke);
+    s.a = sqrt(ph.gama * s.p * ir);
+  };
+
+  const double inv = ph.inv_delta[0];
+  Pt<ND> w[6];
+  load_chunk(cb);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 5; ++p) pt_from_tile(w[p], p);
+  int j = 5;  // next point (relative to cb) to bring into the window
+  double fprev[NV];
+  for (int f = c0 - 1; f < c1; ++f) {
+    if (j % XC == 0) {
+      __syncthreads();
+      load_chunk(cb + j);
+      __syncthreads();
+    }
+    pt_from_tile(w[5], j % XC);
+    ++j;
+    double fh[NV];
+    char_flux<ND, 0, CONV, AVG>(w, ph, fh);
+    if (f >= c0 && active) {
+      const int x = my_base + f;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double v = -(fh[m] - fprev[m]) * inv;
+        if constexpr (ACC) R[m * g.fs + x] += v;
+        else R[m * g.fs + x] = v;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; ++m) fprev[m] = fh[m];
+#pragma unroll
+    for (int p = 0; p < 5; ++p) w[p] = w[p + 1];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Central fourth-order stencils (scheme.py:81-85): D1 weights
 // [1/12, -2/3, 0, 2/3, -1/12], D2 weights [-1/12, 4/3, -5/2, 4/3, -1/12].
 // ---------------------------------------------------------------------------
@@ -359,6 +451,193 @@ __global__ void __launch_bounds__(256) k_update(const double* __restrict__ Q, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// 3-D viscous terms + RK update for the characteristic schemes, tiled 2.5-D.
+// A 64 x 4 block of (x, y) columns marches along z.  Each z plane of the
+// constituent relations u_0..u_2, T (computed here from Q, with a 2-point
+// x/y halo) is staged in one of three rotating LDS slots; the x/y stencils
+// read LDS, the z stencils read a per-thread register queue of the five
+// planes z-2..z+2 holding u, T and the inner derivatives D_x u_0, D_x u_2,
+// D_y u_1, D_y u_2 of the mixed terms D_z(D_x u), D_z(D_y u).  The formulas
+// are those of viscous_terms (scheme.py:299-387).
+// ---------------------------------------------------------------------------
+constexpr int VTX = 64, VTY = 4, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
+
+template <int MODE>
+__global__ void __launch_bounds__(256, 2) k_visc_update3d(const double* __restrict__ Q, double* __restrict__ R,
+                                                           double* __restrict__ Qo, double* __restrict__ Qn, Grid g,
+                                                           Phys ph, double cA, double cB, int first, int sz) {
+  __shared__ double sp[3][4][VPL];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * VTX + tx;
+  const int x0 = blockIdx.x * VTX, y0 = blockIdx.y * VTY;
+  const int z0 = blockIdx.z * sz;
+  const int z1 = min(z0 + sz, g.n[2]);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool inside = x < g.n[0] && y < g.n[1];
+  const double ix = ph.inv_delta[0], iy = ph.inv_delta[1], iz = ph.inv_delta[2];
+
+  // stage plane zp (u0,u1,u2,T with a 2-point x/y halo) into LDS slot `slot`
+  auto stage = [&](int zp, int slot) {
+    for (int p = tid; p < VPL; p += VTX * VTY) {
+      const int xx = p % VPX, yy = p / VPX;
+      const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
+      if (gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1]) {
+        const int o = g.idx(gx, gy, zp);
+        const double rho = Q[o];
+        const double ir = 1.0 / rho;
+        const double u0 = Q[g.fs + o] * ir, u1 = Q[2 * g.fs + o] * ir, u2 = Q[3 * g.fs + o] * ir;
+        const double pr = ph.gm1 * (Q[4 * g.fs + o] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
+        sp[slot][0][p] = u0;
+        sp[slot][1][p] = u1;
+        sp[slot][2][p] = u2;
+        sp[slot][3][p] = pr * ph.gama * ph.Minf * ph.Minf * ir;
+      }
+    }
+  };
+  const int c = (ty + 2) * VPX + tx + 2;  // this thread's point in a slot
+  auto dx1 = [&](const double* f, int o) {
+    return (1.0 / 12.0) * (f[o - 2] - f[o + 2]) + (2.0 / 3.0) * (f[o + 1] - f[o - 1]);
+  };
+  auto dy1 = [&](const double* f, int o) {
+    return (1.0 / 12.0) * (f[o - 2 * VPX] - f[o + 2 * VPX]) + (2.0 / 3.0) * (f[o + VPX] - f[o - VPX]);
+  };
+  auto dx2 = [&](const double* f, int o) {
+    return (-1.0 / 12.0) * (f[o - 2] + f[o + 2]) + (4.0 / 3.0) * (f[o - 1] + f[o + 1]) - 2.5 * f[o];
+  };
+  auto dy2 = [&](const double* f, int o) {
+    return (-1.0 / 12.0) * (f[o - 2 * VPX] + f[o + 2 * VPX]) + (4.0 / 3.0) * (f[o - VPX] + f[o + VPX]) - 2.5 * f[o];
+  };
+  // column quantities of a plane: u0,u1,u2,T, Dx u0, Dx u2, Dy u1, Dy u2
+  auto column = [&](int slot, double* cq) {
+    const double* u0 = sp[slot][0];
+    const double* u1 = sp[slot][1];
+    const double* u2 = sp[slot][2];
+    cq[0] = u0[c];
+    cq[1] = u1[c];
+    cq[2] = u2[c];
+    cq[3] = sp[slot][3][c];
+    cq[4] = ix * dx1(u0, c);
+    cq[5] = ix * dx1(u2, c);
+    cq[6] = iy * dy1(u1, c);
+    cq[7] = iy * dy1(u2, c);
+  };
+
+  double zq[5][8];
+  // prime the pipeline with planes z0-2 .. z0+1
+  for (int k = 0; k < 4; ++k) {
+    const int zp = z0 - 2 + k;
+    __syncthreads();
+    stage(zp, (zp + 3) % 3);
+    __syncthreads();
+    double cq[8];
+    column((zp + 3) % 3, cq);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) zq[k + 1][e] = cq[e];
+  }
+  for (int z = z0; z < z1; ++z) {
+    __syncthreads();
+    stage(z + 2, (z + 2) % 3);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) zq[k][e] = zq[k + 1][e];
+    column((z + 2) % 3, zq[4]);
+    if (!inside) continue;
+    const int sl = z % 3;
+    const double* U[3] = {sp[sl][0], sp[sl][1], sp[sl][2]};
+    const double* T = sp[sl][3];
+    double du[3][3], dd[3][3][3];
+    // x and y first / second derivatives from LDS
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      du[m][0] = ix * dx1(U[m], c);
+      du[m][1] = iy * dy1(U[m], c);
+      dd[m][0][0] = ix * ix * dx2(U[m], c);
+      dd[m][1][1] = iy * iy * dy2(U[m], c);
+    }
+    // z derivatives from the queue (planes z-2..z+2 at indices 0..4)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      du[m][2] = iz * ((1.0 / 12.0) * (zq[0][m] - zq[4][m]) + (2.0 / 3.0) * (zq[3][m] - zq[1][m]));
+      dd[m][2][2] = iz * iz * ((-1.0 / 12.0) * (zq[0][m] + zq[4][m]) + (4.0 / 3.0) * (zq[1][m] + zq[3][m]) - 2.5 * zq[2][m]);
+    }
+    // mixed D_y(D_x u_m), m = 0, 1, from LDS
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const double* f = U[m];
+      const double a2 = ix * dx1(f, c - 2 * VPX), a1 = ix * dx1(f, c - VPX);
+      const double b1 = ix * dx1(f, c + VPX), b2 = ix * dx1(f, c + 2 * VPX);
+      const double v = iy * ((1.0 / 12.0) * (a2 - b2) + (2.0 / 3.0) * (b1 - a1));
+      dd[m][0][1] = v;
+      dd[m][1][0] = v;
+    }
+    // mixed D_z(D_x u_m), m = 0 (queue 4), 2 (queue 5); D_z(D_y u_m), m = 1 (6), 2 (7)
+    auto dzq = [&](int e) {
+      return iz * ((1.0 / 12.0) * (zq[0][e] - zq[4][e]) + (2.0 / 3.0) * (zq[3][e] - zq[1][e]));
+    };
+    dd[0][0][2] = dd[0][2][0] = dzq(4);
+    dd[2][0][2] = dd[2][2][0] = dzq(5);
+    dd[1][1][2] = dd[1][2][1] = dzq(6);
+    dd[2][1][2] = dd[2][2][1] = dzq(7);
+    const double divu = du[0][0] + du[1][1] + du[2][2];
+    double r[5];
+    const int o = g.idx(x, y, z);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) r[m] = R[m * g.fs + o];
+    double energy = ph.cq * (ix * ix * dx2(T, c) + iy * iy * dy2(T, c) +
+                             iz * iz * ((-1.0 / 12.0) * (zq[0][3] + zq[4][3]) + (4.0 / 3.0) * (zq[1][3] + zq[3][3]) -
+                                        2.5 * zq[2][3]));
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double um = zq[2][m];
+      double sk = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sk += dd[k][m][k];
+      double visc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double tt = dd[m][j][j] + dd[j][m][j];
+        if (j == m) tt -= (2.0 / 3.0) * sk;
+        const double dtau = ph.inv_re * tt;
+        const double tau = ph.inv_re * (du[m][j] + du[j][m] - (j == m ? (2.0 / 3.0) * divu : 0.0));
+        visc += dtau;
+        energy += du[m][j] * tau + um * dtau;
+      }
+      r[1 + m] += visc;
+    }
+    r[4] += energy;
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) R[m * g.fs + o] = r[m];
+    } else if constexpr (MODE == 0) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        const double qo = first ? Q[m * g.fs + o] : Qo[m * g.fs + o];
+        Qn[m * g.fs + o] = qo + ph.dt * cA * r[m];
+        Qo[m * g.fs + o] = qo + ph.dt * cB * r[m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        const double tmp = (cA == 0.0 ? 0.0 : cA * Qo[m * g.fs + o]) + ph.dt * r[m];
+        Qo[m * g.fs + o] = tmp;
+        Qn[m * g.fs + o] = Q[m * g.fs + o] + cB * tmp;
+      }
+    }
+  }
+}
+
+void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g, const Phys& ph,
+                          double cA, double cB, int first, int sz, hipStream_t s) {
+  dim3 grid((g.n[0] + VTX - 1) / VTX, (g.n[1] + VTY - 1) / VTY, (g.n[2] + sz - 1) / sz);
+  dim3 block(VTX, VTY);
+  if (mode == 0) k_visc_update3d<0><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
+  else if (mode == 1) k_visc_update3d<1><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
+  else k_visc_update3d<2><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
+}
+
 // interior <-> contiguous copies
 __global__ void __launch_bounds__(256) k_pack(const double* __restrict__ F, double* __restrict__ out, Grid g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -379,6 +658,13 @@ __global__ void __launch_bounds__(256) k_unpack(const double* __restrict__ in, d
 template <int ND, int D, int CONV, int AVG>
 static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
                            hipStream_t s) {
+  if constexpr (D == 0 && ND >= 2) {
+    const int nrows = g.n[1] * g.n[2];
+    dim3 grid((nrows + 63) / 64, (g.n[0] + seg - 1) / seg);
+    if (acc) k_char_sweep_x<ND, CONV, AVG, true><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows);
+    else k_char_sweep_x<ND, CONV, AVG, false><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows);
+    return;
+  }
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= g.n[e];

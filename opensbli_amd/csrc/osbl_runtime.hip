@@ -98,7 +98,9 @@ struct osbl_ctx {
   int nstages = 0;
   double A[5]{}, B[5]{};
   int seg = 32;
+  int vsz = 16;             // z segment of the tiled viscous kernel
   bool need_prims = false;
+  bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
   // profiling
   bool prof = false;
   struct Rec { int fam; hipEvent_t a, b; };
@@ -179,10 +181,16 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
       });
     }
   }
-  timed(c, F_UPDATE, [&] {
-    launch_update(c->nd, conv, c->cfg.viscous != OSBL_VISC_NONE, mode, Q, c->P, c->R, c->Qo, Qn, c->g, c->ph, cA,
-                  cB, first, c->stream);
-  });
+  const bool visc = c->cfg.viscous != OSBL_VISC_NONE;
+  if (c->tiled_visc) {
+    timed(c, F_UPDATE, [&] {
+      launch_visc_update3d(mode, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->vsz, c->stream);
+    });
+  } else {
+    timed(c, F_UPDATE, [&] {
+      launch_update(c->nd, conv, visc, mode, Q, c->P, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->stream);
+    });
+  }
 }
 
 int check_config(const osbl_config* cfg) {
@@ -268,7 +276,8 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   ph.inv_re = cfg->viscous != OSBL_VISC_NONE ? 1.0 / cfg->Re : 0.0;
   ph.cq = cfg->viscous != OSBL_VISC_NONE ? 1.0 / ((cfg->gama - 1.0) * cfg->Minf * cfg->Minf * cfg->Pr * cfg->Re) : 0.0;
   c->nstages = rk_coefficients(cfg->rk, c->A, c->B);
-  c->need_prims = is_central(cfg->conv) || cfg->viscous != OSBL_VISC_NONE;
+  c->tiled_visc = c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE;
+  c->need_prims = is_central(cfg->conv) || (cfg->viscous != OSBL_VISC_NONE && !c->tiled_visc);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t bundle = (size_t)fs * c->nv * sizeof(double);
   auto alloc = [&](double** p, size_t bytes) -> int {
