@@ -1,0 +1,63 @@
+"""The OpenSBLI-style problem-definition surface (opensbli_amd.api) builds
+the same configuration and initial state as the direct problem helpers, and
+refuses what the kernels do not implement."""
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize("scheme", ["central", "weno_z", "weno_js", "teno5", "teno6"])
+def test_tgv_problem_from_script(scheme):
+    from opensbli_amd.apps.taylor_green import build
+    from opensbli_amd.problems import taylor_green
+    prog = build(16, scheme)
+    p = prog.problem()
+    conv = "central_skew" if scheme == "central" else scheme
+    ref, q_ref = taylor_green(16, conv=conv)
+    for k in ["ndim", "n", "conv", "viscous", "rk", "averaging", "bc"]:
+        assert p[k] == ref[k], k
+    for k in ["dt", "gama", "Minf", "Re", "Pr"]:
+        assert abs(p[k] - ref[k]) < 1e-15 * abs(ref[k]), k
+    assert np.allclose(p["delta"], ref["delta"], rtol=1e-15, atol=0)
+    q = prog.initial_state(p)
+    assert np.abs(q - q_ref).max() < 1e-13
+
+
+def test_rejects_unsupported_schemes():
+    from opensbli_amd import api
+    with pytest.raises(NotImplementedError):
+        api.Central(6)
+    with pytest.raises(NotImplementedError):
+        api.LLFWeno(7, formulation="Z")
+    with pytest.raises(NotImplementedError):
+        api.LLFTeno(8)
+    with pytest.raises(NotImplementedError):
+        api.RungeKutta(4)
+    with pytest.raises(NotImplementedError):
+        api.SimulationDataType.set_datatype(api.FloatC)
+
+
+def test_rejects_unknown_equation():
+    from opensbli_amd import api
+    ee = api.EinsteinEquation()
+    sim = api.SimulationEquations()
+    sim.add_equations(ee.expand("Eq(Der(rho,t), - Conservative(rhou_j,x_j) + c_j)", 3, "x", [], []))
+    sim.add_equations(ee.expand("Eq(Der(rhou_i,t), -Conservative(rhou_i*u_j + KD(_i,_j)*p,x_j))", 3, "x", [], []))
+    sim.add_equations(ee.expand("Eq(Der(rhoE,t), - Conservative((p+rhoE)*u_j,x_j))", 3, "x", [], []))
+    block = api.SimulationBlock(3)
+    block.set_block_boundaries([api.PeriodicBC(d, s) for d in range(3) for s in (0, 1)])
+    block.set_equations([sim])
+    c = api.Central(4)
+    r = api.RungeKutta(3)
+    block.set_discretisation_schemes({c.name: c, r.name: r})
+    block.discretise()
+    with pytest.raises(NotImplementedError):
+        api.HIPC(api.TraditionalAlgorithmRK(block))
+
+
+def test_parameter_expressions():
+    from opensbli_amd.apps.taylor_green import build
+    prog = build(32, "central")
+    assert abs(prog.values["Delta0block0"] - 2 * np.pi / 32) < 1e-16
+    from opensbli_amd import api
+    prog.set_values(["niter"], ["ceil(0.2/0.0002)"])
+    assert prog.values["niter"] == 1000

@@ -161,3 +161,22 @@ def test_tgv_symmetry_256_weno_z():
     assert np.max(np.abs(rho_m - q[0])) < 1e-12 * np.max(np.abs(q[0]))
     ru_m = -q[1][:, :, idx]
     assert np.max(np.abs(ru_m - q[1])) < 1e-11 * np.max(np.abs(q[1]))
+
+
+def test_api_script_tgv_weno_z(oracle):
+    """The OpenSBLI-style script (HIPC target) runs on the GPU and matches the oracle."""
+    from opensbli_amd.apps.taylor_green import build
+    _say("case api tgv 16 weno_z 20 steps")
+    prog = build(16, "weno_z", niter=20)
+    out = prog.run()
+    problem = prog.problem()
+    q0 = prog.initial_state(problem)
+    r = oracle.RefSolver(problem)
+    r.set_state(q0)
+    r.run(20)
+    qr = r.get_state()
+    q = np.stack([out[k] for k in ["rho", "rhou0", "rhou1", "rhou2", "rhoE"]])
+    err = rel_l2(q, qr)
+    bar, spread = _bar(problem, q0, oracle, 20, qr)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < bar), (err, bar)
