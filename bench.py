@@ -61,11 +61,17 @@ def main():
     ap.add_argument("--conv", default="weno_z", choices=["weno_z", "weno_js", "teno6", "teno5", "central_skew"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble"])
+    ap.add_argument("--plain-stride", type=int, default=16,
+                    help="ensemble: time the per-instance baseline on every k-th instance and scale by work")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
 
     import torch
     from opensbli_amd.native import Block
@@ -155,6 +161,59 @@ def main():
         "stage_roofline": stage_roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_ensemble(args):
+    """512 uneven 1-D shock tubes (WENO5-Z, RK3, CFL 0.5, t = 0.2): one
+    persistent launch vs a graph-staged launch sequence vs per-instance runs."""
+    import numpy as np
+    import torch
+    from opensbli_amd.ensemble import Ensemble, autotune, ensemble_parameters, plain_runs
+
+    par = ensemble_parameters(512, seed=3)
+    t0 = time.perf_counter()
+    max_n, threads = autotune(par)
+    tune_s = time.perf_counter() - t0
+    e = Ensemble(par)
+    e.set_threads(max_n, threads)
+    e.save()
+    e.run("persistent")  # warm-up
+    e.restore()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms_p = e.run("persistent")
+    t_p = time.perf_counter() - t0
+    e.restore()
+    ms_s = e.run("staged")
+    cu = e.cell_updates()
+    e.close()
+    idx = list(range(0, 512, args.plain_stride))
+    t_plain_sample = plain_runs(par, indices=idx)
+    work = par["n"].astype(float) * par["nsteps"]
+    frac = work[idx].sum() / work.sum()
+    t_plain = t_plain_sample / frac
+    out = {
+        "metric": "batched ensemble speed-up over per-instance launches (512 1-D WENO5-Z shock tubes)",
+        "value": t_plain / (ms_p / 1e3),
+        "unit": "x",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded shock tubes, numpy RandomState(3))",
+        "config": {"workload": "ensemble_512_shock_tubes_weno_z_rk3", "instances": 512,
+                   "n_range": [int(par["n"].min()), int(par["n"].max())], "t_end": 0.2, "cfl": 0.5},
+        "persistent_ms": ms_p,
+        "persistent_wall_s": t_p,
+        "staged_graph_ms": ms_s,
+        "plain_s_estimate": t_plain,
+        "plain_sample": f"every {args.plain_stride}th instance ({len(idx)} runs, {frac:.3f} of the work) "
+                        f"took {t_plain_sample:.2f} s",
+        "cell_updates": cu,
+        "mcell_updates_per_s_persistent": cu / ms_p / 1e3,
+        "mcell_updates_per_s_staged": cu / ms_s / 1e3,
+        "autotune": {"max_n": max_n, "threads": threads, "seconds": tune_s},
     }
     print(json.dumps(out), flush=True)
 

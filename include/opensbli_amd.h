@@ -156,6 +156,35 @@ int osbl_kernel_work(osbl_ctx* ctx, int index, double* bytes, double* flops);
 /* Number of RK stages of the configured temporal scheme. */
 int osbl_num_stages(osbl_ctx* ctx);
 
+/*
+ * Ensembles of independent 1-D problems of uneven size (characteristic
+ * WENO/TENO fluxes, zeroth-order extrapolation ends, RungeKutta(3) or
+ * RungeKuttaLS(3)).  No counterpart in the reference, which would build and
+ * run one generated executable per instance (one OPSC program each,
+ * opsc.py:250-282); these entry points replace that loop of executables.
+ * Instance i has n[i] points, spacing delta[i], its own constant dt[i]
+ * (rk_LS.py:62-67 allows only a constant dt) and nsteps[i] iterations.
+ * base supplies the scheme, averaging, RK scheme, gama, eps and teno_ct.
+ */
+typedef struct osbl_batch osbl_batch;
+int osbl_batch_create(const osbl_config* base, int ninst, const int32_t* n, const double* delta, const double* dt,
+                      const int64_t* nsteps, osbl_batch** out);
+void osbl_batch_destroy(osbl_batch* b);
+/* state of one instance: 3 blocks of n[inst] doubles (rho, rhou0, rhoE) */
+int osbl_batch_set_state(osbl_batch* b, int inst, const double* host);
+int osbl_batch_get_state(osbl_batch* b, int inst, double* host);
+/* integrate every instance to its nsteps: mode 0 = one persistent launch,
+   mode 1 = one launch per RK stage over all running instances, replayed from
+   hipGraphs; returns the device time in *elapsed_ms */
+int osbl_batch_run(osbl_batch* b, int mode, double* elapsed_ms);
+/* persistent mode: threads per workgroup (64..512, multiple of 64) for instances with n <= max_n[k] */
+int osbl_batch_set_threads(osbl_batch* b, int nbuckets, const int32_t* max_n, const int32_t* threads);
+/* staged mode: iterations captured per hipGraph */
+int osbl_batch_set_graph_steps(osbl_batch* b, int steps);
+/* snapshot / restore the ensemble state on the device (for repeated timing) */
+int osbl_batch_save(osbl_batch* b);
+int osbl_batch_restore(osbl_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

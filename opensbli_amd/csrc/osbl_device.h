@@ -68,10 +68,7 @@ struct Pt {
 };
 
 template <int ND>
-__device__ __forceinline__ void load_pt(Pt<ND>& s, const double* __restrict__ Q, long long fs, int off,
-                                         const Phys& ph) {
-#pragma unroll
-  for (int m = 0; m < ND + 2; ++m) s.q[m] = Q[m * fs + off];
+__device__ __forceinline__ void finish_pt(Pt<ND>& s, const Phys& ph) {
   const double ir = 1.0 / s.q[0];
   double ke = 0.0;
 #pragma unroll
@@ -81,6 +78,20 @@ __device__ __forceinline__ void load_pt(Pt<ND>& s, const double* __restrict__ Q,
   }
   s.p = ph.gm1 * (s.q[ND + 1] - 0.5 * s.q[0] * ke);
   s.a = sqrt(ph.gama * s.p * ir);
+}
+
+// raw conservative variables of one point
+template <int ND>
+__device__ __forceinline__ void fetch_q(double (&q)[ND + 2], const double* __restrict__ Q, long long fs, int off) {
+#pragma unroll
+  for (int m = 0; m < ND + 2; ++m) q[m] = Q[m * fs + off];
+}
+
+template <int ND>
+__device__ __forceinline__ void load_pt(Pt<ND>& s, const double* __restrict__ Q, long long fs, int off,
+                                         const Phys& ph) {
+  fetch_q<ND>(s.q, Q, fs, off);
+  finish_pt<ND>(s, ph);
 }
 
 // ---------------------------------------------------------------------------

@@ -18,4 +18,7 @@ void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s);
 void launch_unpack(const double* in, double* F, const Grid& g, hipStream_t s);
 long long halo_points(const Grid& g);
 
+// record the thread-local message osbl_last_error() returns; returns code
+int set_error(int code, const char* msg);
+
 }  // namespace osbl

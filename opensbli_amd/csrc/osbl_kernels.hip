@@ -144,9 +144,15 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
   Pt<ND> w[6];
 #pragma unroll
   for (int p = 0; p < 5; ++p) load_pt<ND>(w[p], Q, g.fs, off + p * st, ph);
+  // the next point's raw values are fetched one iteration ahead
+  double nxt[NV];
+  fetch_q<ND>(nxt, Q, g.fs, off + 5 * st);
   double fprev[NV];
   for (int f = c0 - 1; f < c1; ++f) {
-    load_pt<ND>(w[5], Q, g.fs, off + 5 * st, ph);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) w[5].q[m] = nxt[m];
+    finish_pt<ND>(w[5], ph);
+    if (f + 1 < c1) fetch_q<ND>(nxt, Q, g.fs, off + 6 * st);
     double fh[NV];
     char_flux<ND, D, CONV, AVG>(w, ph, fh);
     if (f >= c0) {

@@ -29,6 +29,15 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+}  // namespace
+
+int osbl::set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+namespace {
+
 #define HIPCHK(call)                                                                  \
   do {                                                                                \
     hipError_t e_ = (call);                                                           \

@@ -22,6 +22,8 @@ EXPORTS = [
     "osbl_abi_version", "osbl_last_error", "osbl_create", "osbl_destroy", "osbl_set_field", "osbl_get_field",
     "osbl_set_field_dev", "osbl_get_field_dev", "osbl_run", "osbl_rhs", "osbl_sync", "osbl_stream",
     "osbl_profile", "osbl_kernel_stats", "osbl_work_per_stage", "osbl_kernel_work", "osbl_num_stages",
+    "osbl_batch_create", "osbl_batch_destroy", "osbl_batch_set_state", "osbl_batch_get_state", "osbl_batch_run",
+    "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
 ]
 
 
@@ -82,6 +84,16 @@ def load():
     L.osbl_work_per_stage.argtypes = [vp, dp, dp]
     L.osbl_kernel_work.argtypes = [vp, ctypes.c_int, dp, dp]
     L.osbl_num_stages.argtypes = [vp]
+    L.osbl_batch_create.argtypes = [ctypes.POINTER(OsblConfig), ctypes.c_int, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    L.osbl_batch_destroy.argtypes = [vp]
+    L.osbl_batch_destroy.restype = None
+    L.osbl_batch_set_state.argtypes = [vp, ctypes.c_int, vp]
+    L.osbl_batch_get_state.argtypes = [vp, ctypes.c_int, vp]
+    L.osbl_batch_run.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_batch_set_threads.argtypes = [vp, ctypes.c_int, vp, vp]
+    L.osbl_batch_set_graph_steps.argtypes = [vp, ctypes.c_int]
+    L.osbl_batch_save.argtypes = [vp]
+    L.osbl_batch_restore.argtypes = [vp]
     _lib = L
     return L
 
