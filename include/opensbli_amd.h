@@ -125,6 +125,10 @@ int osbl_run(osbl_ctx* ctx, int64_t niter);
  */
 int osbl_rhs(osbl_ctx* ctx, double* residual_host);
 
+/* Replay the time loop from hipGraphs of two iterations (default on; the
+   launches are issued directly while profiling is enabled). */
+int osbl_use_graphs(osbl_ctx* ctx, int enable);
+
 /* Wait for all work enqueued on the context's stream. */
 int osbl_sync(osbl_ctx* ctx);
 
@@ -155,6 +159,22 @@ int osbl_kernel_work(osbl_ctx* ctx, int index, double* bytes, double* flops);
 
 /* Number of RK stages of the configured temporal scheme. */
 int osbl_num_stages(osbl_ctx* ctx);
+
+/*
+ * Discrete adjoint of the inviscid central skew-symmetric scheme on periodic
+ * blocks (no counterpart in the reference, which generates forward code
+ * only).  All arrays are device pointers to packed interior fields
+ * (nvar blocks of n[0]*n[1]*n[2] doubles), enqueued on the context's stream.
+ *  osbl_rhs_vjp_dev:      out += (dR/dQ at q)^T mu
+ *  osbl_step_stages_dev:  advance one RK step from the current state and store
+ *                         the input state of every stage in stages
+ *                         (nstages x nvar blocks)
+ *  osbl_adjoint_step_dev: lambda <- (dQ^{n+1}/dQ^n)^T lambda for one
+ *                         RungeKutta(3) step whose stage inputs are in stages
+ */
+int osbl_rhs_vjp_dev(osbl_ctx* ctx, const double* q, const double* mu, double* out);
+int osbl_step_stages_dev(osbl_ctx* ctx, double* stages);
+int osbl_adjoint_step_dev(osbl_ctx* ctx, const double* stages, double* lambda);
 
 /*
  * Ensembles of independent 1-D problems of uneven size (characteristic

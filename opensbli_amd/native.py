@@ -24,6 +24,7 @@ EXPORTS = [
     "osbl_profile", "osbl_kernel_stats", "osbl_work_per_stage", "osbl_kernel_work", "osbl_num_stages",
     "osbl_batch_create", "osbl_batch_destroy", "osbl_batch_set_state", "osbl_batch_get_state", "osbl_batch_run",
     "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
+    "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
 ]
 
 
@@ -94,6 +95,10 @@ def load():
     L.osbl_batch_set_graph_steps.argtypes = [vp, ctypes.c_int]
     L.osbl_batch_save.argtypes = [vp]
     L.osbl_batch_restore.argtypes = [vp]
+    L.osbl_rhs_vjp_dev.argtypes = [vp, vp, vp, vp]
+    L.osbl_step_stages_dev.argtypes = [vp, vp]
+    L.osbl_adjoint_step_dev.argtypes = [vp, vp, vp]
+    L.osbl_use_graphs.argtypes = [vp, ctypes.c_int]
     _lib = L
     return L
 
@@ -194,6 +199,9 @@ class Block:
     def run(self, niter):
         check(load().osbl_run(self.h, int(niter)))
 
+    def use_graphs(self, enable):
+        check(load().osbl_use_graphs(self.h, 1 if enable else 0))
+
     def sync(self):
         check(load().osbl_sync(self.h))
 
@@ -232,6 +240,16 @@ class Block:
             check(L.osbl_kernel_work(self.h, i, ctypes.byref(b), ctypes.byref(f)))
             out[name.value.decode()] = (b.value, f.value)
         return out
+
+    # --- adjoint (device pointers to packed interior arrays) ---
+    def rhs_vjp_dev(self, q_ptr, mu_ptr, out_ptr):
+        check(load().osbl_rhs_vjp_dev(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(mu_ptr), ctypes.c_void_p(out_ptr)))
+
+    def step_stages_dev(self, stages_ptr):
+        check(load().osbl_step_stages_dev(self.h, ctypes.c_void_p(stages_ptr)))
+
+    def adjoint_step_dev(self, stages_ptr, lam_ptr):
+        check(load().osbl_adjoint_step_dev(self.h, ctypes.c_void_p(stages_ptr), ctypes.c_void_p(lam_ptr)))
 
     def num_stages(self):
         return check(load().osbl_num_stages(self.h))

@@ -1,0 +1,104 @@
+"""The hand-written discrete adjoint (central skew scheme, RK3, periodic):
+dot-product test of the RHS vector-Jacobian product, and the gradient of the
+final kinetic energy of the Kelvin-Helmholtz layer with respect to the 16
+mode amplitudes against central finite differences and against PyTorch FP64
+autograd of a plain restatement of the scheme (tests/torch_reference.py)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _kh(n, steps=200):
+    from opensbli_amd.problems import kelvin_helmholtz
+    amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
+    problem, q0 = kelvin_helmholtz(n, amps, conv="central_skew")
+    return problem, q0, amps
+
+
+def test_rhs_vjp_dot_product():
+    import torch
+    from opensbli_amd.native import Block
+    _say("case adjoint vjp dot product 32^2")
+    problem, q0, _ = _kh(32)
+    b = Block(problem)
+    rs = np.random.RandomState(0)
+    dq = rs.normal(size=q0.shape) * np.abs(q0).mean(axis=(1, 2))[:, None, None] * 1e-3
+    mu = rs.normal(size=q0.shape)
+    eps = 1e-4
+
+    def R(q):
+        b.set_state(q)
+        return b.rhs()
+
+    fd = (R(q0 + eps * dq) - R(q0 - eps * dq)) / (2 * eps)
+    lhs = float(np.sum(mu * fd))
+    qd = torch.tensor(q0, dtype=torch.float64, device="cuda").contiguous()
+    md = torch.tensor(mu, dtype=torch.float64, device="cuda").contiguous()
+    out = torch.zeros_like(qd)
+    torch.cuda.synchronize()
+    b.rhs_vjp_dev(qd.data_ptr(), md.data_ptr(), out.data_ptr())
+    b.sync()
+    rhs_ = float(np.sum(out.cpu().numpy() * dq))
+    _say("dot products", lhs, rhs_)
+    assert abs(lhs - rhs_) < 1e-7 * abs(lhs)
+
+
+def _grad_adjoint(n, steps, every):
+    import torch
+    from opensbli_amd.adjoint import integrate, kh_initial_state, kinetic_energy
+    from opensbli_amd.native import Block
+    problem, q0, amps = _kh(n)
+    b = Block(problem)
+    a = torch.tensor(amps, dtype=torch.float64, device="cuda", requires_grad=True)
+    q = kh_initial_state(a, n)
+    qN = integrate(q, b, steps, every)
+    J = kinetic_energy(qN, problem)
+    J.backward()
+    return problem, amps, float(J), a.grad.cpu().numpy(), b
+
+
+def test_gradient_vs_finite_differences():
+    import torch
+    from opensbli_amd.adjoint import integrate, kh_initial_state, kinetic_energy
+    _say("case adjoint gradient vs FD 64^2 200 steps every 20")
+    n, steps = 64, 200
+    problem, amps, J, g, b = _grad_adjoint(n, steps, 20)
+    h = 1e-5
+    for m in [0, 3, 9, 15]:
+        vals = []
+        for sgn in (1, -1):
+            a = amps.copy()
+            a[m] += sgn * h
+            q = kh_initial_state(torch.tensor(a, dtype=torch.float64, device="cuda"), n)
+            with torch.no_grad():
+                vals.append(float(kinetic_energy(integrate(q, b, steps, 200), problem)))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        _say(m, g[m], fd)
+        assert abs(g[m] - fd) < 1e-6 * max(abs(fd), 1e-3 * np.abs(g).max())
+
+
+def test_gradient_vs_torch_autograd():
+    import torch
+    import torch_reference as tr
+    from opensbli_amd.adjoint import kh_initial_state
+    _say("case adjoint gradient vs torch autograd 64^2 200 steps")
+    n, steps = 64, 200
+    problem, amps, J, g, b = _grad_adjoint(n, steps, 20)
+    a = torch.tensor(amps, dtype=torch.float64, device="cuda", requires_grad=True)
+    q = kh_initial_state(a, n)
+    qN = tr.integrate(q, steps, problem["dt"], problem["delta"], problem["gama"])
+    dv = problem["delta"][0] * problem["delta"][1]
+    Jt = 0.5 * ((qN[1] ** 2 + qN[2] ** 2) / qN[0]).sum() * dv
+    Jt.backward()
+    gt = a.grad.cpu().numpy()
+    _say("J", J, float(Jt), "max rel grad diff", np.abs(g - gt).max() / np.abs(gt).max())
+    assert abs(J - float(Jt)) < 1e-12 * abs(J)
+    assert np.abs(g - gt).max() < 1e-8 * np.abs(gt).max()
