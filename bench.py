@@ -61,7 +61,8 @@ def main():
     ap.add_argument("--conv", default="weno_z", choices=["weno_z", "weno_js", "teno6", "teno5", "central_skew"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble"])
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble", "flat_plate", "adjoint"])
+    ap.add_argument("--no-adjoint", action="store_true", help="tgv: skip the adjoint/forward ratio measurement")
     ap.add_argument("--plain-stride", type=int, default=16,
                     help="ensemble: time the per-instance baseline on every k-th instance and scale by work")
     args = ap.parse_args()
@@ -72,6 +73,11 @@ def main():
 
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "flat_plate":
+        return bench_flat_plate(args)
+    if args.workload == "adjoint":
+        print(json.dumps(adjoint_ratio()), flush=True)
+        return
 
     import torch
     from opensbli_amd.native import Block
@@ -136,6 +142,11 @@ def main():
     kernels = {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items() if v[1]}
     blk.close()
 
+    adj = None
+    if not args.no_adjoint:
+        _log("bench: adjoint/forward ratio (2-D Kelvin-Helmholtz 512^2, 200 steps, checkpoints every 20)")
+        adj = adjoint_ratio()
+
     cpu = None
     if not args.no_cpu_baseline:
         _log(f"bench: cpu baseline ({args.cpu_steps} step of {n}^3 on the host)")
@@ -160,7 +171,80 @@ def main():
         "roofline_fp64": roofline_fp64,
         "stage_roofline": stage_roofline,
         "kernels": kernels,
+        "adjoint": adj,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_ratio(n=512, steps=200, every=20, reps=3):
+    """Backward/forward time of the differentiable 2-D Kelvin-Helmholtz run:
+    forward = the checkpointed forward integration, backward = recomputation
+    of every segment plus the hand-written adjoint steps (median of reps)."""
+    import numpy as np
+    import torch
+    from opensbli_amd.adjoint import integrate, kh_initial_state, kinetic_energy
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import kelvin_helmholtz
+    amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
+    problem, _ = kelvin_helmholtz(n, amps, conv="central_skew")
+    b = Block(problem)
+    fw, bw = [], []
+    for rep in range(reps + 1):
+        a = torch.tensor(amps, dtype=torch.float64, device="cuda", requires_grad=True)
+        q = kh_initial_state(a, n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = kinetic_energy(integrate(q, b, steps, every), problem)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if rep:  # the first repetition warms up
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+    b.close()
+    f, w = float(np.median(fw)), float(np.median(bw))
+    return {"workload": f"kelvin_helmholtz_{n}^2_central_skew_rk3_{steps}_steps_checkpoint_{every}",
+            "forward_ms": f * 1e3, "backward_ms": w * 1e3, "ratio": w / f}
+
+
+def bench_flat_plate(args):
+    """The 480x256x128 flat plate (WENO5-Z + Sutherland viscous + metric,
+    RungeKuttaLS(3)): Mcell-updates/s per RK stage and per-kernel times."""
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import flat_plate
+    problem, q0 = flat_plate()
+    blk = Block(problem)
+    blk.set_state(q0)
+    stages = blk.num_stages()
+    blk.run(args.warmup)
+    blk.sync()
+    blk.profile(True)
+    blk.sync()
+    t0 = time.perf_counter()
+    blk.run(args.steps)
+    blk.sync()
+    t1 = time.perf_counter()
+    stats = blk.kernel_stats()
+    blk.profile(False)
+    blk.close()
+    cells = 480 * 256 * 128
+    el = t1 - t0
+    out = {
+        "metric": "Mcell-updates/s per RK stage (FP64, 480x256x128 flat plate)",
+        "value": cells * stages * args.steps / el / 1e6,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (similarity boundary-layer profile + N(0,1e-4) velocity noise, seed 5)",
+        "config": {"workload": "flat_plate_480x256x128_weno_z_sutherland_metric_ls3", "grid": [480, 256, 128]},
+        "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items() if v[1]},
     }
     print(json.dumps(out), flush=True)
 

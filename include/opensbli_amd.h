@@ -51,14 +51,26 @@ extern "C" {
 
 #define OSBL_VISC_NONE 0
 #define OSBL_VISC_CONST 1 /* tau_ij=(1/Re)(...), q_j = D_j T/((gama-1)Minf^2 Pr Re) (taylor_green_vortex.py:18-19) */
+#define OSBL_VISC_SUTHERLAND 2 /* mu = T^1.5 (1+SuthT/RefT)/(T+SuthT/RefT) as a CR (katzer_SBLI.py:12-26) */
 
 #define OSBL_RK3 0        /* RungeKutta(3), 3-register (rk_sbli.py:16-146) */
 #define OSBL_RK3_LS 1     /* RungeKuttaLS(3) (rk_LS.py:91-94) */
 #define OSBL_RK3_LS_SSP 2 /* RungeKuttaLS(3, formulation='SSP') (rk_LS.py:74-89) */
 #define OSBL_RK4_LS 3     /* RungeKuttaLS(4), five stages (rk_LS.py:96-97) */
 
-#define OSBL_BC_PERIODIC 0 /* PeriodicBC (periodic.py:7-56, exchange.py:9-57) */
-#define OSBL_BC_EXTRAP0 1  /* ExtrapolationBC(order=0) (extrapolation.py:29-58) */
+#define OSBL_BC_PERIODIC 0        /* PeriodicBC (periodic.py:7-56, exchange.py:9-57) */
+#define OSBL_BC_EXTRAP0 1         /* ExtrapolationBC(order=0) (extrapolation.py:37-41) */
+#define OSBL_BC_EXTRAP1 2         /* ExtrapolationBC(order=1) (extrapolation.py:42-54) */
+#define OSBL_BC_ZERO_GRADIENT 3   /* ZeroGradientOutletBC (zero_gradient_outlet.py:11-23) */
+#define OSBL_BC_ADIABATIC_WALL 4  /* AdiabaticWallBC (adiabatic_wall.py:28-69) */
+#define OSBL_BC_ISOTHERMAL_WALL 5 /* IsothermalWallBC with rhoE = rho Twall/(gama(gama-1)Minf^2) (isothermal_wall.py:33-88) */
+#define OSBL_BC_DIRICHLET 6       /* DirichletBC, values from osbl_set_bc_plane (dirichlet.py:28-41) */
+#define OSBL_BC_INLET_PEXTRAP 7   /* InletPressureExtrapolateBC, side 0 (inlet_pressure_extrapolate.py:32-66) */
+
+/* stencil closure at faces whose BC is a ModifyCentralDerivative (all but
+   periodic and zero-gradient; bc_core.py:15-16, opensblifunctions.py:523-534) */
+#define OSBL_BS_CARPENTER 0      /* Carpenter() (Carpenter_scheme.py:6-102), the BCs' default */
+#define OSBL_BS_REDUCED_ACCESS 1 /* ReducedAccess() (reduced_access_scheme.py:6-83) */
 
 /*
  * The problem definition, a flat mirror of the constants OpenSBLI declares
@@ -75,6 +87,12 @@ typedef struct osbl_config {
   int32_t rk;          /* OSBL_RK* */
   int32_t bc[3][2];    /* OSBL_BC_* per direction and side */
   double gama, Minf, Re, Pr, dt, eps, teno_ct;
+  int32_t bscheme[3][2]; /* OSBL_BS_* per face (faces that modify the central stencils) */
+  int32_t stretch[3];    /* 1: direction d is stretched along itself; coordinates via osbl_set_coordinates
+                            (MetricsEquation.generate_transformations, metric.py:37-54) */
+  int32_t reserved;
+  double suthT, refT;    /* SuthT, RefT of the Sutherland law */
+  double twall;          /* Twall of the isothermal wall */
 } osbl_config;
 
 typedef struct osbl_ctx osbl_ctx;
@@ -128,6 +146,19 @@ int osbl_rhs(osbl_ctx* ctx, double* residual_host);
 /* Replay the time loop from hipGraphs of two iterations (default on; the
    launches are issued directly while profiling is enabled). */
 int osbl_use_graphs(osbl_ctx* ctx, int enable);
+
+/* Coordinates x_d[0..n[d]) of a stretched direction.  Replaces the metric
+   kernels of MetricsEquation (metric.py:241-273): D_dd = 1/CD(x_d, xi_d) and
+   SD_ddd = CD(D_dd, xi_d) with the faces' stencil closures.  Both faces of d
+   must modify the central stencils. */
+int osbl_set_coordinates(osbl_ctx* ctx, int d, const double* x_host);
+
+/* Per-face plane of conservative values, [nvar][n_e2][n_e1] over the interior
+   extents of the two other directions (e1 < e2): the values a DirichletBC face
+   sets (dirichlet.py:28-41), or the inflow state an InletPressureExtrapolateBC
+   face keeps in its halos (written now; the reference's initial-condition
+   kernel covers the halos, katzer_init.py:233-245). */
+int osbl_set_bc_plane(osbl_ctx* ctx, int d, int side, const double* plane_host);
 
 /* Wait for all work enqueued on the context's stream. */
 int osbl_sync(osbl_ctx* ctx);

@@ -6,7 +6,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libosbl_hip.so")
-SOURCES = ["osbl_kernels.hip", "osbl_runtime.hip", "osbl_batch.hip", "osbl_adjoint.hip"]
+SOURCES = ["osbl_kernels.hip", "osbl_runtime.hip", "osbl_batch.hip", "osbl_adjoint.hip", "osbl_general.hip",
+           "osbl_boundary.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OSBL_ARCH", "gfx950")
 FLAGS = [
@@ -16,16 +17,23 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=()):
-    """Compile every HIP source and link libosbl_hip.so next to this file."""
-    objs = []
+def build(verbose=False, extra=(), jobs=None):
+    """Compile every HIP source (in parallel) and link libosbl_hip.so next to this file."""
+    from concurrent.futures import ThreadPoolExecutor
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        results = list(ex.map(lambda c: subprocess.run(c), cmds))
+    bad = [c for c, r in zip(cmds, results) if r.returncode != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, bad[0])
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)

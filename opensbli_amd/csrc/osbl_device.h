@@ -42,7 +42,19 @@ struct Phys {
   double gama, gm1, Minf, Re, Pr, dt, eps, teno_ct;
   double inv_delta[3];
   double inv_re, cq;  // 1/Re and 1/((gama-1) Minf^2 Pr Re)
+  double suth;        // SuthT/RefT of the Sutherland law
 };
+
+// Near-boundary stencils and metric of a block with non-periodic faces
+// (bc_core.py:15-16, opensblifunctions.py:523-534, metric.py:72-212).
+struct Bnd {
+  int mod[3][2];        // 0: central stencil; 1 + OSBL_BS_* where the face's BC replaces it
+  const double* dm[3];  // D_dd of a direction stretched along itself (n[d] values), or nullptr
+  const double* sd[3];  // SD_ddd = CD(D_dd, xi_d)
+  const double* cw;     // closure rows (6 weights each): Carpenter first derivative rows 0-3 at 0,
+                        // ReducedAccess first derivative rows 0-1 at 24, second derivative rows 0-1 at 36
+};
+constexpr int CW_CARP = 0, CW_RA = 24, CW_D2 = 36, CW_SIZE = 48;
 
 // ---------------------------------------------------------------------------
 // compile-time loop

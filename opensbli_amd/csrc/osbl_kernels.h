@@ -9,11 +9,19 @@ void launch_halo_periodic(double* Q, int nv, const Grid& g, hipStream_t s);
 void launch_extrap0(double* Q, int nv, const Grid& g, int d, int side, hipStream_t s);
 void launch_prims(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
-                  bool acc, hipStream_t s);
+                  const double* met, bool acc, hipStream_t s);
 void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
                    double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s);
 void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g, const Phys& ph,
                           double cA, double cB, int first, int sz, hipStream_t s);
+void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R,
+                       double* Qo, double* Qn, const Grid& g, const Phys& ph, const Bnd& b, double cA, double cB,
+                       int first, int suth, hipStream_t s);
+void launch_prims_mu(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
+void launch_bc_face(int nd, double* Q, double* Q2, const Grid& g, const Phys& ph, int d, int side, int type, int nh,
+                    double twall, const double* plane, hipStream_t s);
+void launch_periodic_dir(double* Q, int nv, const Grid& g, int d, hipStream_t s);
+void launch_fill_face_halo(double* Q, int nv, const Grid& g, int d, int side, const double* plane, hipStream_t s);
 void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s);
 void launch_unpack(const double* in, double* F, const Grid& g, hipStream_t s);
 long long halo_points(const Grid& g);

@@ -116,7 +116,8 @@ __global__ void __launch_bounds__(256) k_prims(const double* __restrict__ Q, dou
 // ---------------------------------------------------------------------------
 template <int ND, int D, int CONV, int AVG, bool ACC>
 __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q, double* __restrict__ R, Grid g,
-                                                     Phys ph, int seg, long long total) {
+                                                     Phys ph, int seg, long long total,
+                                                     const double* __restrict__ met) {
   constexpr int NV = ND + 2;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -157,9 +158,11 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
     char_flux<ND, D, CONV, AVG>(w, ph, fh);
     if (f >= c0) {
       const int x = off + 2 * st;
+      // a stretched direction scales the flux difference by D_dd (opensblifunctions.py:607-616)
+      const double gm = met ? met[f] : 1.0;
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
-        const double v = -(fh[m] - fprev[m]) * inv;
+        const double v = met ? -(gm * ((fh[m] - fprev[m]) * inv)) : -(fh[m] - fprev[m]) * inv;
         if constexpr (ACC) R[m * g.fs + x] += v;
         else R[m * g.fs + x] = v;
       }
@@ -186,7 +189,7 @@ constexpr int XLD = 65;       // padded column stride (doubles) of the LDS tile
 
 template <int ND, int CONV, int AVG, bool ACC>
 __global__ void __launch_bounds__(64, 2) k_char_sweep_x(const double* __restrict__ Q, double* __restrict__ R, Grid g,
-                                                      Phys ph, int seg, int nrows) {
+                                                      Phys ph, int seg, int nrows, const double* __restrict__ met) {
   constexpr int NV = ND + 2;
   __shared__ double tile[NV][XC * XLD];
   const int lane = threadIdx.x;
@@ -250,9 +253,10 @@ __global__ void __launch_bounds__(64, 2) k_char_sweep_x(const double* __restrict
     char_flux<ND, 0, CONV, AVG>(w, ph, fh);
     if (f >= c0 && active) {
       const int x = my_base + f;
+      const double gm = met ? met[f] : 1.0;
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
-        const double v = -(fh[m] - fprev[m]) * inv;
+        const double v = met ? -(gm * ((fh[m] - fprev[m]) * inv)) : -(fh[m] - fprev[m]) * inv;
         if constexpr (ACC) R[m * g.fs + x] += v;
         else R[m * g.fs + x] = v;
       }
@@ -663,12 +667,13 @@ __global__ void __launch_bounds__(256) k_unpack(const double* __restrict__ in, d
 // ---------------------------------------------------------------------------
 template <int ND, int D, int CONV, int AVG>
 static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
+                           const double* met,
                            hipStream_t s) {
   if constexpr (D == 0 && ND >= 2) {
     const int nrows = g.n[1] * g.n[2];
     dim3 grid((nrows + 63) / 64, (g.n[0] + seg - 1) / seg);
-    if (acc) k_char_sweep_x<ND, CONV, AVG, true><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows);
-    else k_char_sweep_x<ND, CONV, AVG, false><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows);
+    if (acc) k_char_sweep_x<ND, CONV, AVG, true><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows, met);
+    else k_char_sweep_x<ND, CONV, AVG, false><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows, met);
     return;
   }
   long long lines = 1;
@@ -677,39 +682,42 @@ static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys
   const long long nseg = (g.n[D] + seg - 1) / seg;
   const long long total = lines * nseg;
   const unsigned blocks = (unsigned)((total + 255) / 256);
-  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total);
-  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total);
+  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met);
+  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met);
 }
 
 template <int ND, int D, int CONV>
 static void launch_sweep_avg(int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
+                             const double* met,
                              hipStream_t s) {
-  if (avg == OSBL_AVG_ROE) launch_sweep_t<ND, D, CONV, OSBL_AVG_ROE>(Q, R, g, ph, seg, acc, s);
-  else launch_sweep_t<ND, D, CONV, OSBL_AVG_SIMPLE>(Q, R, g, ph, seg, acc, s);
+  if (avg == OSBL_AVG_ROE) launch_sweep_t<ND, D, CONV, OSBL_AVG_ROE>(Q, R, g, ph, seg, acc, met, s);
+  else launch_sweep_t<ND, D, CONV, OSBL_AVG_SIMPLE>(Q, R, g, ph, seg, acc, met, s);
 }
 
 template <int ND, int D>
 static void launch_sweep_conv(int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                              const double* met,
                               bool acc, hipStream_t s) {
   switch (conv) {
-    case OSBL_CONV_WENO_JS: launch_sweep_avg<ND, D, OSBL_CONV_WENO_JS>(avg, Q, R, g, ph, seg, acc, s); break;
-    case OSBL_CONV_WENO_Z: launch_sweep_avg<ND, D, OSBL_CONV_WENO_Z>(avg, Q, R, g, ph, seg, acc, s); break;
-    case OSBL_CONV_TENO5: launch_sweep_avg<ND, D, OSBL_CONV_TENO5>(avg, Q, R, g, ph, seg, acc, s); break;
-    default: launch_sweep_avg<ND, D, OSBL_CONV_TENO6>(avg, Q, R, g, ph, seg, acc, s); break;
+    case OSBL_CONV_WENO_JS: launch_sweep_avg<ND, D, OSBL_CONV_WENO_JS>(avg, Q, R, g, ph, seg, acc, met, s); break;
+    case OSBL_CONV_WENO_Z: launch_sweep_avg<ND, D, OSBL_CONV_WENO_Z>(avg, Q, R, g, ph, seg, acc, met, s); break;
+    case OSBL_CONV_TENO5: launch_sweep_avg<ND, D, OSBL_CONV_TENO5>(avg, Q, R, g, ph, seg, acc, met, s); break;
+    default: launch_sweep_avg<ND, D, OSBL_CONV_TENO6>(avg, Q, R, g, ph, seg, acc, met, s); break;
   }
 }
 
 void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                  const double* met,
                   bool acc, hipStream_t s) {
   if (nd == 3) {
-    if (d == 0) launch_sweep_conv<3, 0>(conv, avg, Q, R, g, ph, seg, acc, s);
-    else if (d == 1) launch_sweep_conv<3, 1>(conv, avg, Q, R, g, ph, seg, acc, s);
-    else launch_sweep_conv<3, 2>(conv, avg, Q, R, g, ph, seg, acc, s);
+    if (d == 0) launch_sweep_conv<3, 0>(conv, avg, Q, R, g, ph, seg, met, acc, s);
+    else if (d == 1) launch_sweep_conv<3, 1>(conv, avg, Q, R, g, ph, seg, met, acc, s);
+    else launch_sweep_conv<3, 2>(conv, avg, Q, R, g, ph, seg, met, acc, s);
   } else if (nd == 2) {
-    if (d == 0) launch_sweep_conv<2, 0>(conv, avg, Q, R, g, ph, seg, acc, s);
-    else launch_sweep_conv<2, 1>(conv, avg, Q, R, g, ph, seg, acc, s);
+    if (d == 0) launch_sweep_conv<2, 0>(conv, avg, Q, R, g, ph, seg, met, acc, s);
+    else launch_sweep_conv<2, 1>(conv, avg, Q, R, g, ph, seg, met, acc, s);
   } else {
-    launch_sweep_conv<1, 0>(conv, avg, Q, R, g, ph, seg, acc, s);
+    launch_sweep_conv<1, 0>(conv, avg, Q, R, g, ph, seg, met, acc, s);
   }
 }
 

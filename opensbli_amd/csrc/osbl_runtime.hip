@@ -5,6 +5,7 @@
 // sequence of stream-ordered launches per RK stage:
 //   boundary conditions -> [constituent relations] -> characteristic sweeps
 //   -> fused central/viscous residual + RK update -> boundary conditions.
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -89,6 +90,81 @@ int rk_coefficients(int rk, double* a, double* b) {
 
 bool is_central(int conv) { return conv == OSBL_CONV_CENTRAL_SKEW || conv == OSBL_CONV_CENTRAL_CONS; }
 
+// Closure rows of the central stencils, laid out as Bnd::cw.  Carpenter's
+// first-derivative rows are bc4 = inv(al4) ar4 with the R1, R2 constants of
+// Carpenter_scheme.py:80-102 (solved here in long double); ReducedAccess rows
+// reduced_access_scheme.py:34-40; second-derivative rows of both schemes
+// Carpenter_scheme.py:69-77.
+void closure_table(double* cw) {
+  const long double r1 = -(2177.0L * sqrtl(295369.0L) - 1166427.0L) / 25488.0L;
+  const long double r2 = (66195.0L * sqrtl(53.0L) * sqrtl(5573.0L) - 35909375.0L) / 101952.0L;
+  long double a[4][10] = {
+      {-(216 * r2 + 2160 * r1 - 2125) / 12960, (81 * r2 + 675 * r1 + 415) / 540, -(72 * r2 + 720 * r1 + 445) / 1440,
+       -(108 * r2 + 756 * r1 + 421) / 1296, -0.5L, -(864 * r2 + 6480 * r1 + 305) / 4320,
+       (216 * r2 + 1620 * r1 + 725) / 540, -(864 * r2 + 6480 * r1 + 3335) / 4320, 0, 0},
+      {(81 * r2 + 675 * r1 + 415) / 540, -(4104 * r2 + 32400 * r1 + 11225) / 4320,
+       (1836 * r2 + 14580 * r1 + 7295) / 2160, -(216 * r2 + 2160 * r1 + 655) / 4320, (864 * r2 + 6480 * r1 + 305) / 4320,
+       0, -(864 * r2 + 6480 * r1 + 2315) / 1440, (108 * r2 + 810 * r1 + 415) / 270, 0, 0},
+      {-(72 * r2 + 720 * r1 + 445) / 1440, (1836 * r2 + 14580 * r1 + 7295) / 2160,
+       -(4104 * r2 + 32400 * r1 + 12785) / 4320, (81 * r2 + 675 * r1 + 335) / 540, -(216 * r2 + 1620 * r1 + 725) / 540,
+       (864 * r2 + 6480 * r1 + 2315) / 1440, 0, -(864 * r2 + 6480 * r1 + 785) / 4320, -1.0L / 12, 0},
+      {-(108 * r2 + 756 * r1 + 421) / 1296, -(216 * r2 + 2160 * r1 + 655) / 4320, (81 * r2 + 675 * r1 + 335) / 540,
+       -(216 * r2 + 2160 * r1 - 12085) / 12960, (864 * r2 + 6480 * r1 + 3335) / 4320, -(108 * r2 + 810 * r1 + 415) / 270,
+       (864 * r2 + 6480 * r1 + 785) / 4320, 0, 8.0L / 12, -1.0L / 12}};
+  // Gauss-Jordan elimination of the 4x4 left block over the augmented rows
+  for (int col = 0; col < 4; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < 4; ++r)
+      if (fabsl(a[r][col]) > fabsl(a[piv][col])) piv = r;
+    for (int k = 0; k < 10; ++k) std::swap(a[col][k], a[piv][k]);
+    const long double inv = 1.0L / a[col][col];
+    for (int k = 0; k < 10; ++k) a[col][k] *= inv;
+    for (int r = 0; r < 4; ++r) {
+      if (r == col) continue;
+      const long double f = a[r][col];
+      for (int k = 0; k < 10; ++k) a[r][k] -= f * a[col][k];
+    }
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 6; ++k) cw[CW_CARP + 6 * r + k] = (double)a[r][4 + k];
+  const double ra[2][6] = {{-25.0 / 12.0, 48.0 / 12.0, -36.0 / 12.0, 16.0 / 12.0, -3.0 / 12.0, 0.0},
+                           {-3.0 / 12.0, -10.0 / 12.0, 18.0 / 12.0, -6.0 / 12.0, 1.0 / 12.0, 0.0}};
+  const double d2[2][6] = {{35.0 / 12.0, -104.0 / 12.0, 114.0 / 12.0, -56.0 / 12.0, 11.0 / 12.0, 0.0},
+                           {11.0 / 12.0, -20.0 / 12.0, 6.0 / 12.0, 4.0 / 12.0, -1.0 / 12.0, 0.0}};
+  for (int r = 0; r < 2; ++r)
+    for (int k = 0; k < 6; ++k) {
+      cw[CW_RA + 6 * r + k] = ra[r][k];
+      cw[CW_D2 + 6 * r + k] = d2[r][k];
+    }
+}
+
+// first derivative (without 1/Delta) of a line of N values at index c with
+// closures lo/hi (1 + OSBL_BS_*, or 0), the host twin of the device stencil
+double d1_line(const double* f, int N, int lo, int hi, int c, const double* cw) {
+  if (lo) {
+    const int nr = lo == 1 ? 4 : 2, np = lo == 1 ? 6 : 5;
+    if (c < nr) {
+      const double* w = cw + (lo == 1 ? CW_CARP : CW_RA) + 6 * c;
+      double acc = 0.0;
+      for (int k = 0; k < np; ++k) acc += w[k] * f[c + (k - c)];
+      return acc;
+    }
+  }
+  if (hi) {
+    const int nr = hi == 1 ? 4 : 2, np = hi == 1 ? 6 : 5;
+    const int r = N - 1 - c;
+    if (r < nr) {
+      const double* w = cw + (hi == 1 ? CW_CARP : CW_RA) + 6 * r;
+      double acc = 0.0;
+      for (int k = 0; k < np; ++k) acc += w[k] * f[c + (r - k)];
+      return -acc;
+    }
+  }
+  return (1.0 / 12.0) * (f[c - 2] - f[c + 2]) + (2.0 / 3.0) * (f[c + 1] - f[c - 1]);
+}
+
+bool modifies_stencil(int bc) { return bc != OSBL_BC_PERIODIC && bc != OSBL_BC_ZERO_GRADIENT; }
+
 }  // namespace
 
 struct osbl_ctx {
@@ -110,6 +186,13 @@ struct osbl_ctx {
   int vsz = 16;             // z segment of the tiled viscous kernel
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
+  bool general = false;     // closures, metric or Sutherland viscosity: the general update kernel
+  bool suth = false;
+  Bnd bnd{};
+  double* cw_dev = nullptr;                 // closure rows
+  double* met_dev[3] = {nullptr, nullptr, nullptr};  // D_dd then SD_ddd, n[d] each
+  double* plane_dev[3][2] = {};             // per-face planes
+  int nplane[3][2] = {};
   // profiling
   bool prof = false;
   struct Rec { int fam; hipEvent_t a, b; };
@@ -170,17 +253,37 @@ void drain_records(osbl_ctx* c) {
 
 double* Qcur(osbl_ctx* c) { return c->Qbuf[c->cur]; }
 
+// halo depth a boundary kernel fills: the largest halo of the schemes,
+// -3/+4 for WENO5/TENO (weno.py:17-29), -2/+2 for Central(4) (bc_core.py:136-147)
+int bc_halos(const osbl_ctx* c, int side) { return is_central(c->cfg.conv) ? 2 : (side == 0 ? 3 : 4); }
+
 void apply_bcs(osbl_ctx* c, double* Q) {
-  // Periodic directions: one pass that gives every halo point its periodic
-  // image.  Non-periodic faces are then filled by their own kernels, which
-  // overwrite whatever the wrap put there, in the order of the reference's
-  // boundary list (direction by direction, side 0 then side 1).
-  bool any_periodic = false;
-  for (int d = 0; d < c->nd; ++d) any_periodic |= c->cfg.bc[d][0] == OSBL_BC_PERIODIC;
-  if (any_periodic) timed(c, F_BC, [&] { launch_halo_periodic(Q, c->nv, c->g, c->stream); });
-  for (int d = 0; d < c->nd; ++d)
-    for (int s = 0; s < 2; ++s)
-      if (c->cfg.bc[d][s] == OSBL_BC_EXTRAP0) timed(c, F_BC, [&] { launch_extrap0(Q, c->nv, c->g, d, s, c->stream); });
+  bool all_periodic = true;
+  for (int d = 0; d < c->nd; ++d) all_periodic &= c->cfg.bc[d][0] == OSBL_BC_PERIODIC;
+  if (all_periodic) {
+    // one pass that gives every halo point its periodic image
+    timed(c, F_BC, [&] { launch_halo_periodic(Q, c->nv, c->g, c->stream); });
+    return;
+  }
+  // the reference's order (block.py:199-210): direction by direction, side 0 then side 1
+  double* Q2 = (Q == c->Qbuf[0]) ? c->Qbuf[1] : c->Qbuf[0];
+  for (int d = 0; d < c->nd; ++d) {
+    if (c->cfg.bc[d][0] == OSBL_BC_PERIODIC) {
+      timed(c, F_BC, [&] { launch_periodic_dir(Q, c->nv, c->g, d, c->stream); });
+      continue;
+    }
+    for (int s = 0; s < 2; ++s) {
+      const int bc = c->cfg.bc[d][s];
+      if (bc == OSBL_BC_EXTRAP0) {
+        timed(c, F_BC, [&] { launch_extrap0(Q, c->nv, c->g, d, s, c->stream); });
+      } else {
+        timed(c, F_BC, [&] {
+          launch_bc_face(c->nd, Q, Q2, c->g, c->ph, d, s, bc, bc_halos(c, s), c->cfg.twall, c->plane_dev[d][s],
+                         c->stream);
+        });
+      }
+    }
+  }
 }
 
 // spatial residual of the current state into R (characteristic part only),
@@ -189,16 +292,26 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
   double* Q = Qcur(c);
   double* Qn = c->Qbuf[1 - c->cur];
   const int conv = c->cfg.conv;
-  if (c->need_prims) timed(c, F_PRIMS, [&] { launch_prims(c->nd, Q, c->P, c->g, c->ph, c->stream); });
+  if (c->need_prims) {
+    timed(c, F_PRIMS, [&] {
+      if (c->suth) launch_prims_mu(c->nd, Q, c->P, c->g, c->ph, c->stream);
+      else launch_prims(c->nd, Q, c->P, c->g, c->ph, c->stream);
+    });
+  }
   if (!is_central(conv)) {
     for (int d = 0; d < c->nd; ++d) {
       timed(c, F_SWEEP_X + d, [&] {
-        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, d > 0, c->stream);
+        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[d], d > 0, c->stream);
       });
     }
   }
   const bool visc = c->cfg.viscous != OSBL_VISC_NONE;
-  if (c->tiled_visc) {
+  if (c->general) {
+    timed(c, F_UPDATE, [&] {
+      launch_update_gen(c->nd, conv, visc, mode, Q, c->P, c->R, c->Qo, Qn, c->g, c->ph, c->bnd, cA, cB, first,
+                        c->suth ? 1 : 0, c->stream);
+    });
+  } else if (c->tiled_visc) {
     timed(c, F_UPDATE, [&] {
       launch_visc_update3d(mode, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->vsz, c->stream);
     });
@@ -224,15 +337,30 @@ int check_config(const osbl_config* cfg) {
     return fail(OSBL_E_ARG, "unknown convective scheme %d", cfg->conv);
   if (cfg->averaging != OSBL_AVG_SIMPLE && cfg->averaging != OSBL_AVG_ROE)
     return fail(OSBL_E_ARG, "unknown averaging %d", cfg->averaging);
-  if (cfg->viscous != OSBL_VISC_NONE && cfg->viscous != OSBL_VISC_CONST)
+  if (cfg->viscous < OSBL_VISC_NONE || cfg->viscous > OSBL_VISC_SUTHERLAND)
     return fail(OSBL_E_ARG, "unknown viscous model %d", cfg->viscous);
+  if (cfg->viscous == OSBL_VISC_SUTHERLAND && !(cfg->refT > 0.0 && cfg->suthT >= 0.0))
+    return fail(OSBL_E_ARG, "Sutherland viscosity needs RefT > 0 and SuthT >= 0");
   if (cfg->rk < OSBL_RK3 || cfg->rk > OSBL_RK4_LS) return fail(OSBL_E_ARG, "unknown RK scheme %d", cfg->rk);
   for (int d = 0; d < cfg->ndim; ++d) {
-    for (int s = 0; s < 2; ++s)
-      if (cfg->bc[d][s] != OSBL_BC_PERIODIC && cfg->bc[d][s] != OSBL_BC_EXTRAP0)
-        return fail(OSBL_E_ARG, "unknown boundary condition %d on direction %d side %d", cfg->bc[d][s], d, s);
+    for (int s = 0; s < 2; ++s) {
+      const int bc = cfg->bc[d][s];
+      if (bc < OSBL_BC_PERIODIC || bc > OSBL_BC_INLET_PEXTRAP)
+        return fail(OSBL_E_ARG, "unknown boundary condition %d on direction %d side %d", bc, d, s);
+      if (bc == OSBL_BC_INLET_PEXTRAP && s != 0)
+        return fail(OSBL_E_ARG, "direction %d: the pressure-extrapolating inlet is implemented for side 0 only", d);
+      if (bc == OSBL_BC_ADIABATIC_WALL || bc == OSBL_BC_ISOTHERMAL_WALL || bc == OSBL_BC_INLET_PEXTRAP) {
+        if (cfg->ndim < 2 && bc != OSBL_BC_INLET_PEXTRAP)
+          return fail(OSBL_E_ARG, "wall boundaries need a 2-D or 3-D block");
+      }
+      if (cfg->bscheme[d][s] != OSBL_BS_CARPENTER && cfg->bscheme[d][s] != OSBL_BS_REDUCED_ACCESS)
+        return fail(OSBL_E_ARG, "unknown boundary scheme %d on direction %d side %d", cfg->bscheme[d][s], d, s);
+    }
     if ((cfg->bc[d][0] == OSBL_BC_PERIODIC) != (cfg->bc[d][1] == OSBL_BC_PERIODIC))
       return fail(OSBL_E_ARG, "direction %d: a periodic boundary needs both sides periodic", d);
+    if (cfg->stretch[d] && !(modifies_stencil(cfg->bc[d][0]) && modifies_stencil(cfg->bc[d][1])))
+      return fail(OSBL_E_ARG, "direction %d: a stretched direction needs boundary conditions that close the "
+                              "central stencils on both faces", d);
   }
   if (cfg->gama <= 1.0) return fail(OSBL_E_ARG, "gama must exceed 1");
   if (cfg->viscous != OSBL_VISC_NONE && (cfg->Re <= 0.0 || cfg->Pr <= 0.0 || cfg->Minf <= 0.0))
@@ -291,8 +419,20 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   for (int d = 0; d < 3; ++d) ph.inv_delta[d] = d < c->nd ? 1.0 / cfg->delta[d] : 0.0;
   ph.inv_re = cfg->viscous != OSBL_VISC_NONE ? 1.0 / cfg->Re : 0.0;
   ph.cq = cfg->viscous != OSBL_VISC_NONE ? 1.0 / ((cfg->gama - 1.0) * cfg->Minf * cfg->Minf * cfg->Pr * cfg->Re) : 0.0;
+  ph.suth = cfg->viscous == OSBL_VISC_SUTHERLAND ? cfg->suthT / cfg->refT : 0.0;
   c->nstages = rk_coefficients(cfg->rk, c->A, c->B);
-  c->tiled_visc = c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE;
+  c->suth = cfg->viscous == OSBL_VISC_SUTHERLAND;
+  bool closures = false, stretched = false;
+  for (int d = 0; d < c->nd; ++d) {
+    stretched |= cfg->stretch[d] != 0;
+    for (int s = 0; s < 2; ++s)
+      if (modifies_stencil(cfg->bc[d][s])) {
+        c->bnd.mod[d][s] = 1 + cfg->bscheme[d][s];
+        closures = true;
+      }
+  }
+  c->general = closures || stretched || c->suth;
+  c->tiled_visc = !c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE;
   c->need_prims = is_central(cfg->conv) || (cfg->viscous != OSBL_VISC_NONE && !c->tiled_visc);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t bundle = (size_t)fs * c->nv * sizeof(double);
@@ -307,9 +447,23 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
     osbl_destroy(c);
     return rc;
   }
-  if (c->need_prims && (rc = alloc(&c->P, (size_t)fs * (c->nd + 2) * sizeof(double)))) {
+  if (c->need_prims && (rc = alloc(&c->P, (size_t)fs * (c->nd + 3) * sizeof(double)))) {
     osbl_destroy(c);
     return rc;
+  }
+  {
+    double cw[CW_SIZE];
+    closure_table(cw);
+    if ((rc = alloc(&c->cw_dev, sizeof(cw)))) {
+      osbl_destroy(c);
+      return rc;
+    }
+    if (hipMemcpyAsync(c->cw_dev, cw, sizeof(cw), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      osbl_destroy(c);
+      return fail(OSBL_E_HIP, "closure table upload failed");
+    }
+    c->bnd.cw = c->cw_dev;
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     osbl_destroy(c);
@@ -335,6 +489,11 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->Abuf);
   (void)hipFree(c->Mbuf);
   (void)hipFree(c->pk);
+  (void)hipFree(c->cw_dev);
+  for (int d = 0; d < 3; ++d) {
+    (void)hipFree(c->met_dev[d]);
+    for (int s = 0; s < 2; ++s) (void)hipFree(c->plane_dev[d][s]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -376,6 +535,20 @@ int osbl_get_field(osbl_ctx* c, int var, double* host) {
   return OSBL_OK;
 }
 
+// metric and face planes the configuration asks for must be set before stepping
+static int check_ready(const osbl_ctx* c) {
+  for (int d = 0; d < c->nd; ++d) {
+    if (c->cfg.stretch[d] && !c->bnd.dm[d])
+      return fail(OSBL_E_STATE, "direction %d is stretched: call osbl_set_coordinates first", d);
+    for (int s = 0; s < 2; ++s) {
+      const int bc = c->cfg.bc[d][s];
+      if ((bc == OSBL_BC_DIRICHLET || bc == OSBL_BC_INLET_PEXTRAP) && !c->plane_dev[d][s])
+        return fail(OSBL_E_STATE, "face (%d, %d) needs its plane: call osbl_set_bc_plane first", d, s);
+    }
+  }
+  return OSBL_OK;
+}
+
 static void iterate(osbl_ctx* c, int mode) {
   for (int s = 0; s < c->nstages; ++s) {
     stage(c, mode, c->A[s], c->B[s], s == 0);
@@ -406,6 +579,7 @@ static int capture_pair(osbl_ctx* c, int mode, int start) {
 int osbl_run(osbl_ctx* c, int64_t niter) {
   if (!c) return fail(OSBL_E_ARG, "null context");
   if (niter < 0) return fail(OSBL_E_ARG, "niter must be non-negative");
+  if (int rc = check_ready(c)) return rc;
   const int mode = c->cfg.rk == OSBL_RK3 ? 0 : 1;
   if (niter > 0) apply_bcs(c, Qcur(c));
   int64_t it = 0;
@@ -432,6 +606,60 @@ int osbl_run(osbl_ctx* c, int64_t niter) {
   return OSBL_OK;
 }
 
+int osbl_set_coordinates(osbl_ctx* c, int d, const double* x) {
+  if (!c || !x) return fail(OSBL_E_ARG, "null argument");
+  if (d < 0 || d >= c->nd) return fail(OSBL_E_ARG, "direction %d out of range", d);
+  if (!c->cfg.stretch[d]) return fail(OSBL_E_STATE, "direction %d is not declared stretched", d);
+  const int N = c->g.n[d];
+  double cw[CW_SIZE];
+  closure_table(cw);
+  // metric kernels of MetricsEquation (metric.py:241-273): detJ = CD(x_d, xi_d),
+  // D_dd = 1/detJ, then SD_ddd = CD(D_dd, xi_d), each with the faces' closures
+  std::vector<double> buf(2 * (size_t)N);
+  const double inv = 1.0 / c->cfg.delta[d];
+  const int lo = c->bnd.mod[d][0], hi = c->bnd.mod[d][1];
+  for (int i = 0; i < N; ++i) buf[i] = 1.0 / (inv * d1_line(x, N, lo, hi, i, cw));
+  for (int i = 0; i < N; ++i) buf[N + i] = inv * d1_line(buf.data(), N, lo, hi, i, cw);
+  for (int i = 0; i < N; ++i)
+    if (!std::isfinite(buf[i]) || !std::isfinite(buf[N + i]))
+      return fail(OSBL_E_ARG, "direction %d: the coordinates do not give a finite metric at %d", d, i);
+  if (!c->met_dev[d] && hipMalloc((void**)&c->met_dev[d], 2 * (size_t)N * sizeof(double)) != hipSuccess)
+    return fail(OSBL_E_NOMEM, "metric allocation failed");
+  HIPCHK(hipMemcpyAsync(c->met_dev[d], buf.data(), 2 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->bnd.dm[d] = c->met_dev[d];
+  c->bnd.sd[d] = c->met_dev[d] + N;
+  for (int k = 0; k < 2; ++k)  // launch arguments changed
+    if (c->graph2[k]) {
+      (void)hipGraphExecDestroy(c->graph2[k]);
+      c->graph2[k] = nullptr;
+    }
+  return OSBL_OK;
+}
+
+int osbl_set_bc_plane(osbl_ctx* c, int d, int side, const double* plane) {
+  if (!c || !plane) return fail(OSBL_E_ARG, "null argument");
+  if (d < 0 || d >= c->nd || side < 0 || side > 1) return fail(OSBL_E_ARG, "face (%d, %d) out of range", d, side);
+  const int bc = c->cfg.bc[d][side];
+  if (bc != OSBL_BC_DIRICHLET && bc != OSBL_BC_INLET_PEXTRAP)
+    return fail(OSBL_E_STATE, "face (%d, %d) takes no plane of values", d, side);
+  const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  const size_t n = (size_t)c->nv * c->g.n[e1] * c->g.n[e2];
+  if (!c->plane_dev[d][side] && hipMalloc((void**)&c->plane_dev[d][side], n * sizeof(double)) != hipSuccess)
+    return fail(OSBL_E_NOMEM, "plane allocation failed");
+  HIPCHK(hipMemcpyAsync(c->plane_dev[d][side], plane, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (bc == OSBL_BC_INLET_PEXTRAP)  // both state buffers carry the inflow halos
+    for (int k = 0; k < 2; ++k) launch_fill_face_halo(c->Qbuf[k], c->nv, c->g, d, side, c->plane_dev[d][side], c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 2; ++k)
+    if (c->graph2[k]) {
+      (void)hipGraphExecDestroy(c->graph2[k]);
+      c->graph2[k] = nullptr;
+    }
+  return OSBL_OK;
+}
+
 int osbl_use_graphs(osbl_ctx* c, int enable) {
   if (!c) return fail(OSBL_E_ARG, "null context");
   c->use_graphs = enable != 0;
@@ -440,6 +668,7 @@ int osbl_use_graphs(osbl_ctx* c, int enable) {
 
 int osbl_rhs(osbl_ctx* c, double* residual_host) {
   if (!c || !residual_host) return fail(OSBL_E_ARG, "null argument");
+  if (int rc = check_ready(c)) return rc;
   apply_bcs(c, Qcur(c));
   stage(c, 2, 0.0, 0.0, 0);
   HIPCHK(hipGetLastError());

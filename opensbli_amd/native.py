@@ -13,9 +13,11 @@ LIB_PATH = os.path.join(HERE, "libosbl_hip.so")
 
 CONV = {"central_skew": 0, "central_cons": 1, "weno_js": 2, "weno_z": 3, "teno5": 4, "teno6": 5}
 AVG = {"simple": 0, "roe": 1}
-VISC = {"none": 0, "const": 1}
+VISC = {"none": 0, "const": 1, "sutherland": 2}
 RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
-BC = {"periodic": 0, "extrap0": 1}
+BC = {"periodic": 0, "extrap0": 1, "extrap1": 2, "zero_gradient": 3, "adiabatic_wall": 4,
+      "isothermal_wall": 5, "dirichlet": 6, "inlet_pextrap": 7}
+BSCHEME = {"carpenter": 0, "reduced_access": 1}
 
 # symbols the header declares (checked by the ABI test)
 EXPORTS = [
@@ -25,6 +27,7 @@ EXPORTS = [
     "osbl_batch_create", "osbl_batch_destroy", "osbl_batch_set_state", "osbl_batch_get_state", "osbl_batch_run",
     "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
     "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
+    "osbl_set_coordinates", "osbl_set_bc_plane",
 ]
 
 
@@ -45,6 +48,12 @@ class OsblConfig(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("eps", ctypes.c_double),
         ("teno_ct", ctypes.c_double),
+        ("bscheme", (ctypes.c_int32 * 2) * 3),
+        ("stretch", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32),
+        ("suthT", ctypes.c_double),
+        ("refT", ctypes.c_double),
+        ("twall", ctypes.c_double),
     ]
 
 
@@ -99,6 +108,8 @@ def load():
     L.osbl_step_stages_dev.argtypes = [vp, vp]
     L.osbl_adjoint_step_dev.argtypes = [vp, vp, vp]
     L.osbl_use_graphs.argtypes = [vp, ctypes.c_int]
+    L.osbl_set_coordinates.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_set_bc_plane.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     _lib = L
     return L
 
@@ -132,6 +143,14 @@ def make_config(p):
     c.dt = p["dt"]
     c.eps = p.get("eps", 1e-16)
     c.teno_ct = p.get("teno_ct", 1e-6)
+    bs = p.get("bscheme", [["carpenter", "carpenter"]] * 3)
+    for d in range(3):
+        for s in range(2):
+            c.bscheme[d][s] = BSCHEME[bs[d][s]] if d < nd else 0
+        c.stretch[d] = int(p.get("stretch", [0, 0, 0])[d]) if d < nd else 0
+    c.suthT = p.get("SuthT", 110.4)
+    c.refT = p.get("RefT", 288.0)
+    c.twall = p.get("Twall", 1.0)
     return c
 
 
@@ -156,6 +175,12 @@ class Block:
         h = ctypes.c_void_p()
         check(L.osbl_create(ctypes.byref(self.cfg), ctypes.byref(h)))
         self.h = h
+        for d, x in problem.get("coords", {}).items():
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            check(L.osbl_set_coordinates(self.h, int(d), _dptr(a)))
+        for (d, side), pl in problem.get("planes", {}).items():
+            a = np.ascontiguousarray(pl, dtype=np.float64)
+            check(L.osbl_set_bc_plane(self.h, int(d), int(side), _dptr(a)))
 
     def close(self):
         if getattr(self, "h", None):

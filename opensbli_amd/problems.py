@@ -123,3 +123,80 @@ def kelvin_helmholtz_state(n, amplitudes, sigma=0.05, gama=1.4):
         v = v + a * np.sin(2.0 * math.pi * m * xx) * env
     p = 2.5
     return np.stack([rho, rho * u, rho * v, p / (gama - 1.0) + 0.5 * rho * (u * u + v * v)])
+
+
+def stretched_coordinate(n, length, factor, kind="sinh"):
+    """Wall-clustered coordinate x(j), j = 0..n-1, on [0, length].
+
+    "sinh": length*sinh(by*xi/length)/sinh(by), xi = j*length/(n-1), the map of
+    the reference's flat-plate apps (katzer_SBLI.py:131, transitional_SBLI.py:168);
+    "tanh": length*(1 + tanh(by*(eta - 1))/tanh(by)), eta = j/(n-1), a one-sided
+    form of the channel apps' map (turbulent_channel.py:157).
+    """
+    eta = np.arange(n, dtype=np.float64) / (n - 1)
+    if kind == "sinh":
+        return length * np.sinh(factor * eta) / np.sinh(factor)
+    if kind == "tanh":
+        return length * (1.0 + np.tanh(factor * (eta - 1.0)) / np.tanh(factor))
+    raise ValueError("unknown stretching %r" % kind)
+
+
+def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950.0, Pr=0.72, gama=1.4,
+               Tinf=288.0, SuthT=110.4, stretch_factor=5.0, stretch="sinh", dt=0.025, conv="weno_z", rk="ls3",
+               noise=1e-4, seed=5, bscheme="reduced_access", wall="adiabatic_wall", Twall=None, ndim=None):
+    """Laminar flat-plate boundary layer, wall-normal stretched grid.
+
+    x0 streamwise (inlet with pressure extrapolation, zeroth-order outlet),
+    x1 wall-normal (stretched; no-slip wall below, free-stream Dirichlet
+    above), x2 spanwise periodic (3-D).  The inflow is the compressible
+    similarity profile at Re_delta* (boundary_layer.SimilarityProfile) and the
+    interior velocity carries N(0, noise) perturbations from
+    numpy.random.RandomState(seed).  Viscosity follows Sutherland's law
+    (katzer_SBLI.py:26), the convective flux is the characteristic LLF
+    scheme `conv` with Roe averaging, the viscous terms are Central(4) with the
+    metric of the stretched direction (katzer_SBLI.py:37-70).
+
+    Returns (problem, q0); the problem carries the wall-normal coordinates
+    ("coords") and the planes of the inlet and of the Dirichlet top.
+    """
+    from .boundary_layer import SimilarityProfile
+    nd = len(n) if ndim is None else ndim
+    n = list(n[:nd])
+    lengths = list(lengths[:nd])
+    delta = [lengths[0] / (n[0] - 1), lengths[1] / (n[1] - 1)] + ([lengths[2] / n[2]] if nd == 3 else [])
+    bl = SimilarityProfile(Minf, Re, Pr=Pr, gama=gama, Tw=-1.0 if wall == "adiabatic_wall" else Twall,
+                           Tinf=Tinf, suth=SuthT)
+    y = stretched_coordinate(n[1], lengths[1], stretch_factor, stretch)
+    rho, u, rv, T = bl.evaluate(y)
+    # profile as (nvar, n1) columns
+    cons = [rho, rho * u, rv] + ([np.zeros_like(rho)] if nd == 3 else [])
+    ke = 0.5 * (cons[1] ** 2 + cons[2] ** 2) / rho
+    rhoE = rho * T / (gama * (gama - 1.0) * Minf ** 2) + ke
+    prof = np.stack(cons + [rhoE])                                      # (nvar, n1)
+    shape = tuple(reversed(n))                                          # (z,) y, x
+    bshape = (1,) * (nd - 2) + (n[1], 1)
+    q0 = np.stack([np.broadcast_to(prof[m].reshape(bshape), shape) for m in range(nd + 2)]).copy()
+    if noise:
+        rs = np.random.RandomState(seed)
+        vel = [q0[1 + m] / q0[0] for m in range(nd)]
+        kin0 = 0.5 * sum(v * v for v in vel) * q0[0]
+        for m in range(nd):
+            vel[m] = vel[m] + rs.normal(0.0, noise, shape)
+        kin1 = 0.5 * sum(v * v for v in vel) * q0[0]
+        for m in range(nd):
+            q0[1 + m] = q0[0] * vel[m]
+        q0[nd + 1] = q0[nd + 1] - kin0 + kin1     # keep the temperature profile
+        # the wall rows stay at rest
+        q0[1:nd + 1, ..., 0, :] = 0.0
+    # inlet plane: the clean profile over (x1, x2) -> [nvar][n_e2][n_e1] with e1 = 1, e2 = 2
+    n2 = n[2] if nd == 3 else 1
+    inlet = np.repeat(prof[:, None, :], n2, axis=1)                    # (nvar, n2, n1)
+    # top plane (normal 1): free-stream values of the profile's last row over (x0, x2)
+    top = np.repeat(np.repeat(prof[:, -1][:, None, None], n2, axis=1), n[0], axis=2)  # (nvar, n2, n0)
+    bc = [["inlet_pextrap", "extrap0"], [wall, "dirichlet"]] + ([["periodic", "periodic"]] if nd == 3 else [])
+    bs = [[bscheme, bscheme], [bscheme, bscheme]] + ([["carpenter", "carpenter"]] if nd == 3 else [])
+    problem = dict(ndim=nd, n=n, delta=delta, conv=conv, averaging="roe", viscous="sutherland", rk=rk, bc=bc,
+                   bscheme=bs, stretch=[0, 1, 0][:nd] + [0] * (3 - nd), gama=gama, Minf=Minf, Re=Re, Pr=Pr, dt=dt,
+                   eps=1e-16, teno_ct=1e-6, SuthT=SuthT, RefT=Tinf, Twall=bl.Twall if Twall is None else Twall,
+                   coords={1: y}, planes={(0, 0): inlet, (1, 1): top})
+    return problem, q0

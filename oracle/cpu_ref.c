@@ -41,8 +41,12 @@ struct ref_state {
   double* q[MAXV];
   double* qold[MAXV];  /* Q_RKold (3-register) or tempRK (low storage) */
   double* res[MAXV];
-  double *u[3], *p, *T, *a;
+  double *u[3], *p, *T, *a, *mu;
   double* wk[NWK];
+  int mod[3][2];      /* 1 + REF_BS_* where the face's BC replaces the central stencils, else 0 */
+  double* dm[3];      /* metric D_dd of a stretched direction (n[d] values), else NULL */
+  double* sd[3];      /* SD_ddd = CD(D_dd, xi_d) */
+  double* plane[3][2];/* per-face conservative plane (Dirichlet values / inflow halo state) */
 };
 
 static inline ptrdiff_t IX(const ref_state* s, int i, int j, int k) {
@@ -80,13 +84,137 @@ void ref_central_weights(double* w1, double* w2) {
   for (int i = 0; i < 5; ++i) { w1[i] = W1[i]; w2[i] = W2[i]; }
 }
 
-/* out = D_d f (first derivative) over range R */
+/* ------------------------------------------------------------------------- */
+/* Near-boundary replacements of the central stencils at faces whose BC is a  */
+/* ModifyCentralDerivative (bc_core.py:15-16; opensblifunctions.py:523-534).  */
+/* Rows are the side-0 stencils on f[0..5] at grid points 0, 1, ...; side 1   */
+/* mirrors them on f[N-1], f[N-2], ... with the first derivative negated.     */
+/* ------------------------------------------------------------------------- */
+/* Carpenter 4th-order one-sided closure, bc4 = inv(al4) * ar4 with the R1, R2
+   constants of Carpenter_scheme.py:80-102, evaluated here in long double. */
+static double CARP1[4][6];
+static int carp_ready = 0;
+static void carpenter_init(void) {
+  if (carp_ready) return;
+  const long double R1 = -(2177.0L * sqrtl(295369.0L) - 1166427.0L) / 25488.0L;
+  const long double R2 = (66195.0L * sqrtl(53.0L) * sqrtl(5573.0L) - 35909375.0L) / 101952.0L;
+  long double al[4][4] = {
+      {-(216 * R2 + 2160 * R1 - 2125) / 12960, (81 * R2 + 675 * R1 + 415) / 540, -(72 * R2 + 720 * R1 + 445) / 1440,
+       -(108 * R2 + 756 * R1 + 421) / 1296},
+      {(81 * R2 + 675 * R1 + 415) / 540, -(4104 * R2 + 32400 * R1 + 11225) / 4320,
+       (1836 * R2 + 14580 * R1 + 7295) / 2160, -(216 * R2 + 2160 * R1 + 655) / 4320},
+      {-(72 * R2 + 720 * R1 + 445) / 1440, (1836 * R2 + 14580 * R1 + 7295) / 2160,
+       -(4104 * R2 + 32400 * R1 + 12785) / 4320, (81 * R2 + 675 * R1 + 335) / 540},
+      {-(108 * R2 + 756 * R1 + 421) / 1296, -(216 * R2 + 2160 * R1 + 655) / 4320, (81 * R2 + 675 * R1 + 335) / 540,
+       -(216 * R2 + 2160 * R1 - 12085) / 12960}};
+  long double ar[4][6] = {
+      {-1.0L / 2, -(864 * R2 + 6480 * R1 + 305) / 4320, (216 * R2 + 1620 * R1 + 725) / 540,
+       -(864 * R2 + 6480 * R1 + 3335) / 4320, 0, 0},
+      {(864 * R2 + 6480 * R1 + 305) / 4320, 0, -(864 * R2 + 6480 * R1 + 2315) / 1440, (108 * R2 + 810 * R1 + 415) / 270,
+       0, 0},
+      {-(216 * R2 + 1620 * R1 + 725) / 540, (864 * R2 + 6480 * R1 + 2315) / 1440, 0,
+       -(864 * R2 + 6480 * R1 + 785) / 4320, -1.0L / 12, 0},
+      {(864 * R2 + 6480 * R1 + 3335) / 4320, -(108 * R2 + 810 * R1 + 415) / 270, (864 * R2 + 6480 * R1 + 785) / 4320, 0,
+       8.0L / 12, -1.0L / 12}};
+  /* Gauss-Jordan with partial pivoting on [al | ar] */
+  for (int c = 0; c < 4; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (fabsl(al[r][c]) > fabsl(al[piv][c])) piv = r;
+    if (piv != c) {
+      for (int k = 0; k < 4; ++k) { long double t = al[c][k]; al[c][k] = al[piv][k]; al[piv][k] = t; }
+      for (int k = 0; k < 6; ++k) { long double t = ar[c][k]; ar[c][k] = ar[piv][k]; ar[piv][k] = t; }
+    }
+    long double inv = 1.0L / al[c][c];
+    for (int k = 0; k < 4; ++k) al[c][k] *= inv;
+    for (int k = 0; k < 6; ++k) ar[c][k] *= inv;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      long double f = al[r][c];
+      for (int k = 0; k < 4; ++k) al[r][k] -= f * al[c][k];
+      for (int k = 0; k < 6; ++k) ar[r][k] -= f * ar[c][k];
+    }
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 6; ++k) CARP1[r][k] = (double)ar[r][k];
+  carp_ready = 1;
+}
+/* ReducedAccess first-derivative closure (reduced_access_scheme.py:34-40) */
+static const double RA1[2][6] = {{-25.0 / 12.0, 48.0 / 12.0, -36.0 / 12.0, 16.0 / 12.0, -3.0 / 12.0, 0.0},
+                                  {-3.0 / 12.0, -10.0 / 12.0, 18.0 / 12.0, -6.0 / 12.0, 1.0 / 12.0, 0.0}};
+/* second-derivative closure of both schemes (Carpenter_scheme.py:69-77,
+   reduced_access_scheme.py:75-83) */
+static const double BC2[2][6] = {{35.0 / 12.0, -104.0 / 12.0, 114.0 / 12.0, -56.0 / 12.0, 11.0 / 12.0, 0.0},
+                                 {11.0 / 12.0, -20.0 / 12.0, 6.0 / 12.0, 4.0 / 12.0, -1.0 / 12.0, 0.0}};
+
+int ref_closure_weights(int scheme, int order, double* w) {
+  carpenter_init();
+  if (order == 2) {
+    for (int r = 0; r < 2; ++r)
+      for (int k = 0; k < 6; ++k) w[6 * r + k] = BC2[r][k];
+    return 2;
+  }
+  if (scheme == REF_BS_CARPENTER) {
+    for (int r = 0; r < 4; ++r)
+      for (int k = 0; k < 6; ++k) w[6 * r + k] = CARP1[r][k];
+    return 4;
+  }
+  for (int r = 0; r < 2; ++r)
+    for (int k = 0; k < 6; ++k) w[6 * r + k] = RA1[r][k];
+  return 2;
+}
+
+/* First derivative D_xi f (without 1/Delta) at the point x, grid index c of N
+   along a line of stride st, with the closures lo/hi (1 + REF_BS_*, or 0).
+   Side-0 rows are tested first, as in the Piecewise of
+   opensblifunctions.py:523-534. */
+static double d1_gen(const double* f, ptrdiff_t x, ptrdiff_t st, int N, int lo, int hi, int c) {
+  if (lo) {
+    const int nr = lo - 1 == REF_BS_CARPENTER ? 4 : 2, np = lo - 1 == REF_BS_CARPENTER ? 6 : 5;
+    if (c < nr) {
+      const double* w = lo - 1 == REF_BS_CARPENTER ? CARP1[c] : RA1[c];
+      double acc = 0.0;
+      for (int k = 0; k < np; ++k) acc += w[k] * f[x + (k - c) * st];
+      return acc;
+    }
+  }
+  if (hi) {
+    const int nr = hi - 1 == REF_BS_CARPENTER ? 4 : 2, np = hi - 1 == REF_BS_CARPENTER ? 6 : 5;
+    const int r = N - 1 - c;
+    if (r < nr) {
+      const double* w = hi - 1 == REF_BS_CARPENTER ? CARP1[r] : RA1[r];
+      double acc = 0.0;
+      for (int k = 0; k < np; ++k) acc += w[k] * f[x + (r - k) * st];
+      return -acc;
+    }
+  }
+  return W1[0] * f[x - 2 * st] + W1[1] * f[x - st] + W1[3] * f[x + st] + W1[4] * f[x + 2 * st];
+}
+/* Homogeneous second derivative D_xixi f (without 1/Delta^2) */
+static double d2_gen(const double* f, ptrdiff_t x, ptrdiff_t st, int N, int lo, int hi, int c) {
+  if (lo && c < 2) {
+    double acc = 0.0;
+    for (int k = 0; k < 5; ++k) acc += BC2[c][k] * f[x + (k - c) * st];
+    return acc;
+  }
+  if (hi && N - 1 - c < 2) {
+    const int r = N - 1 - c;
+    double acc = 0.0;
+    for (int k = 0; k < 5; ++k) acc += BC2[r][k] * f[x + (r - k) * st];
+    return acc;
+  }
+  return W2[0] * f[x - 2 * st] + W2[1] * f[x - st] + W2[2] * f[x] + W2[3] * f[x + st] + W2[4] * f[x + 2 * st];
+}
+static inline int coord(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
+
+/* out = D_d f (first derivative in the computational coordinate) over range R */
 static void deriv1(ref_state* s, const double* f, double* out, int d, range3 R) {
   const ptrdiff_t st = s->st[d];
   const double inv = 1.0 / s->c.delta[d];
+  const int N = s->n[d], lo = s->mod[d][0], hi = s->mod[d][1];
   LOOP(R) {
     ptrdiff_t x = IX(s, i, j, k);
-    out[x] = inv * (W1[0] * f[x - 2 * st] + W1[1] * f[x - st] + W1[3] * f[x + st] + W1[4] * f[x + 2 * st]);
+    out[x] = inv * d1_gen(f, x, st, N, lo, hi, coord(d, i, j, k));
   }
 }
 /* out = D_dd f (homogeneous second derivative) over range R */
@@ -94,10 +222,10 @@ static void deriv2(ref_state* s, const double* f, double* out, int d, range3 R) 
   const ptrdiff_t st = s->st[d];
   const double inv = 1.0 / s->c.delta[d];
   const double inv2 = inv * inv;
+  const int N = s->n[d], lo = s->mod[d][0], hi = s->mod[d][1];
   LOOP(R) {
     ptrdiff_t x = IX(s, i, j, k);
-    out[x] = inv2 * (W2[0] * f[x - 2 * st] + W2[1] * f[x - st] + W2[2] * f[x] + W2[3] * f[x + st] +
-                     W2[4] * f[x + 2 * st]);
+    out[x] = inv2 * d2_gen(f, x, st, N, lo, hi, coord(d, i, j, k));
   }
 }
 
@@ -126,7 +254,17 @@ static void constituent_relations(ref_state* s) {
     s->T[x] = pp * g * M2 / rho[x];
     s->a[x] = sqrt(g * pp / rho[x]);
   }
+  if (s->c.viscous == REF_VISC_SUTHERLAND) { /* Eq(mu, T**1.5*(1+SuthT/RefT)/(T+SuthT/RefT)), katzer_SBLI.py:26 */
+    const double sr = s->c.suthT / s->c.refT;
+    LOOP(R) {
+      ptrdiff_t x = IX(s, i, j, k);
+      s->mu[x] = pow(s->T[x], 1.5) * (1.0 + sr) / (s->T[x] + sr);
+    }
+  }
 }
+
+/* metric factor D_dd at grid index c of a stretched direction (1 otherwise) */
+static inline double met1(const ref_state* s, int d, int c) { return s->dm[d] ? s->dm[d][c] : 1.0; }
 
 /* ------------------------------------------------------------------------- */
 /* Central convective terms                                                  */
@@ -176,6 +314,19 @@ static void central_skew(ref_state* s) {
     double* dp = w[D0 + 15];
     deriv1(s, ud, dud, d, I);
     deriv1(s, s->p, dp, d, I);
+    /* a stretched direction scales every derivative by D_dd (metric.py:72-126) */
+    if (s->dm[d]) {
+      double* all[16];
+      int na = 0;
+      for (int t = 0; t < nd + 3; ++t) all[na++] = dprod[t];
+      for (int m = 0; m < nv; ++m) all[na++] = dq[m];
+      all[na++] = dud;
+      all[na++] = dp;
+      for (int t = 0; t < na; ++t) {
+        double* f = all[t];
+        LOOP(I) { ptrdiff_t x = IX(s, i, j, k); f[x] = met1(s, d, coord(d, i, j, k)) * f[x]; }
+      }
+    }
     /* residual contributions of direction d (the generated residual kernel sums over j) */
     LOOP(I) {
       ptrdiff_t x = IX(s, i, j, k);
@@ -217,37 +368,57 @@ static void central_conservative(ref_state* s) {
     deriv1(s, w[1 + nd], w[21 + nd], d, I);
     LOOP(I) {
       ptrdiff_t x = IX(s, i, j, k);
-      s->res[0][x] -= w[20][x];
-      for (int m = 0; m < nd + 1; ++m) s->res[1 + m][x] -= w[21 + m][x];
+      const double g1 = met1(s, d, coord(d, i, j, k));
+      if (s->dm[d]) {
+        s->res[0][x] -= g1 * w[20][x];
+        for (int m = 0; m < nd + 1; ++m) s->res[1 + m][x] -= g1 * w[21 + m][x];
+      } else {
+        s->res[0][x] -= w[20][x];
+        for (int m = 0; m < nd + 1; ++m) s->res[1 + m][x] -= w[21 + m][x];
+      }
     }
   }
 }
 
 /* ------------------------------------------------------------------------- */
-/* Viscous central terms with constant viscosity (TGV stress/heat-flux form,  */
-/* taylor_green_vortex.py:14-19, scheme.py:299-327,360-387).  Every term      */
-/* containing Re is accumulated into the residual (scheme.py:263-267,291-294). */
-/* Mixed derivatives are D_b(D_a u) with a < b (opensblifunctions.py:341-353), */
-/* the inner one evaluated over +-2 halo points in b.                          */
+/* Viscous central terms (taylor_green_vortex.py:14-19 with constant          */
+/* viscosity; katzer_SBLI.py:12-26 with mu(T) a constituent relation),        */
+/* expanded by the product rule of Der.differentiate (parsing.py:141-186):    */
+/*   D_j tau_mj = (mu/Re)(D_jj u_m + D_j D_m u_j - 2/3 d_mj D_m D_k u_k)       */
+/*              + (1/Re) D_j mu (D_j u_m + D_m u_j - 2/3 d_mj D_k u_k)          */
+/*   -D_j q_j   = cq (mu D_jj T + D_j mu D_j T),  cq = 1/((g-1) M^2 Pr Re)       */
+/*   D_j(u_m tau_mj) = D_j u_m tau_mj + u_m D_j tau_mj                          */
+/* Every term containing Re is accumulated into the residual                  */
+/* (scheme.py:263-267,291-294).  Mixed derivatives are D_b(D_a u) with a < b   */
+/* (opensblifunctions.py:341-353), the inner one evaluated over +-2 halo       */
+/* points in b (scheme.py:360-387).  With a stretched direction every          */
+/* derivative is transformed (metric.py:72-212): D_j -> D_jj CD_xi_j,          */
+/* D_jj -> D_jj^2 CD2_xi_j + D_jj SD_jjj CD_xi_j, D_a D_b -> D_aa D_bb CD_ab.   */
 /* ------------------------------------------------------------------------- */
-static void viscous_constant(ref_state* s) {
+static void viscous(ref_state* s) {
   const int nd = s->nd;
   const double Re = s->c.Re, g = s->c.gama, M = s->c.Minf, Pr = s->c.Pr;
   const double inv_re = 1.0 / Re;
   const double cq = 1.0 / ((g - 1.0) * M * M * Pr * Re);
+  const int suth = s->c.viscous == REF_VISC_SUTHERLAND;
   double** w = s->wk;
-  /* work layout: du[m][j] at 0+3m+j, d2u[m][j] at 9+3m+j, d2T[j] at 18+j, mixed at 21.. */
+  /* work layout: du[m][j] at 0+3m+j, d2u[m][j] at 9+3m+j, d2T[j] at 18+j, mixed at 21..29,
+     dT[j] at 30+j, dmu[j] at 33+j */
   double* du[3][3];
   double* d2u[3][3];
   double* d2T[3];
+  double* dT[3];
+  double* dmu[3];
   double* mix[3][3][3]; /* mix[m][a][b], a<b: D_b(D_a u_m) */
   for (int m = 0; m < 3; ++m)
     for (int jj = 0; jj < 3; ++jj) { du[m][jj] = w[3 * m + jj]; d2u[m][jj] = w[9 + 3 * m + jj]; }
-  for (int jj = 0; jj < 3; ++jj) d2T[jj] = w[18 + jj];
+  for (int jj = 0; jj < 3; ++jj) { d2T[jj] = w[18 + jj]; dT[jj] = w[30 + jj]; dmu[jj] = w[33 + jj]; }
   int nm = 21;
   for (int m = 0; m < 3; ++m)
     for (int a_ = 0; a_ < 3; ++a_)
       for (int b = 0; b < 3; ++b) mix[m][a_][b] = (a_ < b && a_ < nd && b < nd) ? w[nm++] : NULL;
+  int metric = 0;
+  for (int d = 0; d < nd; ++d) metric |= s->dm[d] != NULL;
 
   range3 I = rng_all(s, 0);
   for (int m = 0; m < nd; ++m)
@@ -258,36 +429,74 @@ static void viscous_constant(ref_state* s) {
       deriv2(s, s->u[m], d2u[m][a_], a_, I);
     }
   for (int a_ = 0; a_ < nd; ++a_) deriv2(s, s->T, d2T[a_], a_, I);
+  if (suth || metric)
+    for (int a_ = 0; a_ < nd; ++a_) deriv1(s, s->T, dT[a_], a_, I);
+  if (suth)
+    for (int a_ = 0; a_ < nd; ++a_) deriv1(s, s->mu, dmu[a_], a_, I);
   for (int m = 0; m < nd; ++m)
     for (int a_ = 0; a_ < nd; ++a_)
       for (int b = a_ + 1; b < nd; ++b) deriv1(s, du[m][a_], mix[m][a_][b], b, I);
 
   LOOP(I) {
     ptrdiff_t x = IX(s, i, j, k);
-#define DD(mm, aa, bb) ((aa) == (bb) ? d2u[mm][aa][x] : ((aa) < (bb) ? mix[mm][aa][bb][x] : mix[mm][bb][aa][x]))
+    const int cc[3] = {i, j, k};
+    double G[3], S2[3];
+    for (int d = 0; d < 3; ++d) {
+      G[d] = d < nd ? met1(s, d, cc[d]) : 1.0;
+      S2[d] = (d < nd && s->sd[d]) ? s->sd[d][cc[d]] : 0.0;
+    }
+    /* transformed first and second derivatives */
+    double DU[3][3], DD[3][3][3], D2T[3], DT[3], DMU[3];
+    for (int m = 0; m < nd; ++m)
+      for (int a_ = 0; a_ < nd; ++a_) {
+        DU[m][a_] = metric ? G[a_] * du[m][a_][x] : du[m][a_][x];
+        for (int b = 0; b < nd; ++b) {
+          double v;
+          if (a_ == b) v = metric ? G[a_] * G[a_] * d2u[m][a_][x] + G[a_] * S2[a_] * du[m][a_][x] : d2u[m][a_][x];
+          else {
+            const double mx = a_ < b ? mix[m][a_][b][x] : mix[m][b][a_][x];
+            v = metric ? G[a_] * G[b] * mx : mx;
+          }
+          DD[m][a_][b] = v;
+        }
+      }
+    for (int a_ = 0; a_ < nd; ++a_) {
+      D2T[a_] = metric ? G[a_] * G[a_] * d2T[a_][x] + G[a_] * S2[a_] * dT[a_][x] : d2T[a_][x];
+      DT[a_] = (suth || metric) ? (metric ? G[a_] * dT[a_][x] : dT[a_][x]) : 0.0;
+      DMU[a_] = suth ? (metric ? G[a_] * dmu[a_][x] : dmu[a_][x]) : 0.0;
+    }
+    const double mu_ = suth ? s->mu[x] : 1.0;
     double divu = 0.0;
-    for (int kk = 0; kk < nd; ++kk) divu += du[kk][kk][x];
+    for (int kk = 0; kk < nd; ++kk) divu += DU[kk][kk];
     double energy = 0.0;
-    for (int jj = 0; jj < nd; ++jj) energy += cq * d2T[jj][x];
+    for (int jj = 0; jj < nd; ++jj) {
+      if (suth) energy += cq * (mu_ * D2T[jj] + DMU[jj] * DT[jj]);
+      else energy += cq * D2T[jj];
+    }
     for (int m = 0; m < nd; ++m) {
       double visc = 0.0;
       for (int jj = 0; jj < nd; ++jj) {
-        /* D_j tau_mj = (1/Re)(D_jj u_m + D_j D_m u_j - 2/3 delta_mj sum_k D_m D_k u_k) */
-        double t = DD(m, jj, jj) + DD(jj, m, jj);
+        double t = DD[m][jj][jj] + DD[jj][m][jj];
         if (jj == m) {
           double sk = 0.0;
-          for (int kk = 0; kk < nd; ++kk) sk += DD(kk, m, kk);
+          for (int kk = 0; kk < nd; ++kk) sk += DD[kk][m][kk];
           t -= (2.0 / 3.0) * sk;
         }
-        double dtau = inv_re * t;
-        double tau = inv_re * (du[m][jj][x] + du[jj][m][x] - (jj == m ? (2.0 / 3.0) * divu : 0.0));
+        const double sr = DU[m][jj] + DU[jj][m] - (jj == m ? (2.0 / 3.0) * divu : 0.0);
+        double dtau, tau;
+        if (suth) {
+          dtau = inv_re * mu_ * t + inv_re * DMU[jj] * sr;
+          tau = inv_re * mu_ * sr;
+        } else {
+          dtau = inv_re * t;
+          tau = inv_re * sr;
+        }
         visc += dtau;
-        energy += du[m][jj][x] * tau + s->u[m][x] * dtau;
+        energy += DU[m][jj] * tau + s->u[m][x] * dtau;
       }
       s->res[1 + m][x] += visc;
     }
     s->res[nd + 1][x] += energy;
-#undef DD
   }
 }
 
@@ -653,7 +862,9 @@ static void shock_capturing(ref_state* s) {
       double r = 0.0;
       for (int d = 0; d < nd; ++d) {
         const double* w = s->wk[5 * d + m];
-        r += (w[x] - w[x - s->st[d]]) / s->c.delta[d];
+        /* WD(F, x_d) -> D_dd WD(F, xi_d) on a stretched direction (opensblifunctions.py:607-616) */
+        if (s->dm[d]) r += met1(s, d, coord(d, i, j, k)) * ((w[x] - w[x - s->st[d]]) / s->c.delta[d]);
+        else r += (w[x] - w[x - s->st[d]]) / s->c.delta[d];
       }
       s->res[m][x] = -r;
     }
@@ -714,11 +925,141 @@ static void extrapolation0(ref_state* s, double** arrs, int narr, int d, int sid
   }
 }
 
+/* Halo depth a boundary kernel fills (bc_core.py:136-147): the largest halo
+   of the schemes, -3/+4 for WENO5/TENO (weno.py:17-29), -2/+2 for Central(4). */
+static int bc_halos(const ref_state* s, int side) {
+  const int central = s->c.conv == REF_CONV_CENTRAL_SKEW || s->c.conv == REF_CONV_CENTRAL_CONS;
+  if (central) return 2;
+  return side == 0 ? 3 : 4;
+}
+
+/* temperature from the conservative variables at x (ns_physics.py:105-106) */
+static double t_cons(const ref_state* s, ptrdiff_t x) {
+  const int nd = s->nd;
+  const double g = s->c.gama, M = s->c.Minf;
+  double m2 = 0.0;
+  for (int m = 0; m < nd; ++m) m2 += s->q[1 + m][x] * s->q[1 + m][x];
+  return (s->q[nd + 1][x] - 0.5 * m2 / s->q[0][x]) * (g * (g - 1.0) * M * M) / s->q[0][x];
+}
+
+/* value of the per-face plane for variable v at plane indices (ia, ib),
+   clamped onto the interior extents (halo corners are never read) */
+static double plane_value(const ref_state* s, int d, int side, int v, int ia, int ib) {
+  const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  const int n1 = s->n[e1], n2 = s->n[e2];
+  ia = ia < 0 ? 0 : (ia >= n1 ? n1 - 1 : ia);
+  ib = ib < 0 ? 0 : (ib >= n2 ? n2 - 1 : ib);
+  return s->plane[d][side][((size_t)v * n2 + ib) * n1 + ia];
+}
+
+/* One point (x) of the boundary plane of (d, side): st points inward
+   (to_side_factor), (ia, ib) are the plane indices of the other directions. */
+static void bc_point(ref_state* s, int d, int side, ptrdiff_t x, ptrdiff_t st, int ia, int ib) {
+  const int nv = s->nvar, nd = s->nd;
+  const int nh = bc_halos(s, side);
+  const double g = s->c.gama, M = s->c.Minf;
+  double** q = s->q;
+  switch (s->c.bc[d][side]) {
+    case REF_BC_EXTRAP1: /* halo i = 2 q[i-1] - q[i-2] going outward (extrapolation.py:42-54) */
+      for (int v = 0; v < nv; ++v)
+        for (int hh = 1; hh <= nh; ++hh) q[v][x - hh * st] = 2.0 * q[v][x - (hh - 1) * st] - q[v][x - (hh - 2) * st];
+      break;
+    case REF_BC_ZERO_GRADIENT: /* q[0] = q[1]; q[-i] = q[i] (zero_gradient_outlet.py:16-23) */
+      for (int v = 0; v < nv; ++v) {
+        q[v][x] = q[v][x + st];
+        for (int hh = 1; hh <= nh; ++hh) q[v][x - hh * st] = q[v][x + hh * st];
+      }
+      break;
+    case REF_BC_ADIABATIC_WALL: { /* adiabatic_wall.py:28-69 */
+      /* wall: zero momentum; T_wall from Carpenter's one-sided dT/dn = 0 */
+      for (int m = 0; m < nd; ++m) q[1 + m][x] = 0.0;
+      const double T1 = t_cons(s, x + st), T2 = t_cons(s, x + 2 * st), T3 = t_cons(s, x + 3 * st);
+      const double Tw = (6.0 / 11.0) * (3.0 * T1 + (1.0 / 3.0) * T3 - (3.0 / 2.0) * T2);
+      const double pw = q[0][x] * Tw / (g * M * M);
+      q[nd + 1][x] = pw / (g - 1.0);
+      for (int hh = 1; hh <= nh; ++hh) {
+        q[0][x - hh * st] = q[0][x + hh * st];
+        for (int m = 0; m < nd; ++m) q[1 + m][x - hh * st] = -q[1 + m][x + hh * st];
+        q[nd + 1][x - hh * st] = q[nd + 1][x + hh * st];
+      }
+      break;
+    }
+    case REF_BC_ISOTHERMAL_WALL: { /* isothermal_wall.py:33-88 with Eq(rhoE, rho*Twall/(gama*(gama-1)*Minf**2)) */
+      const double Tw = s->c.twall;
+      for (int m = 0; m < nd; ++m) q[1 + m][x] = 0.0;
+      q[nd + 1][x] = q[0][x] * Tw / (g * (g - 1.0) * (M * M));
+      double m2 = 0.0;
+      for (int m = 0; m < nd; ++m) m2 += q[1 + m][x] * q[1 + m][x];
+      const double pw = (g - 1.0) * (q[nd + 1][x] - 0.5 * m2 / q[0][x]);
+      const double Ta = t_cons(s, x + st);
+      double uh[5][3];
+      for (int hh = 1; hh <= nh; ++hh)
+        for (int m = 0; m < nd; ++m) uh[hh - 1][m] = q[1 + m][x + hh * st] / q[0][x + hh * st];
+      for (int hh = 1; hh <= nh; ++hh) {
+        const double Th = (hh + 1) * Tw - hh * Ta;
+        const double rh = pw * g * M * M / Th;
+        double ke = 0.0;
+        for (int m = 0; m < nd; ++m) {
+          q[1 + m][x - hh * st] = -rh * uh[hh - 1][m];
+          ke += uh[hh - 1][m] * uh[hh - 1][m];
+        }
+        q[nd + 1][x - hh * st] = pw / (g - 1.0) + 0.5 * rh * ke;
+        q[0][x - hh * st] = rh;
+      }
+      break;
+    }
+    case REF_BC_DIRICHLET: /* boundary point and halos take the plane values (dirichlet.py:28-41) */
+      for (int v = 0; v < nv; ++v) {
+        const double val = plane_value(s, d, side, v, ia, ib);
+        for (int hh = 0; hh <= nh; ++hh) q[v][x - hh * st] = val;
+      }
+      break;
+    case REF_BC_INLET_PEXTRAP: { /* inlet_pressure_extrapolate.py:32-66, side 0 */
+      const double rb = q[0][x];
+      double ke = 0.0, un = 0.0;
+      for (int m = 0; m < nd; ++m) {
+        const double ub = fabs(q[1 + m][x] / rb);
+        ke += ub * ub;
+        if (m == d) un = ub;
+      }
+      const double pb = (g - 1.0) * (q[nd + 1][x] - 0.5 * rb * ke);
+      const double ab = sqrt(g * pb / rb);
+      const int super = un >= ab;
+      for (int v = 0; v < nv; ++v) q[v][x] = super ? q[v][x - st] : q[v][x];
+      for (int hh = 1; hh <= nh; ++hh) q[nd + 1][x - hh * st] = super ? q[nd + 1][x - hh * st] : q[nd + 1][x];
+      break;
+    }
+    default: break;
+  }
+}
+
+/* Boundary kernel of (d, side) over its plane, the other directions' halos
+   included (bc_core.py:163-171). */
+static void bc_face(ref_state* s, int d, int side) {
+  if (s->c.bc[d][side] == REF_BC_EXTRAP0) {
+    extrapolation0(s, s->q, s->nvar, d, side);
+    return;
+  }
+  const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  const int b = side == 0 ? 0 : s->n[d] - 1;
+  const ptrdiff_t st = (side == 0 ? 1 : -1) * s->st[d];
+  for (int ib = -s->h[e2]; ib < s->n[e2] + s->h[e2]; ++ib)
+    for (int ia = -s->h[e1]; ia < s->n[e1] + s->h[e1]; ++ia) {
+      int c[3];
+      c[d] = b;
+      c[e1] = ia;
+      c[e2] = ib;
+      bc_point(s, d, side, IX(s, c[0], c[1], c[2]), st, ia, ib);
+    }
+}
+
+/* Boundary kernels in the order of SimulationBlock.apply_boundary_conditions
+   (block.py:199-210): direction 0 sides 0, 1, then direction 1, ... */
 static void apply_bcs(ref_state* s) {
   for (int d = 0; d < s->nd; ++d)
     for (int side = 0; side < 2; ++side) {
       if (s->c.bc[d][side] == REF_BC_PERIODIC) periodic_exchange(s, s->q, s->nvar, d, side);
-      else extrapolation0(s, s->q, s->nvar, d, side);
+      else bc_face(s, d, side);
     }
 }
 
@@ -730,7 +1071,7 @@ static void evaluate_rhs(ref_state* s) {
     case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
     default: shock_capturing(s); break;
   }
-  if (s->c.viscous == REF_VISC_CONST) viscous_constant(s);
+  if (s->c.viscous != REF_VISC_NONE) viscous(s);
 }
 
 int ref_rk_coefficients(int rk, double* a, double* b) {
@@ -828,10 +1169,18 @@ int ref_create(const ref_config* cfg, ref_state** out) {
   s->p = (double*)calloc(1, bytes);
   s->T = (double*)calloc(1, bytes);
   s->a = (double*)calloc(1, bytes);
+  s->mu = (double*)calloc(1, bytes);
+  carpenter_init();
+  for (int d = 0; d < s->nd; ++d)
+    for (int side = 0; side < 2; ++side) {
+      const int bc = cfg->bc[d][side];
+      /* every BC but periodic and ZeroGradientOutlet is a ModifyCentralDerivative */
+      if (bc != REF_BC_PERIODIC && bc != REF_BC_ZERO_GRADIENT) s->mod[d][side] = 1 + cfg->bscheme[d][side];
+    }
   /* the residual kernels consume every convective work array before the
      viscous kernels run, so the two groups share one pool */
   int nwk = (cfg->conv == REF_CONV_CENTRAL_SKEW || cfg->conv == REF_CONV_CENTRAL_CONS) ? 32 : 15;
-  if (cfg->viscous != REF_VISC_NONE && nwk < 30) nwk = 30;
+  if (cfg->viscous != REF_VISC_NONE && nwk < 36) nwk = 36;
   for (int w = 0; w < nwk; ++w) s->wk[w] = (double*)calloc(1, bytes);
   *out = s;
   return 0;
@@ -841,8 +1190,12 @@ void ref_destroy(ref_state* s) {
   if (!s) return;
   for (int m = 0; m < MAXV; ++m) { free(s->q[m]); free(s->qold[m]); free(s->res[m]); }
   for (int m = 0; m < 3; ++m) free(s->u[m]);
-  free(s->p); free(s->T); free(s->a);
+  free(s->p); free(s->T); free(s->a); free(s->mu);
   for (int w = 0; w < NWK; ++w) free(s->wk[w]);
+  for (int d = 0; d < 3; ++d) {
+    free(s->dm[d]); free(s->sd[d]);
+    free(s->plane[d][0]); free(s->plane[d][1]);
+  }
   free(s);
 }
 
@@ -866,6 +1219,57 @@ int ref_get_field(const ref_state* s, int var, double* v) {
   return 0;
 }
 
+/* Metric of a direction stretched only along itself (metric.py:72-126,241-273):
+   detJ = CD(x_d, xi_d), D_dd = 1/detJ, then SD_ddd = CD(D_dd, xi_d), both with
+   the faces' stencil closures; both faces must modify the central stencils
+   (the one-sided rows never read the halos). */
+int ref_set_coordinates(ref_state* s, int d, const double* xc) {
+  if (d < 0 || d >= s->nd || !s->c.stretch[d]) return -1;
+  if (!s->mod[d][0] || !s->mod[d][1]) return -2;
+  const int N = s->n[d];
+  const double inv = 1.0 / s->c.delta[d];
+  free(s->dm[d]);
+  free(s->sd[d]);
+  s->dm[d] = (double*)calloc((size_t)N, sizeof(double));
+  s->sd[d] = (double*)calloc((size_t)N, sizeof(double));
+  for (int c = 0; c < N; ++c) s->dm[d][c] = 1.0 / (inv * d1_gen(xc, c, 1, N, s->mod[d][0], s->mod[d][1], c));
+  for (int c = 0; c < N; ++c) s->sd[d][c] = inv * d1_gen(s->dm[d], c, 1, N, s->mod[d][0], s->mod[d][1], c);
+  return 0;
+}
+
+int ref_get_metric(const ref_state* s, int d, double* dm, double* sd) {
+  if (d < 0 || d >= s->nd || !s->dm[d]) return -1;
+  for (int c = 0; c < s->n[d]; ++c) { dm[c] = s->dm[d][c]; sd[c] = s->sd[d][c]; }
+  return 0;
+}
+
+/* The plane of a Dirichlet face, or the inflow state of an inlet face, whose
+   halo points all take it now (the initial condition covers the halos,
+   katzer_init.py:233-245, and the inlet kernel never writes rho, rhou there). */
+int ref_set_bc_plane(ref_state* s, int d, int side, const double* plane) {
+  if (d < 0 || d >= s->nd || side < 0 || side > 1) return -1;
+  const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+  const size_t np = (size_t)s->nvar * s->n[e1] * s->n[e2];
+  free(s->plane[d][side]);
+  s->plane[d][side] = (double*)malloc(np * sizeof(double));
+  memcpy(s->plane[d][side], plane, np * sizeof(double));
+  if (s->c.bc[d][side] == REF_BC_INLET_PEXTRAP) {
+    const int b = side == 0 ? 0 : s->n[d] - 1;
+    const ptrdiff_t st = (side == 0 ? 1 : -1) * s->st[d];
+    for (int ib = -s->h[e2]; ib < s->n[e2] + s->h[e2]; ++ib)
+      for (int ia = -s->h[e1]; ia < s->n[e1] + s->h[e1]; ++ia) {
+        int c[3];
+        c[d] = b;
+        c[e1] = ia;
+        c[e2] = ib;
+        const ptrdiff_t x = IX(s, c[0], c[1], c[2]);
+        for (int v = 0; v < s->nvar; ++v)
+          for (int hh = 1; hh <= H; ++hh) s->q[v][x - hh * st] = plane_value(s, d, side, v, ia, ib);
+      }
+  }
+  return 0;
+}
+
 int ref_rhs(ref_state* s, double* residual) {
   apply_bcs(s);
   constituent_relations(s);
@@ -874,7 +1278,7 @@ int ref_rhs(ref_state* s, double* residual) {
     case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
     default: shock_capturing(s); break;
   }
-  if (s->c.viscous == REF_VISC_CONST) viscous_constant(s);
+  if (s->c.viscous != REF_VISC_NONE) viscous(s);
   ptrdiff_t c = 0;
   for (int m = 0; m < s->nvar; ++m)
     for (int k = 0; k < s->n[2]; ++k)

@@ -25,9 +25,22 @@ enum {
   REF_CONV_TENO6 = 5          /* LLFTeno(6), teno.py:216-234 */
 };
 enum { REF_AVG_SIMPLE = 0, REF_AVG_ROE = 1 };           /* averaging.py:31,62 */
-enum { REF_VISC_NONE = 0, REF_VISC_CONST = 1 };         /* taylor_green_vortex.py:18-19 */
+enum { REF_VISC_NONE = 0, REF_VISC_CONST = 1,          /* taylor_green_vortex.py:18-19 */
+       REF_VISC_SUTHERLAND = 2 };                       /* mu(T) CR, katzer_SBLI.py:26 */
 enum { REF_RK3_SBLI = 0, REF_RK3_LS = 1, REF_RK3_LS_SSP = 2, REF_RK4_LS = 3 };  /* rk_sbli.py, rk_LS.py */
-enum { REF_BC_PERIODIC = 0, REF_BC_EXTRAP0 = 1 };       /* periodic.py, extrapolation.py */
+enum {
+  REF_BC_PERIODIC = 0,        /* periodic.py */
+  REF_BC_EXTRAP0 = 1,         /* ExtrapolationBC(order=0), extrapolation.py:37-41 */
+  REF_BC_EXTRAP1 = 2,         /* ExtrapolationBC(order=1), extrapolation.py:42-54 */
+  REF_BC_ZERO_GRADIENT = 3,   /* ZeroGradientOutletBC, zero_gradient_outlet.py:11-23 */
+  REF_BC_ADIABATIC_WALL = 4,  /* AdiabaticWallBC, adiabatic_wall.py:28-69 */
+  REF_BC_ISOTHERMAL_WALL = 5, /* IsothermalWallBC, isothermal_wall.py:33-88 */
+  REF_BC_DIRICHLET = 6,       /* DirichletBC, dirichlet.py:28-41 */
+  REF_BC_INLET_PEXTRAP = 7    /* InletPressureExtrapolateBC (side 0), inlet_pressure_extrapolate.py:32-66 */
+};
+/* near-boundary replacement of the central stencils (bc_core.py:15-16,
+   opensblifunctions.py:523-534) at faces whose BC modifies them */
+enum { REF_BS_CARPENTER = 0, REF_BS_REDUCED_ACCESS = 1 };
 
 typedef struct ref_config {
   int32_t ndim;
@@ -36,6 +49,11 @@ typedef struct ref_config {
   int32_t conv, averaging, viscous, rk;
   int32_t bc[3][2];
   double gama, Minf, Re, Pr, dt, eps, teno_ct;
+  int32_t bscheme[3][2]; /* REF_BS_* of each face (used when its BC modifies the stencils) */
+  int32_t stretch[3];    /* 1: the coordinate of direction d is given (metric.py:37-54) */
+  int32_t pad_;
+  double suthT, refT;    /* Sutherland constants SuthT, RefT */
+  double twall;          /* Twall of the isothermal wall */
 } ref_config;
 
 typedef struct ref_state ref_state;
@@ -45,6 +63,13 @@ void ref_destroy(ref_state* s);
 /* interior values, x fastest, n0*n1*n2 doubles */
 int ref_set_field(ref_state* s, int var, const double* interior);
 int ref_get_field(const ref_state* s, int var, double* interior);
+/* coordinates x_d (n[d] values) of a stretched direction; computes the
+   metric D_dd = 1/CD(x_d, xi_d) and SD_ddd = CD(D_dd, xi_d) */
+int ref_set_coordinates(ref_state* s, int d, const double* x);
+/* per-face plane of conservative values, [nvar][n_e2][n_e1] over the interior
+   extents of the other two directions (e1 < e2): the Dirichlet values, or the
+   inflow state that an inlet face keeps in its halos */
+int ref_set_bc_plane(ref_state* s, int d, int side, const double* plane);
 /* apply the boundary conditions, then evaluate one RHS into residual[nvar][cells] */
 int ref_rhs(ref_state* s, double* residual);
 /* niter full iterations of the generated time loop (algorithm.py:466-470) */
@@ -63,6 +88,11 @@ void ref_eigensystem(int ndim, int d, double gama, double rho, const double* u, 
 void ref_roe_average(int ndim, double gama, const double* qL, const double* qR, double* out);
 /* central weights on points -2..2 of the first and second derivative */
 void ref_central_weights(double* w1, double* w2);
+/* boundary closure weights of the first (order 1: rows x 6) or second
+   (order 2: 2 x 6) derivative at side 0 for scheme REF_BS_*; returns rows */
+int ref_closure_weights(int scheme, int order, double* w);
+/* metric arrays of direction d after ref_set_coordinates (n[d] values each) */
+int ref_get_metric(const ref_state* s, int d, double* dm, double* sd);
 /* RK coefficients of a temporal scheme: returns the number of stages */
 int ref_rk_coefficients(int rk, double* a, double* b);
 

@@ -13,9 +13,11 @@ _LIBS = {"plain": os.path.join(_HERE, "libosbl_oracle.so"), "fma": os.path.join(
 
 CONV = {"central_skew": 0, "central_cons": 1, "weno_js": 2, "weno_z": 3, "teno5": 4, "teno6": 5}
 AVG = {"simple": 0, "roe": 1}
-VISC = {"none": 0, "const": 1}
+VISC = {"none": 0, "const": 1, "sutherland": 2}
 RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
-BC = {"periodic": 0, "extrap0": 1}
+BC = {"periodic": 0, "extrap0": 1, "extrap1": 2, "zero_gradient": 3, "adiabatic_wall": 4,
+      "isothermal_wall": 5, "dirichlet": 6, "inlet_pextrap": 7}
+BSCHEME = {"carpenter": 0, "reduced_access": 1}
 
 
 class RefConfig(ctypes.Structure):
@@ -35,6 +37,12 @@ class RefConfig(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("eps", ctypes.c_double),
         ("teno_ct", ctypes.c_double),
+        ("bscheme", (ctypes.c_int32 * 2) * 3),
+        ("stretch", ctypes.c_int32 * 3),
+        ("pad_", ctypes.c_int32),
+        ("suthT", ctypes.c_double),
+        ("refT", ctypes.c_double),
+        ("twall", ctypes.c_double),
     ]
 
 
@@ -69,6 +77,10 @@ def lib(variant="plain"):
         L.ref_roe_average.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, dp]
         L.ref_central_weights.argtypes = [dp, dp]
         L.ref_rk_coefficients.argtypes = [ctypes.c_int, dp, dp]
+        L.ref_set_coordinates.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.ref_set_bc_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp]
+        L.ref_closure_weights.argtypes = [ctypes.c_int, ctypes.c_int, dp]
+        L.ref_get_metric.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -98,7 +110,22 @@ def make_config(p):
     c.dt = p["dt"]
     c.eps = p.get("eps", 1e-16)
     c.teno_ct = p.get("teno_ct", 1e-6)
+    bs = p.get("bscheme", [["carpenter"] * 2] * 3)
+    for d in range(3):
+        for s in range(2):
+            c.bscheme[d][s] = BSCHEME[bs[d][s]] if d < p["ndim"] else 0
+        c.stretch[d] = int(p.get("stretch", [0, 0, 0])[d]) if d < p["ndim"] else 0
+    c.suthT = p.get("SuthT", 110.4)
+    c.refT = p.get("RefT", 288.0)
+    c.twall = p.get("Twall", 1.0)
     return c
+
+
+def closure_weights(scheme, order):
+    """Side-0 closure rows (rows x 6) of the first (order 1) or second (order 2) derivative."""
+    w = np.zeros(24)
+    rows = lib().ref_closure_weights(BSCHEME[scheme], order, _ptr(w))
+    return w[: 6 * rows].reshape(rows, 6)
 
 
 class RefSolver:
@@ -115,6 +142,21 @@ class RefSolver:
         if self.L.ref_create(ctypes.byref(self.cfg), ctypes.byref(h)) != 0:
             raise RuntimeError("ref_create failed")
         self.h = h
+        for d, x in problem.get("coords", {}).items():
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            if self.L.ref_set_coordinates(self.h, int(d), _ptr(a)) != 0:
+                raise RuntimeError(f"ref_set_coordinates({d}) failed")
+        for (d, side), pl in problem.get("planes", {}).items():
+            a = np.ascontiguousarray(pl, dtype=np.float64)
+            if self.L.ref_set_bc_plane(self.h, int(d), int(side), _ptr(a)) != 0:
+                raise RuntimeError(f"ref_set_bc_plane({d}, {side}) failed")
+
+    def metric(self, d):
+        n = self.p["n"][d]
+        dm, sd = np.empty(n), np.empty(n)
+        if self.L.ref_get_metric(self.h, d, _ptr(dm), _ptr(sd)) != 0:
+            raise RuntimeError("no metric for direction %d" % d)
+        return dm, sd
 
     def __del__(self):
         if getattr(self, "h", None):

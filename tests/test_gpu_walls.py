@@ -1,0 +1,148 @@
+"""Parity of the HIP path with the oracle on blocks with non-periodic faces:
+stencil closures (Carpenter, ReducedAccess), the metric of a stretched
+direction, Sutherland viscosity and the wall / inlet / outlet / Dirichlet
+boundary conditions, through the C-ABI.
+
+Bar as in test_gpu_parity.py: relative L2 per conservative variable below
+max(1e-10, 4 x the spread between the two oracle builds), after 100 steps.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _smooth_state(n, gama=1.4, Minf=0.5, u0=0.3, seed=0):
+    """A smooth 2-D state (nvar, ny, nx) on the unit square."""
+    rs = np.random.RandomState(seed)
+    y, x = np.meshgrid(np.linspace(0, 1, n[1]), np.linspace(0, 1, n[0]), indexing="ij")
+    a = rs.uniform(0.5, 1.5, 6)
+    rho = 1.0 + 0.05 * np.sin(2 * np.pi * a[0] * x) * np.cos(np.pi * a[1] * y)
+    u = u0 + 0.05 * np.cos(2 * np.pi * a[2] * x) * np.sin(np.pi * a[3] * y)
+    v = 0.03 * np.sin(2 * np.pi * a[4] * x) * np.sin(np.pi * y)
+    p = 1.0 / (gama * Minf ** 2) * (1.0 + 0.05 * np.cos(2 * np.pi * a[5] * x) * np.cos(np.pi * y))
+    E = p / (gama - 1) + 0.5 * rho * (u * u + v * v)
+    return np.stack([rho, rho * u, rho * v, E])
+
+
+def _problem_2d(n, bc, conv, viscous, bscheme, stretch=None, planes=None, rk="rk3", dt=2e-4, **kw):
+    from opensbli_amd.problems import stretched_coordinate
+    p = dict(ndim=2, n=list(n), delta=[1.0 / (n[0] - 1), 1.0 / (n[1] - 1)], conv=conv, averaging="roe",
+             viscous=viscous, rk=rk, bc=bc, bscheme=bscheme, stretch=[0, 1 if stretch else 0, 0], gama=1.4, Minf=0.5,
+             Re=200.0, Pr=0.72, dt=dt, eps=1e-16, SuthT=110.4, RefT=288.0, Twall=1.0)
+    if stretch:
+        p["coords"] = {1: stretched_coordinate(n[1], 1.0, stretch, "sinh")}
+    if planes:
+        p["planes"] = planes
+    p.update(kw)
+    return p
+
+
+def _compare(problem, q0, oracle, steps):
+    from opensbli_amd.native import Block
+    g = Block(problem)
+    g.set_state(q0)
+    r = oracle.RefSolver(problem)
+    r.set_state(q0)
+    f = oracle.RefSolver(problem, variant="fma")
+    f.set_state(q0)
+    if steps == 0:
+        a, b, c = g.rhs(), r.rhs(), f.rhs()
+    else:
+        g.run(steps)
+        r.run(steps)
+        f.run(steps)
+        a, b, c = g.get_state(), r.get_state(), f.get_state()
+    err = rel_l2(a, b)
+    spread = rel_l2(c, b)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.isfinite(a).all()
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), (err, spread)
+    return a
+
+
+@pytest.mark.parametrize("bscheme", ["carpenter", "reduced_access"])
+@pytest.mark.parametrize("steps", [0, 100])
+def test_dirichlet_box_sutherland_metric(bscheme, steps, oracle):
+    """Central conservative convection, Sutherland viscosity, stretched y,
+    Dirichlet on all faces: every closure row and metric term is exercised."""
+    n = (40, 36)
+    _say(f"case dirichlet box bscheme={bscheme} steps={steps}")
+    q0 = _smooth_state(n)
+    planes = {(0, 0): q0[:, :, 0][:, None, :], (0, 1): q0[:, :, -1][:, None, :],
+              (1, 0): q0[:, 0, :][:, None, :], (1, 1): q0[:, -1, :][:, None, :]}
+    p = _problem_2d(n, [["dirichlet"] * 2, ["dirichlet"] * 2], "central_cons", "sutherland",
+                    [[bscheme] * 2, [bscheme] * 2, ["carpenter"] * 2], stretch=1.5, planes=planes)
+    _compare(p, q0, oracle, steps)
+
+
+BC_SETS = {
+    "inlet_wall": ([["inlet_pextrap", "zero_gradient"], ["isothermal_wall", "extrap1"]], "weno_z", "sutherland", None),
+    "channel": ([["periodic", "periodic"], ["isothermal_wall", "isothermal_wall"]], "central_skew", "const", None),
+    "adiabatic_dirichlet": ([["extrap0", "extrap0"], ["adiabatic_wall", "dirichlet"]], "weno_z", "const", 1.2),
+    "teno_walls": ([["extrap1", "extrap0"], ["adiabatic_wall", "adiabatic_wall"]], "teno6", "sutherland", None),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BC_SETS))
+@pytest.mark.parametrize("bscheme", ["carpenter", "reduced_access"])
+def test_bc_sets_100_steps(name, bscheme, oracle):
+    bc, conv, visc, stretch = BC_SETS[name]
+    n = (48, 40)
+    _say(f"case bc set {name} bscheme={bscheme} conv={conv} visc={visc} steps=100")
+    q0 = _smooth_state(n)
+    # walls at rest
+    for side, row in ((0, 0), (1, -1)):
+        if bc[1][side].endswith("wall"):
+            q0[3, row, :] -= 0.5 * (q0[1, row, :] ** 2 + q0[2, row, :] ** 2) / q0[0, row, :]
+            q0[1:3, row, :] = 0.0
+    planes = {}
+    if bc[0][0] == "inlet_pextrap":
+        planes[(0, 0)] = q0[:, :, 0][:, None, :]
+    if bc[1][1] == "dirichlet":
+        planes[(1, 1)] = q0[:, -1, :][:, None, :]
+    p = _problem_2d(n, bc, conv, visc, [[bscheme] * 2, [bscheme] * 2, ["carpenter"] * 2], stretch=stretch,
+                    planes=planes, Twall=1.0)
+    _compare(p, q0, oracle, 100)
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_flat_plate_rhs_and_100_steps(nd, oracle):
+    from opensbli_amd.problems import flat_plate
+    n = (64, 48) if nd == 2 else (40, 32, 8)
+    _say(f"case flat plate nd={nd} n={n}")
+    p, q0 = flat_plate(n=n, ndim=nd)
+    _compare(p, q0, oracle, 0)
+    _compare(p, q0, oracle, 100)
+
+
+def test_flat_plate_full_size_50_steps():
+    """The 480x256x128 flat plate: 50 steps stay finite, the wall stays at rest."""
+    import time
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import flat_plate
+    _say("case flat plate 480x256x128 50 steps")
+    p, q0 = flat_plate()
+    g = Block(p)
+    g.set_state(q0)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(50)
+    g.sync()
+    _say("50 steps", time.perf_counter() - t0, "s")
+    q = g.get_state()
+    assert np.isfinite(q).all()
+    assert np.all(q[1:4, :, 0, :] == 0.0)
+    # the laminar layer changes little in 50 steps away from the inlet
+    assert np.abs(q[0] - q0[0]).max() < 0.05
