@@ -213,11 +213,129 @@ __global__ void __launch_bounds__(128) k_adj_stage(const double* __restrict__ Qs
   }
 }
 
+// 2-D variant of k_adj_stage with the stencil inputs staged through LDS: a
+// 64 x 4 block of cells loads the (64+4) x (4+4) tile of q, u and the adjoint
+// weights mu = alpha A + beta B once (periodic wrap), and every cell's four
+// neighbours per direction are then read from LDS.  Same arithmetic as
+// k_adj_stage.
+constexpr int ATX = 64, ATY = 4, APX = ATX + 4, APY = ATY + 4, ANP = APX * APY;
+
+__global__ void __launch_bounds__(256) k_adj_stage2d(const double* __restrict__ Qs, const double* __restrict__ A,
+                                                     const double* __restrict__ B, double alpha, double beta,
+                                                     double c, double* __restrict__ out1, double* __restrict__ out2,
+                                                     int n0, int n1, Phys ph) {
+  constexpr int NV = 4, E = 3;
+  __shared__ double s_mu[NV][ANP];
+  __shared__ double s_q[NV][ANP];
+  __shared__ double s_u[2][ANP];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * ATX + tx;
+  const int x0 = blockIdx.x * ATX, y0 = blockIdx.y * ATY;
+  const long long cells = (long long)n0 * n1;
+  for (int p = tid; p < ANP; p += ATX * ATY) {
+    const int xx = p % APX, yy = p / APX;
+    int gx = (x0 - 2 + xx) % n0;
+    int gy = (y0 - 2 + yy) % n1;
+    gx = gx < 0 ? gx + n0 : gx;
+    gy = gy < 0 ? gy + n1 : gy;
+    const long long y = gx + (long long)n0 * gy;
+    double q[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      q[m] = Qs[m * cells + y];
+      s_q[m][p] = q[m];
+      double v = alpha * A[m * cells + y];
+      if (B) v += beta * B[m * cells + y];
+      s_mu[m][p] = v;
+    }
+    const double ir = 1.0 / q[0];
+    s_u[0][p] = q[1] * ir;
+    s_u[1][p] = q[2] * ir;
+  }
+  __syncthreads();
+  const int i = x0 + tx, j0 = y0 + ty;
+  if (i >= n0 || j0 >= n1) return;
+  const int cc = (ty + 2) * APX + tx + 2;
+  const long long x = i + (long long)n0 * j0;
+  double q[NV], u[2], mu[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    q[m] = s_q[m][cc];
+    mu[m] = s_mu[m][cc];
+  }
+  u[0] = s_u[0][cc];
+  u[1] = s_u[1][cc];
+  const double p = ph.gm1 * (q[E] - 0.5 * q[0] * (u[0] * u[0] + u[1] * u[1]));
+  double gq[NV], gu[2], gp = 0.0;
+#pragma unroll
+  for (int m = 0; m < NV; ++m) gq[m] = 0.0;
+#pragma unroll
+  for (int jd = 0; jd < 2; ++jd) {
+    const double inv = ph.inv_delta[jd];
+    constexpr double W1A = 1.0 / 12.0, W1B = 2.0 / 3.0;
+    const int st = jd == 0 ? 1 : APX;
+    const int offs[4] = {-2 * st, -st, st, 2 * st};
+    const double wts[4] = {W1A, -W1B, W1B, -W1A};
+    double dmu[NV], dmuu[NV], dq[NV], dmuq = 0.0, duj = 0.0;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) { dmu[m] = 0.0; dmuu[m] = 0.0; dq[m] = 0.0; }
+#pragma unroll
+    for (int sI = 0; sI < 4; ++sI) {
+      const int y = cc + offs[sI];
+      const double w = wts[sI];
+      const double uj = s_u[jd][y];
+      double sq = 0.0;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double my = s_mu[m][y];
+        const double qy = s_q[m][y];
+        dmu[m] += w * my;
+        dmuu[m] += w * (my * uj);
+        dq[m] += w * qy;
+        sq += my * qy;
+      }
+      dmuq += w * sq;
+      duj += w * uj;
+    }
+#pragma unroll
+    for (int m = 0; m < NV; ++m) { dmu[m] *= inv; dmuu[m] *= inv; dq[m] *= inv; }
+    dmuq *= inv;
+    duj *= inv;
+    double gu_j = 0.5 * dmuq + p * dmu[E];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      gq[m] += 0.5 * u[jd] * dmu[m] + 0.5 * dmuu[m] - 0.5 * mu[m] * duj;
+      gu_j += 0.5 * q[m] * dmu[m] - 0.5 * mu[m] * dq[m];
+    }
+    gu[jd] = gu_j;
+    gp += dmu[1 + jd] + u[jd] * dmu[E];
+  }
+  const double ir = 1.0 / q[0];
+  const double uu = u[0] * u[0] + u[1] * u[1];
+  double v[NV];
+  v[0] = gq[0] + gp * ph.gm1 * 0.5 * uu;
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    v[0] -= gu[d] * u[d] * ir;
+    v[1 + d] = gq[1 + d] + gu[d] * ir - gp * ph.gm1 * u[d];
+  }
+  v[E] = gq[E] + gp * ph.gm1;
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    const double ab = A[m * cells + x] + (B ? B[m * cells + x] : 0.0);
+    out1[m * cells + x] = c * ab + v[m];
+    if (out2) out2[m * cells + x] = ab;
+  }
+}
+
 void launch_adj_stage(int nd, const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
                       double* out1, double* out2, const Grid& g, const Phys& ph, hipStream_t s) {
+  if (nd == 2) {
+    dim3 grid((g.n[0] + ATX - 1) / ATX, (g.n[1] + ATY - 1) / ATY);
+    k_adj_stage2d<<<grid, dim3(ATX, ATY), 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0], g.n[1], ph);
+    return;
+  }
   dim3 grid((g.n[0] + 127) / 128, g.n[1], g.n[2]);
   if (nd == 3) k_adj_stage<3><<<grid, 128, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0], g.n[1], g.n[2], ph);
-  else if (nd == 2) k_adj_stage<2><<<grid, 128, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0], g.n[1], g.n[2], ph);
   else k_adj_stage<1><<<grid, 128, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0], g.n[1], g.n[2], ph);
 }
 
