@@ -11,15 +11,23 @@ with one change: the code-generation target ``OPSC(alg)`` becomes
     RoeAverage, SimpleAverage          averaging.py:31,62
     RungeKutta, RungeKuttaLS           rk_sbli.py:16, rk_LS.py:15
     PeriodicBC, ExtrapolationBC        periodic.py:7, extrapolation.py:8
+    ZeroGradientOutletBC, AdiabaticWallBC, IsothermalWallBC, DirichletBC,
+    InletPressureExtrapolateBC         zero_gradient_outlet.py:5, adiabatic_wall.py:10,
+                                       isothermal_wall.py:11, dirichlet.py:6,
+                                       inlet_pressure_extrapolate.py:11
+    Carpenter, ReducedAccess           Carpenter_scheme.py:6, reduced_access_scheme.py:6
+    MetricsEquation                    metric.py:20 (directions stretched along themselves)
     GridBasedInitialisation            gridbasedinit.py:15
+    Initialise_Katzer                  katzer_init.py:183
     TraditionalAlgorithmRK             algorithm.py:336
     substitute_simulation_parameters   helperfunctions.py:130
 
 Instead of expanding the Einstein notation symbolically, HIPC recognises the
 equation systems the GPU kernels implement (the compressible Euler/Navier-
 Stokes equations in skew-symmetric or conservative form, the constituent
-relations and the constant-viscosity stress tensor and heat flux of the
-reference's apps) and refuses anything else with the reference's error types.
+relations, the constant or Sutherland-viscosity stress tensor and heat flux of
+the reference's apps) and refuses anything else with the reference's error
+types.
 The initial condition equations are evaluated on the host with SymPy.
 """
 import math
@@ -128,6 +136,12 @@ class SimulationEquations(_EquationSet):
     def __init__(self, order=None, **kwargs):
         super().__init__()
 
+    def apply_metrics(self, metriceq):
+        """Transform the equations with the block's metric (opensbliequations.py:93-110)."""
+        if metriceq.stretching_metric is None:
+            raise ValueError("generate_transformations must be called before apply_metrics")
+        self.metric = metriceq
+
     @property
     def time_advance_arrays(self):
         return ["rho", "rhou", "rhoE"]
@@ -140,6 +154,27 @@ class ConstituentRelations(_EquationSet):
 class GridBasedInitialisation(_EquationSet):
     """Initial-condition equations evaluated over the grid before the time
     loop (gridbasedinit.py:15-67); here evaluated on the host with SymPy."""
+
+
+class Initialise_Katzer:
+    """Laminar compressible boundary-layer initial state (katzer_init.py:183):
+    the similarity solution at (Re, xMach, Tinf) with an adiabatic wall, on the
+    wall-normal coordinate given by `coordinate_evaluations`.  One wall
+    direction (bl_directions[d][0] True) is implemented."""
+
+    def __init__(self, bl_directions, n_coeffs, Re, xMach, Tinf, coordinate_evaluations=None):
+        self.bl_directions = list(bl_directions)
+        flags = [bool(b[0]) for b in self.bl_directions]
+        if sum(flags) < 1:
+            raise ValueError("Provide the directions to apply a boundary layer profile in.")
+        if sum(flags) > 1:
+            raise NotImplementedError("boundary layers on two walls are not implemented by the HIPC target")
+        if n_coeffs < 10:
+            raise ValueError("Higher number of polynomial coefficients are required for a good polynomial fit.")
+        self.wall_direction = flags.index(True)
+        self.Re, self.xMach, self.Tinf = float(Re), float(xMach), float(Tinf)
+        self.coordinate_evaluations = list(coordinate_evaluations or [])
+        self.equations = self.coordinate_evaluations
 
 
 # ---------------------------------------------------------------------------
@@ -253,13 +288,112 @@ class PeriodicBC:
         self.direction, self.side, self.kind = direction, side, "periodic"
 
 
-class ExtrapolationBC:
-    """Extrapolation boundary (extrapolation.py:8); zeroth order."""
+class Carpenter:
+    """Carpenter's 4th-order one-sided closure of the central stencils
+    (Carpenter_scheme.py:6), the default of every stencil-modifying BC."""
+    name = "carpenter"
+
+
+class ReducedAccess:
+    """The (-4, 4)-access closure (reduced_access_scheme.py:6)."""
+    name = "reduced_access"
+
+
+def _closure(scheme):
+    if scheme is None:
+        return "carpenter"
+    if isinstance(scheme, (Carpenter, ReducedAccess)) or scheme in (Carpenter, ReducedAccess):
+        return scheme.name
+    raise NotImplementedError("boundary scheme %r is not implemented by the HIPC target" % (scheme,))
+
+
+class _BC:
+    kind = None
+    equations = None
+
+    def __init__(self, direction, side, scheme=None, plane=True):
+        if not plane:
+            raise NotImplementedError("split boundary conditions are not implemented by the HIPC target")
+        self.direction, self.side = direction, side
+        self.scheme = _closure(scheme)
+
+
+class ExtrapolationBC(_BC):
+    """Extrapolation boundary (extrapolation.py:8): order 0 or 1."""
 
     def __init__(self, direction, side, order, scheme=None, plane=True):
-        if order != 0:
-            raise NotImplementedError("the HIPC target implements zeroth-order extrapolation only")
-        self.direction, self.side, self.kind = direction, side, "extrap0"
+        super().__init__(direction, side, scheme, plane)
+        if order not in (0, 1):
+            raise ValueError("Only zeroth order and linear extrapolation currently implemented.")
+        self.order = order
+        self.kind = "extrap0" if order == 0 else "extrap1"
+
+
+class ZeroGradientOutletBC(_BC):
+    """q[boundary] = q[boundary - 1], halos mirrored (zero_gradient_outlet.py:5);
+    leaves the central stencils unmodified."""
+    kind = "zero_gradient"
+
+
+class AdiabaticWallBC(_BC):
+    """No-slip wall with dT/dn = 0 (adiabatic_wall.py:10)."""
+    kind = "adiabatic_wall"
+
+
+class IsothermalWallBC(_BC):
+    """No-slip wall at T = Twall (isothermal_wall.py:11).  `equations` must set
+    rhoE = rho*Twall/(gama*(gama-1)*Minf**2), the form of the reference's apps."""
+    kind = "isothermal_wall"
+
+    def __init__(self, direction, side, equations, scheme=None, plane=True):
+        super().__init__(direction, side, scheme, plane)
+        self.equations = list(flatten([equations]))
+
+
+class DirichletBC(_BC):
+    """Fixed values on the boundary plane and its halos (dirichlet.py:6).  The
+    equations may depend on the coordinates, grid indexes and constants."""
+    kind = "dirichlet"
+
+    def __init__(self, direction, side, equations, scheme=None, plane=True):
+        super().__init__(direction, side, scheme, plane)
+        self.equations = list(flatten([equations]))
+
+
+class InletPressureExtrapolateBC(_BC):
+    """Inlet with pressure extrapolation when subsonic (inlet_pressure_extrapolate.py:11)."""
+    kind = "inlet_pextrap"
+
+    def __init__(self, direction, side, scheme=None, plane=True):
+        super().__init__(direction, side, scheme, plane)
+        if side != 0:
+            raise ValueError("Only implemented this BC for inlet side 0.")
+
+
+# ---------------------------------------------------------------------------
+# metric
+# ---------------------------------------------------------------------------
+class MetricsEquation:
+    """Coordinate transformation of the derivatives (metric.py:20).  The HIPC
+    target implements directions stretched along themselves, parameters
+    (stretched, False): the metric is D_dd = 1/CD(x_d, xi_d) with
+    SD_ddd = CD(D_dd, xi_d), computed from the block's coordinates."""
+
+    def __init__(self, **kwargs):
+        self.stretching_metric = None
+        self.curvilinear_metric = None
+        self.ndim = None
+
+    def generate_transformations(self, ndim, coordinate_symbol, parameters, max_order):
+        if len(flatten(list(parameters))) != ndim * 2:
+            raise ValueError("The parameters for stretching provided should match the number of dimensions")
+        self.ndim = ndim
+        self.stretching_metric = [bool(p[0]) for p in parameters]
+        self.curvilinear_metric = [bool(p[1]) for p in parameters]
+        if any(self.curvilinear_metric):
+            raise NotImplementedError("curvilinear (non-diagonal) metrics are not implemented by the HIPC target")
+        self.coordinate_symbol = coordinate_symbol
+        return
 
 
 # ---------------------------------------------------------------------------
@@ -427,6 +561,11 @@ VISCOUS = {"rhou_i": {(1, "Der(tau_i_j,x_j)")}, "rhoE": {(1, "Der(u_i*tau_i_j,x_
 STRESS = "Eq(tau_i_j,(1.0/Re)*(Der(u_i,x_j)+Der(u_j,x_i)-(2/3)*KD(_i,_j)*Der(u_k,x_k)))"
 HEAT_POS = "Eq(q_j,(1.0/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
 HEAT_NEG = "Eq(q_j,(-1.0/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
+# Sutherland viscosity as a constituent relation (katzer_SBLI.py:12-26)
+STRESS_MU = "Eq(tau_i_j,(mu/Re)*(Der(u_i,x_j)+Der(u_j,x_i)-(2/3)*KD(_i,_j)*Der(u_k,x_k)))"
+HEAT_MU_POS = "Eq(q_j,(mu/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
+HEAT_MU_NEG = "Eq(q_j,(-mu/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
+SUTHERLAND = "(T**(1.5)*(1.0+SuthT/RefT)/(T+SuthT/RefT))"
 CONSTITUENT = {
     "u_i": "rhou_i/rho",
     "p": "(gama-1)*(rhoE-rho*(1/2)*(KD(_i,_j)*u_i*u_j))",
@@ -506,12 +645,16 @@ def _recognise(block):
         for var, want in VISCOUS.items():
             if not want <= terms[var]:
                 raise NotImplementedError("partial viscous terms are not supported")
-        if _norm(STRESS) not in subs:
-            raise NotImplementedError("only the constant-viscosity stress tensor (1.0/Re)*(...) is supported")
-        heat_ok = (heat == 1 and _norm(HEAT_POS) in subs) or (heat == -1 and _norm(HEAT_NEG) in subs)
+        if _norm(STRESS) in subs:
+            heat_ok = (heat == 1 and _norm(HEAT_POS) in subs) or (heat == -1 and _norm(HEAT_NEG) in subs)
+            viscous = "const"
+        elif _norm(STRESS_MU) in subs:
+            heat_ok = (heat == 1 and _norm(HEAT_MU_POS) in subs) or (heat == -1 and _norm(HEAT_MU_NEG) in subs)
+            viscous = "sutherland"
+        else:
+            raise NotImplementedError("only the stress tensors (1.0/Re)*(...) and (mu/Re)*(...) are supported")
         if not heat_ok:
             raise NotImplementedError("heat flux form not supported")
-        viscous = "const"
     # constituent relations
     have = {}
     for c in crs:
@@ -519,13 +662,23 @@ def _recognise(block):
             lhs, rhs = _parse_eq(e.text)
             have[lhs] = rhs
     need = ["u_i", "p"] + (["a"] if conv not in ("central_skew", "central_cons") else []) + \
-        (["T"] if viscous != "none" else [])
+        (["T"] if viscous != "none" else []) + (["mu"] if viscous == "sutherland" else [])
     for k in need:
         if k not in have:
             raise ValueError("Constituent relation is not found for %s" % k)
-        if have[k] != _norm(CONSTITUENT[k]):
+        want = SUTHERLAND if k == "mu" else CONSTITUENT[k]
+        if have[k] != _norm(want):
             raise NotImplementedError("constituent relation for %s not supported: %s" % (k, have[k]))
-    return dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk)
+    stretch = [0, 0, 0]
+    metric = getattr(sim[0], "metric", None)
+    if metric is not None:
+        if metric.ndim != ndim:
+            raise ValueError("the metric was generated for %d dimensions on a %d-D block" % (metric.ndim, ndim))
+        if not [s for s in spatial if isinstance(s, Central)]:
+            raise ValueError("A Central scheme should be instantiated for metrics.")
+        for d in range(ndim):
+            stretch[d] = int(metric.stretching_metric[d])
+    return dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch)
 
 
 # ---------------------------------------------------------------------------
@@ -587,48 +740,143 @@ class HIPC:
         except KeyError as e:
             raise ValueError("simulation parameter %s has no value" % e)
         p = dict(ndim=nd, n=n, delta=delta, dt=dt, gama=float(v.get("gama", 1.4)),
-                 bc=[[b.boundaries[d][s].kind for s in range(2)] for d in range(nd)], **self.scheme)
+                 bc=[[b.boundaries[d][s].kind for s in range(2)] for d in range(nd)],
+                 bscheme=[[getattr(b.boundaries[d][s], "scheme", "carpenter") for s in range(2)] for d in range(nd)],
+                 **self.scheme)
         for k in ("Minf", "Re", "Pr", "eps"):
             if k in v:
                 p[k] = float(v[k])
         if "TENO_CT" in v:
             p["teno_ct"] = float(v["TENO_CT"])
+        if p["viscous"] == "sutherland":
+            for k in ("SuthT", "RefT"):
+                if k not in v:
+                    raise ValueError("simulation parameter %s has no value" % k)
+                p[k] = float(v[k])
+        if any(p["bc"][d][s] == "isothermal_wall" for d in range(nd) for s in range(2)):
+            if "Twall" not in v:
+                raise ValueError("simulation parameter Twall has no value")
+            p["Twall"] = float(v["Twall"])
         return p
 
-    def initial_state(self, problem):
-        """Evaluate the GridBasedInitialisation equations over the interior grid."""
-        inits = [e for e in self.block.equation_classes if isinstance(e, GridBasedInitialisation)]
-        if not inits:
-            raise ValueError("no GridBasedInitialisation on the block")
+    # ----------------------------------------------------------------------
+    def _env(self, problem):
+        env = {sympy.Symbol(k): val for k, val in self.values.items()}
         nd = problem["ndim"]
-        shape = tuple(reversed(problem["n"]))
-        env = {}
-        for k, val in self.values.items():
-            env[sympy.Symbol(k)] = val
         grids = np.meshgrid(*[np.arange(n) for n in reversed(problem["n"])], indexing="ij")
         for d in range(nd):
             env[sympy.Symbol("idx%d" % d)] = grids[nd - 1 - d].astype(np.float64)
         env[sympy.pi] = math.pi
-        out = {}
-        for eq in flatten([e.equations for e in inits]):
+        return env
+
+    @staticmethod
+    def _evaluate(equations, env, shape, out, forbid=()):
+        for eq in equations:
             rhs = eq.rhs
             syms = sorted(rhs.free_symbols, key=lambda s: s.name)
+            bad = [s for s in syms if s.name in forbid]
+            if bad:
+                raise NotImplementedError("boundary values depending on the solution (%s) are not implemented "
+                                          "by the HIPC target" % bad)
             missing = [s for s in syms if s not in env]
             if missing:
-                raise ValueError("initial condition uses undefined symbols %s" % missing)
+                raise ValueError("equation uses undefined symbols %s" % missing)
             f = sympy.lambdify(syms, rhs, modules="numpy")
             val = np.broadcast_to(np.asarray(f(*[env[s] for s in syms]), dtype=np.float64), shape).copy()
             env[eq.lhs] = val
             out[str(eq.lhs)] = val
+
+    def initial_state(self, problem):
+        """Evaluate the initial condition over the interior grid: the
+        GridBasedInitialisation equations, or the Initialise_Katzer profile."""
+        nd = problem["ndim"]
+        shape = tuple(reversed(problem["n"]))
         names = ["rho"] + ["rhou%d" % d for d in range(nd)] + ["rhoE"]
+        env = self._env(problem)
+        out = {}
+        katz = [e for e in self.block.equation_classes if isinstance(e, Initialise_Katzer)]
+        inits = [e for e in self.block.equation_classes if isinstance(e, GridBasedInitialisation)]
+        if katz:
+            k = katz[0]
+            self._evaluate(k.coordinate_evaluations, env, shape, out)
+            wd = k.wall_direction
+            if wd != 1:
+                raise NotImplementedError("the boundary-layer initialisation expects the wall normal in x1")
+            if "x%d" % wd not in out:
+                raise ValueError("Initialise_Katzer needs the coordinate DataObject(x%d)" % wd)
+            from .boundary_layer import SimilarityProfile
+            bl = SimilarityProfile(k.xMach, k.Re, Pr=0.72, gama=1.4, Tw=-1.0, Tinf=k.Tinf)
+            rho, u, rv, T = bl.evaluate(out["x%d" % wd])
+            gama, Minf = problem["gama"], problem.get("Minf", 1.0)
+            out["rho"], out["rhou0"], out["rhou1"] = rho, rho * u, rv
+            if nd == 3:
+                out["rhou2"] = np.zeros(shape)
+            out["rhoE"] = rho * T / (gama * (gama - 1) * Minf ** 2) + 0.5 * (out["rhou0"] ** 2 + rv ** 2) / rho
+        elif inits:
+            self._evaluate(list(flatten([e.equations for e in inits])), env, shape, out)
+        else:
+            raise ValueError("no GridBasedInitialisation on the block")
         for nm in names:
             if nm not in out:
                 raise ValueError("initial condition does not define %s" % nm)
+        self._fields = (env, out)
         return np.stack([out[nm] for nm in names])
+
+    def _attach_boundary_data(self, problem, q0):
+        """Coordinates of stretched directions and the planes of Dirichlet and
+        inlet faces, from the initial-condition evaluation."""
+        nd = problem["ndim"]
+        env, fields = self._fields
+        shape = tuple(reversed(problem["n"]))
+        coords = {}
+        for d in range(nd):
+            if problem["stretch"][d]:
+                name = "x%d" % d
+                if name not in fields:
+                    raise ValueError("a stretched direction needs its coordinate DataObject(%s) in the "
+                                     "initialisation" % name)
+                line = np.moveaxis(fields[name], nd - 1 - d, -1).reshape(-1, problem["n"][d])[0]
+                coords[d] = np.ascontiguousarray(line)
+        if coords:
+            problem["coords"] = coords
+        planes = {}
+        conserved = {"rho", "rhoE"} | {"rhou%d" % d for d in range(nd)}
+        for d in range(nd):
+            for s in range(2):
+                bc = self.block.boundaries[d][s]
+                if bc.kind not in ("dirichlet", "inlet_pextrap", "isothermal_wall"):
+                    continue
+                if bc.kind == "isothermal_wall":
+                    eqs = [e for e in bc.equations if str(e.lhs) == "rhoE"]
+                    want = sympy.Symbol("rho") * sympy.Symbol("Twall") / (
+                        sympy.Symbol("gama") * (sympy.Symbol("gama") - 1.0) * sympy.Symbol("Minf") ** 2.0)
+                    if len(eqs) != 1 or sympy.simplify(eqs[0].rhs - want) != 0:
+                        raise NotImplementedError("the isothermal wall implements rhoE = rho*Twall/"
+                                                  "(gama*(gama-1)*Minf**2) only")
+                    continue
+                if bc.kind == "inlet_pextrap":
+                    src = q0
+                else:
+                    out = {}
+                    env2 = dict(env)
+                    self._evaluate(bc.equations, env2, shape, out, forbid=conserved)
+                    missing = [nm for nm in ["rho"] + ["rhou%d" % k for k in range(nd)] + ["rhoE"] if nm not in out]
+                    if missing:
+                        raise ValueError("DirichletBC on direction %d side %d does not set %s" % (d, s, missing))
+                    src = np.stack([out[nm] for nm in ["rho"] + ["rhou%d" % k for k in range(nd)] + ["rhoE"]])
+                face = np.take(src, 0 if s == 0 else -1, axis=1 + (nd - 1 - d))
+                e1, e2 = [e for e in range(3) if e != d][:2]
+                n1 = problem["n"][e1] if e1 < nd else 1
+                n2 = problem["n"][e2] if e2 < nd else 1
+                planes[(d, s)] = np.ascontiguousarray(face.reshape(nd + 2, n2, n1))
+        if planes:
+            problem["planes"] = planes
+        return problem
 
     def run(self, niter=None, return_block=False):
         problem = self.problem()
         q0 = self.initial_state(problem)
+        problem = self._attach_boundary_data(problem, q0)
         blk = native.Block(problem)
         blk.set_state(q0)
         n = int(self.values.get("niter", 0) if niter is None else niter)

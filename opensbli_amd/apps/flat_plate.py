@@ -1,0 +1,117 @@
+"""Laminar flat-plate boundary layer through the OpenSBLI-style API.
+
+The structure of the reference's SBLI scripts (katzer_SBLI.py,
+transitional_SBLI.py): conservative Navier-Stokes with a characteristic
+shock-capturing scheme, Sutherland viscosity as a constituent relation,
+StoreSome(4) viscous terms, a wall-normal stretched grid through
+MetricsEquation, InletPressureExtrapolateBC / ExtrapolationBC in x0, a no-slip
+adiabatic wall and a free-stream Dirichlet top in x1 (ReducedAccess closures),
+periodic x2, the Initialise_Katzer similarity profile, RungeKuttaLS(3).
+With OPSC(alg) replaced by HIPC(alg).
+"""
+from opensbli_amd.api import (AdiabaticWallBC, Central, ConstituentRelations, ConstantObject, DataObject, Double,
+                              DirichletBC, EinsteinEquation, ExtrapolationBC, GridVariable, HIPC,
+                              InletPressureExtrapolateBC, Initialise_Katzer, LLFWeno, MetricsEquation, PeriodicBC,
+                              ReducedAccess, RoeAverage, RungeKuttaLS, SimulationBlock, SimulationDataType,
+                              SimulationEquations, StoreSome, TraditionalAlgorithmRK, parse_expr,
+                              substitute_simulation_parameters)
+
+
+def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025", scheme="Z", top=None):
+    ndim = len(n)
+    sc1 = "**{'scheme':'Weno'}"
+    mass = "Eq(Der(rho,t), - Conservative(rhou_j,x_j,%s))" % sc1
+    momentum = "Eq(Der(rhou_i,t) , -Conservative(rhou_i*u_j + KD(_i,_j)*p,x_j , %s) + Der(tau_i_j,x_j) )" % sc1
+    energy = "Eq(Der(rhoE,t), - Conservative((p+rhoE)*u_j,x_j, %s) - Der(q_j,x_j) + Der(u_i*tau_i_j ,x_j) )" % sc1
+    stress_tensor = "Eq(tau_i_j, (mu/Re)*(Der(u_i,x_j)+ Der(u_j,x_i) - (2/3)* KD(_i,_j)* Der(u_k,x_k)))"
+    heat_flux = "Eq(q_j, (-mu/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
+    substitutions = [stress_tensor, heat_flux]
+    constants = ["Re", "Pr", "gama", "Minf", "SuthT", "RefT"]
+    coordinate_symbol = "x"
+    velocity = "Eq(u_i, rhou_i/rho)"
+    pressure = "Eq(p, (gama-1)*(rhoE - rho*(1/2)*(KD(_i,_j)*u_i*u_j)))"
+    speed_of_sound = "Eq(a, (gama*p/rho)**0.5)"
+    temperature = "Eq(T, p*gama*Minf*Minf/(rho))"
+    viscosity = "Eq(mu, (T**(1.5)*(1.0+SuthT/RefT)/(T+SuthT/RefT)))"
+
+    eq = EinsteinEquation()
+    base_eqns = [eq.expand(b, ndim, coordinate_symbol, substitutions, constants) for b in (mass, momentum, energy)]
+    constituent_eqns = [eq.expand(c, ndim, coordinate_symbol, substitutions, constants)
+                        for c in (velocity, pressure, speed_of_sound, temperature, viscosity)]
+
+    block = SimulationBlock(ndim, block_number=0)
+    metriceq = MetricsEquation()
+    metriceq.generate_transformations(ndim, coordinate_symbol, [(False, False), (True, False)] +
+                                      [(False, False)] * (ndim - 2), 2)
+    simulation_eq = SimulationEquations()
+    constituent = ConstituentRelations()
+    for e in base_eqns:
+        simulation_eq.add_equations(e)
+    for e in constituent_eqns:
+        constituent.add_equations(e)
+    simulation_eq.apply_metrics(metriceq)
+
+    schemes = {}
+    LLF = LLFWeno(5, formulation=scheme, averaging=RoeAverage([0, 1]))
+    schemes[LLF.name] = LLF
+    cent = StoreSome(4, "u0 u1 u2 T" if ndim == 3 else "u0 u1 T")
+    schemes[cent.name] = cent
+    rk = RungeKuttaLS(3)
+    schemes[rk.name] = rk
+    block.set_discretisation_schemes(schemes)
+
+    local_dict = {"block": block, "GridVariable": GridVariable, "DataObject": DataObject}
+    boundaries = [[0, 0] for _ in range(ndim)]
+    boundaries[0][0] = InletPressureExtrapolateBC(0, 0, scheme=ReducedAccess())
+    boundaries[0][1] = ExtrapolationBC(0, 1, order=0, scheme=ReducedAccess())
+    boundaries[1][0] = AdiabaticWallBC(1, 0, scheme=ReducedAccess())
+    # free-stream top: the similarity profile's edge values (rho, rho*u, rho*v, rhoE)
+    top = top or (1.0, 1.0, 0.00564966816, 1.0 / (1.4 * 0.4 * 4.0) + 0.5 * (1.0 + 0.00564966816 ** 2))
+    upper = [parse_expr("Eq(DataObject(rho), %.15f)" % top[0], local_dict=local_dict),
+             parse_expr("Eq(DataObject(rhou0), %.15f)" % top[1], local_dict=local_dict),
+             parse_expr("Eq(DataObject(rhou1), %.15f)" % top[2], local_dict=local_dict)]
+    if ndim == 3:
+        upper.append(parse_expr("Eq(DataObject(rhou2), 0.0)", local_dict=local_dict))
+    upper.append(parse_expr("Eq(DataObject(rhoE), %.15f)" % top[3], local_dict=local_dict))
+    boundaries[1][1] = DirichletBC(1, 1, upper, scheme=ReducedAccess())
+    if ndim == 3:
+        boundaries[2][0] = PeriodicBC(2, 0)
+        boundaries[2][1] = PeriodicBC(2, 1)
+    block.set_block_boundaries(boundaries)
+
+    for con in ("Lx1", "by"):
+        local_dict[con] = ConstantObject(con)
+    gridx0 = parse_expr("Eq(DataObject(x0), block.deltas[0]*block.grid_indexes[0])", local_dict=local_dict)
+    gridx1 = parse_expr("Eq(DataObject(x1), Lx1*sinh(by*block.deltas[1]*block.grid_indexes[1]/Lx1)/sinh(by))",
+                        local_dict=local_dict)
+    coordinate_evaluation = [gridx0, gridx1]
+    polynomial_directions = [(False, DataObject("x0")), (True, DataObject("x1"))]
+    if ndim == 3:
+        coordinate_evaluation.append(parse_expr("Eq(DataObject(x2), block.deltas[2]*block.grid_indexes[2])",
+                                                local_dict=local_dict))
+        polynomial_directions.append((False, DataObject("x2")))
+    initial = Initialise_Katzer(polynomial_directions, 50, 950.0, 2.0, 288.0,
+                                coordinate_evaluations=coordinate_evaluation)
+
+    block.set_equations([constituent, simulation_eq, initial, metriceq])
+    block.discretise()
+    alg = TraditionalAlgorithmRK(block)
+    SimulationDataType.set_datatype(Double)
+    prog = HIPC(alg)
+    names = ["gama", "Minf", "Pr", "Re", "dt", "niter", "block0np0", "block0np1", "Delta0block0", "Delta1block0",
+             "SuthT", "RefT", "eps", "Lx1", "by"]
+    vals = ["1.4", "2.0", "0.72", "950.0", dt, str(niter), str(n[0]), str(n[1]),
+            "%r/(block0np0-1)" % lengths[0], "%r/(block0np1-1)" % lengths[1], "110.4", "288.0", "1e-15",
+            repr(lengths[1]), "5.0"]
+    if ndim == 3:
+        names += ["block0np2", "Delta2block0"]
+        vals += [str(n[2]), "%r/block0np2" % lengths[2]]
+    substitute_simulation_parameters(names, vals)
+    return prog
+
+
+if __name__ == "__main__":
+    import sys
+    prog = build(niter=int(sys.argv[1]) if len(sys.argv) > 1 else 50)
+    res = prog.run()
+    print({k: float(abs(v).max()) for k, v in res.items()})

@@ -61,3 +61,37 @@ def test_parameter_expressions():
     from opensbli_amd import api
     prog.set_values(["niter"], ["ceil(0.2/0.0002)"])
     assert prog.values["niter"] == 1000
+
+
+def test_flat_plate_script_matches_problem():
+    """The SBLI-style script (metric, Sutherland, walls, Katzer profile) gives
+    the same block as problems.flat_plate without the noise."""
+    from opensbli_amd.apps import flat_plate as fp
+    from opensbli_amd.problems import flat_plate
+    prog = fp.build(n=(40, 32, 8))
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    ref_p, ref_q = flat_plate(n=(40, 32, 8), noise=0.0)
+    for k in ("ndim", "n", "conv", "averaging", "viscous", "rk", "bc", "stretch", "gama", "Minf", "Re", "Pr",
+              "SuthT", "RefT"):
+        assert p[k] == ref_p[k], k
+    assert np.allclose(p["delta"], ref_p["delta"], rtol=1e-15)
+    assert p["bscheme"][:2] == ref_p["bscheme"][:2]
+    assert np.abs(p["coords"][1] - ref_p["coords"][1]).max() < 1e-12
+    for f in ((0, 0), (1, 1)):
+        assert np.abs(p["planes"][f] - ref_p["planes"][f]).max() < 1e-10
+    assert np.abs(q0 - ref_q).max() < 1e-15
+
+
+def test_facade_rejects_unsupported_boundaries():
+    from opensbli_amd import api
+    with pytest.raises(ValueError):
+        api.InletPressureExtrapolateBC(0, 1)
+    with pytest.raises(ValueError):
+        api.ExtrapolationBC(0, 0, order=2)
+    m = api.MetricsEquation()
+    with pytest.raises(NotImplementedError):
+        m.generate_transformations(2, "x", [(True, True), (True, True)], 2)
+    with pytest.raises(ValueError):
+        m.generate_transformations(2, "x", [(False, False)], 2)

@@ -146,3 +146,26 @@ def test_flat_plate_full_size_50_steps():
     assert np.all(q[1:4, :, 0, :] == 0.0)
     # the laminar layer changes little in 50 steps away from the inlet
     assert np.abs(q[0] - q0[0]).max() < 0.05
+
+
+def test_flat_plate_script_on_gpu(oracle):
+    """The SBLI-style script through HIPC.run against the oracle on the same block."""
+    from opensbli_amd.apps import flat_plate as fp
+    _say("case flat plate script 40x32x8 100 steps")
+    prog = fp.build(n=(40, 32, 8), niter=100)
+    res = prog.run()
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(100)
+    qr = r.get_state()
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhou2", "rhoE")])
+    f = oracle.RefSolver(p, variant="fma")
+    f.set_state(q0)
+    f.run(100)
+    spread = rel_l2(f.get_state(), qr)
+    err = rel_l2(q, qr)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), (err, spread)
