@@ -110,14 +110,31 @@ __device__ __forceinline__ void load_pt(Pt<ND>& s, const double* __restrict__ Q,
 // Interface state between points i and i+1 (the averaged symbols of the
 // eigensystem).  Roe: averaging.py:70-114.  Simple: averaging.py:38-59.
 // ---------------------------------------------------------------------------
-template <int ND>
-struct Avg {
-  double rho, a, u[ND];
-  double ia, ir;           // 1/a, 1/rho (inv_AVG_a, inv_AVG_rho, shock_capturing.py:319-331)
-  double phi;              // (gama-1)|u|^2/2
-  double alpha, bta;       // rho/(a sqrt2), 1/(rho a sqrt2)
-  double Hh;               // a^2/(gama-1) + |u|^2/2  (1-D eigensystem)
+template <int ND, class T = double>
+struct AvgT {
+  T rho, a, u[ND];
+  T ia, ir;           // 1/a, 1/rho (inv_AVG_a, inv_AVG_rho, shock_capturing.py:319-331)
+  T phi;              // (gama-1)|u|^2/2
+  T alpha, bta;       // rho/(a sqrt2), 1/(rho a sqrt2)
+  T Hh;               // a^2/(gama-1) + |u|^2/2  (1-D eigensystem)
 };
+template <int ND>
+using Avg = AvgT<ND, double>;
+
+// the derived symbols of an averaged state (rho, a, u)
+template <int ND, class T>
+__device__ __forceinline__ void finish_avg(AvgT<ND, T>& v, const Phys& ph) {
+  T uu = v.u[0] * v.u[0];
+#pragma unroll
+  for (int m = 1; m < ND; ++m) uu = uu + v.u[m] * v.u[m];
+  v.ia = 1.0 / v.a;
+  v.ir = 1.0 / v.rho;
+  v.phi = ph.gm1 * 0.5 * uu;
+  constexpr double RS2 = 0.70710678118654752440;  // 1/sqrt(2.0)
+  v.alpha = v.rho * v.ia * RS2;
+  v.bta = v.ir * v.ia * RS2;
+  v.Hh = v.a * v.a / ph.gm1 + 0.5 * uu;
+}
 
 template <int ND, int AVG>
 __device__ __forceinline__ void interface_state(Avg<ND>& v, const Pt<ND>& L, const Pt<ND>& R, const Phys& ph) {
@@ -140,16 +157,7 @@ __device__ __forceinline__ void interface_state(Avg<ND>& v, const Pt<ND>& L, con
     for (int m = 0; m < ND; ++m) v.u[m] = 0.5 * (L.u[m] + R.u[m]);
     v.a = 0.5 * (L.a + R.a);
   }
-  double uu = 0.0;
-#pragma unroll
-  for (int m = 0; m < ND; ++m) uu += v.u[m] * v.u[m];
-  v.ia = 1.0 / v.a;
-  v.ir = 1.0 / v.rho;
-  v.phi = ph.gm1 * 0.5 * uu;
-  constexpr double RS2 = 0.70710678118654752440;  // 1/sqrt(2.0)
-  v.alpha = v.rho * v.ia * RS2;
-  v.bta = v.ir * v.ia * RS2;
-  v.Hh = v.a * v.a / ph.gm1 + 0.5 * uu;
+  finish_avg<ND>(v, ph);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,12 +199,13 @@ __host__ __device__ constexpr unsigned rmask() {
   }
 }
 
-// row K of the left eigenvector matrix
-template <int ND, int D, int K>
-__device__ __forceinline__ void lrow(const Avg<ND>& v, const Phys& ph, double l[ND + 2]) {
+// row K of the left eigenvector matrix (T: double, or a dual number type for
+// the derivatives with respect to the averaged state)
+template <int ND, int D, int K, class T = double>
+__device__ __forceinline__ void lrow(const AvgT<ND, T>& v, const Phys& ph, T l[ND + 2]) {
   const double gm1 = ph.gm1;
   if constexpr (ND == 3) {
-    const double ia2 = v.ia * v.ia;
+    const T ia2 = v.ia * v.ia;
     if constexpr (K == D) {
       l[0] = 1.0 - v.phi * ia2;
       l[1] = gm1 * v.u[0] * ia2;
@@ -210,14 +219,17 @@ __device__ __forceinline__ void lrow(const Avg<ND>& v, const Phys& ph, double l[
       l[1 + c] = s * v.ir;
     } else {
       constexpr double sg = (K == 3) ? 1.0 : -1.0;
-      const double th = v.u[D];
+      const T th = v.u[D];
       l[0] = v.bta * (v.phi - sg * th * v.a);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) l[1 + m] = -v.bta * (gm1 * v.u[m] - (m == D ? sg * v.a : 0.0));
+      for (int m = 0; m < 3; ++m) {
+        if (m == D) l[1 + m] = -v.bta * (gm1 * v.u[m] - sg * v.a);
+        else l[1 + m] = -v.bta * (gm1 * v.u[m]);
+      }
       l[4] = v.bta * gm1;
     }
   } else if constexpr (ND == 2) {
-    const double ia2 = v.ia * v.ia;
+    const T ia2 = v.ia * v.ia;
     if constexpr (K == 0) {
       l[0] = 1.0 - v.phi * ia2;
       l[1] = gm1 * v.u[0] * ia2;
@@ -230,16 +242,19 @@ __device__ __forceinline__ void lrow(const Avg<ND>& v, const Phys& ph, double l[
       l[1 + c] = s * v.ir;
     } else {
       constexpr double sg = (K == 2) ? 1.0 : -1.0;
-      const double th = v.u[D];
+      const T th = v.u[D];
       l[0] = v.bta * (v.phi - sg * v.a * th);
 #pragma unroll
-      for (int m = 0; m < 2; ++m) l[1 + m] = -v.bta * (gm1 * v.u[m] - (m == D ? sg * v.a : 0.0));
+      for (int m = 0; m < 2; ++m) {
+        if (m == D) l[1 + m] = -v.bta * (gm1 * v.u[m] - sg * v.a);
+        else l[1 + m] = -v.bta * (gm1 * v.u[m]);
+      }
       l[3] = v.bta * gm1;
     }
   } else {
-    const double u = v.u[0], a = v.a, Hh = v.Hh;
-    const double den = 2.0 * Hh - u * u;
-    const double iden = 1.0 / den;
+    const T u = v.u[0], a = v.a, Hh = v.Hh;
+    const T den = 2.0 * Hh - u * u;
+    const T iden = 1.0 / den;
     if constexpr (K == 0) {
       l[0] = u * (2.0 * Hh + a * u - u * u) * v.ia * 0.5 * iden;
       l[1] = (-Hh - a * u + 0.5 * u * u) * v.ia * iden;
@@ -257,8 +272,8 @@ __device__ __forceinline__ void lrow(const Avg<ND>& v, const Phys& ph, double l[
 }
 
 // column K of the right eigenvector matrix
-template <int ND, int D, int K>
-__device__ __forceinline__ void rcol(const Avg<ND>& v, const Phys& ph, double r[ND + 2]) {
+template <int ND, int D, int K, class T = double>
+__device__ __forceinline__ void rcol(const AvgT<ND, T>& v, const Phys& ph, T r[ND + 2]) {
   const double igm1 = 1.0 / ph.gm1;
   if constexpr (ND == 3) {
     if constexpr (K == D) {
@@ -276,7 +291,10 @@ __device__ __forceinline__ void rcol(const Avg<ND>& v, const Phys& ph, double r[
       constexpr double sg = (K == 3) ? 1.0 : -1.0;
       r[0] = v.alpha;
 #pragma unroll
-      for (int m = 0; m < 3; ++m) r[1 + m] = v.alpha * (v.u[m] + (m == D ? sg * v.a : 0.0));
+      for (int m = 0; m < 3; ++m) {
+        if (m == D) r[1 + m] = v.alpha * (v.u[m] + sg * v.a);
+        else r[1 + m] = v.alpha * v.u[m];
+      }
       r[4] = v.alpha * ((v.phi + v.a * v.a) * igm1 + sg * v.u[D] * v.a);
     }
   } else if constexpr (ND == 2) {
@@ -294,11 +312,14 @@ __device__ __forceinline__ void rcol(const Avg<ND>& v, const Phys& ph, double r[
       constexpr double sg = (K == 2) ? 1.0 : -1.0;
       r[0] = v.alpha;
 #pragma unroll
-      for (int m = 0; m < 2; ++m) r[1 + m] = v.alpha * (v.u[m] + (m == D ? sg * v.a : 0.0));
+      for (int m = 0; m < 2; ++m) {
+        if (m == D) r[1 + m] = v.alpha * (v.u[m] + sg * v.a);
+        else r[1 + m] = v.alpha * v.u[m];
+      }
       r[3] = v.alpha * ((v.phi + v.a * v.a) * igm1 + sg * v.a * v.u[D]);
     }
   } else {
-    const double u = v.u[0], a = v.a, Hh = v.Hh;
+    const T u = v.u[0], a = v.a, Hh = v.Hh;
     if constexpr (K == 0) {
       r[0] = 1.0; r[1] = u - a; r[2] = Hh - u * a;
     } else if constexpr (K == 1) {

@@ -30,6 +30,9 @@ void launch_rhs_vjp(int nd, const double* Q, const double* P, const double* M, d
 void launch_adj_stage(int nd, const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
                       double* out1, double* out2, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_pack_all(const double* F, double* out, const Grid& g, int nv, hipStream_t s);
+void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
+                           double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, int seg,
+                           bool acc, hipStream_t s);
 void launch_lincomb(double* out, double a, const double* x, double b, const double* y, long long n, hipStream_t s);
 
 // record the thread-local message osbl_last_error() returns; returns code

@@ -184,6 +184,7 @@ struct osbl_ctx {
   double A[5]{}, B[5]{};
   int seg = 32;
   int vsz = 16;             // z segment of the tiled viscous kernel
+  int vjp_seg = 8;          // cells per thread of the characteristic VJP sweeps
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
   bool general = false;     // closures, metric or Sutherland viscosity: the general update kernel
@@ -783,11 +784,12 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
 int osbl_num_stages(osbl_ctx* c) { return c ? c->nstages : fail(OSBL_E_ARG, "null context"); }
 
 // ---------------------------------------------------------------------------
-// discrete adjoint (central skew convective terms, inviscid, periodic, RK3)
+// discrete adjoint (inviscid central skew or LLF WENO5-Z, periodic, RK3)
 // ---------------------------------------------------------------------------
 static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
-  if (c->cfg.conv != OSBL_CONV_CENTRAL_SKEW || c->cfg.viscous != OSBL_VISC_NONE)
-    return fail(OSBL_E_ARG, "the adjoint covers the inviscid central skew-symmetric scheme");
+  if ((c->cfg.conv != OSBL_CONV_CENTRAL_SKEW && c->cfg.conv != OSBL_CONV_WENO_Z) || c->cfg.viscous != OSBL_VISC_NONE)
+    return fail(OSBL_E_ARG, "the adjoint covers the inviscid central skew-symmetric and WENO5-Z schemes "
+                            "(TENO's cut-off is not differentiable)");
   for (int d = 0; d < c->nd; ++d)
     if (c->cfg.bc[d][0] != OSBL_BC_PERIODIC) return fail(OSBL_E_ARG, "the adjoint needs periodic boundaries");
   if (need_rk3 && c->cfg.rk != OSBL_RK3) return fail(OSBL_E_ARG, "the adjoint step covers RungeKutta(3)");
@@ -814,11 +816,24 @@ static void vjp(osbl_ctx* c, const double* q, const double* mu, double* out) {
   launch_rhs_vjp(c->nd, c->Abuf, c->P, c->Mbuf, out, c->g, c->ph, c->stream);
 }
 
+// out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B), out2 = A + B (optional)
+static void adj_stage(osbl_ctx* c, const double* Qs, const double* A, const double* B, double alpha, double beta,
+                      double cc, double* out1, double* out2) {
+  if (c->cfg.conv == OSBL_CONV_WENO_Z)
+    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->vjp_seg,
+                          false, c->stream);
+  else
+    launch_adj_stage(c->nd, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->stream);
+}
+
 int osbl_rhs_vjp_dev(osbl_ctx* c, const double* q_dev, const double* mu_dev, double* out_dev) {
   if (!c || !q_dev || !mu_dev || !out_dev) return fail(OSBL_E_ARG, "null argument");
   int rc = adjoint_ready(c, false);
   if (rc) return rc;
-  vjp(c, q_dev, mu_dev, out_dev);
+  if (c->cfg.conv == OSBL_CONV_WENO_Z)
+    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, c->g,
+                          c->ph, c->vjp_seg, true, c->stream);
+  else vjp(c, q_dev, mu_dev, out_dev);
   HIPCHK(hipGetLastError());
   return OSBL_OK;
 }
@@ -852,15 +867,14 @@ int osbl_adjoint_step_dev(osbl_ctx* c, const double* stages_dev, double* lam) {
   const double dt = c->ph.dt;
   const double* a = c->A;  // rknew
   const double* b = c->B;  // rkold
-  hipStream_t s = c->stream;
   // Q3 = Qo2 + dt a2 R(Q2), Qo2 = lam's other path:  lam_Q2 = VJP(Q2; dt a2 lam)
-  launch_adj_stage(c->nd, Q2, lam, nullptr, dt * a[2], 0.0, 0.0, l2, nullptr, c->g, c->ph, s);
+  adj_stage(c, Q2, lam, nullptr, dt * a[2], 0.0, 0.0, l2, nullptr);
   // Q2 = Qo1 + dt a1 R(Q1), Qo2 = Qo1 + dt b1 R(Q1):
   //   lam_Q1 = VJP(Q1; dt a1 lam_Q2 + dt b1 lam_Qo2),  lam_Qo1 = lam_Q2 + lam_Qo2
-  launch_adj_stage(c->nd, Q1, l2, lam, dt * a[1], dt * b[1], 0.0, l1, lo, c->g, c->ph, s);
+  adj_stage(c, Q1, l2, lam, dt * a[1], dt * b[1], 0.0, l1, lo);
   // Q1 = Qn + dt a0 R(Qn), Qo1 = Qn + dt b0 R(Qn):
   //   lam_Qn = lam_Q1 + lam_Qo1 + VJP(Qn; dt a0 lam_Q1 + dt b0 lam_Qo1)
-  launch_adj_stage(c->nd, Qn, l1, lo, dt * a[0], dt * b[0], 1.0, lam, nullptr, c->g, c->ph, s);
+  adj_stage(c, Qn, l1, lo, dt * a[0], dt * b[0], 1.0, lam, nullptr);
   HIPCHK(hipGetLastError());
   return OSBL_OK;
 }
