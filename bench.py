@@ -76,7 +76,9 @@ def main():
     if args.workload == "flat_plate":
         return bench_flat_plate(args)
     if args.workload == "adjoint":
-        print(json.dumps(adjoint_ratio()), flush=True)
+        out = adjoint_ratio()
+        out["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
+        print(json.dumps(out), flush=True)
         return
 
     import torch
@@ -146,6 +148,7 @@ def main():
     if not args.no_adjoint:
         _log("bench: adjoint/forward ratio (2-D Kelvin-Helmholtz 512^2, 200 steps, checkpoints every 20)")
         adj = adjoint_ratio()
+        adj["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -177,7 +180,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def adjoint_ratio(n=512, steps=200, every=20, reps=3):
+def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew"):
     """Backward/forward time of the differentiable 2-D Kelvin-Helmholtz run:
     forward = the checkpointed forward integration, backward = recomputation
     of every segment plus the hand-written adjoint steps (median of reps)."""
@@ -187,7 +190,7 @@ def adjoint_ratio(n=512, steps=200, every=20, reps=3):
     from opensbli_amd.native import Block
     from opensbli_amd.problems import kelvin_helmholtz
     amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
-    problem, _ = kelvin_helmholtz(n, amps, conv="central_skew")
+    problem, _ = kelvin_helmholtz(n, amps, conv=conv)
     b = Block(problem)
     fw, bw = [], []
     for rep in range(reps + 1):
@@ -206,7 +209,7 @@ def adjoint_ratio(n=512, steps=200, every=20, reps=3):
             bw.append(t2 - t1)
     b.close()
     f, w = float(np.median(fw)), float(np.median(bw))
-    return {"workload": f"kelvin_helmholtz_{n}^2_central_skew_rk3_{steps}_steps_checkpoint_{every}",
+    return {"workload": f"kelvin_helmholtz_{n}^2_{conv}_rk3_{steps}_steps_checkpoint_{every}",
             "forward_ms": f * 1e3, "backward_ms": w * 1e3, "ratio": w / f}
 
 
