@@ -32,6 +32,7 @@ struct Grid {
   int planes;
   int plane;  // doubles per plane
   long long fs;  // doubles per field (padded)
+  int periodic_x;  // x faces periodic
   __host__ __device__ __forceinline__ int idx(int i, int j, int k) const {
     return (i + OX) + pitch * ((j + h[1]) + rows * (k + h[2]));
   }

@@ -269,6 +269,122 @@ __global__ void __launch_bounds__(64, 2) k_char_sweep_x(const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Characteristic sweep along x with the lanes of a wave along x.  A wave owns
+// one grid line (one (y, z) row) and walks it in chunks of 64 cells: the
+// chunk's 69 points (x0-2 .. x0+66) are loaded with coalesced reads, their
+// constituent relations computed once and kept in LDS, then lane l evaluates
+// the flux at interface x0+l+1/2 from LDS and the residual of
+// cell x0+l is its difference with lane l-1's flux (a shuffle; lane 0 takes
+// the previous chunk's last flux).  Stores are coalesced 512-byte rows.  On a
+// periodic line the flux left of cell 0 is the line's last interface flux, so
+// cell 0 is finished after the walk; otherwise it is evaluated first.
+// ---------------------------------------------------------------------------
+constexpr int XW = 4;             // waves (lines) per block
+constexpr int XP = 64 + 5;        // points staged per chunk
+
+template <int ND, int CONV, int AVG, bool ACC>
+__global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restrict__ Q, double* __restrict__ R,
+                                                          Grid g, Phys ph, int nrows, int periodic,
+                                                          const double* __restrict__ met) {
+  constexpr int NV = ND + 2;
+  constexpr int NF = NV + ND + 2;  // q, u, p, a
+  __shared__ double sh[XW][NF][XP];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = blockIdx.x * XW + wv;
+  if (row >= nrows) return;  // whole waves only: no block-wide barrier below
+  double (*S)[XP] = sh[wv];
+  const int base = g.idx(0, row % g.n[1], row / g.n[1]);
+  const int N = g.n[0];
+  const double inv = ph.inv_delta[0];
+  auto stage_point = [&](int slot, int x) {
+    Pt<ND> p;
+    load_pt<ND>(p, Q, g.fs, base + x, ph);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) S[m][slot] = p.q[m];
+#pragma unroll
+    for (int m = 0; m < ND; ++m) S[NV + m][slot] = p.u[m];
+    S[NV + ND][slot] = p.p;
+    S[NV + ND + 1][slot] = p.a;
+  };
+  auto get_point = [&](Pt<ND>& p, int slot) {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) p.q[m] = S[m][slot];
+#pragma unroll
+    for (int m = 0; m < ND; ++m) p.u[m] = S[NV + m][slot];
+    p.p = S[NV + ND][slot];
+    p.a = S[NV + ND + 1][slot];
+  };
+  // flux at interface x0 + lane + 1/2 (points x0+lane-2 .. x0+lane+3 = slots lane .. lane+5)
+  auto flux = [&](double fh[NV]) {
+    Pt<ND> w[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) get_point(w[q], lane + q);
+    char_flux<ND, 0, CONV, AVG>(w, ph, fh);
+  };
+  const int xmax = N + g.h[0] - 1;  // last stored point of the line
+  auto stage_chunk = [&](int x0) {
+    // slots 0..68 hold points x0-2 .. x0+66 (clamped to the stored line: the
+    // clamped slots only feed lanes whose cells lie beyond the line)
+    __builtin_amdgcn_wave_barrier();
+    stage_point(lane, min(x0 - 2 + lane, xmax));
+    if (lane < XP - 64) stage_point(64 + lane, min(x0 + 62 + lane, xmax));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  auto emit = [&](int x, const double* r) {
+    const double gm = met ? met[x] : 1.0;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      const double v = met ? -(gm * (r[m] * inv)) : -r[m] * inv;
+      if constexpr (ACC) R[m * g.fs + base + x] += v;
+      else R[m * g.fs + base + x] = v;
+    }
+  };
+  double prev[NV];  // flux left of the chunk's first cell (valid in every lane)
+  double first[NV];
+  if (!periodic) {
+    // flux at interface -1/2 (points -3..2): lane 0 of a chunk staged at x0 = -1
+    stage_chunk(-1);
+    double fh[NV];
+    flux(fh);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) prev[m] = __shfl(fh[m], 0);
+  }
+  for (int x0 = 0; x0 < N; x0 += 64) {
+    stage_chunk(x0);
+    double fh[NV];
+    flux(fh);
+    const int x = x0 + lane;
+    double left[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      const double up = __shfl_up(fh[m], 1);
+      left[m] = lane == 0 ? prev[m] : up;
+    }
+    const int last = min(63, N - 1 - x0);  // lane holding the chunk's last valid interface
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      if (x0 == 0) first[m] = fh[m];
+      prev[m] = __shfl(fh[m], last);
+    }
+    if (x < N && !(periodic && x == 0)) {
+      double r[NV];
+#pragma unroll
+      for (int m = 0; m < NV; ++m) r[m] = fh[m] - left[m];
+      emit(x, r);
+    }
+  }
+  if (periodic && lane == 0) {
+    // the line's last interface flux is the flux left of cell 0
+    double r[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) r[m] = first[m] - prev[m];
+    emit(0, r);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Central fourth-order stencils (scheme.py:81-85): D1 weights
 // [1/12, -2/3, 0, 2/3, -1/12], D2 weights [-1/12, 4/3, -5/2, 4/3, -1/12].
 // ---------------------------------------------------------------------------
@@ -671,9 +787,9 @@ static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys
                            hipStream_t s) {
   if constexpr (D == 0 && ND >= 2) {
     const int nrows = g.n[1] * g.n[2];
-    dim3 grid((nrows + 63) / 64, (g.n[0] + seg - 1) / seg);
-    if (acc) k_char_sweep_x<ND, CONV, AVG, true><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows, met);
-    else k_char_sweep_x<ND, CONV, AVG, false><<<grid, 64, 0, s>>>(Q, R, g, ph, seg, nrows, met);
+    const unsigned blocks = (unsigned)((nrows + XW - 1) / XW);
+    if (acc) k_char_sweep_xl<ND, CONV, AVG, true><<<blocks, 64 * XW, 0, s>>>(Q, R, g, ph, nrows, g.periodic_x, met);
+    else k_char_sweep_xl<ND, CONV, AVG, false><<<blocks, 64 * XW, 0, s>>>(Q, R, g, ph, nrows, g.periodic_x, met);
     return;
   }
   long long lines = 1;

@@ -398,6 +398,7 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
     g.h[d] = d < c->nd ? halo : 0;
   }
   g.pitch = ((OX + g.n[0] + g.h[0]) + 15) / 16 * 16;
+  g.periodic_x = cfg->bc[0][0] == OSBL_BC_PERIODIC;
   g.rows = g.n[1] + 2 * g.h[1];
   g.planes = g.n[2] + 2 * g.h[2];
   g.plane = g.pitch * g.rows;
