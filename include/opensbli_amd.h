@@ -93,6 +93,10 @@ typedef struct osbl_config {
   int32_t reserved;
   double suthT, refT;    /* SuthT, RefT of the Sutherland law */
   double twall;          /* Twall of the isothermal wall */
+  int32_t sensor;        /* 1: modified Ducros sensor with the adaptive TENO cut-off
+                            (shock_sensors.py:12-49, teno.py:430-443, LLFTeno(formulation='adaptive')) */
+  int32_t reserved2;
+  double teno_a1, teno_a2, sensor_eps;  /* teno_a1, teno_a2, epsilon of the sensor */
 } osbl_config;
 
 typedef struct osbl_ctx osbl_ctx;

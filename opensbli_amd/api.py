@@ -242,12 +242,38 @@ class LLFTeno:
     def __init__(self, order, formulation=None, physics=None, averaging=None, sensor=None, store_sensor=False):
         if order not in (5, 6):
             raise NotImplementedError("Only 5th and 6th order TENO are supported by the HIPC target.")
-        if formulation is not None:
-            raise NotImplementedError("adaptive TENO is not supported by the HIPC target yet")
+        if sensor is None and formulation is not None:
+            raise ValueError("Storage array for the shock sensor is required.")
+        if formulation not in (None, "adaptive"):
+            raise NotImplementedError("TENO formulation %r is not implemented by the HIPC target" % formulation)
+        self.adaptive = formulation == "adaptive"
+        self.sensor_array = sensor
+        self.store_sensor = store_sensor
         self.conv = "teno5" if order == 5 else "teno6"
         self.averaging = (averaging or SimpleAverage([0, 1])).kind
         self.name = "TenoDerivative"
         self.schemetype = "Spatial"
+
+
+class _SensorEquation:
+    """The modified Ducros sensor of ShockSensor.ducros_equations as a
+    constituent relation (recognised by name)."""
+
+    def __init__(self, name, ndim):
+        self.text = "Eq(%s,DUCROS)" % name
+        self.ndim = ndim
+        self.substitutions = ()
+
+
+class ShockSensor:
+    """Modified Ducros shock sensor (shock_sensors.py:8): theta =
+    1/2 (1 - tanh(2.5 (1 + 100 div u))) div^2 / (div^2 + |curl u|^2 + epsilon);
+    `epsilon` is a simulation constant."""
+
+    def ducros_equations(self, block, coordinate_symbol, metrics=None, name="theta"):
+        if block.ndim < 2:
+            raise ValueError("vorticity is not defined for one-dimensional cases.")
+        return [_SensorEquation(name, block.ndim)], DataObject(name)
 
 
 class RungeKutta:
@@ -669,6 +695,13 @@ def _recognise(block):
         want = SUTHERLAND if k == "mu" else CONSTITUENT[k]
         if have[k] != _norm(want):
             raise NotImplementedError("constituent relation for %s not supported: %s" % (k, have[k]))
+    sensor = 0
+    tenos = [s for s in spatial if isinstance(s, LLFTeno)]
+    if tenos and tenos[0].adaptive and conv in ("teno5", "teno6"):
+        name = str(tenos[0].sensor_array)
+        if have.get(name) != "DUCROS":
+            raise ValueError("the adaptive TENO needs the shock sensor %s among the constituent relations" % name)
+        sensor = 1
     stretch = [0, 0, 0]
     metric = getattr(sim[0], "metric", None)
     if metric is not None:
@@ -678,7 +711,7 @@ def _recognise(block):
             raise ValueError("A Central scheme should be instantiated for metrics.")
         for d in range(ndim):
             stretch[d] = int(metric.stretching_metric[d])
-    return dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch)
+    return dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch, sensor=sensor)
 
 
 # ---------------------------------------------------------------------------
@@ -753,6 +786,11 @@ class HIPC:
                 if k not in v:
                     raise ValueError("simulation parameter %s has no value" % k)
                 p[k] = float(v[k])
+        if p.get("sensor"):
+            for k, name in (("teno_a1", "teno_a1"), ("teno_a2", "teno_a2"), ("sensor_eps", "epsilon")):
+                if name not in v:
+                    raise ValueError("simulation parameter %s has no value" % name)
+                p[k] = float(v[name])
         if any(p["bc"][d][s] == "isothermal_wall" for d in range(nd) for s in range(2)):
             if "Twall" not in v:
                 raise ValueError("simulation parameter Twall has no value")

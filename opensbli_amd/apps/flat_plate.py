@@ -11,15 +11,18 @@ With OPSC(alg) replaced by HIPC(alg).
 """
 from opensbli_amd.api import (AdiabaticWallBC, Central, ConstituentRelations, ConstantObject, DataObject, Double,
                               DirichletBC, EinsteinEquation, ExtrapolationBC, GridVariable, HIPC,
-                              InletPressureExtrapolateBC, Initialise_Katzer, LLFWeno, MetricsEquation, PeriodicBC,
-                              ReducedAccess, RoeAverage, RungeKuttaLS, SimulationBlock, SimulationDataType,
-                              SimulationEquations, StoreSome, TraditionalAlgorithmRK, parse_expr,
+                              InletPressureExtrapolateBC, Initialise_Katzer, LLFTeno, LLFWeno, MetricsEquation,
+                              PeriodicBC, ReducedAccess, RoeAverage, RungeKuttaLS, ShockSensor, SimulationBlock,
+                              SimulationDataType, SimulationEquations, StoreSome, TraditionalAlgorithmRK, parse_expr,
                               substitute_simulation_parameters)
 
 
 def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025", scheme="Z", top=None):
+    """scheme: "Z" or "JS" (LLFWeno), or "teno5"/"teno6" (adaptive LLFTeno with
+    the Ducros sensor, as katzer_SBLI.py:55-62)."""
     ndim = len(n)
-    sc1 = "**{'scheme':'Weno'}"
+    teno = scheme.startswith("teno")
+    sc1 = "**{'scheme':'%s'}" % ("Teno" if teno else "Weno")
     mass = "Eq(Der(rho,t), - Conservative(rhou_j,x_j,%s))" % sc1
     momentum = "Eq(Der(rhou_i,t) , -Conservative(rhou_i*u_j + KD(_i,_j)*p,x_j , %s) + Der(tau_i_j,x_j) )" % sc1
     energy = "Eq(Der(rhoE,t), - Conservative((p+rhoE)*u_j,x_j, %s) - Der(q_j,x_j) + Der(u_i*tau_i_j ,x_j) )" % sc1
@@ -52,7 +55,14 @@ def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025",
     simulation_eq.apply_metrics(metriceq)
 
     schemes = {}
-    LLF = LLFWeno(5, formulation=scheme, averaging=RoeAverage([0, 1]))
+    if teno:
+        SS = ShockSensor()
+        shock_sensor, sensor_array = SS.ducros_equations(block, coordinate_symbol, metriceq)
+        constituent.add_equations(shock_sensor)
+        LLF = LLFTeno(int(scheme[-1]), formulation="adaptive", averaging=RoeAverage([0, 1]), sensor=sensor_array,
+                      store_sensor=True)
+    else:
+        LLF = LLFWeno(5, formulation=scheme, averaging=RoeAverage([0, 1]))
     schemes[LLF.name] = LLF
     cent = StoreSome(4, "u0 u1 u2 T" if ndim == 3 else "u0 u1 T")
     schemes[cent.name] = cent
@@ -99,10 +109,10 @@ def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025",
     SimulationDataType.set_datatype(Double)
     prog = HIPC(alg)
     names = ["gama", "Minf", "Pr", "Re", "dt", "niter", "block0np0", "block0np1", "Delta0block0", "Delta1block0",
-             "SuthT", "RefT", "eps", "Lx1", "by"]
+             "SuthT", "RefT", "eps", "Lx1", "by", "teno_a1", "teno_a2", "epsilon", "TENO_CT"]
     vals = ["1.4", "2.0", "0.72", "950.0", dt, str(niter), str(n[0]), str(n[1]),
             "%r/(block0np0-1)" % lengths[0], "%r/(block0np1-1)" % lengths[1], "110.4", "288.0", "1e-15",
-            repr(lengths[1]), "5.0"]
+            repr(lengths[1]), "5.0", "10.5", "4.5", "1.0e-30", "1e-6"]
     if ndim == 3:
         names += ["block0np2", "Delta2block0"]
         vals += [str(n[2]), "%r/block0np2" % lengths[2]]

@@ -58,7 +58,7 @@ __device__ __forceinline__ void stage_segment(const double* __restrict__ Q, doub
     finish_pt<ND>(w[5], ph);
     if (f + 1 < c1) fetch_q<ND>(nxt, Q, fs, off + 6);
     double fh[NV];
-    char_flux<ND, 0, CONV, AVG>(w, ph, fh);
+    char_flux<ND, 0, CONV, AVG>(w, ph, fh, ph.teno_ct);
     if (f >= c0) {
       const int x = off + 2;
 #pragma unroll

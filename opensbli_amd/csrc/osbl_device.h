@@ -44,7 +44,29 @@ struct Phys {
   double inv_delta[3];
   double inv_re, cq;  // 1/Re and 1/((gama-1) Minf^2 Pr Re)
   double suth;        // SuthT/RefT of the Sutherland law
+  double teno_a1, teno_a2, sensor_eps;  // adaptive TENO cut-off and Ducros sensor constants
 };
+
+// 10^n for an integer n by repeated products (exact for |n| <= 22)
+__host__ __device__ __forceinline__ double pow10i(int n) {
+  double r = 1.0;
+  const int m = n < 0 ? -n : n;
+  for (int i = 0; i < m; ++i) r *= 10.0;
+  return n < 0 ? 1.0 / r : r;
+}
+
+// adaptive TENO cut-off from the sensor at an interface's left point
+// (teno.py:430-443): C_T = 1/10^floor(a1 - a2 (1 - (1-theta)^4 (1 + 4 theta))),
+// each operation rounded separately so that floor() sees the same value as a
+// build without contraction
+__device__ __forceinline__ double adaptive_ct(double m, const Phys& ph) {
+  const double t = __dsub_rn(1.0, m);
+  const double t2 = __dmul_rn(t, t);
+  const double t4 = __dmul_rn(t2, t2);
+  const double q = __dmul_rn(t4, __dadd_rn(1.0, __dmul_rn(4.0, m)));
+  const double beta = __dsub_rn(ph.teno_a1, __dmul_rn(ph.teno_a2, __dsub_rn(1.0, q)));
+  return 1.0 / pow10i((int)floor(beta));
+}
 
 // Near-boundary stencils and metric of a block with non-periodic faces
 // (bc_core.py:15-16, opensblifunctions.py:523-534, metric.py:72-212).
@@ -493,7 +515,7 @@ __device__ __forceinline__ double recon_pair(const double fp[6], const double fm
 // Euler flux vector.
 // ---------------------------------------------------------------------------
 template <int ND, int D, int CONV, int AVG>
-__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2]) {
+__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2], double ct) {
   constexpr int NV = ND + 2;
   Avg<ND> v;
   interface_state<ND, AVG>(v, w[2], w[3], ph);
@@ -524,7 +546,7 @@ __device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, 
       fp[p] = 0.5 * (cf + lam * cs);
       fm[p] = 0.5 * (cf - lam * cs);
     }
-    const double rc = recon_pair<CONV>(fp, fm, ph.eps, ph.teno_ct);
+    const double rc = recon_pair<CONV>(fp, fm, ph.eps, ct);
     double r[NV];
     rcol<ND, D, K>(v, ph, r);
     sfor<0, NV>([&](auto mc) {

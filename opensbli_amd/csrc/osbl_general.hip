@@ -16,15 +16,16 @@ namespace osbl {
 
 // Derivative in the computational coordinate (without 1/Delta) at the point
 // x, grid index c of N along direction d (stride st).  ORDER 1 or 2.  f(y)
-// evaluates the differentiated expression at storage offset y.  Side-0 rows
-// are tested first, as in the reference's Piecewise.
+// evaluates the differentiated expression at storage offset y.  A closure row
+// replaces the stencil only at its own grid index (side-0 rows tested first,
+// as in the reference's Piecewise); halo points keep the central stencil.
 template <int ORDER, class F>
 __device__ __forceinline__ double dxi(const Bnd& b, int d, int c, int N, int x, int st, F&& f) {
   const int lo = b.mod[d][0], hi = b.mod[d][1];
   if constexpr (ORDER == 1) {
     if (lo) {
       const int nr = lo == 1 ? 4 : 2, np = lo == 1 ? 6 : 5;
-      if (c < nr) {
+      if (c >= 0 && c < nr) {
         const double* w = b.cw + (lo == 1 ? CW_CARP : CW_RA) + 6 * c;
         double acc = 0.0;
         for (int k = 0; k < np; ++k) acc += w[k] * f(x + (k - c) * st);
@@ -34,7 +35,7 @@ __device__ __forceinline__ double dxi(const Bnd& b, int d, int c, int N, int x, 
     if (hi) {
       const int nr = hi == 1 ? 4 : 2, np = hi == 1 ? 6 : 5;
       const int r = N - 1 - c;
-      if (r < nr) {
+      if (r >= 0 && r < nr) {
         const double* w = b.cw + (hi == 1 ? CW_CARP : CW_RA) + 6 * r;
         double acc = 0.0;
         for (int k = 0; k < np; ++k) acc += w[k] * f(x + (r - k) * st);
@@ -43,13 +44,13 @@ __device__ __forceinline__ double dxi(const Bnd& b, int d, int c, int N, int x, 
     }
     return (1.0 / 12.0) * (f(x - 2 * st) - f(x + 2 * st)) + (2.0 / 3.0) * (f(x + st) - f(x - st));
   } else {
-    if (lo && c < 2) {
+    if (lo && c >= 0 && c < 2) {
       const double* w = b.cw + CW_D2 + 6 * c;
       double acc = 0.0;
       for (int k = 0; k < 5; ++k) acc += w[k] * f(x + (k - c) * st);
       return acc;
     }
-    if (hi && N - 1 - c < 2) {
+    if (hi && N - 1 - c >= 0 && N - 1 - c < 2) {
       const int r = N - 1 - c;
       const double* w = b.cw + CW_D2 + 6 * r;
       double acc = 0.0;
@@ -286,6 +287,61 @@ __global__ void __launch_bounds__(256) k_prims_mu(const double* __restrict__ Q, 
   P[ND * g.fs + x] = p;
   P[(ND + 1) * g.fs + x] = T;
   P[(ND + 2) * g.fs + x] = pow(T, 1.5) * (1.0 + ph.suth) / (T + ph.suth);
+}
+
+// Modified Ducros sensor (shock_sensors.py:12-49) as a constituent relation:
+// theta = 1/2 (1 - tanh(2.5 (1 + 100 div u))) div^2 / (div^2 + |curl u|^2 + epsilon)
+// over the interior and one halo point on each side (the sweeps read it at
+// every interface's left point, interfaces -1/2 .. N-1/2), with the block's
+// central derivatives (closures, metric; halo points of a stretched
+// direction take the mirrored metric of metric.py:359-379).
+template <int ND>
+__global__ void __launch_bounds__(256) k_sensor(const double* __restrict__ Q, double* __restrict__ TH, Grid g,
+                                                Phys ph, Bnd b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
+  const int j = ND >= 2 ? (int)blockIdx.y - 1 : 0;
+  const int k = ND >= 3 ? (int)blockIdx.z - 1 : 0;
+  if (i > g.n[0]) return;
+  const int x = g.idx(i, j, k);
+  const int st[3] = {1, g.pitch, g.plane};
+  const int ci[3] = {i, j, k};
+  double du[3][3] = {{0.0}};
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double G = 1.0;
+    if (b.dm[d]) {
+      int c = ci[d];
+      const int N = g.n[d];
+      if (c < 0) c = -c;
+      if (c > N - 1) c = 2 * (N - 1) - c;
+      G = b.dm[d][c];
+    }
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      const double* qm = Q + (1 + m) * g.fs;
+      const double v = ph.inv_delta[d] * dxi<1>(b, d, ci[d], g.n[d], x, st[d], [&](int y) { return qm[y] / Q[y]; });
+      du[m][d] = G * v;
+    }
+  }
+  double div = 0.0, w2;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) div += du[d][d];
+  if constexpr (ND == 2) {
+    const double w = du[1][0] - du[0][1];
+    w2 = w * w;
+  } else {
+    const double wa = du[2][1] - du[1][2], wb = du[0][2] - du[2][0], wc = du[1][0] - du[0][1];
+    w2 = wa * wa + wb * wb + wc * wc;
+  }
+  const double filt = 0.5 * (1.0 - tanh(2.5 * (1.0 + 100.0 * div)));
+  TH[x] = filt * (div * div) / (div * div + w2 + ph.sensor_eps);
+}
+
+void launch_sensor(int nd, const double* Q, double* theta, const Grid& g, const Phys& ph, const Bnd& b,
+                   hipStream_t s) {
+  dim3 grid((g.n[0] + 2 + 255) / 256, nd >= 2 ? g.n[1] + 2 : 1, nd >= 3 ? g.n[2] + 2 : 1);
+  if (nd == 3) k_sensor<3><<<grid, 256, 0, s>>>(Q, theta, g, ph, b);
+  else k_sensor<2><<<grid, 256, 0, s>>>(Q, theta, g, ph, b);
 }
 
 template <int ND, int CONV, bool VISC>

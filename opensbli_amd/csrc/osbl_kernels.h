@@ -9,7 +9,8 @@ void launch_halo_periodic(double* Q, int nv, const Grid& g, hipStream_t s);
 void launch_extrap0(double* Q, int nv, const Grid& g, int d, int side, hipStream_t s);
 void launch_prims(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
-                  const double* met, bool acc, hipStream_t s);
+                  const double* met, const double* theta, bool acc, hipStream_t s);
+void launch_sensor(int nd, const double* Q, double* theta, const Grid& g, const Phys& ph, const Bnd& b, hipStream_t s);
 void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
                    double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s);
 void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g, const Phys& ph,

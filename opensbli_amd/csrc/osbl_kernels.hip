@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(256) k_prims(const double* __restrict__ Q, dou
 template <int ND, int D, int CONV, int AVG, bool ACC>
 __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q, double* __restrict__ R, Grid g,
                                                      Phys ph, int seg, long long total,
-                                                     const double* __restrict__ met) {
+                                                     const double* __restrict__ met,
+                                                     const double* __restrict__ theta) {
   constexpr int NV = ND + 2;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -155,7 +156,9 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
     finish_pt<ND>(w[5], ph);
     if (f + 1 < c1) fetch_q<ND>(nxt, Q, g.fs, off + 6 * st);
     double fh[NV];
-    char_flux<ND, D, CONV, AVG>(w, ph, fh);
+    // interface between window points 2 and 3: its left point is at off + 2 st
+    const double ct = theta ? adaptive_ct(theta[off + 2 * st], ph) : ph.teno_ct;
+    char_flux<ND, D, CONV, AVG>(w, ph, fh, ct);
     if (f >= c0) {
       const int x = off + 2 * st;
       // a stretched direction scales the flux difference by D_dd (opensblifunctions.py:607-616)
@@ -176,99 +179,6 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
 }
 
 // ---------------------------------------------------------------------------
-// Characteristic sweep along x for ND >= 2.  A wave owns 64 consecutive grid
-// lines (y,z rows) and one x segment of each; every lane slides the 6-point
-// window along its own row.  Rows are strided in memory, so the conservative
-// variables are staged through LDS in chunks of XC points: one chunk is a
-// 64-row x XC-point tile per variable, read from HBM in 64-byte row pieces
-// and stored column-major (lane = row) so that each lane's reads from LDS are
-// bank-conflict free.
-// ---------------------------------------------------------------------------
-constexpr int XC = 8;         // points per chunk
-constexpr int XLD = 65;       // padded column stride (doubles) of the LDS tile
-
-template <int ND, int CONV, int AVG, bool ACC>
-__global__ void __launch_bounds__(64, 2) k_char_sweep_x(const double* __restrict__ Q, double* __restrict__ R, Grid g,
-                                                      Phys ph, int seg, int nrows, const double* __restrict__ met) {
-  constexpr int NV = ND + 2;
-  __shared__ double tile[NV][XC * XLD];
-  const int lane = threadIdx.x;
-  const int row0 = blockIdx.x * 64;
-  const int row = row0 + lane;
-  const bool active = row < nrows;
-  const int c0 = blockIdx.y * seg;
-  const int c1 = min(c0 + seg, g.n[0]);
-  const int cb = c0 - 3;           // first point of the window sequence
-  const int last = c1 + 2;         // last point needed
-  // row r -> (j, k)
-  auto row_base = [&](int r) { return g.idx(0, r % g.n[1], r / g.n[1]); };
-  const int my_base = active ? row_base(row) : 0;
-
-  auto load_chunk = [&](int first) {
-    // 64 rows x XC points x NV variables; lane -> (row slot = lane/XC + 8*t, col = lane%XC)
-    const int col = lane % XC;
-    const int p = first + col;
-#pragma unroll
-    for (int t = 0; t < 64 / (64 / XC); ++t) {
-      const int rs = lane / XC + (64 / XC) * t;
-      const int r = row0 + rs;
-      if (r < nrows && p <= last) {
-        const int off = row_base(r) + p;
-#pragma unroll
-        for (int m = 0; m < NV; ++m) tile[m][col * XLD + rs] = Q[m * g.fs + off];
-      }
-    }
-  };
-  auto pt_from_tile = [&](Pt<ND>& s, int col) {
-#pragma unroll
-    for (int m = 0; m < NV; ++m) s.q[m] = tile[m][col * XLD + lane];
-    const double ir = 1.0 / s.q[0];
-    double ke = 0.0;
-#pragma unroll
-    for (int m = 0; m < ND; ++m) {
-      s.u[m] = s.q[1 + m] * ir;
-      ke += s.u[m] * s.u[m];
-    }
-    s.p = ph.gm1 * (s.q[ND + 1] - 0.5 * s.q[0] * ke);
-    s.a = sqrt(ph.gama * s.p * ir);
-  };
-
-  const double inv = ph.inv_delta[0];
-  Pt<ND> w[6];
-  load_chunk(cb);
-  __syncthreads();
-#pragma unroll
-  for (int p = 0; p < 5; ++p) pt_from_tile(w[p], p);
-  int j = 5;  // next point (relative to cb) to bring into the window
-  double fprev[NV];
-  for (int f = c0 - 1; f < c1; ++f) {
-    if (j % XC == 0) {
-      __syncthreads();
-      load_chunk(cb + j);
-      __syncthreads();
-    }
-    pt_from_tile(w[5], j % XC);
-    ++j;
-    double fh[NV];
-    char_flux<ND, 0, CONV, AVG>(w, ph, fh);
-    if (f >= c0 && active) {
-      const int x = my_base + f;
-      const double gm = met ? met[f] : 1.0;
-#pragma unroll
-      for (int m = 0; m < NV; ++m) {
-        const double v = met ? -(gm * ((fh[m] - fprev[m]) * inv)) : -(fh[m] - fprev[m]) * inv;
-        if constexpr (ACC) R[m * g.fs + x] += v;
-        else R[m * g.fs + x] = v;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < NV; ++m) fprev[m] = fh[m];
-#pragma unroll
-    for (int p = 0; p < 5; ++p) w[p] = w[p + 1];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Characteristic sweep along x with the lanes of a wave along x.  A wave owns
 // one grid line (one (y, z) row) and walks it in chunks of 64 cells: the
 // chunk's 69 points (x0-2 .. x0+66) are loaded with coalesced reads, their
@@ -285,7 +195,8 @@ constexpr int XP = 64 + 5;        // points staged per chunk
 template <int ND, int CONV, int AVG, bool ACC>
 __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restrict__ Q, double* __restrict__ R,
                                                           Grid g, Phys ph, int nrows, int periodic,
-                                                          const double* __restrict__ met) {
+                                                          const double* __restrict__ met,
+                                                          const double* __restrict__ theta) {
   constexpr int NV = ND + 2;
   constexpr int NF = NV + ND + 2;  // q, u, p, a
   __shared__ double sh[XW][NF][XP];
@@ -315,13 +226,14 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
     p.a = S[NV + ND + 1][slot];
   };
   // flux at interface x0 + lane + 1/2 (points x0+lane-2 .. x0+lane+3 = slots lane .. lane+5)
-  auto flux = [&](double fh[NV]) {
+  const int xmax = N + g.h[0] - 1;  // last stored point of the line
+  auto flux = [&](double fh[NV], int xi) {
     Pt<ND> w[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) get_point(w[q], lane + q);
-    char_flux<ND, 0, CONV, AVG>(w, ph, fh);
+    const double ct = theta ? adaptive_ct(theta[base + min(xi, xmax)], ph) : ph.teno_ct;
+    char_flux<ND, 0, CONV, AVG>(w, ph, fh, ct);
   };
-  const int xmax = N + g.h[0] - 1;  // last stored point of the line
   auto stage_chunk = [&](int x0) {
     // slots 0..68 hold points x0-2 .. x0+66 (clamped to the stored line: the
     // clamped slots only feed lanes whose cells lie beyond the line)
@@ -347,14 +259,14 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
     // flux at interface -1/2 (points -3..2): lane 0 of a chunk staged at x0 = -1
     stage_chunk(-1);
     double fh[NV];
-    flux(fh);
+    flux(fh, -1 + lane);
 #pragma unroll
     for (int m = 0; m < NV; ++m) prev[m] = __shfl(fh[m], 0);
   }
   for (int x0 = 0; x0 < N; x0 += 64) {
     stage_chunk(x0);
     double fh[NV];
-    flux(fh);
+    flux(fh, x0 + lane);
     const int x = x0 + lane;
     double left[NV];
 #pragma unroll
@@ -783,13 +695,14 @@ __global__ void __launch_bounds__(256) k_unpack(const double* __restrict__ in, d
 // ---------------------------------------------------------------------------
 template <int ND, int D, int CONV, int AVG>
 static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
-                           const double* met,
-                           hipStream_t s) {
+                           const double* met, const double* theta, hipStream_t s) {
   if constexpr (D == 0 && ND >= 2) {
     const int nrows = g.n[1] * g.n[2];
     const unsigned blocks = (unsigned)((nrows + XW - 1) / XW);
-    if (acc) k_char_sweep_xl<ND, CONV, AVG, true><<<blocks, 64 * XW, 0, s>>>(Q, R, g, ph, nrows, g.periodic_x, met);
-    else k_char_sweep_xl<ND, CONV, AVG, false><<<blocks, 64 * XW, 0, s>>>(Q, R, g, ph, nrows, g.periodic_x, met);
+    if (acc)
+      k_char_sweep_xl<ND, CONV, AVG, true><<<blocks, 64 * XW, 0, s>>>(Q, R, g, ph, nrows, g.periodic_x, met, theta);
+    else
+      k_char_sweep_xl<ND, CONV, AVG, false><<<blocks, 64 * XW, 0, s>>>(Q, R, g, ph, nrows, g.periodic_x, met, theta);
     return;
   }
   long long lines = 1;
@@ -798,42 +711,39 @@ static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys
   const long long nseg = (g.n[D] + seg - 1) / seg;
   const long long total = lines * nseg;
   const unsigned blocks = (unsigned)((total + 255) / 256);
-  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met);
-  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met);
+  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met, theta);
+  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met, theta);
 }
 
 template <int ND, int D, int CONV>
 static void launch_sweep_avg(int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg, bool acc,
-                             const double* met,
-                             hipStream_t s) {
-  if (avg == OSBL_AVG_ROE) launch_sweep_t<ND, D, CONV, OSBL_AVG_ROE>(Q, R, g, ph, seg, acc, met, s);
-  else launch_sweep_t<ND, D, CONV, OSBL_AVG_SIMPLE>(Q, R, g, ph, seg, acc, met, s);
+                             const double* met, const double* theta, hipStream_t s) {
+  if (avg == OSBL_AVG_ROE) launch_sweep_t<ND, D, CONV, OSBL_AVG_ROE>(Q, R, g, ph, seg, acc, met, theta, s);
+  else launch_sweep_t<ND, D, CONV, OSBL_AVG_SIMPLE>(Q, R, g, ph, seg, acc, met, theta, s);
 }
 
 template <int ND, int D>
 static void launch_sweep_conv(int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
-                              const double* met,
-                              bool acc, hipStream_t s) {
+                              const double* met, const double* theta, bool acc, hipStream_t s) {
   switch (conv) {
-    case OSBL_CONV_WENO_JS: launch_sweep_avg<ND, D, OSBL_CONV_WENO_JS>(avg, Q, R, g, ph, seg, acc, met, s); break;
-    case OSBL_CONV_WENO_Z: launch_sweep_avg<ND, D, OSBL_CONV_WENO_Z>(avg, Q, R, g, ph, seg, acc, met, s); break;
-    case OSBL_CONV_TENO5: launch_sweep_avg<ND, D, OSBL_CONV_TENO5>(avg, Q, R, g, ph, seg, acc, met, s); break;
-    default: launch_sweep_avg<ND, D, OSBL_CONV_TENO6>(avg, Q, R, g, ph, seg, acc, met, s); break;
+    case OSBL_CONV_WENO_JS: launch_sweep_avg<ND, D, OSBL_CONV_WENO_JS>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
+    case OSBL_CONV_WENO_Z: launch_sweep_avg<ND, D, OSBL_CONV_WENO_Z>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
+    case OSBL_CONV_TENO5: launch_sweep_avg<ND, D, OSBL_CONV_TENO5>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
+    default: launch_sweep_avg<ND, D, OSBL_CONV_TENO6>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
   }
 }
 
 void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
-                  const double* met,
-                  bool acc, hipStream_t s) {
+                  const double* met, const double* theta, bool acc, hipStream_t s) {
   if (nd == 3) {
-    if (d == 0) launch_sweep_conv<3, 0>(conv, avg, Q, R, g, ph, seg, met, acc, s);
-    else if (d == 1) launch_sweep_conv<3, 1>(conv, avg, Q, R, g, ph, seg, met, acc, s);
-    else launch_sweep_conv<3, 2>(conv, avg, Q, R, g, ph, seg, met, acc, s);
+    if (d == 0) launch_sweep_conv<3, 0>(conv, avg, Q, R, g, ph, seg, met, theta, acc, s);
+    else if (d == 1) launch_sweep_conv<3, 1>(conv, avg, Q, R, g, ph, seg, met, theta, acc, s);
+    else launch_sweep_conv<3, 2>(conv, avg, Q, R, g, ph, seg, met, theta, acc, s);
   } else if (nd == 2) {
-    if (d == 0) launch_sweep_conv<2, 0>(conv, avg, Q, R, g, ph, seg, met, acc, s);
-    else launch_sweep_conv<2, 1>(conv, avg, Q, R, g, ph, seg, met, acc, s);
+    if (d == 0) launch_sweep_conv<2, 0>(conv, avg, Q, R, g, ph, seg, met, theta, acc, s);
+    else launch_sweep_conv<2, 1>(conv, avg, Q, R, g, ph, seg, met, theta, acc, s);
   } else {
-    launch_sweep_conv<1, 0>(conv, avg, Q, R, g, ph, seg, met, acc, s);
+    launch_sweep_conv<1, 0>(conv, avg, Q, R, g, ph, seg, met, theta, acc, s);
   }
 }
 

@@ -180,6 +180,7 @@ struct osbl_ctx {
   double* R = nullptr;
   double* P = nullptr;
   double* stage_buf = nullptr;  // contiguous interior staging for host copies
+  double* Theta = nullptr;      // Ducros sensor (adaptive TENO)
   int nstages = 0;
   double A[5]{}, B[5]{};
   int seg = 32;
@@ -299,10 +300,12 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
       else launch_prims(c->nd, Q, c->P, c->g, c->ph, c->stream);
     });
   }
+  if (c->Theta) timed(c, F_PRIMS, [&] { launch_sensor(c->nd, Q, c->Theta, c->g, c->ph, c->bnd, c->stream); });
   if (!is_central(conv)) {
     for (int d = 0; d < c->nd; ++d) {
       timed(c, F_SWEEP_X + d, [&] {
-        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[d], d > 0, c->stream);
+        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[d], c->Theta, d > 0,
+                     c->stream);
       });
     }
   }
@@ -364,6 +367,11 @@ int check_config(const osbl_config* cfg) {
                               "central stencils on both faces", d);
   }
   if (cfg->gama <= 1.0) return fail(OSBL_E_ARG, "gama must exceed 1");
+  if (cfg->sensor) {
+    if (cfg->conv != OSBL_CONV_TENO5 && cfg->conv != OSBL_CONV_TENO6)
+      return fail(OSBL_E_ARG, "the adaptive cut-off (Ducros sensor) applies to the TENO schemes");
+    if (cfg->ndim < 2) return fail(OSBL_E_ARG, "the Ducros sensor needs a 2-D or 3-D block");
+  }
   if (cfg->viscous != OSBL_VISC_NONE && (cfg->Re <= 0.0 || cfg->Pr <= 0.0 || cfg->Minf <= 0.0))
     return fail(OSBL_E_ARG, "viscous terms need positive Re, Pr and Minf");
   return OSBL_OK;
@@ -422,6 +430,9 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   ph.inv_re = cfg->viscous != OSBL_VISC_NONE ? 1.0 / cfg->Re : 0.0;
   ph.cq = cfg->viscous != OSBL_VISC_NONE ? 1.0 / ((cfg->gama - 1.0) * cfg->Minf * cfg->Minf * cfg->Pr * cfg->Re) : 0.0;
   ph.suth = cfg->viscous == OSBL_VISC_SUTHERLAND ? cfg->suthT / cfg->refT : 0.0;
+  ph.teno_a1 = cfg->teno_a1;
+  ph.teno_a2 = cfg->teno_a2;
+  ph.sensor_eps = cfg->sensor_eps;
   c->nstages = rk_coefficients(cfg->rk, c->A, c->B);
   c->suth = cfg->viscous == OSBL_VISC_SUTHERLAND;
   bool closures = false, stretched = false;
@@ -450,6 +461,10 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
     return rc;
   }
   if (c->need_prims && (rc = alloc(&c->P, (size_t)fs * (c->nd + 3) * sizeof(double)))) {
+    osbl_destroy(c);
+    return rc;
+  }
+  if (cfg->sensor && (rc = alloc(&c->Theta, (size_t)fs * sizeof(double)))) {
     osbl_destroy(c);
     return rc;
   }
@@ -492,6 +507,7 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->Mbuf);
   (void)hipFree(c->pk);
   (void)hipFree(c->cw_dev);
+  (void)hipFree(c->Theta);
   for (int d = 0; d < 3; ++d) {
     (void)hipFree(c->met_dev[d]);
     for (int s = 0; s < 2; ++s) (void)hipFree(c->plane_dev[d][s]);

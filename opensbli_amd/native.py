@@ -54,6 +54,11 @@ class OsblConfig(ctypes.Structure):
         ("suthT", ctypes.c_double),
         ("refT", ctypes.c_double),
         ("twall", ctypes.c_double),
+        ("sensor", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
+        ("teno_a1", ctypes.c_double),
+        ("teno_a2", ctypes.c_double),
+        ("sensor_eps", ctypes.c_double),
     ]
 
 
@@ -151,6 +156,10 @@ def make_config(p):
     c.suthT = p.get("SuthT", 110.4)
     c.refT = p.get("RefT", 288.0)
     c.twall = p.get("Twall", 1.0)
+    c.sensor = 1 if p.get("sensor") else 0
+    c.teno_a1 = p.get("teno_a1", 10.5)
+    c.teno_a2 = p.get("teno_a2", 4.5)
+    c.sensor_eps = p.get("sensor_eps", 1.0e-30)
     return c
 
 

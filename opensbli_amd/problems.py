@@ -143,7 +143,8 @@ def stretched_coordinate(n, length, factor, kind="sinh"):
 
 def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950.0, Pr=0.72, gama=1.4,
                Tinf=288.0, SuthT=110.4, stretch_factor=5.0, stretch="sinh", dt=0.025, conv="weno_z", rk="ls3",
-               noise=1e-4, seed=5, bscheme="reduced_access", wall="adiabatic_wall", Twall=None, ndim=None):
+               noise=1e-4, seed=5, bscheme="reduced_access", wall="adiabatic_wall", Twall=None, ndim=None,
+               sensor=False, teno_a1=10.5, teno_a2=4.5, sensor_eps=1.0e-30):
     """Laminar flat-plate boundary layer, wall-normal stretched grid.
 
     x0 streamwise (inlet with pressure extrapolation, zeroth-order outlet),
@@ -153,8 +154,10 @@ def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950
     interior velocity carries N(0, noise) perturbations from
     numpy.random.RandomState(seed).  Viscosity follows Sutherland's law
     (katzer_SBLI.py:26), the convective flux is the characteristic LLF
-    scheme `conv` with Roe averaging, the viscous terms are Central(4) with the
-    metric of the stretched direction (katzer_SBLI.py:37-70).
+    scheme `conv` with Roe averaging (sensor=True: TENO with the Ducros sensor
+    and the adaptive cut-off, katzer_SBLI.py:55-62), the viscous terms are
+    Central(4) with the metric of the stretched direction
+    (katzer_SBLI.py:37-70).
 
     Returns (problem, q0); the problem carries the wall-normal coordinates
     ("coords") and the planes of the inlet and of the Dirichlet top.
@@ -199,4 +202,6 @@ def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950
                    bscheme=bs, stretch=[0, 1, 0][:nd] + [0] * (3 - nd), gama=gama, Minf=Minf, Re=Re, Pr=Pr, dt=dt,
                    eps=1e-16, teno_ct=1e-6, SuthT=SuthT, RefT=Tinf, Twall=bl.Twall if Twall is None else Twall,
                    coords={1: y}, planes={(0, 0): inlet, (1, 1): top})
+    if sensor:
+        problem.update(sensor=1, teno_a1=teno_a1, teno_a2=teno_a2, sensor_eps=sensor_eps)
     return problem, q0

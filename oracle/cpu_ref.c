@@ -41,7 +41,7 @@ struct ref_state {
   double* q[MAXV];
   double* qold[MAXV];  /* Q_RKold (3-register) or tempRK (low storage) */
   double* res[MAXV];
-  double *u[3], *p, *T, *a, *mu;
+  double *u[3], *p, *T, *a, *mu, *theta;
   double* wk[NWK];
   int mod[3][2];      /* 1 + REF_BS_* where the face's BC replaces the central stencils, else 0 */
   double* dm[3];      /* metric D_dd of a stretched direction (n[d] values), else NULL */
@@ -166,12 +166,13 @@ int ref_closure_weights(int scheme, int order, double* w) {
 
 /* First derivative D_xi f (without 1/Delta) at the point x, grid index c of N
    along a line of stride st, with the closures lo/hi (1 + REF_BS_*, or 0).
-   Side-0 rows are tested first, as in the Piecewise of
-   opensblifunctions.py:523-534. */
+   A closure row replaces the stencil only at its own grid index (the
+   Equality(idx, loc) conditions of opensblifunctions.py:523-534, side 0
+   tested first); halo points keep the central stencil. */
 static double d1_gen(const double* f, ptrdiff_t x, ptrdiff_t st, int N, int lo, int hi, int c) {
   if (lo) {
     const int nr = lo - 1 == REF_BS_CARPENTER ? 4 : 2, np = lo - 1 == REF_BS_CARPENTER ? 6 : 5;
-    if (c < nr) {
+    if (c >= 0 && c < nr) {
       const double* w = lo - 1 == REF_BS_CARPENTER ? CARP1[c] : RA1[c];
       double acc = 0.0;
       for (int k = 0; k < np; ++k) acc += w[k] * f[x + (k - c) * st];
@@ -181,7 +182,7 @@ static double d1_gen(const double* f, ptrdiff_t x, ptrdiff_t st, int N, int lo, 
   if (hi) {
     const int nr = hi - 1 == REF_BS_CARPENTER ? 4 : 2, np = hi - 1 == REF_BS_CARPENTER ? 6 : 5;
     const int r = N - 1 - c;
-    if (r < nr) {
+    if (r >= 0 && r < nr) {
       const double* w = hi - 1 == REF_BS_CARPENTER ? CARP1[r] : RA1[r];
       double acc = 0.0;
       for (int k = 0; k < np; ++k) acc += w[k] * f[x + (r - k) * st];
@@ -192,12 +193,12 @@ static double d1_gen(const double* f, ptrdiff_t x, ptrdiff_t st, int N, int lo, 
 }
 /* Homogeneous second derivative D_xixi f (without 1/Delta^2) */
 static double d2_gen(const double* f, ptrdiff_t x, ptrdiff_t st, int N, int lo, int hi, int c) {
-  if (lo && c < 2) {
+  if (lo && c >= 0 && c < 2) {
     double acc = 0.0;
     for (int k = 0; k < 5; ++k) acc += BC2[c][k] * f[x + (k - c) * st];
     return acc;
   }
-  if (hi && N - 1 - c < 2) {
+  if (hi && N - 1 - c >= 0 && N - 1 - c < 2) {
     const int r = N - 1 - c;
     double acc = 0.0;
     for (int k = 0; k < 5; ++k) acc += BC2[r][k] * f[x + (r - k) * st];
@@ -235,6 +236,8 @@ static void deriv2(ref_state* s, const double* f, double* out, int d, range3 R) 
 /* generated code evaluates them over interior + halotype, which is a subset, */
 /* and values outside that subset are never read.                             */
 /* ------------------------------------------------------------------------- */
+static void ducros_sensor(ref_state* s);
+
 static void constituent_relations(ref_state* s) {
   const int nd = s->nd;
   const double g = s->c.gama, M2 = s->c.Minf * s->c.Minf;
@@ -254,6 +257,7 @@ static void constituent_relations(ref_state* s) {
     s->T[x] = pp * g * M2 / rho[x];
     s->a[x] = sqrt(g * pp / rho[x]);
   }
+  if (s->c.sensor) ducros_sensor(s);
   if (s->c.viscous == REF_VISC_SUTHERLAND) { /* Eq(mu, T**1.5*(1+SuthT/RefT)/(T+SuthT/RefT)), katzer_SBLI.py:26 */
     const double sr = s->c.suthT / s->c.refT;
     LOOP(R) {
@@ -263,8 +267,57 @@ static void constituent_relations(ref_state* s) {
   }
 }
 
-/* metric factor D_dd at grid index c of a stretched direction (1 otherwise) */
-static inline double met1(const ref_state* s, int d, int c) { return s->dm[d] ? s->dm[d][c] : 1.0; }
+/* metric factor D_dd at grid index c of a stretched direction (1 otherwise);
+   halo points take the mirrored values of the metric boundary kernel
+   (metric.py:359-379: D[-t] = D[t], D[N-1+t] = D[N-1-t]) */
+static inline double met1(const ref_state* s, int d, int c) {
+  if (!s->dm[d]) return 1.0;
+  const int N = s->n[d];
+  if (c < 0) c = -c;
+  if (c > N - 1) c = 2 * (N - 1) - c;
+  return s->dm[d][c];
+}
+
+/* Modified Ducros sensor as a constituent relation (shock_sensors.py:12-49):
+   theta = 1/2 (1 - tanh(2.5 (1 + 100 div u))) div^2 / (div^2 + |curl u|^2 + epsilon),
+   with the block's central derivatives (closures and metric included),
+   evaluated over the interior and three halo points. */
+static void ducros_sensor(ref_state* s) {
+  const int nd = s->nd;
+  range3 R = rng_all(s, 3);
+  LOOP(R) {
+    ptrdiff_t x = IX(s, i, j, k);
+    const int cc[3] = {i, j, k};
+    double du[3][3] = {{0.0}};
+    for (int m = 0; m < nd; ++m)
+      for (int d = 0; d < nd; ++d)
+        du[m][d] = met1(s, d, cc[d]) * ((1.0 / s->c.delta[d]) *
+                   d1_gen(s->u[m], x, s->st[d], s->n[d], s->mod[d][0], s->mod[d][1], cc[d]));
+    double div = 0.0, w2;
+    for (int d = 0; d < nd; ++d) div += du[d][d];
+    if (nd == 2) {
+      const double w = du[1][0] - du[0][1];
+      w2 = w * w;
+    } else {
+      const double wa = du[2][1] - du[1][2], wb = du[0][2] - du[2][0], wc = du[1][0] - du[0][1];
+      w2 = wa * wa + wb * wb + wc * wc;
+    }
+    const double filt = 0.5 * (1.0 - tanh(2.5 * (1.0 + 100.0 * div)));
+    s->theta[x] = filt * (div * div) / (div * div + w2 + s->c.sensor_eps);
+  }
+}
+
+/* adaptive TENO cut-off at an interface from the sensor at its left point
+   (teno.py:430-443): C_T = 1/10^floor(a1 - a2 (1 - (1-theta)^4 (1 + 4 theta))) */
+static double adaptive_ct(const ref_state* s, double m) {
+  const double t = 1.0 - m;
+  const double t4 = (t * t) * (t * t);
+  const double beta = s->c.teno_a1 - s->c.teno_a2 * (1.0 - t4 * (1.0 + 4.0 * m));
+  const int n = (int)floor(beta);
+  double r = 1.0; /* 10^|n| by products, exact for |n| <= 22 */
+  for (int i = 0; i < (n < 0 ? -n : n); ++i) r *= 10.0;
+  return n < 0 ? r : 1.0 / r;
+}
 
 /* ------------------------------------------------------------------------- */
 /* Central convective terms                                                  */
@@ -840,7 +893,8 @@ static void reconstruction_direction(ref_state* s, int d) {
         fp[pp] = 0.5 * (CF[kk][pp] + lmax[kk] * CS[kk][pp]);
         fm[pp] = 0.5 * (CF[kk][pp] - lmax[kk] * CS[kk][pp]);
       }
-      recon[kk] = ref_recon_side(conv, 1, fp, s->c.eps, s->c.teno_ct) + ref_recon_side(conv, -1, fm, s->c.eps, s->c.teno_ct);
+      const double ct = s->c.sensor ? adaptive_ct(s, s->theta[x]) : s->c.teno_ct;
+      recon[kk] = ref_recon_side(conv, 1, fp, s->c.eps, ct) + ref_recon_side(conv, -1, fm, s->c.eps, ct);
     }
     for (int m = 0; m < nv; ++m) {
       double fl = 0.0;
@@ -1146,6 +1200,7 @@ static void time_step(ref_state* s) {
 /* ------------------------------------------------------------------------- */
 int ref_create(const ref_config* cfg, ref_state** out) {
   if (cfg->ndim < 1 || cfg->ndim > 3) return -1;
+  if (cfg->sensor && cfg->ndim < 2) return -1; /* vorticity needs two directions */
   ref_state* s = (ref_state*)calloc(1, sizeof(ref_state));
   s->c = *cfg;
   s->nd = cfg->ndim;
@@ -1170,6 +1225,7 @@ int ref_create(const ref_config* cfg, ref_state** out) {
   s->T = (double*)calloc(1, bytes);
   s->a = (double*)calloc(1, bytes);
   s->mu = (double*)calloc(1, bytes);
+  s->theta = (double*)calloc(1, bytes);
   carpenter_init();
   for (int d = 0; d < s->nd; ++d)
     for (int side = 0; side < 2; ++side) {
@@ -1190,7 +1246,7 @@ void ref_destroy(ref_state* s) {
   if (!s) return;
   for (int m = 0; m < MAXV; ++m) { free(s->q[m]); free(s->qold[m]); free(s->res[m]); }
   for (int m = 0; m < 3; ++m) free(s->u[m]);
-  free(s->p); free(s->T); free(s->a); free(s->mu);
+  free(s->p); free(s->T); free(s->a); free(s->mu); free(s->theta);
   for (int w = 0; w < NWK; ++w) free(s->wk[w]);
   for (int d = 0; d < 3; ++d) {
     free(s->dm[d]); free(s->sd[d]);
@@ -1234,6 +1290,15 @@ int ref_set_coordinates(ref_state* s, int d, const double* xc) {
   s->sd[d] = (double*)calloc((size_t)N, sizeof(double));
   for (int c = 0; c < N; ++c) s->dm[d][c] = 1.0 / (inv * d1_gen(xc, c, 1, N, s->mod[d][0], s->mod[d][1], c));
   for (int c = 0; c < N; ++c) s->sd[d][c] = inv * d1_gen(s->dm[d], c, 1, N, s->mod[d][0], s->mod[d][1], c);
+  return 0;
+}
+
+int ref_get_sensor(const ref_state* s, double* v) {
+  if (!s->c.sensor) return -1;
+  ptrdiff_t c = 0;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) v[c++] = s->theta[IX(s, i, j, k)];
   return 0;
 }
 

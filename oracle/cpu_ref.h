@@ -54,6 +54,9 @@ typedef struct ref_config {
   int32_t pad_;
   double suthT, refT;    /* Sutherland constants SuthT, RefT */
   double twall;          /* Twall of the isothermal wall */
+  int32_t sensor;        /* 1: Ducros sensor theta with the adaptive TENO cut-off */
+  int32_t pad2_;
+  double teno_a1, teno_a2, sensor_eps;  /* teno_a1, teno_a2, epsilon of the sensor */
 } ref_config;
 
 typedef struct ref_state ref_state;
@@ -91,6 +94,8 @@ void ref_central_weights(double* w1, double* w2);
 /* boundary closure weights of the first (order 1: rows x 6) or second
    (order 2: 2 x 6) derivative at side 0 for scheme REF_BS_*; returns rows */
 int ref_closure_weights(int scheme, int order, double* w);
+/* the Ducros sensor of the last RHS evaluation over the interior (x fastest) */
+int ref_get_sensor(const ref_state* s, double* theta);
 /* metric arrays of direction d after ref_set_coordinates (n[d] values each) */
 int ref_get_metric(const ref_state* s, int d, double* dm, double* sd);
 /* RK coefficients of a temporal scheme: returns the number of stages */

@@ -43,6 +43,11 @@ class RefConfig(ctypes.Structure):
         ("suthT", ctypes.c_double),
         ("refT", ctypes.c_double),
         ("twall", ctypes.c_double),
+        ("sensor", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32),
+        ("teno_a1", ctypes.c_double),
+        ("teno_a2", ctypes.c_double),
+        ("sensor_eps", ctypes.c_double),
     ]
 
 
@@ -81,6 +86,7 @@ def lib(variant="plain"):
         L.ref_set_bc_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp]
         L.ref_closure_weights.argtypes = [ctypes.c_int, ctypes.c_int, dp]
         L.ref_get_metric.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+        L.ref_get_sensor.argtypes = [ctypes.c_void_p, dp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -118,6 +124,10 @@ def make_config(p):
     c.suthT = p.get("SuthT", 110.4)
     c.refT = p.get("RefT", 288.0)
     c.twall = p.get("Twall", 1.0)
+    c.sensor = 1 if p.get("sensor") else 0
+    c.teno_a1 = p.get("teno_a1", 10.5)
+    c.teno_a2 = p.get("teno_a2", 4.5)
+    c.sensor_eps = p.get("sensor_eps", 1.0e-30)
     return c
 
 
@@ -150,6 +160,12 @@ class RefSolver:
             a = np.ascontiguousarray(pl, dtype=np.float64)
             if self.L.ref_set_bc_plane(self.h, int(d), int(side), _ptr(a)) != 0:
                 raise RuntimeError(f"ref_set_bc_plane({d}, {side}) failed")
+
+    def sensor(self):
+        out = np.empty(self.shape)
+        if self.L.ref_get_sensor(self.h, _ptr(out)) != 0:
+            raise RuntimeError("no sensor on this block")
+        return out
 
     def metric(self, d):
         n = self.p["n"][d]

@@ -95,3 +95,14 @@ def test_facade_rejects_unsupported_boundaries():
         m.generate_transformations(2, "x", [(True, True), (True, True)], 2)
     with pytest.raises(ValueError):
         m.generate_transformations(2, "x", [(False, False)], 2)
+
+
+def test_flat_plate_adaptive_teno_script():
+    """The Katzer-style script with the Ducros sensor and adaptive TENO5."""
+    from opensbli_amd.apps import flat_plate as fp
+    from opensbli_amd.problems import flat_plate
+    prog = fp.build(n=(40, 32), scheme="teno5")
+    p = prog.problem()
+    ref_p, _ = flat_plate(n=(40, 32), ndim=2, noise=0.0, conv="teno5", sensor=True)
+    for k in ("conv", "sensor", "teno_a1", "teno_a2", "sensor_eps", "viscous", "stretch", "bc"):
+        assert p[k] == ref_p[k], k
