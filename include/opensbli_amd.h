@@ -72,6 +72,16 @@ extern "C" {
 #define OSBL_BS_CARPENTER 0      /* Carpenter() (Carpenter_scheme.py:6-102), the BCs' default */
 #define OSBL_BS_REDUCED_ACCESS 1 /* ReducedAccess() (reduced_access_scheme.py:6-83) */
 
+/* filters run after every full time step, in this order (UserDefinedEquations
+   placed InTheSimulation, algorithm.py:412-470); 2-D and 3-D blocks */
+#define OSBL_FILTER_WENO 1     /* WENOFilter(block, 5, ...) (WENO_filter.py:15-339) */
+#define OSBL_FILTER_SFD 2      /* SFD(block, chifilt, omegafilt) (SFD.py:27-86) */
+#define OSBL_FILTER_BINOMIAL 4 /* BinomialFilter(block, order, grid_condition, sigma) (binomial_filter.py:15-123) */
+
+/* dissipation sensor of the WENO filter (WENO_filter.py:265-275) */
+#define OSBL_WF_DUCROS 0   /* modified Ducros sensor, maximum over +-2 points per direction */
+#define OSBL_WF_CONSTANT 1 /* no flow sensor: shock_filter_control alone */
+
 /*
  * The problem definition, a flat mirror of the constants OpenSBLI declares
  * (ConstantsToDeclare, kernel.py:20-35; substitute_simulation_parameters,
@@ -90,13 +100,20 @@ typedef struct osbl_config {
   int32_t bscheme[3][2]; /* OSBL_BS_* per face (faces that modify the central stencils) */
   int32_t stretch[3];    /* 1: direction d is stretched along itself; coordinates via osbl_set_coordinates
                             (MetricsEquation.generate_transformations, metric.py:37-54) */
-  int32_t reserved;
+  int32_t filters;       /* OSBL_FILTER_* bitmask */
   double suthT, refT;    /* SuthT, RefT of the Sutherland law */
   double twall;          /* Twall of the isothermal wall */
   int32_t sensor;        /* 1: modified Ducros sensor with the adaptive TENO cut-off
                             (shock_sensors.py:12-49, teno.py:430-443, LLFTeno(formulation='adaptive')) */
   int32_t reserved2;
   double teno_a1, teno_a2, sensor_eps;  /* teno_a1, teno_a2, epsilon of the sensor */
+  int32_t binomial_order;  /* even order of the binomial filter, 2..8 (binomial_filter.py:16-24) */
+  int32_t wf_sensor;       /* OSBL_WF_*: dissipation_sensor of the WENO filter */
+  int32_t wf_mach;         /* 1: Mach_correction of the WENO filter (Yee sensor, WENO_filter.py:178-196) */
+  int32_t reserved3;
+  double sigma_filt;       /* sigma_filt: blending weight of the binomial filter */
+  double chi_filt, omega_filt;  /* chi_filt, omega_filt of SFD */
+  double shock_filter_control;  /* shock_filter_control: global scale of the WENO filter */
 } osbl_config;
 
 typedef struct osbl_ctx osbl_ctx;
@@ -163,6 +180,20 @@ int osbl_set_coordinates(osbl_ctx* ctx, int d, const double* x_host);
    face keeps in its halos (written now; the reference's initial-condition
    kernel covers the halos, katzer_init.py:233-245). */
 int osbl_set_bc_plane(osbl_ctx* ctx, int d, int side, const double* plane_host);
+
+/* Grid condition of the binomial filter: one byte per interior cell (x
+   fastest), nonzero where the filter applies -- the boolean expression the
+   reference evaluates from the coordinates (binomial_filter.py:67-75),
+   evaluated by the caller.  NULL applies it everywhere (the default). */
+int osbl_set_filter_mask(osbl_ctx* ctx, const uint8_t* mask_host);
+
+/* SFD filtered solution vfilt of one conservative variable (interior points,
+   x fastest): get for a restart file, set to restart from one ("write out
+   vfilt for restart with sfd", SFD.py:14-16).  Variables not set take the
+   state at the first osbl_run (the "Initialize the filter" kernel,
+   SFD.py:50-61). */
+int osbl_get_sfd_field(osbl_ctx* ctx, int var, double* host);
+int osbl_set_sfd_field(osbl_ctx* ctx, int var, const double* host);
 
 /* Wait for all work enqueued on the context's stream. */
 int osbl_sync(osbl_ctx* ctx);

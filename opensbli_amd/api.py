@@ -20,6 +20,7 @@ with one change: the code-generation target ``OPSC(alg)`` becomes
     GridBasedInitialisation            gridbasedinit.py:15
     Initialise_Katzer                  katzer_init.py:183
     TraditionalAlgorithmRK             algorithm.py:336
+    WENOFilter, BinomialFilter, SFD    WENO_filter.py:15, binomial_filter.py:15, SFD.py:27
     substitute_simulation_parameters   helperfunctions.py:130
 
 Instead of expanding the Einstein notation symbolically, HIPC recognises the
@@ -276,6 +277,66 @@ class ShockSensor:
         return [_SensorEquation(name, block.ndim)], DataObject(name)
 
 
+# ---------------------------------------------------------------------------
+# filters after the full time step (filters/*.py)
+# ---------------------------------------------------------------------------
+class _Filter:
+    def __init__(self):
+        # the script passes equation_classes to block.set_equations, as for
+        # the reference's filter objects
+        self.equation_classes = [self]
+
+
+class WENOFilter(_Filter):
+    """WENO non-linear filter (WENO_filter.py:15-46): the dissipative part of
+    a characteristic WENO5-JS reconstruction (SimpleAverage) against the
+    sixth-order central flux, scaled by the non-linear weights' sensor, a
+    dissipation sensor ('Ducros' or 'Constant', optionally Yee's low-Mach
+    correction) and the constant shock_filter_control; off within 6 points of
+    walls.  Applied after every full time step."""
+
+    def __init__(self, block, order, metrics=None, dissipation_sensor="Ducros", Mach_correction=False):
+        super().__init__()
+        if dissipation_sensor not in ("Ducros", "Constant"):
+            raise NotImplementedError("Please enter a valid dissipation sensor option: 'Ducros', or 'Constant'.")
+        if order != 5:
+            raise NotImplementedError("the HIPC target implements the WENO filter of order 5")
+        if block.ndim < 2:
+            raise ValueError("vorticity is not defined for one-dimensional cases.")
+        block.shock_filter = True
+        self.order = order
+        self.metrics = metrics
+        self.sensor = "ducros" if dissipation_sensor == "Ducros" else "constant"
+        self.mach_correction = bool(Mach_correction)
+
+
+class BinomialFilter(_Filter):
+    """Binomial filter of even order (binomial_filter.py:15-34): weights
+    binomial(N, i)/2^N per direction, averaged over the directions and blended
+    with weight sigma_filt, where grid_condition (a boolean expression of the
+    coordinates and grid indices) holds.  Applied last in every time step."""
+
+    def __init__(self, block, order, grid_condition=None, sigma=0.05):
+        super().__init__()
+        if order % 2 != 0:
+            raise ValueError("The filter is only defined for even orders n.")
+        if order > 10:
+            raise ValueError("Increase the number of halo points in scheme.py for high order filters.")
+        self.order = order
+        self.grid_condition = grid_condition
+        self.sigma = sigma
+
+
+class SFD(_Filter):
+    """Selective frequency damping (SFD.py:27-37): chi_filt, omega_filt;
+    the filtered solution starts as the initial state."""
+
+    def __init__(self, block, chifilt=0.1, omegafilt=1.0 / 0.75):
+        super().__init__()
+        self.chi = chifilt
+        self.omega = omegafilt
+
+
 class RungeKutta:
     """3-register RK3 (rk_sbli.py:16)."""
 
@@ -479,6 +540,7 @@ class SimulationBlock:
         self.discretisation_schemes = {}
         self.io = []
         self.discretised = False
+        self.shock_filter = False
 
     def set_block_boundaries(self, bclist):
         flat = flatten(list(bclist))
@@ -791,6 +853,21 @@ class HIPC:
                 if name not in v:
                     raise ValueError("simulation parameter %s has no value" % name)
                 p[k] = float(v[name])
+        filters = {}
+        for f in self.block.equation_classes:
+            if isinstance(f, WENOFilter):
+                if float(v.get("sensor_theta", 4.0)) != 4.0:
+                    raise NotImplementedError("the HIPC target implements sensor_theta = 4")
+                filters["weno"] = {"sensor": f.sensor, "mach_correction": f.mach_correction,
+                                   "control": float(v.get("shock_filter_control", 1.0))}
+                if f.sensor == "ducros":
+                    p["sensor_eps"] = float(v.get("epsilon", 1.0e-12))
+            elif isinstance(f, SFD):
+                filters["sfd"] = {"chi": float(v.get("chi_filt", f.chi)), "omega": float(v.get("omega_filt", f.omega))}
+            elif isinstance(f, BinomialFilter):
+                filters["binomial"] = {"order": f.order, "sigma": float(v.get("sigma_filt", f.sigma))}
+        if filters:
+            p["filters"] = filters
         if any(p["bc"][d][s] == "isothermal_wall" for d in range(nd) for s in range(2)):
             if "Twall" not in v:
                 raise ValueError("simulation parameter Twall has no value")
@@ -909,6 +986,19 @@ class HIPC:
                 planes[(d, s)] = np.ascontiguousarray(face.reshape(nd + 2, n2, n1))
         if planes:
             problem["planes"] = planes
+        binom = [f for f in self.block.equation_classes if isinstance(f, BinomialFilter)]
+        if binom and binom[0].grid_condition is not None:
+            # the grid condition over the interior, from the coordinates and grid indices
+            cond = sympy.sympify(binom[0].grid_condition)
+            syms = sorted(cond.free_symbols, key=lambda z: z.name)
+            look = dict(env)
+            look.update({sympy.Symbol(k): val for k, val in fields.items()})
+            missing = [z for z in syms if z not in look]
+            if missing:
+                raise ValueError("grid_condition uses undefined symbols %s" % missing)
+            f = sympy.lambdify(syms, cond, modules="numpy")
+            mask = np.broadcast_to(np.asarray(f(*[look[z] for z in syms]), dtype=bool), shape)
+            problem["filters"]["binomial"]["mask"] = np.ascontiguousarray(mask, dtype=np.uint8)
         return problem
 
     def run(self, niter=None, return_block=False):

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libosbl_hip.so")
 SOURCES = ["osbl_kernels.hip", "osbl_runtime.hip", "osbl_batch.hip", "osbl_adjoint.hip", "osbl_general.hip",
-           "osbl_boundary.hip", "osbl_adjoint_char.hip"]
+           "osbl_boundary.hip", "osbl_adjoint_char.hip", "osbl_filters.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OSBL_ARCH", "gfx950")
 FLAGS = [

@@ -423,6 +423,47 @@ __device__ __forceinline__ double weno5js(double fm2, double fm1, double f0, dou
   return (A0 * r0 + A1 * r1 + A2 * r2) / (A0 + A1 + A2);
 }
 
+// WENO5-JS as above, also returning the shock sensor of its non-linear
+// weights (weno.py:415-431, normalised, sensor_theta = 4):
+// rj = sum_r |omega_r/d_r - 1|^4 / (|1/min d - 1|^4 + 2) = (...)/6563.
+__device__ __forceinline__ double weno5js_sensor(double fm2, double fm1, double f0, double f1, double f2, double& rj) {
+  const double t0 = f0 - 2.0 * f1 + f2, s0 = 3.0 * f0 - 4.0 * f1 + f2;
+  const double t1 = fm1 - 2.0 * f0 + f1, s1 = fm1 - f1;
+  const double t2 = fm2 - 2.0 * fm1 + f0, s2 = fm2 - 4.0 * fm1 + 3.0 * f0;
+  constexpr double C13 = 13.0 / 12.0;
+  const double e0 = 1.0e-6 + (C13 * t0 * t0 + 0.25 * s0 * s0);
+  const double e1 = 1.0e-6 + (C13 * t1 * t1 + 0.25 * s1 * s1);
+  const double e2 = 1.0e-6 + (C13 * t2 * t2 + 0.25 * s2 * s2);
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double A0 = 0.3 * (q1 * q2), A1 = 0.6 * (q0 * q2), A2 = 0.1 * (q0 * q1);
+  const double ia = 1.0 / (A0 + A1 + A2);
+  const double w0 = A0 * ia, w1 = A1 * ia, w2 = A2 * ia;
+  const double z0 = w0 / 0.3 - 1.0, z1 = w1 / 0.6 - 1.0, z2 = w2 / 0.1 - 1.0;
+  const double y0 = z0 * z0, y1 = z1 * z1, y2 = z2 * z2;
+  rj = (y0 * y0 + y1 * y1 + y2 * y2) / 6563.0;
+  constexpr double S6 = 1.0 / 6.0;
+  const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
+  const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
+  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  return w0 * r0 + w1 * r1 + w2 * r2;
+}
+
+// Flux of the WENO non-linear filter for one characteristic field
+// (shock_capturing.py:82-96, 401-416, 438-443; WENO_filter.py:324-326): the
+// WENO-JS reconstructions of f+ and f- summed, minus the sixth-order central
+// flux (1, -8, 37, 37, -8, 1)/60 of the characteristic fluxes on points
+// -2..3, times rj = max(sensor(f+ side), sensor(f- side)).
+__device__ __forceinline__ double weno_filter_flux(const double fp[6], const double fm[6], const double cf[6]) {
+  double rr, rl;
+  const double rec = weno5js_sensor(fp[0], fp[1], fp[2], fp[3], fp[4], rr) +
+                     weno5js_sensor(fm[5], fm[4], fm[3], fm[2], fm[1], rl);
+  const double central = (cf[0] - 8.0 * cf[1] + 37.0 * cf[2] + 37.0 * cf[3] - 8.0 * cf[4] + cf[5]) / 60.0;
+  return fmax(rr, rl) * (rec - central);
+}
+
+// scheme tag of the WENO filter's characteristic sweeps (not a convective scheme)
+constexpr int CONV_WFILTER = 16;
+
 __device__ __forceinline__ double pow6(double x) {
   const double x2 = x * x;
   return x2 * x2 * x2;
@@ -532,6 +573,7 @@ __device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, 
     for (int p = 0; p < 6; ++p) lam = fmax(lam, abs_lambda<ND, D, K>(w[p]));
     // coefficient of p in L_k . F - u_D L_k . Q
     double fp[6], fm[6];
+    [[maybe_unused]] double cfa[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
       double cs = 0.0;
@@ -545,8 +587,11 @@ __device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, 
       const double cf = w[p].u[D] * cs + w[p].p * lp;
       fp[p] = 0.5 * (cf + lam * cs);
       fm[p] = 0.5 * (cf - lam * cs);
+      if constexpr (CONV == CONV_WFILTER) cfa[p] = cf;
     }
-    const double rc = recon_pair<CONV>(fp, fm, ph.eps, ct);
+    double rc;
+    if constexpr (CONV == CONV_WFILTER) rc = weno_filter_flux(fp, fm, cfa);
+    else rc = recon_pair<CONV>(fp, fm, ph.eps, ct);
     double r[NV];
     rcol<ND, D, K>(v, ph, r);
     sfor<0, NV>([&](auto mc) {

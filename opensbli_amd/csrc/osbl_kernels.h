@@ -34,6 +34,11 @@ void launch_pack_all(const double* F, double* out, const Grid& g, int nv, hipStr
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
                            double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, int seg,
                            bool acc, hipStream_t s);
+void launch_weno_filter_apply(int nd, double* Q, const double* R, double* theta, const Grid& g, const Phys& ph,
+                              bool ducros, bool mach, double control, int walls, hipStream_t s);
+void launch_sfd(int nd, double* Q, double* F, const Grid& g, double c, double E, hipStream_t s);
+void launch_binomial(int nd, const double* Q, double* Qn, const Grid& g, const double* w, int N, double sigma,
+                     const unsigned char* mask, hipStream_t s);
 void launch_lincomb(double* out, double a, const double* x, double b, const double* y, long long n, hipStream_t s);
 
 // record the thread-local message osbl_last_error() returns; returns code

@@ -729,6 +729,10 @@ static void launch_sweep_conv(int conv, int avg, const double* Q, double* R, con
     case OSBL_CONV_WENO_JS: launch_sweep_avg<ND, D, OSBL_CONV_WENO_JS>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
     case OSBL_CONV_WENO_Z: launch_sweep_avg<ND, D, OSBL_CONV_WENO_Z>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
     case OSBL_CONV_TENO5: launch_sweep_avg<ND, D, OSBL_CONV_TENO5>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
+    case CONV_WFILTER:
+      // the WENO filter reconstructs with SimpleAverage (WENO_filter.py:324)
+      if constexpr (ND >= 2) launch_sweep_t<ND, D, CONV_WFILTER, OSBL_AVG_SIMPLE>(Q, R, g, ph, seg, acc, met, theta, s);
+      break;
     default: launch_sweep_avg<ND, D, OSBL_CONV_TENO6>(avg, Q, R, g, ph, seg, acc, met, theta, s); break;
   }
 }

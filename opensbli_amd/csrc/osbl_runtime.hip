@@ -45,9 +45,9 @@ namespace {
     if (e_ != hipSuccess) return fail(OSBL_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-enum Family { F_BC = 0, F_PRIMS, F_SWEEP_X, F_SWEEP_Y, F_SWEEP_Z, F_UPDATE, F_COUNT };
+enum Family { F_BC = 0, F_PRIMS, F_SWEEP_X, F_SWEEP_Y, F_SWEEP_Z, F_UPDATE, F_FILTER, F_COUNT };
 const char* kFamilyNames[F_COUNT] = {"boundary", "constituent_relations", "char_sweep_x", "char_sweep_y",
-                                     "char_sweep_z", "residual_rk_update"};
+                                     "char_sweep_z", "residual_rk_update", "filters"};
 
 int rk_coefficients(int rk, double* a, double* b) {
   switch (rk) {
@@ -180,7 +180,13 @@ struct osbl_ctx {
   double* R = nullptr;
   double* P = nullptr;
   double* stage_buf = nullptr;  // contiguous interior staging for host copies
-  double* Theta = nullptr;      // Ducros sensor (adaptive TENO)
+  double* Theta = nullptr;      // Ducros sensor (adaptive TENO, WENO filter)
+  double* QF = nullptr;         // SFD filtered solution
+  bool sfd_set[5] = {};         // vfilt of variable m holds a value
+  unsigned char* fmask = nullptr;  // binomial filter grid condition (interior cells)
+  double binom_w[11]{};
+  int walls = 0;                // wall faces, bit 2d+side
+  double sfd_c = 0.0, sfd_E = 0.0;
   int nstages = 0;
   double A[5]{}, B[5]{};
   int seg = 32;
@@ -257,7 +263,13 @@ double* Qcur(osbl_ctx* c) { return c->Qbuf[c->cur]; }
 
 // halo depth a boundary kernel fills: the largest halo of the schemes,
 // -3/+4 for WENO5/TENO (weno.py:17-29), -2/+2 for Central(4) (bc_core.py:136-147)
-int bc_halos(const osbl_ctx* c, int side) { return is_central(c->cfg.conv) ? 2 : (side == 0 ? 3 : 4); }
+// halo points a non-periodic face fills: the block's boundary halos are those
+// of its schemes (bc_core.py:145-166), the characteristic reconstruction's
+// (3 below, 4 above) when the block has one -- the WENO filter included
+int bc_halos(const osbl_ctx* c, int side) {
+  const bool central = is_central(c->cfg.conv) && !(c->cfg.filters & OSBL_FILTER_WENO);
+  return central ? 2 : (side == 0 ? 3 : 4);
+}
 
 void apply_bcs(osbl_ctx* c, double* Q) {
   bool all_periodic = true;
@@ -374,6 +386,35 @@ int check_config(const osbl_config* cfg) {
   }
   if (cfg->viscous != OSBL_VISC_NONE && (cfg->Re <= 0.0 || cfg->Pr <= 0.0 || cfg->Minf <= 0.0))
     return fail(OSBL_E_ARG, "viscous terms need positive Re, Pr and Minf");
+  if (cfg->filters & ~(OSBL_FILTER_WENO | OSBL_FILTER_SFD | OSBL_FILTER_BINOMIAL))
+    return fail(OSBL_E_ARG, "unknown filter bits 0x%x", cfg->filters);
+  if (cfg->filters && cfg->ndim < 2)
+    return fail(OSBL_E_ARG, "the filters are defined for 2-D and 3-D blocks (their conservative variable lists)");
+  if (cfg->filters & OSBL_FILTER_WENO) {
+    if (cfg->wf_sensor != OSBL_WF_DUCROS && cfg->wf_sensor != OSBL_WF_CONSTANT)
+      return fail(OSBL_E_ARG, "unknown WENO filter dissipation sensor %d", cfg->wf_sensor);
+  }
+  if (cfg->filters & OSBL_FILTER_SFD) {
+    if (!(cfg->omega_filt != 0.0) || !(cfg->chi_filt / cfg->omega_filt != -1.0))
+      return fail(OSBL_E_ARG, "SFD needs omega_filt != 0 and chi_filt/omega_filt != -1");
+  }
+  if (cfg->filters & OSBL_FILTER_BINOMIAL) {
+    const int N = cfg->binomial_order;
+    if (N < 2 || N % 2) return fail(OSBL_E_ARG, "the binomial filter is defined for even orders (got %d)", N);
+    if (N > 8)
+      return fail(OSBL_E_ARG, "binomial order %d reads %d halo points; this build keeps at most 4", N, N / 2);
+    const bool central = is_central(cfg->conv) && !(cfg->filters & OSBL_FILTER_WENO);
+    for (int d = 0; d < cfg->ndim; ++d)
+      for (int s = 0; s < 2; ++s) {
+        const int bc = cfg->bc[d][s];
+        if (bc == OSBL_BC_PERIODIC) continue;  // the exchange fills every halo point
+        // the reference's boundary kernels fill the block's boundary halos only
+        const int fill = central ? 2 : (s == 0 ? 3 : 4);
+        if (N / 2 > fill)
+          return fail(OSBL_E_ARG, "binomial order %d reads %d halo points; face (%d, %d) fills %d", N, N / 2, d, s,
+                      fill);
+      }
+  }
   return OSBL_OK;
 }
 
@@ -398,7 +439,11 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   c->nd = cfg->ndim;
   c->nv = cfg->ndim + 2;
   HIPCHK(hipGetDevice(&c->device));
-  const int halo = is_central(cfg->conv) && cfg->viscous == OSBL_VISC_NONE ? 2 : 4;
+  // 2 halo points serve the central stencils, 4 the characteristic
+  // reconstructions (3 below, 4 above) and binomial filters above order 4
+  const bool narrow = is_central(cfg->conv) && cfg->viscous == OSBL_VISC_NONE && !(cfg->filters & OSBL_FILTER_WENO) &&
+                      !((cfg->filters & OSBL_FILTER_BINOMIAL) && cfg->binomial_order > 4);
+  const int halo = narrow ? 2 : 4;
   Grid& g = c->g;
   g.nd = c->nd;
   for (int d = 0; d < 3; ++d) {
@@ -464,9 +509,31 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
     osbl_destroy(c);
     return rc;
   }
-  if (cfg->sensor && (rc = alloc(&c->Theta, (size_t)fs * sizeof(double)))) {
+  const bool wf_ducros = (cfg->filters & OSBL_FILTER_WENO) && cfg->wf_sensor == OSBL_WF_DUCROS;
+  if ((cfg->sensor || wf_ducros) && (rc = alloc(&c->Theta, (size_t)fs * sizeof(double)))) {
     osbl_destroy(c);
     return rc;
+  }
+  if ((cfg->filters & OSBL_FILTER_SFD) && (rc = alloc(&c->QF, bundle))) {
+    osbl_destroy(c);
+    return rc;
+  }
+  for (int d = 0; d < c->nd; ++d)
+    for (int s = 0; s < 2; ++s)
+      if (cfg->bc[d][s] == OSBL_BC_ADIABATIC_WALL || cfg->bc[d][s] == OSBL_BC_ISOTHERMAL_WALL)
+        c->walls |= 1 << (2 * d + s);
+  if (cfg->filters & OSBL_FILTER_SFD) {
+    c->sfd_c = cfg->chi_filt / cfg->omega_filt;
+    c->sfd_E = std::exp(-(cfg->chi_filt + cfg->omega_filt) * cfg->dt);
+  }
+  if (cfg->filters & OSBL_FILTER_BINOMIAL) {
+    // binomial(N, t)/2^N (binomial_filter.py:36-42)
+    const int N = cfg->binomial_order;
+    double b = 1.0;
+    for (int t = 0; t <= N; ++t) {
+      c->binom_w[t] = b / std::ldexp(1.0, N);
+      b = b * (N - t) / (t + 1);
+    }
   }
   {
     double cw[CW_SIZE];
@@ -508,6 +575,8 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->pk);
   (void)hipFree(c->cw_dev);
   (void)hipFree(c->Theta);
+  (void)hipFree(c->QF);
+  (void)hipFree(c->fmask);
   for (int d = 0; d < 3; ++d) {
     (void)hipFree(c->met_dev[d]);
     for (int s = 0; s < 2; ++s) (void)hipFree(c->plane_dev[d][s]);
@@ -567,12 +636,46 @@ static int check_ready(const osbl_ctx* c) {
   return OSBL_OK;
 }
 
+// the filters after the full time step (algorithm.py:412-470), in the order
+// WENO filter, SFD, binomial filter
+static void filters(osbl_ctx* c) {
+  const int f = c->cfg.filters;
+  if (f & OSBL_FILTER_WENO) {
+    double* Q = Qcur(c);
+    const bool ducros = c->cfg.wf_sensor == OSBL_WF_DUCROS;
+    if (ducros) timed(c, F_FILTER, [&] { launch_sensor(c->nd, Q, c->Theta, c->g, c->ph, c->bnd, c->stream); });
+    for (int d = 0; d < c->nd; ++d) {
+      timed(c, F_FILTER, [&] {
+        launch_sweep(c->nd, d, CONV_WFILTER, OSBL_AVG_SIMPLE, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[d], nullptr,
+                     d > 0, c->stream);
+      });
+    }
+    timed(c, F_FILTER, [&] {
+      launch_weno_filter_apply(c->nd, Q, c->R, c->Theta, c->g, c->ph, ducros, c->cfg.wf_mach != 0,
+                               c->cfg.shock_filter_control, c->walls, c->stream);
+    });
+  }
+  if (f & OSBL_FILTER_SFD)
+    timed(c, F_FILTER, [&] { launch_sfd(c->nd, Qcur(c), c->QF, c->g, c->sfd_c, c->sfd_E, c->stream); });
+  if (f & OSBL_FILTER_BINOMIAL) {
+    timed(c, F_FILTER, [&] {
+      launch_binomial(c->nd, Qcur(c), c->Qbuf[1 - c->cur], c->g, c->binom_w, c->cfg.binomial_order, c->cfg.sigma_filt,
+                      c->fmask, c->stream);
+    });
+    c->cur ^= 1;
+  }
+}
+
 static void iterate(osbl_ctx* c, int mode) {
+  // with filters the interior changes after the last stage's boundary
+  // conditions, so the iteration starts with them as the reference's does
+  if (c->cfg.filters) apply_bcs(c, Qcur(c));
   for (int s = 0; s < c->nstages; ++s) {
     stage(c, mode, c->A[s], c->B[s], s == 0);
     c->cur ^= 1;
     apply_bcs(c, Qcur(c));
   }
+  if (c->cfg.filters) filters(c);
 }
 
 // capture two iterations starting from buffer `start` into an executable graph
@@ -600,12 +703,22 @@ int osbl_run(osbl_ctx* c, int64_t niter) {
   if (int rc = check_ready(c)) return rc;
   const int mode = c->cfg.rk == OSBL_RK3 ? 0 : 1;
   if (niter > 0) apply_bcs(c, Qcur(c));
+  if ((c->cfg.filters & OSBL_FILTER_SFD) && niter > 0) {
+    // "Initialize the filter" before the time loop: vfilt = v (SFD.py:50-61)
+    for (int m = 0; m < c->nv; ++m) {
+      if (c->sfd_set[m]) continue;
+      HIPCHK(hipMemcpyAsync(c->QF + m * c->g.fs, Qcur(c) + m * c->g.fs, c->g.fs * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
+      c->sfd_set[m] = true;
+    }
+  }
   int64_t it = 0;
   // Launch-bound blocks replay two iterations from a graph (kernels only); with
   // profiling on, every launch is issued directly so it can be timed.
   if (c->use_graphs && !c->prof && niter >= 2) {
     const int start = c->cur;
-    const int parity_after = (2 * c->nstages) & 1;  // buffers swap once per stage
+    // buffers swap once per stage and once per binomial filter pass
+    const int parity_after = (2 * (c->nstages + ((c->cfg.filters & OSBL_FILTER_BINOMIAL) ? 1 : 0))) & 1;
     if (!c->graph2[start]) {
       int rc = capture_pair(c, mode, start);
       if (rc) return rc;
@@ -652,6 +765,53 @@ int osbl_set_coordinates(osbl_ctx* c, int d, const double* x) {
       (void)hipGraphExecDestroy(c->graph2[k]);
       c->graph2[k] = nullptr;
     }
+  return OSBL_OK;
+}
+
+int osbl_set_filter_mask(osbl_ctx* c, const uint8_t* mask) {
+  if (!c) return fail(OSBL_E_ARG, "null context");
+  if (!(c->cfg.filters & OSBL_FILTER_BINOMIAL)) return fail(OSBL_E_STATE, "the block has no binomial filter");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!mask) {
+    HIPCHK(hipFree(c->fmask));
+    c->fmask = nullptr;
+    return OSBL_OK;
+  }
+  const size_t n = (size_t)cells(c);
+  if (!c->fmask && hipMalloc((void**)&c->fmask, n) != hipSuccess) {
+    c->fmask = nullptr;
+    return fail(OSBL_E_NOMEM, "hipMalloc of %zu bytes failed", n);
+  }
+  HIPCHK(hipMemcpy(c->fmask, mask, n, hipMemcpyHostToDevice));
+  // graphs hold the mask pointer as a kernel argument: capture again
+  for (int k = 0; k < 2; ++k)
+    if (c->graph2[k]) {
+      (void)hipGraphExecDestroy(c->graph2[k]);
+      c->graph2[k] = nullptr;
+    }
+  return OSBL_OK;
+}
+
+int osbl_get_sfd_field(osbl_ctx* c, int var, double* host) {
+  if (!c || !host) return fail(OSBL_E_ARG, "null argument");
+  if (!(c->cfg.filters & OSBL_FILTER_SFD)) return fail(OSBL_E_STATE, "the block has no SFD filter");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable index %d out of range [0,%d)", var, c->nv);
+  launch_pack(c->QF + var * c->g.fs, c->stage_buf, c->g, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, c->stage_buf, cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+int osbl_set_sfd_field(osbl_ctx* c, int var, const double* host) {
+  if (!c || !host) return fail(OSBL_E_ARG, "null argument");
+  if (!(c->cfg.filters & OSBL_FILTER_SFD)) return fail(OSBL_E_STATE, "the block has no SFD filter");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable index %d out of range [0,%d)", var, c->nv);
+  HIPCHK(hipMemcpyAsync(c->stage_buf, host, cells(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  launch_unpack(c->stage_buf, c->QF + var * c->g.fs, c->g, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->sfd_set[var] = true;
   return OSBL_OK;
 }
 
@@ -785,6 +945,7 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
     case F_SWEEP_X: b = 16.0 * nv * n; f = interface_flops(c) * n; break;
     case F_SWEEP_Y:
     case F_SWEEP_Z: b = 24.0 * nv * n; f = interface_flops(c) * n; break;
+    case F_FILTER: b = 16.0 * nv * n; break;
     case F_UPDATE: {
       // read Q (central) or R, the constituent relations when viscous, the RK register; write Q and register
       double rd = (is_central(c->cfg.conv) ? nv : nv) + (c->need_prims ? c->nd + 2 : 0) + nv;

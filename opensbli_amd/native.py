@@ -18,6 +18,8 @@ RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
 BC = {"periodic": 0, "extrap0": 1, "extrap1": 2, "zero_gradient": 3, "adiabatic_wall": 4,
       "isothermal_wall": 5, "dirichlet": 6, "inlet_pextrap": 7}
 BSCHEME = {"carpenter": 0, "reduced_access": 1}
+FILTERS = {"weno": 1, "sfd": 2, "binomial": 4}
+WF_SENSOR = {"ducros": 0, "constant": 1}
 
 # symbols the header declares (checked by the ABI test)
 EXPORTS = [
@@ -27,7 +29,8 @@ EXPORTS = [
     "osbl_batch_create", "osbl_batch_destroy", "osbl_batch_set_state", "osbl_batch_get_state", "osbl_batch_run",
     "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
     "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
-    "osbl_set_coordinates", "osbl_set_bc_plane",
+    "osbl_set_coordinates", "osbl_set_bc_plane", "osbl_set_filter_mask", "osbl_get_sfd_field",
+    "osbl_set_sfd_field",
 ]
 
 
@@ -50,7 +53,7 @@ class OsblConfig(ctypes.Structure):
         ("teno_ct", ctypes.c_double),
         ("bscheme", (ctypes.c_int32 * 2) * 3),
         ("stretch", ctypes.c_int32 * 3),
-        ("reserved", ctypes.c_int32),
+        ("filters", ctypes.c_int32),
         ("suthT", ctypes.c_double),
         ("refT", ctypes.c_double),
         ("twall", ctypes.c_double),
@@ -59,6 +62,14 @@ class OsblConfig(ctypes.Structure):
         ("teno_a1", ctypes.c_double),
         ("teno_a2", ctypes.c_double),
         ("sensor_eps", ctypes.c_double),
+        ("binomial_order", ctypes.c_int32),
+        ("wf_sensor", ctypes.c_int32),
+        ("wf_mach", ctypes.c_int32),
+        ("reserved3", ctypes.c_int32),
+        ("sigma_filt", ctypes.c_double),
+        ("chi_filt", ctypes.c_double),
+        ("omega_filt", ctypes.c_double),
+        ("shock_filter_control", ctypes.c_double),
     ]
 
 
@@ -115,6 +126,9 @@ def load():
     L.osbl_use_graphs.argtypes = [vp, ctypes.c_int]
     L.osbl_set_coordinates.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_set_bc_plane.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+    L.osbl_set_filter_mask.argtypes = [vp, vp]
+    L.osbl_get_sfd_field.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_set_sfd_field.argtypes = [vp, ctypes.c_int, dp]
     _lib = L
     return L
 
@@ -160,6 +174,23 @@ def make_config(p):
     c.teno_a1 = p.get("teno_a1", 10.5)
     c.teno_a2 = p.get("teno_a2", 4.5)
     c.sensor_eps = p.get("sensor_eps", 1.0e-30)
+    # filters: {"weno": {"sensor", "mach_correction", "control"}, "sfd": {"chi", "omega"},
+    #           "binomial": {"order", "sigma", "mask"}}, defaults as the reference's constructors
+    f = p.get("filters") or {}
+    bits = 0
+    for k in f:
+        if k not in FILTERS:
+            raise ValueError(f"unknown filter {k!r}")
+        bits |= FILTERS[k]
+    c.filters = bits
+    b, sf, w = f.get("binomial") or {}, f.get("sfd") or {}, f.get("weno") or {}
+    c.binomial_order = int(b.get("order", 2))
+    c.sigma_filt = float(b.get("sigma", 0.05))
+    c.chi_filt = float(sf.get("chi", 0.1))
+    c.omega_filt = float(sf.get("omega", 1.0 / 0.75))
+    c.wf_sensor = WF_SENSOR[w.get("sensor", "ducros")]
+    c.wf_mach = 1 if w.get("mach_correction") else 0
+    c.shock_filter_control = float(w.get("control", 1.0))
     return c
 
 
@@ -190,6 +221,35 @@ class Block:
         for (d, side), pl in problem.get("planes", {}).items():
             a = np.ascontiguousarray(pl, dtype=np.float64)
             check(L.osbl_set_bc_plane(self.h, int(d), int(side), _dptr(a)))
+        mask = ((problem.get("filters") or {}).get("binomial") or {}).get("mask")
+        if mask is not None:
+            self.set_filter_mask(mask)
+
+    def set_filter_mask(self, mask):
+        """Grid condition of the binomial filter (interior cells, nonzero where it applies); None clears it."""
+        if mask is None:
+            check(load().osbl_set_filter_mask(self.h, None))
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.shape != self.shape:
+            raise ValueError(f"mask has shape {m.shape}, expected {self.shape}")
+        check(load().osbl_set_filter_mask(self.h, m.ctypes.data))
+
+    def sfd_field(self):
+        """The SFD filtered solution vfilt, (nvar,) + shape."""
+        out = np.empty((self.nvar,) + self.shape)
+        for m in range(self.nvar):
+            a = np.empty(self.shape)
+            check(load().osbl_get_sfd_field(self.h, m, _dptr(a)))
+            out[m] = a
+        return out
+
+    def set_sfd_field(self, qf):
+        for m in range(self.nvar):
+            a = np.ascontiguousarray(qf[m], dtype=np.float64)
+            if a.shape != self.shape:
+                raise ValueError(f"field {m} has shape {a.shape}, expected {self.shape}")
+            check(load().osbl_set_sfd_field(self.h, m, _dptr(a)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -47,6 +47,12 @@ struct ref_state {
   double* dm[3];      /* metric D_dd of a stretched direction (n[d] values), else NULL */
   double* sd[3];      /* SD_ddd = CD(D_dd, xi_d) */
   double* plane[3][2];/* per-face conservative plane (Dirichlet values / inflow halo state) */
+  double* qf[MAXV];   /* SFD filtered solution vfilt */
+  int sfd_set[MAXV];  /* vfilt of variable m holds a value (set, or initialised from the state) */
+  double* kappa;      /* WENO filter dissipation sensor (interior only; halos stay 0) */
+  double* msens;      /* Yee Mach sensor of the WENO filter */
+  double* tmp[MAXV];  /* binomial filter output */
+  unsigned char* fmask; /* binomial filter grid condition (interior cells) or NULL */
 };
 
 static inline ptrdiff_t IX(const ref_state* s, int i, int j, int k) {
@@ -278,32 +284,37 @@ static inline double met1(const ref_state* s, int d, int c) {
   return s->dm[d][c];
 }
 
-/* Modified Ducros sensor as a constituent relation (shock_sensors.py:12-49):
-   theta = 1/2 (1 - tanh(2.5 (1 + 100 div u))) div^2 / (div^2 + |curl u|^2 + epsilon),
-   with the block's central derivatives (closures and metric included),
-   evaluated over the interior and three halo points. */
-static void ducros_sensor(ref_state* s) {
+/* Modified Ducros sensor (shock_sensors.py:12-49) at one point:
+   1/2 (1 - tanh(2.5 (1 + 100 div u))) div^2 / (div^2 + |curl u|^2 + epsilon),
+   with the block's central derivatives (closures and metric included). */
+static double ducros_at(const ref_state* s, ptrdiff_t x, int i, int j, int k) {
   const int nd = s->nd;
+  const int cc[3] = {i, j, k};
+  double du[3][3] = {{0.0}};
+  for (int m = 0; m < nd; ++m)
+    for (int d = 0; d < nd; ++d)
+      du[m][d] = met1(s, d, cc[d]) * ((1.0 / s->c.delta[d]) *
+                 d1_gen(s->u[m], x, s->st[d], s->n[d], s->mod[d][0], s->mod[d][1], cc[d]));
+  double div = 0.0, w2;
+  for (int d = 0; d < nd; ++d) div += du[d][d];
+  if (nd == 2) {
+    const double w = du[1][0] - du[0][1];
+    w2 = w * w;
+  } else {
+    const double wa = du[2][1] - du[1][2], wb = du[0][2] - du[2][0], wc = du[1][0] - du[0][1];
+    w2 = wa * wa + wb * wb + wc * wc;
+  }
+  const double filt = 0.5 * (1.0 - tanh(2.5 * (1.0 + 100.0 * div)));
+  return filt * (div * div) / (div * div + w2 + s->c.sensor_eps);
+}
+
+/* The sensor as a constituent relation of the adaptive TENO scheme, over the
+   interior and three halo points. */
+static void ducros_sensor(ref_state* s) {
   range3 R = rng_all(s, 3);
   LOOP(R) {
     ptrdiff_t x = IX(s, i, j, k);
-    const int cc[3] = {i, j, k};
-    double du[3][3] = {{0.0}};
-    for (int m = 0; m < nd; ++m)
-      for (int d = 0; d < nd; ++d)
-        du[m][d] = met1(s, d, cc[d]) * ((1.0 / s->c.delta[d]) *
-                   d1_gen(s->u[m], x, s->st[d], s->n[d], s->mod[d][0], s->mod[d][1], cc[d]));
-    double div = 0.0, w2;
-    for (int d = 0; d < nd; ++d) div += du[d][d];
-    if (nd == 2) {
-      const double w = du[1][0] - du[0][1];
-      w2 = w * w;
-    } else {
-      const double wa = du[2][1] - du[1][2], wb = du[0][2] - du[2][0], wc = du[1][0] - du[0][1];
-      w2 = wa * wa + wb * wb + wc * wc;
-    }
-    const double filt = 0.5 * (1.0 - tanh(2.5 * (1.0 + 100.0 * div)));
-    s->theta[x] = filt * (div * div) / (div * div + w2 + s->c.sensor_eps);
+    s->theta[x] = ducros_at(s, x, i, j, k);
   }
 }
 
@@ -591,7 +602,7 @@ static const double WENO_C_L[3][3] = {{11.0 / 6.0, -7.0 / 6.0, 1.0 / 3.0}, {1.0 
                                       {-1.0 / 6.0, 5.0 / 6.0, 1.0 / 3.0}};
 static const double WENO_D_L[3] = {1.0 / 10.0, 3.0 / 5.0, 3.0 / 10.0};
 
-static double weno5(int conv, int side, const double* f) {
+static double weno5(int conv, int side, const double* f, double* omega_out) {
   int shift = side > 0 ? 0 : 1;
   double beta[3], alpha[3];
   js_beta(f, shift, beta);
@@ -612,6 +623,7 @@ static double weno5(int conv, int side, const double* f) {
   double recon = 0.0;
   for (int r = 0; r < 3; ++r) {
     double omega = alpha[r] * inv_alpha_sum;
+    if (omega_out) omega_out[r] = omega;
     double eno = 0.0;
     for (int jj = 0; jj < 3; ++jj) eno += c[r][jj] * f[-r + jj + shift];
     recon += omega * eno;
@@ -711,7 +723,7 @@ static double teno(int conv, int side, const double* f, double eps, double ct) {
 
 double ref_recon_side(int conv, int side, const double* f6, double eps, double ct) {
   const double* f = f6 + 2;
-  if (conv == REF_CONV_WENO_JS || conv == REF_CONV_WENO_Z) return weno5(conv, side, f);
+  if (conv == REF_CONV_WENO_JS || conv == REF_CONV_WENO_Z) return weno5(conv, side, f, NULL);
   return teno(conv, side, f, eps, ct);
 }
 
@@ -723,6 +735,30 @@ int ref_smoothness(int conv, int side, const double* f6, double* beta) {
   }
   teno_beta(side, f, beta);
   return conv == REF_CONV_TENO6 ? 4 : 3;
+}
+
+/* Shock sensor of one WENO-JS reconstruction from its non-linear weights
+   (weno.py:415-431, normalised, sensor_theta = 4):
+   sum_r |omega_r/d_r - 1|^4 / (|1/min(d) - 1|^4 + k - 1). */
+static double weno_sensor(const double* omega, const double* d) {
+  double s = 0.0;
+  for (int r = 0; r < 3; ++r) s += pow(fabs(omega[r] / d[r] - 1.0), 4.0);
+  return s / (pow(fabs(1.0 / 0.1 - 1.0), 4.0) + 2.0);
+}
+
+/* WENO filter flux of one characteristic field (shock_capturing.py:82-96,
+   401-416, 438-443): the WENO-JS reconstructions of f+ (right) and f- (left)
+   summed, minus the sixth-order central flux of the characteristic fluxes on
+   points -2..3, weighted by rj = max(sensor(right), sensor(left)). */
+double ref_weno_filter_flux(const double* fp6, const double* fm6, const double* cf6, double* rj_out) {
+  double om[3];
+  const double right = weno5(REF_CONV_WENO_JS, 1, fp6 + 2, om);
+  double rj = weno_sensor(om, WENO_D_R);
+  const double left = weno5(REF_CONV_WENO_JS, -1, fm6 + 2, om);
+  rj = fmax(rj, weno_sensor(om, WENO_D_L));
+  const double central = (cf6[0] - 8.0 * cf6[1] + 37.0 * cf6[2] + 37.0 * cf6[3] - 8.0 * cf6[4] + cf6[5]) / 60.0;
+  if (rj_out) *rj_out = rj;
+  return rj * ((right + left) - central);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -824,18 +860,21 @@ void ref_roe_average(int nd, double g, const double* qL, const double* qR, doubl
 /* Characteristic LLF reconstruction kernel of one direction                  */
 /* (shock_capturing.py:357-448, weno.py:486-521, teno.py:489-530), evaluated  */
 /* at i+1/2 for i in [-1, N) along d; the flux is written into wk[5d+m].      */
-static void reconstruction_direction(ref_state* s, int d) {
+/* filter = 1: the WENO filter's reconstruction (WENO-JS, SimpleAverage, the
+   filter flux of ref_weno_filter_flux; WENO_filter.py:324-326) */
+static void reconstruction_direction(ref_state* s, int d, int filter) {
   const int nd = s->nd, nv = s->nvar;
   const ptrdiff_t st = s->st[d];
   const double g = s->c.gama;
   const int conv = s->c.conv;
+  const int averaging = filter ? REF_AVG_SIMPLE : s->c.averaging;
   range3 R = rng_all(s, 0);
   R.lo[d] = -1;
   double** out = s->wk + 5 * d;
   LOOP(R) {
     ptrdiff_t x = IX(s, i, j, k);
     double avg[5], ua[3];
-    if (s->c.averaging == REF_AVG_ROE) {
+    if (averaging == REF_AVG_ROE) {
       double qL[7], qR[7];
       for (int m = 0; m < nv; ++m) { qL[m] = s->q[m][x]; qR[m] = s->q[m][x + st]; }
       /* the Roe average uses the velocity datasets u_m = rhou_m/rho */
@@ -893,6 +932,10 @@ static void reconstruction_direction(ref_state* s, int d) {
         fp[pp] = 0.5 * (CF[kk][pp] + lmax[kk] * CS[kk][pp]);
         fm[pp] = 0.5 * (CF[kk][pp] - lmax[kk] * CS[kk][pp]);
       }
+      if (filter) {
+        recon[kk] = ref_weno_filter_flux(fp, fm, CF[kk], NULL);
+        continue;
+      }
       const double ct = s->c.sensor ? adaptive_ct(s, s->theta[x]) : s->c.teno_ct;
       recon[kk] = ref_recon_side(conv, 1, fp, s->c.eps, ct) + ref_recon_side(conv, -1, fm, s->c.eps, ct);
     }
@@ -906,9 +949,9 @@ static void reconstruction_direction(ref_state* s, int d) {
 
 /* Shock-capturing residual (shock_capturing.py:21-34, opensblifunctions.py:575-593): */
 /* R_m = -sum_d (wk_dm[i] - wk_dm[i-1]) / Delta_d, assigned.                   */
-static void shock_capturing(ref_state* s) {
+static void shock_capturing(ref_state* s, int filter) {
   const int nd = s->nd, nv = s->nvar;
-  for (int d = 0; d < nd; ++d) reconstruction_direction(s, d);
+  for (int d = 0; d < nd; ++d) reconstruction_direction(s, d, filter);
   range3 I = rng_all(s, 0);
   LOOP(I) {
     ptrdiff_t x = IX(s, i, j, k);
@@ -983,7 +1026,8 @@ static void extrapolation0(ref_state* s, double** arrs, int narr, int d, int sid
    of the schemes, -3/+4 for WENO5/TENO (weno.py:17-29), -2/+2 for Central(4). */
 static int bc_halos(const ref_state* s, int side) {
   const int central = s->c.conv == REF_CONV_CENTRAL_SKEW || s->c.conv == REF_CONV_CENTRAL_CONS;
-  if (central) return 2;
+  /* the WENO filter's reconstruction halos join the block's boundary halos */
+  if (central && !(s->c.filters & REF_FILTER_WENO)) return 2;
   return side == 0 ? 3 : 4;
 }
 
@@ -1123,9 +1167,142 @@ static void evaluate_rhs(ref_state* s) {
   switch (s->c.conv) {
     case REF_CONV_CENTRAL_SKEW: central_skew(s); break;
     case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
-    default: shock_capturing(s); break;
+    default: shock_capturing(s, 0); break;
   }
   if (s->c.viscous != REF_VISC_NONE) viscous(s);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Filters after the full time step (InTheSimulation, algorithm.py:412-470)   */
+/* ------------------------------------------------------------------------- */
+
+/* WENO non-linear filter (WENO_filter.py:145-339): constituent relations u, p,
+   a (and the Yee Mach sensor) on the post-step state; the dissipation sensor
+   kappa over the interior (the sensor array's halos are never written, so they
+   hold zeros); the filter residual -sum_d (F_d[i] - F_d[i-1])/Delta_d of the
+   filter fluxes; then Q += Wall * shock_filter_control * kappa_max * dt * R,
+   with kappa_max the maximum of kappa over the points +-2 along each direction
+   and Wall = 0 within 6 points of a wall face. */
+static void weno_filter(ref_state* s) {
+  const int nd = s->nd, nv = s->nvar;
+  const double g = s->c.gama;
+  range3 R = rng_all(s, H);
+  LOOP(R) {
+    ptrdiff_t x = IX(s, i, j, k);
+    const double inv_rho = 1.0 / s->q[0][x];
+    double m2 = 0.0, uu = 0.0;
+    for (int m = 0; m < nd; ++m) {
+      const double um = s->q[1 + m][x] * inv_rho;
+      s->u[m][x] = um;
+      m2 += s->q[1 + m][x] * s->q[1 + m][x];
+      uu += um * um;
+    }
+    const double pp = (g - 1.0) * (s->q[nd + 1][x] - 0.5 * m2 * inv_rho);
+    s->p[x] = pp;
+    s->a[x] = sqrt(g * pp * inv_rho);
+    if (s->c.wf_mach) { /* WENO_filter.py:178-196 */
+      const double inv_a = 1.0 / sqrt(g * pp * inv_rho);
+      const double M = sqrt(uu) * inv_a, M2 = M * M;
+      s->msens[x] = fmin(0.5 * M2 * sqrt(4.0 + (1.0 - M2) * (1.0 - M2)) / (1.0 + M2), 1.0);
+    }
+  }
+  range3 I = rng_all(s, 0);
+  if (s->c.wf_sensor == REF_WF_DUCROS) {
+    LOOP(I) {
+      ptrdiff_t x = IX(s, i, j, k);
+      const double th = ducros_at(s, x, i, j, k);
+      s->kappa[x] = s->c.wf_mach ? s->msens[x] * th : th;
+    }
+  }
+  shock_capturing(s, 1);
+  int wall[3][2] = {{0}};
+  for (int d = 0; d < nd; ++d)
+    for (int side = 0; side < 2; ++side)
+      wall[d][side] = s->c.bc[d][side] == REF_BC_ADIABATIC_WALL || s->c.bc[d][side] == REF_BC_ISOTHERMAL_WALL;
+  LOOP(I) {
+    ptrdiff_t x = IX(s, i, j, k);
+    const int cc[3] = {i, j, k};
+    double kmax;
+    if (s->c.wf_sensor == REF_WF_DUCROS) {
+      kmax = s->kappa[x];
+      for (int d = 0; d < nd; ++d)
+        for (int o = -2; o <= 2; ++o) kmax = fmax(kmax, s->kappa[x + o * s->st[d]]);
+    } else {
+      kmax = s->c.wf_mach ? s->msens[x] : 1.0;
+    }
+    double w = 1.0;
+    for (int d = 0; d < nd; ++d) {
+      if (wall[d][0] && cc[d] <= 5) w = 0.0;
+      if (wall[d][1] && cc[d] >= s->n[d] - 6) w = 0.0;
+    }
+    for (int m = 0; m < nv; ++m)
+      s->q[m][x] = s->q[m][x] + w * s->c.shock_filter_control * kmax * s->c.dt * s->res[m][x];
+  }
+}
+
+/* Selective frequency damping (SFD.py:64-86), pointwise over the interior:
+   vfilt' = (v (1 - E) + vfilt (c + E)) / (c + 1),
+   v'     = (v (E c + 1) + vfilt c (1 - E)) / (c + 1),
+   c = chi/omega, E = exp(-(chi + omega) dt). */
+static void sfd_filter(ref_state* s) {
+  const int nv = s->nvar;
+  const double chi = s->c.chi_filt, om = s->c.omega_filt;
+  range3 I = rng_all(s, 0);
+  for (int m = 0; m < nv; ++m) {
+    double *q = s->q[m], *f = s->qf[m];
+    LOOP(I) {
+      ptrdiff_t x = IX(s, i, j, k);
+      const double v = q[x], old = f[x];
+      const double E = exp(-(chi + om) * s->c.dt);
+      f[x] = (v * (1.0 - E) + old * (chi / om + E)) / (chi / om + 1.0);
+      q[x] = (v * (E * chi / om + 1.0) + old * chi / om * (1.0 - E)) / (chi / om + 1.0);
+    }
+  }
+}
+
+/* Binomial filter of even order N (binomial_filter.py:36-110): per direction
+   the weighted sum with weights binomial(N, i)/2^N on points -N/2..N/2, the
+   directional values averaged, then q = (1 - sigma) q + sigma q_f where the
+   grid condition holds.  Every point reads the pre-filter state (the
+   reference writes q in place while reading its neighbours, which makes its
+   result depend on the loop order; this is the order-free reading). */
+static void binomial_filter(ref_state* s) {
+  const int nd = s->nd, nv = s->nvar, N = s->c.binomial_order;
+  double w[11];
+  double c = 1.0;
+  for (int i = 0; i <= N; ++i) {
+    w[i] = c / pow(2.0, N);
+    c = c * (N - i) / (i + 1);
+  }
+  const double sg = s->c.sigma_filt;
+  range3 I = rng_all(s, 0);
+  for (int m = 0; m < nv; ++m) {
+    const double* q = s->q[m];
+    double* o = s->tmp[m];
+    LOOP(I) {
+      ptrdiff_t x = IX(s, i, j, k);
+      double sum = 0.0;
+      for (int d = 0; d < nd; ++d) {
+        double fd = 0.0;
+        for (int t = 0; t <= N; ++t) fd += w[t] * q[x + (t - N / 2) * s->st[d]];
+        sum += fd;
+      }
+      const double qf = sum / (double)nd;
+      int on = 1;
+      if (s->fmask) on = s->fmask[(size_t)i + (size_t)s->n[0] * ((size_t)j + (size_t)s->n[1] * (size_t)k)] != 0;
+      o[x] = on ? (1.0 - sg) * q[x] + sg * qf : q[x];
+    }
+  }
+  for (int m = 0; m < nv; ++m) {
+    double *q = s->q[m], *o = s->tmp[m];
+    LOOP(I) { ptrdiff_t x = IX(s, i, j, k); q[x] = o[x]; }
+  }
+}
+
+static void filters(ref_state* s) {
+  if (s->c.filters & REF_FILTER_WENO) weno_filter(s);
+  if (s->c.filters & REF_FILTER_SFD) sfd_filter(s);
+  if (s->c.filters & REF_FILTER_BINOMIAL) binomial_filter(s);
 }
 
 int ref_rk_coefficients(int rk, double* a, double* b) {
@@ -1195,12 +1372,18 @@ static void time_step(ref_state* s) {
     }
     apply_bcs(s);
   }
+  filters(s);
 }
 
 /* ------------------------------------------------------------------------- */
 int ref_create(const ref_config* cfg, ref_state** out) {
   if (cfg->ndim < 1 || cfg->ndim > 3) return -1;
   if (cfg->sensor && cfg->ndim < 2) return -1; /* vorticity needs two directions */
+  /* the filters define their fields for 2-D and 3-D blocks only */
+  if (cfg->filters && cfg->ndim < 2) return -1;
+  if ((cfg->filters & REF_FILTER_BINOMIAL) &&
+      (cfg->binomial_order < 2 || cfg->binomial_order > 10 || cfg->binomial_order % 2)) return -1;
+  if ((cfg->filters & REF_FILTER_SFD) && !(cfg->omega_filt != 0.0)) return -1;
   ref_state* s = (ref_state*)calloc(1, sizeof(ref_state));
   s->c = *cfg;
   s->nd = cfg->ndim;
@@ -1226,6 +1409,14 @@ int ref_create(const ref_config* cfg, ref_state** out) {
   s->a = (double*)calloc(1, bytes);
   s->mu = (double*)calloc(1, bytes);
   s->theta = (double*)calloc(1, bytes);
+  if (cfg->filters & REF_FILTER_WENO) {
+    s->kappa = (double*)calloc(1, bytes);
+    s->msens = (double*)calloc(1, bytes);
+  }
+  for (int m = 0; m < s->nvar; ++m) {
+    if (cfg->filters & REF_FILTER_SFD) s->qf[m] = (double*)calloc(1, bytes);
+    if (cfg->filters & REF_FILTER_BINOMIAL) s->tmp[m] = (double*)calloc(1, bytes);
+  }
   carpenter_init();
   for (int d = 0; d < s->nd; ++d)
     for (int side = 0; side < 2; ++side) {
@@ -1236,6 +1427,7 @@ int ref_create(const ref_config* cfg, ref_state** out) {
   /* the residual kernels consume every convective work array before the
      viscous kernels run, so the two groups share one pool */
   int nwk = (cfg->conv == REF_CONV_CENTRAL_SKEW || cfg->conv == REF_CONV_CENTRAL_CONS) ? 32 : 15;
+  if ((cfg->filters & REF_FILTER_WENO) && nwk < 15) nwk = 15;
   if (cfg->viscous != REF_VISC_NONE && nwk < 36) nwk = 36;
   for (int w = 0; w < nwk; ++w) s->wk[w] = (double*)calloc(1, bytes);
   *out = s;
@@ -1247,6 +1439,8 @@ void ref_destroy(ref_state* s) {
   for (int m = 0; m < MAXV; ++m) { free(s->q[m]); free(s->qold[m]); free(s->res[m]); }
   for (int m = 0; m < 3; ++m) free(s->u[m]);
   free(s->p); free(s->T); free(s->a); free(s->mu); free(s->theta);
+  free(s->kappa); free(s->msens); free(s->fmask);
+  for (int m = 0; m < MAXV; ++m) { free(s->qf[m]); free(s->tmp[m]); }
   for (int w = 0; w < NWK; ++w) free(s->wk[w]);
   for (int d = 0; d < 3; ++d) {
     free(s->dm[d]); free(s->sd[d]);
@@ -1341,7 +1535,7 @@ int ref_rhs(ref_state* s, double* residual) {
   switch (s->c.conv) {
     case REF_CONV_CENTRAL_SKEW: central_skew(s); break;
     case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
-    default: shock_capturing(s); break;
+    default: shock_capturing(s, 0); break;
   }
   if (s->c.viscous != REF_VISC_NONE) viscous(s);
   ptrdiff_t c = 0;
@@ -1353,6 +1547,48 @@ int ref_rhs(ref_state* s, double* residual) {
 }
 
 int ref_run(ref_state* s, int64_t niter) {
+  if ((s->c.filters & REF_FILTER_SFD) && niter > 0) {
+    /* "Initialize the filter" before the time loop: vfilt = v (SFD.py:50-61) */
+    range3 I = rng_all(s, 0);
+    for (int m = 0; m < s->nvar; ++m) {
+      if (s->sfd_set[m]) continue;
+      double *f = s->qf[m], *q = s->q[m];
+      LOOP(I) { ptrdiff_t x = IX(s, i, j, k); f[x] = q[x]; }
+      s->sfd_set[m] = 1;
+    }
+  }
   for (int64_t it = 0; it < niter; ++it) time_step(s);
+  return 0;
+}
+
+int ref_set_filter_mask(ref_state* s, const uint8_t* mask) {
+  if (!(s->c.filters & REF_FILTER_BINOMIAL)) return -1;
+  const size_t n = (size_t)s->n[0] * s->n[1] * s->n[2];
+  free(s->fmask);
+  s->fmask = NULL;
+  if (!mask) return 0;
+  s->fmask = (unsigned char*)malloc(n);
+  memcpy(s->fmask, mask, n);
+  return 0;
+}
+
+int ref_get_sfd_field(const ref_state* s, int var, double* v) {
+  if (!(s->c.filters & REF_FILTER_SFD) || var < 0 || var >= s->nvar) return -1;
+  const double* f = s->qf[var];
+  ptrdiff_t c = 0;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) v[c++] = f[IX(s, i, j, k)];
+  return 0;
+}
+
+int ref_set_sfd_field(ref_state* s, int var, const double* v) {
+  if (!(s->c.filters & REF_FILTER_SFD) || var < 0 || var >= s->nvar) return -1;
+  double* f = s->qf[var];
+  ptrdiff_t c = 0;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) f[IX(s, i, j, k)] = v[c++];
+  s->sfd_set[var] = 1;
   return 0;
 }

@@ -41,6 +41,12 @@ enum {
 /* near-boundary replacement of the central stencils (bc_core.py:15-16,
    opensblifunctions.py:523-534) at faces whose BC modifies them */
 enum { REF_BS_CARPENTER = 0, REF_BS_REDUCED_ACCESS = 1 };
+/* filters applied after every full time step (InTheSimulation kernels,
+   algorithm.py:412-470), in this order: WENO filter, SFD, binomial filter */
+enum { REF_FILTER_WENO = 1,       /* WENOFilter, WENO_filter.py:15-339 */
+       REF_FILTER_SFD = 2,        /* SFD, SFD.py:27-86 */
+       REF_FILTER_BINOMIAL = 4 }; /* BinomialFilter, binomial_filter.py:15-123 */
+enum { REF_WF_DUCROS = 0, REF_WF_CONSTANT = 1 };  /* dissipation_sensor, WENO_filter.py:265-275 */
 
 typedef struct ref_config {
   int32_t ndim;
@@ -51,12 +57,19 @@ typedef struct ref_config {
   double gama, Minf, Re, Pr, dt, eps, teno_ct;
   int32_t bscheme[3][2]; /* REF_BS_* of each face (used when its BC modifies the stencils) */
   int32_t stretch[3];    /* 1: the coordinate of direction d is given (metric.py:37-54) */
-  int32_t pad_;
+  int32_t filters;       /* REF_FILTER_* bitmask */
   double suthT, refT;    /* Sutherland constants SuthT, RefT */
   double twall;          /* Twall of the isothermal wall */
   int32_t sensor;        /* 1: Ducros sensor theta with the adaptive TENO cut-off */
   int32_t pad2_;
   double teno_a1, teno_a2, sensor_eps;  /* teno_a1, teno_a2, epsilon of the sensor */
+  int32_t binomial_order;  /* even order N of the binomial filter, 2..10 */
+  int32_t wf_sensor;       /* REF_WF_*: dissipation sensor of the WENO filter */
+  int32_t wf_mach;         /* 1: Yee low-Mach correction of the WENO filter */
+  int32_t pad3_;
+  double sigma_filt;       /* binomial blending weight sigma_filt */
+  double chi_filt, omega_filt;   /* SFD damping coefficient and cut frequency */
+  double shock_filter_control;   /* global scale of the WENO filter */
 } ref_config;
 
 typedef struct ref_state ref_state;
@@ -98,6 +111,17 @@ int ref_closure_weights(int scheme, int order, double* w);
 int ref_get_sensor(const ref_state* s, double* theta);
 /* metric arrays of direction d after ref_set_coordinates (n[d] values each) */
 int ref_get_metric(const ref_state* s, int d, double* dm, double* sd);
+/* binomial filter grid condition: 1 where the filter applies (interior cells,
+   x fastest); without a mask it applies everywhere */
+int ref_set_filter_mask(ref_state* s, const uint8_t* mask);
+/* SFD filtered solution vfilt (interior, x fastest); it starts as the state at
+   the first ref_run unless set */
+int ref_get_sfd_field(const ref_state* s, int var, double* interior);
+int ref_set_sfd_field(ref_state* s, int var, const double* interior);
+/* WENO filter flux of one characteristic field at an interface: fp, fm the
+   split fluxes f+ and f- and cf the characteristic flux on points -2..3; returns
+   rj (recon(f+) + recon(f-) - central6(cf)) and writes rj */
+double ref_weno_filter_flux(const double* fp6, const double* fm6, const double* cf6, double* rj);
 /* RK coefficients of a temporal scheme: returns the number of stages */
 int ref_rk_coefficients(int rk, double* a, double* b);
 

@@ -18,6 +18,23 @@ RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
 BC = {"periodic": 0, "extrap0": 1, "extrap1": 2, "zero_gradient": 3, "adiabatic_wall": 4,
       "isothermal_wall": 5, "dirichlet": 6, "inlet_pextrap": 7}
 BSCHEME = {"carpenter": 0, "reduced_access": 1}
+FILTERS = {"weno": 1, "sfd": 2, "binomial": 4}
+WF_SENSOR = {"ducros": 0, "constant": 1}
+
+
+def filter_fields(p):
+    """Filter settings of a problem dict: (bitmask, binomial order, WENO-filter
+    sensor, Mach correction, sigma, chi, omega, control)."""
+    f = p.get("filters", {}) or {}
+    mask = 0
+    for k in f:
+        mask |= FILTERS[k]
+    b = f.get("binomial", {}) or {}
+    s = f.get("sfd", {}) or {}
+    w = f.get("weno", {}) or {}
+    return (mask, int(b.get("order", 2)), WF_SENSOR[w.get("sensor", "ducros")], 1 if w.get("mach_correction") else 0,
+            float(b.get("sigma", 0.05)), float(s.get("chi", 0.1)), float(s.get("omega", 1.0 / 0.75)),
+            float(w.get("control", 1.0)))
 
 
 class RefConfig(ctypes.Structure):
@@ -39,7 +56,7 @@ class RefConfig(ctypes.Structure):
         ("teno_ct", ctypes.c_double),
         ("bscheme", (ctypes.c_int32 * 2) * 3),
         ("stretch", ctypes.c_int32 * 3),
-        ("pad_", ctypes.c_int32),
+        ("filters", ctypes.c_int32),
         ("suthT", ctypes.c_double),
         ("refT", ctypes.c_double),
         ("twall", ctypes.c_double),
@@ -48,6 +65,14 @@ class RefConfig(ctypes.Structure):
         ("teno_a1", ctypes.c_double),
         ("teno_a2", ctypes.c_double),
         ("sensor_eps", ctypes.c_double),
+        ("binomial_order", ctypes.c_int32),
+        ("wf_sensor", ctypes.c_int32),
+        ("wf_mach", ctypes.c_int32),
+        ("pad3_", ctypes.c_int32),
+        ("sigma_filt", ctypes.c_double),
+        ("chi_filt", ctypes.c_double),
+        ("omega_filt", ctypes.c_double),
+        ("shock_filter_control", ctypes.c_double),
     ]
 
 
@@ -87,6 +112,11 @@ def lib(variant="plain"):
         L.ref_closure_weights.argtypes = [ctypes.c_int, ctypes.c_int, dp]
         L.ref_get_metric.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         L.ref_get_sensor.argtypes = [ctypes.c_void_p, dp]
+        L.ref_set_filter_mask.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_get_sfd_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.ref_set_sfd_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.ref_weno_filter_flux.argtypes = [dp, dp, dp, dp]
+        L.ref_weno_filter_flux.restype = ctypes.c_double
         _libs[variant] = L
     return _libs[variant]
 
@@ -128,7 +158,17 @@ def make_config(p):
     c.teno_a1 = p.get("teno_a1", 10.5)
     c.teno_a2 = p.get("teno_a2", 4.5)
     c.sensor_eps = p.get("sensor_eps", 1.0e-30)
+    (c.filters, c.binomial_order, c.wf_sensor, c.wf_mach, c.sigma_filt, c.chi_filt, c.omega_filt,
+     c.shock_filter_control) = filter_fields(p)
     return c
+
+
+def weno_filter_flux(fp, fm, cf):
+    """(filter flux, rj) of one characteristic field from f+, f-, CF on points -2..3."""
+    a, b, c = (np.ascontiguousarray(v, dtype=np.float64) for v in (fp, fm, cf))
+    rj = np.zeros(1)
+    v = lib().ref_weno_filter_flux(_ptr(a), _ptr(b), _ptr(c), _ptr(rj))
+    return v, rj[0]
 
 
 def closure_weights(scheme, order):
@@ -160,6 +200,24 @@ class RefSolver:
             a = np.ascontiguousarray(pl, dtype=np.float64)
             if self.L.ref_set_bc_plane(self.h, int(d), int(side), _ptr(a)) != 0:
                 raise RuntimeError(f"ref_set_bc_plane({d}, {side}) failed")
+        mask = ((problem.get("filters") or {}).get("binomial") or {}).get("mask")
+        if mask is not None:
+            self._mask = np.ascontiguousarray(mask, dtype=np.uint8)
+            assert self._mask.shape == self.shape
+            self.L.ref_set_filter_mask(self.h, self._mask.ctypes.data)
+
+    def sfd_field(self):
+        out = np.empty((self.nvar,) + self.shape)
+        for m in range(self.nvar):
+            if self.L.ref_get_sfd_field(self.h, m, _ptr(out[m])) != 0:
+                raise RuntimeError("no SFD on this block")
+        return out
+
+    def set_sfd_field(self, qf):
+        for m in range(self.nvar):
+            a = np.ascontiguousarray(qf[m], dtype=np.float64)
+            if self.L.ref_set_sfd_field(self.h, m, _ptr(a)) != 0:
+                raise RuntimeError("no SFD on this block")
 
     def sensor(self):
         out = np.empty(self.shape)

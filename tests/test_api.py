@@ -106,3 +106,40 @@ def test_flat_plate_adaptive_teno_script():
     ref_p, _ = flat_plate(n=(40, 32), ndim=2, noise=0.0, conv="teno5", sensor=True)
     for k in ("conv", "sensor", "teno_a1", "teno_a2", "sensor_eps", "viscous", "stretch", "bc"):
         assert p[k] == ref_p[k], k
+
+
+def test_filters_from_script():
+    """WENOFilter, SFD and BinomialFilter on the TGV script: the problem
+    carries the filters with the reference's default constants, substituted
+    values override them, and the binomial grid condition becomes a mask."""
+    from opensbli_amd.apps.taylor_green import build
+    prog = build(16, "central", filters=("weno", "sfd", "binomial"))
+    p = prog.problem()
+    f = p["filters"]
+    assert f["weno"] == {"sensor": "ducros", "mach_correction": False, "control": 1.0}
+    assert p["sensor_eps"] == 1.0e-12
+    assert f["sfd"] == {"chi": 0.1, "omega": 1.0 / 0.75}
+    assert f["binomial"] == {"order": 4, "sigma": 0.05}
+    from opensbli_amd.api import substitute_simulation_parameters
+    substitute_simulation_parameters(["sigma_filt", "shock_filter_control", "epsilon"], ["0.2", "0.5", "1e-10"])
+    p = prog.problem()
+    assert p["filters"]["binomial"]["sigma"] == 0.2 and p["filters"]["weno"]["control"] == 0.5
+    assert p["sensor_eps"] == 1e-10
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    mask = p["filters"]["binomial"]["mask"]
+    assert mask.shape == (16, 16, 16) and mask.dtype == np.uint8
+    assert mask[:, :, :8].all() and not mask[:, :, 8:].any()
+
+
+def test_filter_constructors_follow_the_reference_checks():
+    from opensbli_amd import api
+    b2 = api.SimulationBlock(2)
+    with pytest.raises(ValueError, match="even orders"):
+        api.BinomialFilter(b2, 3)
+    with pytest.raises(ValueError, match="halo points"):
+        api.BinomialFilter(b2, 12)
+    with pytest.raises(NotImplementedError, match="dissipation sensor"):
+        api.WENOFilter(b2, 5, dissipation_sensor="Yee")
+    w = api.WENOFilter(b2, 5, dissipation_sensor="Constant", Mach_correction=True)
+    assert b2.shock_filter and w.sensor == "constant" and w.mach_correction
