@@ -24,11 +24,29 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# measured HBM bytes per launch of each kernel family (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes over this benchmark, scaled by a plain copy's readings);
+# the commands and raw figures are in profiles/hbm_traffic_weno_z_tgv256.md
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_weno_z_tgv256.json")
 FP64_PEAK_TFS = 78.6
 
 
 def _log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(family, n, conv):
+    """HBM bytes per launch of a kernel family from the committed measurement,
+    when it covers this workload; None otherwise."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("grid") != n or t.get("scheme") != conv:
+        return None
+    fam = t.get("families", {}).get(family)
+    return None if fam is None else fam.get("bytes_per_launch")
 
 
 def cpu_baseline(n, conv, steps=1):
@@ -124,7 +142,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": kb / avg_s / 1e9 / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(fam, n, args.conv),
         "avg_launch_ms": avg_s * 1e3,
         "launches": launches,
     }

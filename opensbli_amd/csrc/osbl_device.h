@@ -159,21 +159,40 @@ __device__ __forceinline__ void finish_avg(AvgT<ND, T>& v, const Phys& ph) {
   v.Hh = v.a * v.a / ph.gm1 + 0.5 * uu;
 }
 
+// the per-point factors of the Roe average: sqrt(rho) and
+// sqrt(rho) H = (rhoE + p)/sqrt(rho).  Sweeps that visit every point once
+// compute them once per point and pass them in (RoeF); the result is the same.
+struct RoeF {
+  double sL, hL, sR, hR;
+};
+template <int ND>
+__device__ __forceinline__ void roe_factors(const Pt<ND>& P, double& sq, double& h) {
+  sq = sqrt(P.q[0]);
+  h = (P.q[ND + 1] + P.p) / sq;
+}
+
+template <int ND>
+__device__ __forceinline__ void roe_state(Avg<ND>& v, const Pt<ND>& L, const Pt<ND>& R, const RoeF& f,
+                                          const Phys& ph) {
+  const double inv = 1.0 / (f.sR + f.sL);
+  v.rho = f.sL * f.sR;  // sqrt(rho_L rho_R)
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    v.u[m] = (f.sL * L.u[m] + f.sR * R.u[m]) * inv;
+    ke += v.u[m] * v.u[m];
+  }
+  const double hh = (f.hL + f.hR) * inv;
+  v.a = sqrt(ph.gm1 * (hh - 0.5 * ke));
+}
+
 template <int ND, int AVG>
 __device__ __forceinline__ void interface_state(Avg<ND>& v, const Pt<ND>& L, const Pt<ND>& R, const Phys& ph) {
   if constexpr (AVG == OSBL_AVG_ROE) {
-    const double sL = sqrt(L.q[0]), sR = sqrt(R.q[0]);
-    const double inv = 1.0 / (sR + sL);
-    v.rho = sL * sR;  // sqrt(rho_L rho_R)
-    double ke = 0.0;
-#pragma unroll
-    for (int m = 0; m < ND; ++m) {
-      v.u[m] = (sL * L.u[m] + sR * R.u[m]) * inv;
-      ke += v.u[m] * v.u[m];
-    }
-    // sqrt(rho) H = (rhoE + p)/sqrt(rho)
-    const double hh = ((L.q[ND + 1] + L.p) / sL + (R.q[ND + 1] + R.p) / sR) * inv;
-    v.a = sqrt(ph.gm1 * (hh - 0.5 * ke));
+    RoeF f;
+    roe_factors<ND>(L, f.sL, f.hL);
+    roe_factors<ND>(R, f.sR, f.hR);
+    roe_state<ND>(v, L, R, f, ph);
   } else {
     v.rho = 0.5 * (L.q[0] + R.q[0]);
 #pragma unroll
@@ -382,7 +401,9 @@ __device__ __forceinline__ double abs_lambda(const Pt<ND>& s) {
 // alpha_r = d_r (1 + tau^2/b_r^2), b_r = eps + beta_r; multiplying every
 // alpha_r by b0^2 b1^2 b2^2 leaves omega_r = alpha_r / sum(alpha) unchanged
 // and needs one division instead of four.
-__device__ __forceinline__ double weno5z(double fm2, double fm1, double f0, double f1, double f2) {
+// numerator and denominator of the WENO5-Z value (value = num/den)
+__device__ __forceinline__ void weno5z_nd(double fm2, double fm1, double f0, double f1, double f2, double& num,
+                                          double& den) {
   const double t0 = f0 - 2.0 * f1 + f2, s0 = 3.0 * f0 - 4.0 * f1 + f2;
   const double t1 = fm1 - 2.0 * f0 + f1, s1 = fm1 - f1;
   const double t2 = fm2 - 2.0 * fm1 + f0, s2 = fm2 - 4.0 * fm1 + 3.0 * f0;
@@ -401,12 +422,30 @@ __device__ __forceinline__ double weno5z(double fm2, double fm1, double f0, doub
   const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
   const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
   const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
-  return (A0 * r0 + A1 * r1 + A2 * r2) / (A0 + A1 + A2);
+  num = A0 * r0 + A1 * r1 + A2 * r2;
+  den = A0 + A1 + A2;
+}
+
+__device__ __forceinline__ double weno5z(double fm2, double fm1, double f0, double f1, double f2) {
+  double n, d;
+  weno5z_nd(fm2, fm1, f0, f1, f2, n, d);
+  return n / d;
+}
+
+// n1/d1 + n2/d2 with one division: (n1 d2 + n2 d1)/(d1 d2).  The weights'
+// denominators grow like beta^6 (at least ~1e-85 from eps), so d1 d2 is a
+// normal number unless |f| exceeds ~1e12; then the two quotients are formed
+// separately.
+__device__ __forceinline__ double sum_quotients(double n1, double d1, double n2, double d2) {
+  const double dd = d1 * d2;
+  if (dd < 1.0e300) return (n1 * d2 + n2 * d1) / dd;
+  return n1 / d1 + n2 / d2;
 }
 
 // WENO5-JS (weno.py:341-369) with eps = 1e-6: alpha_r = d_r/(eps+beta_r)^2,
 // rewritten as d_r * prod_{s!=r} (eps+beta_s)^2 with one division.
-__device__ __forceinline__ double weno5js(double fm2, double fm1, double f0, double f1, double f2) {
+__device__ __forceinline__ void weno5js_nd(double fm2, double fm1, double f0, double f1, double f2, double& num,
+                                           double& den) {
   const double t0 = f0 - 2.0 * f1 + f2, s0 = 3.0 * f0 - 4.0 * f1 + f2;
   const double t1 = fm1 - 2.0 * f0 + f1, s1 = fm1 - f1;
   const double t2 = fm2 - 2.0 * fm1 + f0, s2 = fm2 - 4.0 * fm1 + 3.0 * f0;
@@ -420,7 +459,14 @@ __device__ __forceinline__ double weno5js(double fm2, double fm1, double f0, dou
   const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
   const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
   const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
-  return (A0 * r0 + A1 * r1 + A2 * r2) / (A0 + A1 + A2);
+  num = A0 * r0 + A1 * r1 + A2 * r2;
+  den = A0 + A1 + A2;
+}
+
+__device__ __forceinline__ double weno5js(double fm2, double fm1, double f0, double f1, double f2) {
+  double n, d;
+  weno5js_nd(fm2, fm1, f0, f1, f2, n, d);
+  return n / d;
 }
 
 // WENO5-JS as above, also returning the shock sensor of its non-linear
@@ -537,9 +583,15 @@ __device__ __forceinline__ double teno(double fm2, double fm1, double f0, double
 template <int CONV>
 __device__ __forceinline__ double recon_pair(const double fp[6], const double fm[6], double eps, double ct) {
   if constexpr (CONV == OSBL_CONV_WENO_Z) {
-    return weno5z(fp[0], fp[1], fp[2], fp[3], fp[4]) + weno5z(fm[5], fm[4], fm[3], fm[2], fm[1]);
+    double n1, d1, n2, d2;
+    weno5z_nd(fp[0], fp[1], fp[2], fp[3], fp[4], n1, d1);
+    weno5z_nd(fm[5], fm[4], fm[3], fm[2], fm[1], n2, d2);
+    return sum_quotients(n1, d1, n2, d2);
   } else if constexpr (CONV == OSBL_CONV_WENO_JS) {
-    return weno5js(fp[0], fp[1], fp[2], fp[3], fp[4]) + weno5js(fm[5], fm[4], fm[3], fm[2], fm[1]);
+    double n1, d1, n2, d2;
+    weno5js_nd(fp[0], fp[1], fp[2], fp[3], fp[4], n1, d1);
+    weno5js_nd(fm[5], fm[4], fm[3], fm[2], fm[1], n2, d2);
+    return sum_quotients(n1, d1, n2, d2);
   } else if constexpr (CONV == OSBL_CONV_TENO6) {
     return teno<true, true>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct) +
            teno<true, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct);
@@ -555,11 +607,17 @@ __device__ __forceinline__ double recon_pair(const double fp[6], const double fm
 // through F = u_D Q + p (e_{1+D} + u_D e_last), which holds exactly for the
 // Euler flux vector.
 // ---------------------------------------------------------------------------
-template <int ND, int D, int CONV, int AVG>
-__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2], double ct) {
+template <int ND, int D, int CONV, int AVG, bool CACHED = false>
+__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2], double ct,
+                                          const RoeF& rf = RoeF{}) {
   constexpr int NV = ND + 2;
   Avg<ND> v;
-  interface_state<ND, AVG>(v, w[2], w[3], ph);
+  if constexpr (CACHED && AVG == OSBL_AVG_ROE) {
+    roe_state<ND>(v, w[2], w[3], rf, ph);
+    finish_avg<ND>(v, ph);
+  } else {
+    interface_state<ND, AVG>(v, w[2], w[3], ph);
+  }
 #pragma unroll
   for (int m = 0; m < NV; ++m) fh[m] = 0.0;
   sfor<0, NV>([&](auto kc) {

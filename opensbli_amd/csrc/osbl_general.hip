@@ -71,6 +71,12 @@ __device__ __forceinline__ double dxi(const Bnd& b, int d, int c, int N, int x, 
 // D_j -> G_j D, D_jj -> G_j^2 D2 + G_j S_j D, D_a D_b -> G_a G_b D_b(D_a).
 // P holds u_0..u_{ND-1}, p, T, mu.
 template <int ND>
+__device__ __forceinline__ void viscous_assemble(double mu, const double um_[3], const double (&DU)[ND][ND],
+                                                 const double (&DD)[ND][ND][ND], const double (&D2T)[ND],
+                                                 const double (&DT)[ND], const double (&DMU)[ND], const Phys& ph,
+                                                 bool suth, double vr[ND + 2]);
+
+template <int ND>
 __device__ __forceinline__ void viscous_gen(const double* __restrict__ P, long long fs, int x, const int st[3],
                                             const int ci[3], const Grid& g, const Bnd& b, const Phys& ph, bool suth,
                                             double vr[ND + 2]) {
@@ -135,7 +141,17 @@ __device__ __forceinline__ void viscous_gen(const double* __restrict__ P, long l
       DMU[j] = metric ? G[j] * dm1 : dm1;
     }
   }
-  const double mu = suth ? MU[x] : 1.0;
+  const double um[3] = {U[0][x], ND > 1 ? U[1][x] : 0.0, ND > 2 ? U[2][x] : 0.0};
+  viscous_assemble<ND>(suth ? MU[x] : 1.0, um, DU, DD, D2T, DT, DMU, ph, suth, vr);
+}
+
+// The viscous residual from the transformed derivatives at a point (the
+// second half of viscous_gen, shared with the tiled kernel).
+template <int ND>
+__device__ __forceinline__ void viscous_assemble(double mu, const double um_[3], const double (&DU)[ND][ND],
+                                                 const double (&DD)[ND][ND][ND], const double (&D2T)[ND],
+                                                 const double (&DT)[ND], const double (&DMU)[ND], const Phys& ph,
+                                                 bool suth, double vr[ND + 2]) {
   double divu = 0.0;
 #pragma unroll
   for (int k = 0; k < ND; ++k) divu += DU[k][k];
@@ -147,7 +163,7 @@ __device__ __forceinline__ void viscous_gen(const double* __restrict__ P, long l
   }
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
-    const double um = U[m][x];
+    const double um = um_[m];
     double visc = 0.0;
 #pragma unroll
     for (int j = 0; j < ND; ++j) {
@@ -261,6 +277,233 @@ __global__ void __launch_bounds__(128) k_update_gen(const double* __restrict__ Q
       Qn[m * g.fs + x] = Q[m * g.fs + x] + cB * tmp;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// 3-D viscous terms of the general path, tiled 2.5-D like k_visc_update3d: a
+// 64 x 4 block of (x, y) columns marches along z; u_0..u_2, T (and mu) of each
+// z plane are staged from P into one of three rotating LDS slots (2-point
+// x/y halo); the z stencils read a per-thread queue of the planes z-2..z+2
+// holding u, T, mu and the inner derivatives D_x u_0, D_x u_2, D_y u_1,
+// D_y u_2 of the mixed terms.  The viscous terms are assigned to R (the
+// stage's first residual contribution).  Points inside the closure rows of
+// an x or y face are left to k_visc_shell; z must have neither closures nor a
+// metric (checked by the caller).  The derivative and assembly formulas are
+// those of viscous_gen.
+// ---------------------------------------------------------------------------
+constexpr int GTX = 64, GTY = 4, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
+
+// closure rows of the first (Carpenter 4, ReducedAccess 2) and second (2) derivatives
+__host__ __device__ __forceinline__ int closure_rows(int mod) { return mod == 0 ? 0 : (mod == 1 ? 4 : 2); }
+
+template <bool SUTH>
+__global__ void __launch_bounds__(GTX * GTY, 2) k_visc_gen3d(const double* __restrict__ P, double* __restrict__ R,
+                                                            Grid g, Phys ph, Bnd b, int sz) {
+  constexpr int NF = SUTH ? 5 : 4;  // u0, u1, u2, T (, mu)
+  constexpr int NQ = NF + 4;        // + D_x u0, D_x u2, D_y u1, D_y u2
+  __shared__ double sp[3][NF][GPL];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * GTX + tx;
+  const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
+  const int z0 = blockIdx.z * sz;
+  const int z1 = min(z0 + sz, g.n[2]);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool inside = x < g.n[0] && y < g.n[1];
+  const double ix = ph.inv_delta[0], iy = ph.inv_delta[1], iz = ph.inv_delta[2];
+  const bool slow = x < closure_rows(b.mod[0][0]) || g.n[0] - 1 - x < closure_rows(b.mod[0][1]) ||
+                    y < closure_rows(b.mod[1][0]) || g.n[1] - 1 - y < closure_rows(b.mod[1][1]);
+  const bool metric = b.dm[0] != nullptr || b.dm[1] != nullptr;
+  const double G0 = (b.dm[0] && inside) ? b.dm[0][x] : 1.0, S0 = (b.sd[0] && inside) ? b.sd[0][x] : 0.0;
+  const double G1 = (b.dm[1] && inside) ? b.dm[1][y] : 1.0, S1 = (b.sd[1] && inside) ? b.sd[1][y] : 0.0;
+
+  auto stage = [&](int zp, int slot) {
+    for (int p = tid; p < GPL; p += GTX * GTY) {
+      const int xx = p % GPX, yy = p / GPX;
+      const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
+      if (gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1]) {
+        const int o = g.idx(gx, gy, zp);
+        sp[slot][0][p] = P[o];
+        sp[slot][1][p] = P[g.fs + o];
+        sp[slot][2][p] = P[2 * g.fs + o];
+        sp[slot][3][p] = P[4 * g.fs + o];
+        if constexpr (SUTH) sp[slot][4][p] = P[5 * g.fs + o];
+      }
+    }
+  };
+  const int c = (ty + 2) * GPX + tx + 2;
+  auto d1 = [](double fm2, double fm1, double fp1, double fp2) {
+    return (1.0 / 12.0) * (fm2 - fp2) + (2.0 / 3.0) * (fp1 - fm1);
+  };
+  auto d2 = [](double fm2, double fm1, double f0, double fp1, double fp2) {
+    return (-1.0 / 12.0) * (fm2 + fp2) + (4.0 / 3.0) * (fm1 + fp1) - 2.5 * f0;
+  };
+  auto dx1 = [&](const double* f, int o) { return d1(f[o - 2], f[o - 1], f[o + 1], f[o + 2]); };
+  auto dy1 = [&](const double* f, int o) { return d1(f[o - 2 * GPX], f[o - GPX], f[o + GPX], f[o + 2 * GPX]); };
+  auto dx2 = [&](const double* f, int o) { return d2(f[o - 2], f[o - 1], f[o], f[o + 1], f[o + 2]); };
+  auto dy2 = [&](const double* f, int o) {
+    return d2(f[o - 2 * GPX], f[o - GPX], f[o], f[o + GPX], f[o + 2 * GPX]);
+  };
+  auto column = [&](int slot, double* cq) {
+#pragma unroll
+    for (int e = 0; e < NF; ++e) cq[e] = sp[slot][e][c];
+    cq[NF + 0] = ix * dx1(sp[slot][0], c);
+    cq[NF + 1] = ix * dx1(sp[slot][2], c);
+    cq[NF + 2] = iy * dy1(sp[slot][1], c);
+    cq[NF + 3] = iy * dy1(sp[slot][2], c);
+  };
+
+  // one loop stages every plane zs = z0-2 .. z1+1 (slot (zs+3) % 3) and
+  // appends its column quantities to the queue, so every plane's are computed
+  // by the same code; plane z = zs - 2 is finished once z+2 is in
+  double zq[5][NQ] = {};
+  for (int zs = z0 - 2; zs < z1 + 2; ++zs) {
+    __syncthreads();
+    stage(zs, (zs + 3) % 3);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int e = 0; e < NQ; ++e) zq[k][e] = zq[k + 1][e];
+    column((zs + 3) % 3, zq[4]);
+    const int z = zs - 2;
+    if (z < z0 || !inside || slow) continue;
+    const int o = g.idx(x, y, z);
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    {
+      const int sl = (z + 3) % 3;
+      const double* U[3] = {sp[sl][0], sp[sl][1], sp[sl][2]};
+      const double* T = sp[sl][3];
+      const double G[3] = {G0, G1, 1.0}, S[3] = {S0, S1, 0.0};
+      auto qz1 = [&](int e) { return iz * d1(zq[0][e], zq[1][e], zq[3][e], zq[4][e]); };
+      auto qz2 = [&](int e) { return iz * iz * d2(zq[0][e], zq[1][e], zq[2][e], zq[3][e], zq[4][e]); };
+      double DU[3][3], DD[3][3][3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double du[3], dd[3];
+        du[0] = ix * dx1(U[m], c);
+        dd[0] = ix * ix * dx2(U[m], c);
+        du[1] = iy * dy1(U[m], c);
+        dd[1] = iy * iy * dy2(U[m], c);
+        du[2] = qz1(m);
+        dd[2] = qz2(m);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          DU[m][j] = metric ? G[j] * du[j] : du[j];
+          DD[m][j][j] = metric ? G[j] * G[j] * dd[j] + G[j] * S[j] * du[j] : dd[j];
+        }
+      }
+      // mixed D_y(D_x u_m), m = 0, 1
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const double* f = U[m];
+        double v = iy * d1(ix * dx1(f, c - 2 * GPX), ix * dx1(f, c - GPX), ix * dx1(f, c + GPX),
+                           ix * dx1(f, c + 2 * GPX));
+        if (metric) v = G[0] * G[1] * v;
+        DD[m][0][1] = v;
+        DD[m][1][0] = v;
+      }
+      // mixed D_z(D_x u_m), m = 0, 2 and D_z(D_y u_m), m = 1, 2 from the queue
+      {
+        double v = qz1(NF + 0);
+        if (metric) v = G[0] * G[2] * v;
+        DD[0][0][2] = DD[0][2][0] = v;
+        v = qz1(NF + 1);
+        if (metric) v = G[0] * G[2] * v;
+        DD[2][0][2] = DD[2][2][0] = v;
+        v = qz1(NF + 2);
+        if (metric) v = G[1] * G[2] * v;
+        DD[1][1][2] = DD[1][2][1] = v;
+        v = qz1(NF + 3);
+        if (metric) v = G[1] * G[2] * v;
+        DD[2][1][2] = DD[2][2][1] = v;
+      }
+      double D2T[3], DT[3], DMU[3];
+      {
+        double t1[3], t2[3];
+        t1[0] = ix * dx1(T, c);
+        t2[0] = ix * ix * dx2(T, c);
+        t1[1] = iy * dy1(T, c);
+        t2[1] = iy * iy * dy2(T, c);
+        t1[2] = qz1(3);
+        t2[2] = qz2(3);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          DT[j] = 0.0;
+          DMU[j] = 0.0;
+          if (SUTH || metric) {
+            DT[j] = metric ? G[j] * t1[j] : t1[j];
+            D2T[j] = metric ? G[j] * G[j] * t2[j] + G[j] * S[j] * t1[j] : t2[j];
+          } else {
+            D2T[j] = t2[j];
+          }
+        }
+        if constexpr (SUTH) {
+          const double* MU = sp[sl][4];
+          const double m1[3] = {ix * dx1(MU, c), iy * dy1(MU, c), qz1(4)};
+#pragma unroll
+          for (int j = 0; j < 3; ++j) DMU[j] = metric ? G[j] * m1[j] : m1[j];
+        }
+      }
+      const double um[3] = {zq[2][0], zq[2][1], zq[2][2]};
+      viscous_assemble<3>(SUTH ? zq[2][4] : 1.0, um, DU, DD, D2T, DT, DMU, ph, SUTH, r);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) R[m * g.fs + o] = r[m];
+  }
+}
+
+// The viscous terms (assigned to R) at the points k_visc_gen3d leaves out:
+// the closure rows of the x or y faces.  part 0: rows y in ys[] (all x);
+// part 1: columns x in xs[] at the other rows.
+struct ShellRows {
+  int ys[8], ny, xs[8], nx;
+};
+
+__global__ void __launch_bounds__(128) k_visc_shell(const double* __restrict__ P, double* __restrict__ R, Grid g,
+                                                    Phys ph, Bnd b, int suth, ShellRows sr, int part) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.z;
+  int i, j;
+  if (part == 0) {
+    if (t >= g.n[0]) return;
+    i = t;
+    j = sr.ys[blockIdx.y];
+  } else {
+    if (t >= g.n[1]) return;
+    for (int q = 0; q < sr.ny; ++q)
+      if (sr.ys[q] == t) return;
+    i = sr.xs[blockIdx.y];
+    j = t;
+  }
+  const int x = g.idx(i, j, k);
+  const int st[3] = {1, g.pitch, g.plane};
+  const int ci[3] = {i, j, k};
+  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  viscous_gen<3>(P, g.fs, x, st, ci, g, b, ph, suth != 0, r);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) R[m * g.fs + x] = r[m];
+}
+
+void launch_visc_gen3d(const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b, int suth, int sz,
+                       hipStream_t s) {
+  dim3 grid((g.n[0] + GTX - 1) / GTX, (g.n[1] + GTY - 1) / GTY, (g.n[2] + sz - 1) / sz);
+  dim3 block(GTX, GTY);
+  if (suth) k_visc_gen3d<true><<<grid, block, 0, s>>>(P, R, g, ph, b, sz);
+  else k_visc_gen3d<false><<<grid, block, 0, s>>>(P, R, g, ph, b, sz);
+  // the closure rows and columns (distinct indices; a face's rows may overlap
+  // the other face's on tiny blocks, hence the de-duplication)
+  ShellRows sr{};
+  auto add = [](int* list, int& n, int v) {
+    for (int q = 0; q < n; ++q)
+      if (list[q] == v) return;
+    list[n++] = v;
+  };
+  for (int r = 0; r < closure_rows(b.mod[1][0]); ++r) add(sr.ys, sr.ny, r);
+  for (int r = 0; r < closure_rows(b.mod[1][1]); ++r) add(sr.ys, sr.ny, g.n[1] - 1 - r);
+  for (int r = 0; r < closure_rows(b.mod[0][0]); ++r) add(sr.xs, sr.nx, r);
+  for (int r = 0; r < closure_rows(b.mod[0][1]); ++r) add(sr.xs, sr.nx, g.n[0] - 1 - r);
+  if (sr.ny) k_visc_shell<<<dim3((g.n[0] + 127) / 128, sr.ny, g.n[2]), 128, 0, s>>>(P, R, g, ph, b, suth, sr, 0);
+  if (sr.nx) k_visc_shell<<<dim3((g.n[1] + 127) / 128, sr.nx, g.n[2]), 128, 0, s>>>(P, R, g, ph, b, suth, sr, 1);
 }
 
 // Constituent relations with the Sutherland viscosity over the padded domain:

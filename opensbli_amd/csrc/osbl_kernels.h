@@ -10,6 +10,8 @@ void launch_extrap0(double* Q, int nv, const Grid& g, int d, int side, hipStream
 void launch_prims(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_sweep(int nd, int d, int conv, int avg, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
                   const double* met, const double* theta, bool acc, hipStream_t s);
+void launch_sweep_z_rk(int conv, int avg, int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g,
+                       const Phys& ph, int seg, const double* theta, double cA, double cB, int first, hipStream_t s);
 void launch_sensor(int nd, const double* Q, double* theta, const Grid& g, const Phys& ph, const Bnd& b, hipStream_t s);
 void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R, double* Qo,
                    double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s);
@@ -18,6 +20,8 @@ void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, doub
 void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R,
                        double* Qo, double* Qn, const Grid& g, const Phys& ph, const Bnd& b, double cA, double cB,
                        int first, int suth, hipStream_t s);
+void launch_visc_gen3d(const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b, int suth, int sz,
+                       hipStream_t s);
 void launch_prims_mu(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_bc_face(int nd, double* Q, double* Q2, const Grid& g, const Phys& ph, int d, int side, int type, int nh,
                     double twall, const double* plane, hipStream_t s);

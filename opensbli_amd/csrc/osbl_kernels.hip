@@ -113,12 +113,24 @@ __global__ void __launch_bounds__(256) k_prims(const double* __restrict__ Q, dou
 // once, and the residual of a cell is the difference of its two interface
 // fluxes (shock_capturing.py:21-34).  Threads of a wave are adjacent in x,
 // except for D = 0, where they are adjacent lines.
+//
+// RKM >= 0: the sweep is the stage's last contribution to the residual, so a
+// cell's complete residual R + (this direction's difference) goes straight
+// into the RK update (RKM = 0: 3-register stage, 1: low-storage stage, as in
+// k_update) and R is not written back.
 // ---------------------------------------------------------------------------
-template <int ND, int D, int CONV, int AVG, bool ACC>
+struct RkArgs {
+  double* Qo;
+  double* Qn;
+  double cA, cB;
+  int first;
+};
+
+template <int ND, int D, int CONV, int AVG, bool ACC, int RKM = -1>
 __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q, double* __restrict__ R, Grid g,
                                                      Phys ph, int seg, long long total,
                                                      const double* __restrict__ met,
-                                                     const double* __restrict__ theta) {
+                                                     const double* __restrict__ theta, RkArgs rk) {
   constexpr int NV = ND + 2;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -150,6 +162,9 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
   double nxt[NV];
   fetch_q<ND>(nxt, Q, g.fs, off + 5 * st);
   double fprev[NV];
+  // Roe factors of the interface's two points, carried along the window
+  [[maybe_unused]] RoeF rf{};
+  if constexpr (AVG == OSBL_AVG_ROE) roe_factors<ND>(w[2], rf.sL, rf.hL);
   for (int f = c0 - 1; f < c1; ++f) {
 #pragma unroll
     for (int m = 0; m < NV; ++m) w[5].q[m] = nxt[m];
@@ -158,7 +173,12 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
     double fh[NV];
     // interface between window points 2 and 3: its left point is at off + 2 st
     const double ct = theta ? adaptive_ct(theta[off + 2 * st], ph) : ph.teno_ct;
-    char_flux<ND, D, CONV, AVG>(w, ph, fh, ct);
+    if constexpr (AVG == OSBL_AVG_ROE) roe_factors<ND>(w[3], rf.sR, rf.hR);
+    char_flux<ND, D, CONV, AVG, true>(w, ph, fh, ct, rf);
+    if constexpr (AVG == OSBL_AVG_ROE) {
+      rf.sL = rf.sR;
+      rf.hL = rf.hR;
+    }
     if (f >= c0) {
       const int x = off + 2 * st;
       // a stretched direction scales the flux difference by D_dd (opensblifunctions.py:607-616)
@@ -166,8 +186,21 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
         const double v = met ? -(gm * ((fh[m] - fprev[m]) * inv)) : -(fh[m] - fprev[m]) * inv;
-        if constexpr (ACC) R[m * g.fs + x] += v;
-        else R[m * g.fs + x] = v;
+        if constexpr (RKM == 0) {
+          const double r = R[m * g.fs + x] + v;
+          const double qo = rk.first ? w[2].q[m] : rk.Qo[m * g.fs + x];
+          rk.Qn[m * g.fs + x] = qo + ph.dt * rk.cA * r;
+          rk.Qo[m * g.fs + x] = qo + ph.dt * rk.cB * r;
+        } else if constexpr (RKM == 1) {
+          const double r = R[m * g.fs + x] + v;
+          const double tmp = (rk.cA == 0.0 ? 0.0 : rk.cA * rk.Qo[m * g.fs + x]) + ph.dt * r;
+          rk.Qo[m * g.fs + x] = tmp;
+          rk.Qn[m * g.fs + x] = w[2].q[m] + rk.cB * tmp;
+        } else if constexpr (ACC) {
+          R[m * g.fs + x] += v;
+        } else {
+          R[m * g.fs + x] = v;
+        }
       }
     }
 #pragma unroll
@@ -198,7 +231,8 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
                                                           const double* __restrict__ met,
                                                           const double* __restrict__ theta) {
   constexpr int NV = ND + 2;
-  constexpr int NF = NV + ND + 2;  // q, u, p, a
+  constexpr bool ROE = AVG == OSBL_AVG_ROE;
+  constexpr int NF = NV + ND + 2 + (ROE ? 2 : 0);  // q, u, p, a (, sqrt(rho), sqrt(rho) H)
   __shared__ double sh[XW][NF][XP];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int row = blockIdx.x * XW + wv;
@@ -216,6 +250,7 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
     for (int m = 0; m < ND; ++m) S[NV + m][slot] = p.u[m];
     S[NV + ND][slot] = p.p;
     S[NV + ND + 1][slot] = p.a;
+    if constexpr (ROE) roe_factors<ND>(p, S[NV + ND + 2][slot], S[NV + ND + 3][slot]);
   };
   auto get_point = [&](Pt<ND>& p, int slot) {
 #pragma unroll
@@ -232,7 +267,14 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
 #pragma unroll
     for (int q = 0; q < 6; ++q) get_point(w[q], lane + q);
     const double ct = theta ? adaptive_ct(theta[base + min(xi, xmax)], ph) : ph.teno_ct;
-    char_flux<ND, 0, CONV, AVG>(w, ph, fh, ct);
+    [[maybe_unused]] RoeF rf{};
+    if constexpr (ROE) {
+      rf.sL = S[NV + ND + 2][lane + 2];
+      rf.hL = S[NV + ND + 3][lane + 2];
+      rf.sR = S[NV + ND + 2][lane + 3];
+      rf.hR = S[NV + ND + 3][lane + 3];
+    }
+    char_flux<ND, 0, CONV, AVG, true>(w, ph, fh, ct, rf);
   };
   auto stage_chunk = [&](int x0) {
     // slots 0..68 hold points x0-2 .. x0+66 (clamped to the stored line: the
@@ -448,6 +490,7 @@ __device__ __forceinline__ void central_convective(const double* __restrict__ Q,
 //   MODE 1: low-storage stage (rk_LS.py:155-166): Qo = cA*Qo + dt*R,
 //           Qn = Q + cB*Qo (Qo not read when cA == 0).
 //   MODE 2: residual only (written to R).
+//   MODE 3 (k_visc_update3d only): the viscous terms alone, assigned to R.
 // For characteristic schemes the convective residual is already in R.
 // ---------------------------------------------------------------------------
 template <int ND, int CONV, bool VISC, int MODE>
@@ -623,7 +666,7 @@ __global__ void __launch_bounds__(256, 2) k_visc_update3d(const double* __restri
     double r[5];
     const int o = g.idx(x, y, z);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) r[m] = R[m * g.fs + o];
+    for (int m = 0; m < 5; ++m) r[m] = MODE == 3 ? 0.0 : R[m * g.fs + o];
     double energy = ph.cq * (ix * ix * dx2(T, c) + iy * iy * dy2(T, c) +
                              iz * iz * ((-1.0 / 12.0) * (zq[0][3] + zq[4][3]) + (4.0 / 3.0) * (zq[1][3] + zq[3][3]) -
                                         2.5 * zq[2][3]));
@@ -646,7 +689,7 @@ __global__ void __launch_bounds__(256, 2) k_visc_update3d(const double* __restri
       r[1 + m] += visc;
     }
     r[4] += energy;
-    if constexpr (MODE == 2) {
+    if constexpr (MODE == 2 || MODE == 3) {
 #pragma unroll
       for (int m = 0; m < 5; ++m) R[m * g.fs + o] = r[m];
     } else if constexpr (MODE == 0) {
@@ -673,7 +716,8 @@ void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, doub
   dim3 block(VTX, VTY);
   if (mode == 0) k_visc_update3d<0><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
   else if (mode == 1) k_visc_update3d<1><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
-  else k_visc_update3d<2><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
+  else if (mode == 2) k_visc_update3d<2><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
+  else k_visc_update3d<3><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
 }
 
 // interior <-> contiguous copies
@@ -711,8 +755,41 @@ static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys
   const long long nseg = (g.n[D] + seg - 1) / seg;
   const long long total = lines * nseg;
   const unsigned blocks = (unsigned)((total + 255) / 256);
-  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met, theta);
-  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met, theta);
+  const RkArgs none{nullptr, nullptr, 0.0, 0.0, 0};
+  if (acc) k_char_sweep<ND, D, CONV, AVG, true><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met, theta, none);
+  else k_char_sweep<ND, D, CONV, AVG, false><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, met, theta, none);
+}
+
+// the last (z) sweep of a 3-D stage fused with the RK update
+template <int CONV, int AVG>
+static void launch_sweep_rk_t(int mode, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                              const double* theta, const RkArgs& rk, hipStream_t s) {
+  const long long lines = (long long)g.n[0] * g.n[1];
+  const long long nseg = (g.n[2] + seg - 1) / seg;
+  const long long total = lines * nseg;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  if (mode == 0)
+    k_char_sweep<3, 2, CONV, AVG, true, 0><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, nullptr, theta, rk);
+  else
+    k_char_sweep<3, 2, CONV, AVG, true, 1><<<blocks, 256, 0, s>>>(Q, R, g, ph, seg, total, nullptr, theta, rk);
+}
+
+template <int CONV>
+static void launch_sweep_rk_avg(int avg, int mode, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
+                                const double* theta, const RkArgs& rk, hipStream_t s) {
+  if (avg == OSBL_AVG_ROE) launch_sweep_rk_t<CONV, OSBL_AVG_ROE>(mode, Q, R, g, ph, seg, theta, rk, s);
+  else launch_sweep_rk_t<CONV, OSBL_AVG_SIMPLE>(mode, Q, R, g, ph, seg, theta, rk, s);
+}
+
+void launch_sweep_z_rk(int conv, int avg, int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g,
+                       const Phys& ph, int seg, const double* theta, double cA, double cB, int first, hipStream_t s) {
+  const RkArgs rk{Qo, Qn, cA, cB, first};
+  switch (conv) {
+    case OSBL_CONV_WENO_JS: launch_sweep_rk_avg<OSBL_CONV_WENO_JS>(avg, mode, Q, R, g, ph, seg, theta, rk, s); break;
+    case OSBL_CONV_WENO_Z: launch_sweep_rk_avg<OSBL_CONV_WENO_Z>(avg, mode, Q, R, g, ph, seg, theta, rk, s); break;
+    case OSBL_CONV_TENO5: launch_sweep_rk_avg<OSBL_CONV_TENO5>(avg, mode, Q, R, g, ph, seg, theta, rk, s); break;
+    default: launch_sweep_rk_avg<OSBL_CONV_TENO6>(avg, mode, Q, R, g, ph, seg, theta, rk, s); break;
+  }
 }
 
 template <int ND, int D, int CONV>

@@ -194,6 +194,7 @@ struct osbl_ctx {
   int vjp_seg = 8;          // cells per thread of the characteristic VJP sweeps
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
+  bool tiled_gen = false;   // the same on the general path (closures/metric/mu(T) in x and y only)
   bool general = false;     // closures, metric or Sutherland viscosity: the general update kernel
   bool suth = false;
   Bnd bnd{};
@@ -313,6 +314,41 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
     });
   }
   if (c->Theta) timed(c, F_PRIMS, [&] { launch_sensor(c->nd, Q, c->Theta, c->g, c->ph, c->bnd, c->stream); });
+  if (c->tiled_visc && mode != 2) {
+    // 3-D characteristic scheme with viscous terms: the viscous kernel assigns
+    // R, the x and y sweeps accumulate, and the z sweep adds the last
+    // difference and applies the RK update, so the update needs no pass of its own
+    timed(c, F_UPDATE, [&] { launch_visc_update3d(3, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->vsz, c->stream); });
+    for (int d = 0; d < 2; ++d) {
+      timed(c, F_SWEEP_X + d, [&] {
+        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, nullptr, c->Theta, true,
+                     c->stream);
+      });
+    }
+    timed(c, F_SWEEP_Z, [&] {
+      launch_sweep_z_rk(conv, c->cfg.averaging, mode, Q, c->R, c->Qo, Qn, c->g, c->ph, c->seg, c->Theta, cA, cB,
+                        first, c->stream);
+    });
+    return;
+  }
+  if (c->tiled_gen && mode != 2) {
+    // the same schedule on the general path: tiled viscous terms with closures,
+    // metric and mu(T) (z free of both), sweeps with the metric, RK in the z sweep
+    timed(c, F_UPDATE, [&] {
+      launch_visc_gen3d(c->P, c->R, c->g, c->ph, c->bnd, c->suth ? 1 : 0, c->vsz, c->stream);
+    });
+    for (int d = 0; d < 2; ++d) {
+      timed(c, F_SWEEP_X + d, [&] {
+        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[d], c->Theta, true,
+                     c->stream);
+      });
+    }
+    timed(c, F_SWEEP_Z, [&] {
+      launch_sweep_z_rk(conv, c->cfg.averaging, mode, Q, c->R, c->Qo, Qn, c->g, c->ph, c->seg, c->Theta, cA, cB,
+                        first, c->stream);
+    });
+    return;
+  }
   if (!is_central(conv)) {
     for (int d = 0; d < c->nd; ++d) {
       timed(c, F_SWEEP_X + d, [&] {
@@ -491,6 +527,8 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   }
   c->general = closures || stretched || c->suth;
   c->tiled_visc = !c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE;
+  c->tiled_gen = c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE &&
+                 !c->bnd.mod[2][0] && !c->bnd.mod[2][1] && !cfg->stretch[2];
   c->need_prims = is_central(cfg->conv) || (cfg->viscous != OSBL_VISC_NONE && !c->tiled_visc);
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t bundle = (size_t)fs * c->nv * sizeof(double);
@@ -942,11 +980,20 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
   switch (family) {
     case F_BC: b = 16.0 * nv * (double)halo_points(c->g); break;
     case F_PRIMS: b = 8.0 * (nv + c->nd + 2) * n; f = 6.0 * nv * n; break;
-    case F_SWEEP_X: b = 16.0 * nv * n; f = interface_flops(c) * n; break;
-    case F_SWEEP_Y:
-    case F_SWEEP_Z: b = 24.0 * nv * n; f = interface_flops(c) * n; break;
+    case F_SWEEP_X: b = (c->tiled_visc || c->tiled_gen ? 24.0 : 16.0) * nv * n; f = interface_flops(c) * n; break;
+    case F_SWEEP_Y: b = 24.0 * nv * n; f = interface_flops(c) * n; break;
+    case F_SWEEP_Z:
+      // with the fused RK update: read Q, R and the register, write Q and the register
+      b = (c->tiled_visc || c->tiled_gen ? 40.0 : 24.0) * nv * n;
+      f = (interface_flops(c) + (c->tiled_visc || c->tiled_gen ? 6.0 * nv : 0.0)) * n;
+      break;
     case F_FILTER: b = 16.0 * nv * n; break;
     case F_UPDATE: {
+      if (c->tiled_visc || c->tiled_gen) {  // viscous terms only: read Q (or the CRs), write R
+        b = 16.0 * nv * n;
+        f = viscous_flops(c) * n;
+        break;
+      }
       // read Q (central) or R, the constituent relations when viscous, the RK register; write Q and register
       double rd = (is_central(c->cfg.conv) ? nv : nv) + (c->need_prims ? c->nd + 2 : 0) + nv;
       b = 8.0 * (rd + 2.0 * nv) * n;

@@ -1,0 +1,126 @@
+"""Summaries of rocprofv3 hardware-counter runs of the time loop.
+
+HBM traffic per kernel launch comes from two counter passes over the same
+program (FETCH_SIZE, then WRITE_SIZE, both in KiB per dispatch), each scaled
+by what the same counter reads for a plain device-to-device copy of known size
+(the counters' definitions count 64-byte requests and bubbles, so a copy's
+reading is not its byte count).  Kernels are grouped into the families
+osbl_kernel_stats reports.
+
+    python -m opensbli_amd.profiling --fetch pf_counter_collection.csv \\
+        --write pw_counter_collection.csv --copy-fetch cf_counter_collection.csv \\
+        --copy-write cw_counter_collection.csv --copy-bytes 1073741824 \\
+        --grid 256 --scheme weno_z --json out.json --md out.md
+"""
+import argparse
+import collections
+import csv
+import json
+
+# kernel-name fragments -> family of osbl_kernel_stats (first match wins)
+FAMILIES = [
+    ("k_char_sweep_xl", "char_sweep_x"),
+    ("k_char_sweep<3, 0,", "char_sweep_x"),
+    ("k_char_sweep<3, 1,", "char_sweep_y"),
+    ("k_char_sweep<3, 2,", "char_sweep_z"),
+    ("k_visc_update3d", "residual_rk_update"),
+    ("k_visc_gen3d", "residual_rk_update"),
+    ("k_visc_shell", "residual_rk_update"),
+    ("k_update", "residual_rk_update"),
+    ("k_prims", "constituent_relations"),
+    ("k_sensor", "constituent_relations"),
+    ("k_halo_periodic", "boundary"),
+    ("k_periodic_dir", "boundary"),
+    ("k_bc_face", "boundary"),
+    ("k_extrap0", "boundary"),
+]
+
+
+def family(kernel_name):
+    for frag, fam in FAMILIES:
+        if frag in kernel_name:
+            return fam
+    return None
+
+
+def read_counter(path, counter):
+    """{kernel name: [value per dispatch]} of one counter from a rocprofv3
+    counter_collection.csv."""
+    out = collections.defaultdict(list)
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter:
+                out[row["Kernel_Name"]].append(float(row["Counter_Value"]))
+    return out
+
+
+def copy_factor(path, counter, true_bytes):
+    """true bytes / counted bytes for the largest copy dispatch in the file."""
+    vals = read_counter(path, counter)
+    best = 0.0
+    for name, v in vals.items():
+        if "copy" in name.lower():
+            best = max(best, max(v))
+    if best <= 0.0:
+        raise ValueError("no copy kernel with %s in %s" % (counter, path))
+    return true_bytes / (best * 1024.0)
+
+
+def summarise(fetch, write, copy_fetch, copy_write, copy_bytes):
+    ff = copy_factor(copy_fetch, "FETCH_SIZE", copy_bytes)
+    fw = copy_factor(copy_write, "WRITE_SIZE", copy_bytes)
+    fe, wr = read_counter(fetch, "FETCH_SIZE"), read_counter(write, "WRITE_SIZE")
+    fams = collections.defaultdict(lambda: {"fetch": [], "write": [], "kernels": set()})
+    for name, vals in fe.items():
+        fam = family(name)
+        if fam:
+            fams[fam]["fetch"] += vals
+            fams[fam]["kernels"].add(name.split("(")[0])
+    for name, vals in wr.items():
+        fam = family(name)
+        if fam:
+            fams[fam]["write"] += vals
+    out = {}
+    for fam, d in fams.items():
+        if not d["fetch"] or not d["write"]:
+            continue
+        f_b = ff * 1024.0 * sum(d["fetch"]) / len(d["fetch"])
+        w_b = fw * 1024.0 * sum(d["write"]) / len(d["write"])
+        out[fam] = {"fetch_bytes": f_b, "write_bytes": w_b, "bytes_per_launch": f_b + w_b,
+                    "dispatches": len(d["fetch"]), "kernels": sorted(d["kernels"])}
+    return out, {"fetch_factor": ff, "write_factor": fw, "copy_bytes": copy_bytes}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--copy-fetch", required=True)
+    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--copy-bytes", type=float, required=True, help="bytes the copy reads (and writes)")
+    ap.add_argument("--grid", type=int, required=True)
+    ap.add_argument("--scheme", required=True)
+    ap.add_argument("--json")
+    ap.add_argument("--md")
+    a = ap.parse_args()
+    fams, cal = summarise(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes)
+    doc = {"grid": a.grid, "scheme": a.scheme, "calibration": cal, "families": fams}
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(doc, f, indent=1, sort_keys=True)
+    cells = float(a.grid) ** 3
+    lines = ["| family | fetch B/launch | write B/launch | total B/cell | dispatches |", "|---|---|---|---|---|"]
+    for fam in sorted(fams):
+        d = fams[fam]
+        lines.append("| %s | %.4g | %.4g | %.1f | %d |" % (fam, d["fetch_bytes"], d["write_bytes"],
+                                                         d["bytes_per_launch"] / cells, d["dispatches"]))
+    table = "\n".join(lines)
+    if a.md:
+        with open(a.md, "w") as f:
+            f.write(table + "\n")
+    print(table)
+    print("calibration: FETCH x%.3f, WRITE x%.3f" % (cal["fetch_factor"], cal["write_factor"]))
+
+
+if __name__ == "__main__":
+    main()

@@ -169,3 +169,47 @@ def test_flat_plate_script_on_gpu(oracle):
     err = rel_l2(q, qr)
     _say("rel L2", err, "oracle self-spread", spread)
     assert np.all(err < np.maximum(TOL, 4.0 * spread)), (err, spread)
+
+
+def _smooth_state_3d(n, gama=1.4, Minf=0.5, seed=0):
+    rs = np.random.RandomState(seed)
+    z, y, x = np.meshgrid(*[np.linspace(0, 1, m) for m in reversed(n)], indexing="ij")
+    a = rs.uniform(0.5, 1.5, 8)
+    rho = 1.0 + 0.05 * np.sin(2 * np.pi * a[0] * x) * np.cos(np.pi * a[1] * y) * np.cos(2 * np.pi * z)
+    u = 0.3 + 0.05 * np.cos(2 * np.pi * a[2] * x) * np.sin(np.pi * a[3] * y)
+    v = 0.03 * np.sin(2 * np.pi * a[4] * x) * np.sin(np.pi * y) * np.cos(2 * np.pi * z)
+    w = 0.04 * np.cos(2 * np.pi * a[6] * x) * np.sin(2 * np.pi * z)
+    p = 1.0 / (gama * Minf ** 2) * (1.0 + 0.05 * np.cos(2 * np.pi * a[5] * x) * np.cos(np.pi * y))
+    E = p / (gama - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, E])
+
+
+@pytest.mark.parametrize("case", ["xy_closures_const", "xy_closures_sutherland_stretch", "z_walls"])
+def test_general_3d_viscous_schedules(case, oracle):
+    """3-D characteristic blocks on the general path: closures in x and y (the
+    tiled viscous kernel with its closure shell, the RK update in the z
+    sweep), and walls in z (the per-point general kernel)."""
+    n = (36, 28, 20)
+    _say(f"case general 3-D {case}")
+    q0 = _smooth_state_3d(n)
+    bs = [["carpenter", "reduced_access"], ["reduced_access", "carpenter"], ["carpenter", "carpenter"]]
+    if case == "z_walls":
+        bc = [["periodic"] * 2, ["periodic"] * 2, ["adiabatic_wall", "isothermal_wall"]]
+        q0[4] -= 0.5 * q0[3] ** 2 / q0[0]
+        q0[3] = 0.0
+        visc, stretch = "const", None
+    else:
+        top = "extrap1" if "stretch" in case else "zero_gradient"  # a stretched direction needs closures on both faces
+        bc = [["extrap0", "extrap1"], ["adiabatic_wall", top], ["periodic"] * 2]
+        for row in (0,):
+            q0[4, :, row, :] -= 0.5 * (q0[1, :, row, :] ** 2 + q0[2, :, row, :] ** 2 + q0[3, :, row, :] ** 2) / q0[0, :, row, :]
+            q0[1:4, :, row, :] = 0.0
+        visc = "const" if case == "xy_closures_const" else "sutherland"
+        stretch = 1.3 if "stretch" in case else None
+    from opensbli_amd.problems import stretched_coordinate
+    p = dict(ndim=3, n=list(n), delta=[1.0 / (m - 1) for m in n], conv="weno_js", averaging="roe", viscous=visc,
+             rk="rk3" if case != "z_walls" else "ls3", bc=bc, bscheme=bs, stretch=[0, 1 if stretch else 0, 0],
+             gama=1.4, Minf=0.5, Re=200.0, Pr=0.72, dt=2e-4, eps=1e-16, SuthT=110.4, RefT=288.0, Twall=1.0)
+    if stretch:
+        p["coords"] = {1: stretched_coordinate(n[1], 1.0, stretch, "sinh")}
+    _compare(p, q0, oracle, 100)
