@@ -102,17 +102,21 @@ struct Pt {
   double p, a;
 };
 
+// Every operation is rounded on its own (no contraction into FMAs), so a
+// point's values are bit-identical wherever a kernel evaluates them -- in a
+// sweep's prologue or in its loop -- and a direction along which the state is
+// uniform keeps exactly zero flux differences.
 template <int ND>
 __device__ __forceinline__ void finish_pt(Pt<ND>& s, const Phys& ph) {
   const double ir = 1.0 / s.q[0];
   double ke = 0.0;
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
-    s.u[m] = s.q[1 + m] * ir;
-    ke += s.u[m] * s.u[m];
+    s.u[m] = __dmul_rn(s.q[1 + m], ir);
+    ke = __dadd_rn(ke, __dmul_rn(s.u[m], s.u[m]));
   }
-  s.p = ph.gm1 * (s.q[ND + 1] - 0.5 * s.q[0] * ke);
-  s.a = sqrt(ph.gama * s.p * ir);
+  s.p = __dmul_rn(ph.gm1, __dsub_rn(s.q[ND + 1], __dmul_rn(__dmul_rn(0.5, s.q[0]), ke)));
+  s.a = sqrt(__dmul_rn(__dmul_rn(ph.gama, s.p), ir));
 }
 
 // raw conservative variables of one point

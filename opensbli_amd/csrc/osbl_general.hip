@@ -285,19 +285,19 @@ __global__ void __launch_bounds__(128) k_update_gen(const double* __restrict__ Q
 // z plane are staged from P into one of three rotating LDS slots (2-point
 // x/y halo); the z stencils read a per-thread queue of the planes z-2..z+2
 // holding u, T, mu and the inner derivatives D_x u_0, D_x u_2, D_y u_1,
-// D_y u_2 of the mixed terms.  The viscous terms are assigned to R (the
-// stage's first residual contribution).  Points inside the closure rows of
+// D_y u_2 of the mixed terms.  The viscous terms are added to R (which holds
+// the x sweep's flux differences).  Points inside the closure rows of
 // an x or y face are left to k_visc_shell; z must have neither closures nor a
 // metric (checked by the caller).  The derivative and assembly formulas are
 // those of viscous_gen.
 // ---------------------------------------------------------------------------
-constexpr int GTX = 64, GTY = 4, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
+constexpr int GTX = 64, GTY = 8, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
 
 // closure rows of the first (Carpenter 4, ReducedAccess 2) and second (2) derivatives
 __host__ __device__ __forceinline__ int closure_rows(int mod) { return mod == 0 ? 0 : (mod == 1 ? 4 : 2); }
 
 template <bool SUTH>
-__global__ void __launch_bounds__(GTX * GTY, 2) k_visc_gen3d(const double* __restrict__ P, double* __restrict__ R,
+__global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __restrict__ P, double* __restrict__ R,
                                                             Grid g, Phys ph, Bnd b, int sz) {
   constexpr int NF = SUTH ? 5 : 4;  // u0, u1, u2, T (, mu)
   constexpr int NQ = NF + 4;        // + D_x u0, D_x u2, D_y u1, D_y u2
@@ -368,7 +368,9 @@ __global__ void __launch_bounds__(GTX * GTY, 2) k_visc_gen3d(const double* __res
     const int z = zs - 2;
     if (z < z0 || !inside || slow) continue;
     const int o = g.idx(x, y, z);
-    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double r[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) r[m] = R[m * g.fs + o];
     {
       const int sl = (z + 3) % 3;
       const double* U[3] = {sp[sl][0], sp[sl][1], sp[sl][2]};
@@ -452,7 +454,7 @@ __global__ void __launch_bounds__(GTX * GTY, 2) k_visc_gen3d(const double* __res
   }
 }
 
-// The viscous terms (assigned to R) at the points k_visc_gen3d leaves out:
+// The viscous terms (added to R) at the points k_visc_gen3d leaves out:
 // the closure rows of the x or y faces.  part 0: rows y in ys[] (all x);
 // part 1: columns x in xs[] at the other rows.
 struct ShellRows {
@@ -478,7 +480,9 @@ __global__ void __launch_bounds__(128) k_visc_shell(const double* __restrict__ P
   const int x = g.idx(i, j, k);
   const int st[3] = {1, g.pitch, g.plane};
   const int ci[3] = {i, j, k};
-  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double r[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) r[m] = R[m * g.fs + x];
   viscous_gen<3>(P, g.fs, x, st, ci, g, b, ph, suth != 0, r);
 #pragma unroll
   for (int m = 0; m < 5; ++m) R[m * g.fs + x] = r[m];

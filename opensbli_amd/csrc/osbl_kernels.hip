@@ -542,10 +542,10 @@ __global__ void __launch_bounds__(256) k_update(const double* __restrict__ Q, co
 // D_y u_1, D_y u_2 of the mixed terms D_z(D_x u), D_z(D_y u).  The formulas
 // are those of viscous_terms (scheme.py:299-387).
 // ---------------------------------------------------------------------------
-constexpr int VTX = 64, VTY = 4, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
+constexpr int VTX = 64, VTY = 8, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
 
 template <int MODE>
-__global__ void __launch_bounds__(256, 2) k_visc_update3d(const double* __restrict__ Q, double* __restrict__ R,
+__global__ void __launch_bounds__(VTX * VTY, 1) k_visc_update3d(const double* __restrict__ Q, double* __restrict__ R,
                                                            double* __restrict__ Qo, double* __restrict__ Qn, Grid g,
                                                            Phys ph, double cA, double cB, int first, int sz) {
   __shared__ double sp[3][4][VPL];

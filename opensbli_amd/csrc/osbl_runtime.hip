@@ -190,7 +190,7 @@ struct osbl_ctx {
   int nstages = 0;
   double A[5]{}, B[5]{};
   int seg = 32;
-  int vsz = 16;             // z segment of the tiled viscous kernel
+  int vsz = 32;             // z segment of the tiled viscous kernels
   int vjp_seg = 8;          // cells per thread of the characteristic VJP sweeps
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
@@ -315,16 +315,18 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
   }
   if (c->Theta) timed(c, F_PRIMS, [&] { launch_sensor(c->nd, Q, c->Theta, c->g, c->ph, c->bnd, c->stream); });
   if (c->tiled_visc && mode != 2) {
-    // 3-D characteristic scheme with viscous terms: the viscous kernel assigns
-    // R, the x and y sweeps accumulate, and the z sweep adds the last
+    // 3-D characteristic scheme with viscous terms: the x sweep assigns R, the
+    // viscous kernel and the y sweep accumulate, and the z sweep adds the last
     // difference and applies the RK update, so the update needs no pass of its own
-    timed(c, F_UPDATE, [&] { launch_visc_update3d(3, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->vsz, c->stream); });
-    for (int d = 0; d < 2; ++d) {
-      timed(c, F_SWEEP_X + d, [&] {
-        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, nullptr, c->Theta, true,
-                     c->stream);
-      });
-    }
+    timed(c, F_SWEEP_X, [&] {
+      launch_sweep(c->nd, 0, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, nullptr, c->Theta, false,
+                   c->stream);
+    });
+    timed(c, F_UPDATE, [&] { launch_visc_update3d(2, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->vsz, c->stream); });
+    timed(c, F_SWEEP_Y, [&] {
+      launch_sweep(c->nd, 1, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, nullptr, c->Theta, true,
+                   c->stream);
+    });
     timed(c, F_SWEEP_Z, [&] {
       launch_sweep_z_rk(conv, c->cfg.averaging, mode, Q, c->R, c->Qo, Qn, c->g, c->ph, c->seg, c->Theta, cA, cB,
                         first, c->stream);
@@ -334,15 +336,17 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
   if (c->tiled_gen && mode != 2) {
     // the same schedule on the general path: tiled viscous terms with closures,
     // metric and mu(T) (z free of both), sweeps with the metric, RK in the z sweep
+    timed(c, F_SWEEP_X, [&] {
+      launch_sweep(c->nd, 0, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[0], c->Theta, false,
+                   c->stream);
+    });
     timed(c, F_UPDATE, [&] {
       launch_visc_gen3d(c->P, c->R, c->g, c->ph, c->bnd, c->suth ? 1 : 0, c->vsz, c->stream);
     });
-    for (int d = 0; d < 2; ++d) {
-      timed(c, F_SWEEP_X + d, [&] {
-        launch_sweep(c->nd, d, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[d], c->Theta, true,
-                     c->stream);
-      });
-    }
+    timed(c, F_SWEEP_Y, [&] {
+      launch_sweep(c->nd, 1, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[1], c->Theta, true,
+                   c->stream);
+    });
     timed(c, F_SWEEP_Z, [&] {
       launch_sweep_z_rk(conv, c->cfg.averaging, mode, Q, c->R, c->Qo, Qn, c->g, c->ph, c->seg, c->Theta, cA, cB,
                         first, c->stream);
@@ -980,7 +984,7 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
   switch (family) {
     case F_BC: b = 16.0 * nv * (double)halo_points(c->g); break;
     case F_PRIMS: b = 8.0 * (nv + c->nd + 2) * n; f = 6.0 * nv * n; break;
-    case F_SWEEP_X: b = (c->tiled_visc || c->tiled_gen ? 24.0 : 16.0) * nv * n; f = interface_flops(c) * n; break;
+    case F_SWEEP_X: b = 16.0 * nv * n; f = interface_flops(c) * n; break;
     case F_SWEEP_Y: b = 24.0 * nv * n; f = interface_flops(c) * n; break;
     case F_SWEEP_Z:
       // with the fused RK update: read Q, R and the register, write Q and the register
@@ -989,8 +993,8 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
       break;
     case F_FILTER: b = 16.0 * nv * n; break;
     case F_UPDATE: {
-      if (c->tiled_visc || c->tiled_gen) {  // viscous terms only: read Q (or the CRs), write R
-        b = 16.0 * nv * n;
+      if (c->tiled_visc || c->tiled_gen) {  // viscous terms only: read Q (or the CRs), RMW R
+        b = 24.0 * nv * n;
         f = viscous_flops(c) * n;
         break;
       }
