@@ -378,18 +378,21 @@ __device__ __forceinline__ void rcol(const AvgT<ND, T>& v, const Phys& ph, T r[N
 
 // |lambda_K| at a stencil point from its own u_D and a (shock_capturing.py:512-536)
 template <int ND, int D, int K>
-__device__ __forceinline__ double abs_lambda(const Pt<ND>& s) {
-  const double ud = s.u[D];
+__device__ __forceinline__ double abs_lambda_ua(double ud, double a) {
   if constexpr (ND == 1) {
-    if constexpr (K == 0) return fabs(ud - s.a);
+    if constexpr (K == 0) return fabs(ud - a);
     else if constexpr (K == 1) return fabs(ud);
-    else return fabs(ud + s.a);
+    else return fabs(ud + a);
   } else {
     constexpr int NV = ND + 2;
-    if constexpr (K == NV - 2) return fabs(ud + s.a);
-    else if constexpr (K == NV - 1) return fabs(ud - s.a);
+    if constexpr (K == NV - 2) return fabs(ud + a);
+    else if constexpr (K == NV - 1) return fabs(ud - a);
     else return fabs(ud);
   }
+}
+template <int ND, int D, int K>
+__device__ __forceinline__ double abs_lambda(const Pt<ND>& s) {
+  return abs_lambda_ua<ND, D, K>(s.u[D], s.a);
 }
 
 // ---------------------------------------------------------------------------
@@ -422,12 +425,12 @@ __device__ __forceinline__ void weno5z_nd(double fm2, double fm1, double f0, dou
   const double A0 = 0.3 * (q0 + tt) * (q1 * q2);
   const double A1 = 0.6 * (q1 + tt) * (q0 * q2);
   const double A2 = 0.1 * (q2 + tt) * (q0 * q1);
-  constexpr double S6 = 1.0 / 6.0;
-  const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
-  const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
-  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  // the candidates' common factor 1/6 moves into the denominator
+  const double r0 = 2.0 * f0 + 5.0 * f1 - f2;
+  const double r1 = -fm1 + 5.0 * f0 + 2.0 * f1;
+  const double r2 = 2.0 * fm2 - 7.0 * fm1 + 11.0 * f0;
   num = A0 * r0 + A1 * r1 + A2 * r2;
-  den = A0 + A1 + A2;
+  den = 6.0 * (A0 + A1 + A2);
 }
 
 __device__ __forceinline__ double weno5z(double fm2, double fm1, double f0, double f1, double f2) {
@@ -459,12 +462,12 @@ __device__ __forceinline__ void weno5js_nd(double fm2, double fm1, double f0, do
   const double e2 = 1.0e-6 + (C13 * t2 * t2 + 0.25 * s2 * s2);
   const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
   const double A0 = 0.3 * (q1 * q2), A1 = 0.6 * (q0 * q2), A2 = 0.1 * (q0 * q1);
-  constexpr double S6 = 1.0 / 6.0;
-  const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
-  const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
-  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  // the candidates' common factor 1/6 moves into the denominator
+  const double r0 = 2.0 * f0 + 5.0 * f1 - f2;
+  const double r1 = -fm1 + 5.0 * f0 + 2.0 * f1;
+  const double r2 = 2.0 * fm2 - 7.0 * fm1 + 11.0 * f0;
   num = A0 * r0 + A1 * r1 + A2 * r2;
-  den = A0 + A1 + A2;
+  den = 6.0 * (A0 + A1 + A2);
 }
 
 __device__ __forceinline__ double weno5js(double fm2, double fm1, double f0, double f1, double f2) {
@@ -606,21 +609,64 @@ __device__ __forceinline__ double recon_pair(const double fp[6], const double fm
 }
 
 // ---------------------------------------------------------------------------
+// The six stencil points of an interface, as char_flux reads them: from a
+// sliding register window (RegWindow), or from the LDS slots of a staged
+// chunk (LdsWindow, fields q, u, p, a in rows of `stride` doubles).  With the
+// LDS window every characteristic field re-reads the points it needs instead
+// of holding all six in registers across the fields.
+// ---------------------------------------------------------------------------
+template <int ND>
+struct RegWindow {
+  static constexpr bool STREAM = false;
+  const Pt<ND>* w;
+  __device__ __forceinline__ double q(int p, int m) const { return w[p].q[m]; }
+  __device__ __forceinline__ double u(int p, int m) const { return w[p].u[m]; }
+  __device__ __forceinline__ double pr(int p) const { return w[p].p; }
+  __device__ __forceinline__ double a(int p) const { return w[p].a; }
+  __device__ __forceinline__ void point(int p, Pt<ND>& out) const { out = w[p]; }
+};
+
+template <int ND>
+struct LdsWindow {
+  static constexpr bool STREAM = true;
+  const double* s;  // field 0 of point 0
+  int stride;       // doubles between fields
+  __device__ __forceinline__ double f(int p, int e) const { return s[e * stride + p]; }
+  __device__ __forceinline__ double q(int p, int m) const { return f(p, m); }
+  __device__ __forceinline__ double u(int p, int m) const { return f(p, ND + 2 + m); }
+  __device__ __forceinline__ double pr(int p) const { return f(p, 2 * ND + 2); }
+  __device__ __forceinline__ double a(int p) const { return f(p, 2 * ND + 3); }
+  __device__ __forceinline__ void point(int p, Pt<ND>& out) const {
+#pragma unroll
+    for (int m = 0; m < ND + 2; ++m) out.q[m] = q(p, m);
+#pragma unroll
+    for (int m = 0; m < ND; ++m) out.u[m] = u(p, m);
+    out.p = pr(p);
+    out.a = a(p);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // LLF characteristic flux at the interface between window points 2 and 3
 // (shock_capturing.py:357-448).  CF_k = L_k . F is formed from CS_k = L_k . Q
 // through F = u_D Q + p (e_{1+D} + u_D e_last), which holds exactly for the
 // Euler flux vector.
 // ---------------------------------------------------------------------------
-template <int ND, int D, int CONV, int AVG, bool CACHED = false>
-__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2], double ct,
-                                          const RoeF& rf = RoeF{}) {
+template <int ND, int D, int CONV, int AVG, bool CACHED = false, class W>
+__device__ __forceinline__ void char_flux_w(const W& w, const Phys& ph, double fh[ND + 2], double ct,
+                                            const RoeF& rf = RoeF{}) {
   constexpr int NV = ND + 2;
   Avg<ND> v;
-  if constexpr (CACHED && AVG == OSBL_AVG_ROE) {
-    roe_state<ND>(v, w[2], w[3], rf, ph);
-    finish_avg<ND>(v, ph);
-  } else {
-    interface_state<ND, AVG>(v, w[2], w[3], ph);
+  {
+    Pt<ND> L, R;
+    w.point(2, L);
+    w.point(3, R);
+    if constexpr (CACHED && AVG == OSBL_AVG_ROE) {
+      roe_state<ND>(v, L, R, rf, ph);
+      finish_avg<ND>(v, ph);
+    } else {
+      interface_state<ND, AVG>(v, L, R, ph);
+    }
   }
 #pragma unroll
   for (int m = 0; m < NV; ++m) fh[m] = 0.0;
@@ -630,26 +676,32 @@ __device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, 
     constexpr unsigned RM = rmask<ND, D, K>();
     double l[NV];
     lrow<ND, D, K>(v, ph, l);
+    // the split fluxes 1/2 (CF +- lam CS) are formed from the half row L_k/2:
+    // halving is exact, so CS/2 and CF/2 come out exactly and the two
+    // products by 1/2 per point disappear
+#pragma unroll
+    for (int m = 0; m < NV; ++m) l[m] *= 0.5;
     double lam = 0.0;
 #pragma unroll
-    for (int p = 0; p < 6; ++p) lam = fmax(lam, abs_lambda<ND, D, K>(w[p]));
+    for (int p = 0; p < 6; ++p) lam = fmax(lam, abs_lambda_ua<ND, D, K>(w.u(p, D), w.a(p)));
     // coefficient of p in L_k . F - u_D L_k . Q
     double fp[6], fm[6];
     [[maybe_unused]] double cfa[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-      double cs = 0.0;
+      double hcs = 0.0;
       sfor<0, NV>([&](auto mc) {
         constexpr int M = decltype(mc)::value;
-        if constexpr ((LM >> M) & 1u) cs += l[M] * w[p].q[M];
+        if constexpr ((LM >> M) & 1u) hcs += l[M] * w.q(p, M);
       });
-      double lp = 0.0;
-      if constexpr ((LM >> (1 + D)) & 1u) lp = l[1 + D];
-      if constexpr ((LM >> (NV - 1)) & 1u) lp += w[p].u[D] * l[NV - 1];
-      const double cf = w[p].u[D] * cs + w[p].p * lp;
-      fp[p] = 0.5 * (cf + lam * cs);
-      fm[p] = 0.5 * (cf - lam * cs);
-      if constexpr (CONV == CONV_WFILTER) cfa[p] = cf;
+      const double ud = w.u(p, D);
+      double hlp = 0.0;
+      if constexpr ((LM >> (1 + D)) & 1u) hlp = l[1 + D];
+      if constexpr ((LM >> (NV - 1)) & 1u) hlp += ud * l[NV - 1];
+      const double hcf = ud * hcs + w.pr(p) * hlp;
+      fp[p] = __fma_rn(lam, hcs, hcf);
+      fm[p] = __fma_rn(-lam, hcs, hcf);
+      if constexpr (CONV == CONV_WFILTER) cfa[p] = 2.0 * hcf;
     }
     double rc;
     if constexpr (CONV == CONV_WFILTER) rc = weno_filter_flux(fp, fm, cfa);
@@ -660,7 +712,15 @@ __device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, 
       constexpr int M = decltype(mc)::value;
       if constexpr ((RM >> M) & 1u) fh[M] += r[M] * rc;
     });
+    // LDS window: the next field reloads its points (nothing stays live across fields)
+    if constexpr (W::STREAM) asm volatile("" ::: "memory");
   });
+}
+
+template <int ND, int D, int CONV, int AVG, bool CACHED = false>
+__device__ __forceinline__ void char_flux(const Pt<ND> (&w)[6], const Phys& ph, double fh[ND + 2], double ct,
+                                          const RoeF& rf = RoeF{}) {
+  char_flux_w<ND, D, CONV, AVG, CACHED>(RegWindow<ND>{w}, ph, fh, ct, rf);
 }
 
 }  // namespace osbl

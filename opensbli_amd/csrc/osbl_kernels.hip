@@ -252,20 +252,10 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
     S[NV + ND + 1][slot] = p.a;
     if constexpr (ROE) roe_factors<ND>(p, S[NV + ND + 2][slot], S[NV + ND + 3][slot]);
   };
-  auto get_point = [&](Pt<ND>& p, int slot) {
-#pragma unroll
-    for (int m = 0; m < NV; ++m) p.q[m] = S[m][slot];
-#pragma unroll
-    for (int m = 0; m < ND; ++m) p.u[m] = S[NV + m][slot];
-    p.p = S[NV + ND][slot];
-    p.a = S[NV + ND + 1][slot];
-  };
   // flux at interface x0 + lane + 1/2 (points x0+lane-2 .. x0+lane+3 = slots lane .. lane+5)
   const int xmax = N + g.h[0] - 1;  // last stored point of the line
   auto flux = [&](double fh[NV], int xi) {
-    Pt<ND> w[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) get_point(w[q], lane + q);
+    const LdsWindow<ND> w{&S[0][lane], XP};
     const double ct = theta ? adaptive_ct(theta[base + min(xi, xmax)], ph) : ph.teno_ct;
     [[maybe_unused]] RoeF rf{};
     if constexpr (ROE) {
@@ -274,7 +264,7 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
       rf.sR = S[NV + ND + 2][lane + 3];
       rf.hR = S[NV + ND + 3][lane + 3];
     }
-    char_flux<ND, 0, CONV, AVG, true>(w, ph, fh, ct, rf);
+    char_flux_w<ND, 0, CONV, AVG, true>(w, ph, fh, ct, rf);
   };
   auto stage_chunk = [&](int x0) {
     // slots 0..68 hold points x0-2 .. x0+66 (clamped to the stored line: the
