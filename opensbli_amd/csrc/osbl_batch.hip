@@ -326,6 +326,7 @@ int osbl_batch_create(const osbl_config* base, int ninst, const int32_t* n, cons
   b->ninst = ninst;
   b->ph.gama = base->gama;
   b->ph.gm1 = base->gama - 1.0;
+  b->ph.igm1 = 1.0 / b->ph.gm1;
   b->ph.eps = base->eps;
   b->ph.teno_ct = base->teno_ct;
   b->mode_rk = base->rk == OSBL_RK3 ? 0 : 1;

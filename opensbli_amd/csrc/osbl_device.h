@@ -45,6 +45,7 @@ struct Phys {
   double inv_re, cq;  // 1/Re and 1/((gama-1) Minf^2 Pr Re)
   double suth;        // SuthT/RefT of the Sutherland law
   double teno_a1, teno_a2, sensor_eps;  // adaptive TENO cut-off and Ducros sensor constants
+  double igm1;        // 1/(gama-1)
 };
 
 // 10^n for an integer n by repeated products (exact for |n| <= 22)
@@ -154,13 +155,21 @@ __device__ __forceinline__ void finish_avg(AvgT<ND, T>& v, const Phys& ph) {
   T uu = v.u[0] * v.u[0];
 #pragma unroll
   for (int m = 1; m < ND; ++m) uu = uu + v.u[m] * v.u[m];
-  v.ia = 1.0 / v.a;
-  v.ir = 1.0 / v.rho;
+  if constexpr (std::is_same_v<T, double>) {
+    // 1/a and 1/rho from one division
+    const double t = 1.0 / (v.a * v.rho);
+    v.ia = v.rho * t;
+    v.ir = v.a * t;
+  } else {
+    // dual numbers (the adjoint's tangents): two plain reciprocals are cheaper
+    v.ia = 1.0 / v.a;
+    v.ir = 1.0 / v.rho;
+  }
   v.phi = ph.gm1 * 0.5 * uu;
   constexpr double RS2 = 0.70710678118654752440;  // 1/sqrt(2.0)
   v.alpha = v.rho * v.ia * RS2;
   v.bta = v.ir * v.ia * RS2;
-  v.Hh = v.a * v.a / ph.gm1 + 0.5 * uu;
+  v.Hh = v.a * v.a * ph.igm1 + 0.5 * uu;
 }
 
 // the per-point factors of the Roe average: sqrt(rho) and
@@ -172,7 +181,7 @@ struct RoeF {
 template <int ND>
 __device__ __forceinline__ void roe_factors(const Pt<ND>& P, double& sq, double& h) {
   sq = sqrt(P.q[0]);
-  h = (P.q[ND + 1] + P.p) / sq;
+  h = (P.q[ND + 1] + P.p) * (sq / P.q[0]);
 }
 
 template <int ND>
@@ -320,7 +329,7 @@ __device__ __forceinline__ void lrow(const AvgT<ND, T>& v, const Phys& ph, T l[N
 // column K of the right eigenvector matrix
 template <int ND, int D, int K, class T = double>
 __device__ __forceinline__ void rcol(const AvgT<ND, T>& v, const Phys& ph, T r[ND + 2]) {
-  const double igm1 = 1.0 / ph.gm1;
+  const double igm1 = ph.igm1;
   if constexpr (ND == 3) {
     if constexpr (K == D) {
       r[0] = 1.0;

@@ -505,6 +505,7 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   Phys& ph = c->ph;
   ph.gama = cfg->gama;
   ph.gm1 = cfg->gama - 1.0;
+  ph.igm1 = 1.0 / ph.gm1;
   ph.Minf = cfg->Minf;
   ph.Re = cfg->Re;
   ph.Pr = cfg->Pr;
