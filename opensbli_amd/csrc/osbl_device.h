@@ -531,9 +531,99 @@ __device__ __forceinline__ double pow6(double x) {
   return x2 * x2 * x2;
 }
 
-// TENO5 / TENO6 (teno.py:57-234, 407-465), literal weights:
+// TENO5 / TENO6 (teno.py:57-234, 407-465):
 // alpha_r = (1 + tau/(eps + beta_r))^6, delta_r = 0 if alpha_r*(1/sum alpha) < C_T,
 // omega_r = d_r delta_r / sum(d delta).
+// Numerator and denominator of the value (value = num/den).  The four (three)
+// quotients tau/(eps + beta_r) share one division: with e_r = eps + beta_r,
+// 1/e_0 = e_1 e_2 e_3 / (e_0 e_1 e_2 e_3) and so on, unless that product
+// leaves the normal range (then each quotient is formed on its own).  The
+// cut-off compares alpha_r * (1/sum alpha) with C_T as written.
+template <bool SIX, bool QUIRK>
+__device__ __forceinline__ void teno_nd(double fm2, double fm1, double f0, double f1, double f2, double f3,
+                                        double eps, double ct, double& num, double& den) {
+  constexpr double C13 = 13.0 / 12.0;
+  double q;
+  q = fm1 - f1;
+  double b0 = 0.25 * q * q;
+  q = fm1 - 2.0 * f0 + f1;
+  b0 += C13 * q * q;
+  q = 3.0 * f0 - 4.0 * f1 + f2;
+  double b1 = 0.25 * q * q;
+  q = f0 - 2.0 * f1 + f2;
+  b1 += C13 * q * q;
+  q = fm2 - 4.0 * fm1 + 3.0 * f0;
+  double b2 = 0.25 * q * q;
+  q = fm2 - 2.0 * fm1 + f0;
+  b2 += C13 * q * q;
+  constexpr double S6 = 1.0 / 6.0;
+  const double r0 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
+  const double r1 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
+  const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
+  const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  if constexpr (SIX) {
+    q = -11.0 * f0 + 18.0 * f1 - 9.0 * f2 + 2.0 * f3;
+    double b3 = (1.0 / 36.0) * q * q;
+    q = 2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3;
+    b3 += C13 * q * q;
+    q = -f0 + 3.0 * f1 - 3.0 * f2 + f3;
+    if constexpr (QUIRK) b3 += (781.0 / 720.0) * q;
+    else b3 += (781.0 / 720.0) * q * q;
+    const double r3 = (3.0 * f0 + 13.0 * f1 - 5.0 * f2 + f3) * (1.0 / 12.0);
+    const double tau = fabs(b3 - (b0 + b2 - 4.0 * b1) * S6);
+    const double e3 = eps + b3;
+    const double p01 = e0 * e1, p23 = e2 * e3;
+    const double pr = p01 * p23;
+    double a0, a1, a2, a3;
+    if (fabs(pr) > 1.0e-300 && fabs(pr) < 1.0e300) {
+      const double tp = tau / pr;
+      a0 = pow6(1.0 + (e1 * p23) * tp);
+      a1 = pow6(1.0 + (e0 * p23) * tp);
+      a2 = pow6(1.0 + (e3 * p01) * tp);
+      a3 = pow6(1.0 + (e2 * p01) * tp);
+    } else {
+      a0 = pow6(1.0 + tau / e0);
+      a1 = pow6(1.0 + tau / e1);
+      a2 = pow6(1.0 + tau / e2);
+      a3 = pow6(1.0 + tau / e3);
+    }
+    const double ias = 1.0 / (a0 + a1 + a2 + a3);
+    const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
+    const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
+    const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
+    const double d3 = (a3 * ias < ct) ? 0.0 : 1.0;
+    constexpr double D0 = 231.0 / 500.0, D1 = 3.0 / 10.0, D2 = 27.0 / 500.0, D3 = 23.0 / 125.0;
+    const double w0 = D0 * d0, w1 = D1 * d1, w2 = D2 * d2, w3 = D3 * d3;
+    num = w0 * r0 + w1 * r1 + w2 * r2 + w3 * r3;
+    den = w0 + w1 + w2 + w3;
+  } else {
+    const double tau = fabs(b0 - b2);
+    const double p01 = e0 * e1;
+    const double pr = p01 * e2;
+    double a0, a1, a2;
+    if (fabs(pr) > 1.0e-300 && fabs(pr) < 1.0e300) {
+      const double tp = tau / pr;
+      a0 = pow6(1.0 + (e1 * e2) * tp);
+      a1 = pow6(1.0 + (e0 * e2) * tp);
+      a2 = pow6(1.0 + p01 * tp);
+    } else {
+      a0 = pow6(1.0 + tau / e0);
+      a1 = pow6(1.0 + tau / e1);
+      a2 = pow6(1.0 + tau / e2);
+    }
+    const double ias = 1.0 / (a0 + a1 + a2);
+    const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
+    const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
+    const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
+    constexpr double D0 = 11.0 / 20.0, D1 = 4.0 / 10.0, D2 = 1.0 / 20.0;
+    const double w0 = D0 * d0, w1 = D1 * d1, w2 = D2 * d2;
+    num = w0 * r0 + w1 * r1 + w2 * r2;
+    den = w0 + w1 + w2;
+  }
+}
+
+// The same with one division per quotient (kernels that keep their stencil
+// in registers: the shared-division form needs more registers than they have).
 template <bool SIX, bool QUIRK>
 __device__ __forceinline__ double teno(double fm2, double fm1, double f0, double f1, double f2, double f3,
                                        double eps, double ct) {
@@ -596,7 +686,7 @@ __device__ __forceinline__ double teno(double fm2, double fm1, double f0, double
 
 // Sum of the right reconstruction of f+ and the left reconstruction of f-
 // (shock_capturing.py:257-265) for one characteristic field.
-template <int CONV>
+template <int CONV, bool SHARED = true>
 __device__ __forceinline__ double recon_pair(const double fp[6], const double fm[6], double eps, double ct) {
   if constexpr (CONV == OSBL_CONV_WENO_Z) {
     double n1, d1, n2, d2;
@@ -608,12 +698,22 @@ __device__ __forceinline__ double recon_pair(const double fp[6], const double fm
     weno5js_nd(fp[0], fp[1], fp[2], fp[3], fp[4], n1, d1);
     weno5js_nd(fm[5], fm[4], fm[3], fm[2], fm[1], n2, d2);
     return sum_quotients(n1, d1, n2, d2);
-  } else if constexpr (CONV == OSBL_CONV_TENO6) {
+  } else if constexpr (!SHARED && CONV == OSBL_CONV_TENO6) {
     return teno<true, true>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct) +
            teno<true, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct);
-  } else {
+  } else if constexpr (!SHARED) {
     return teno<false, false>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct) +
            teno<false, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct);
+  } else if constexpr (CONV == OSBL_CONV_TENO6) {
+    double n1, d1, n2, d2;
+    teno_nd<true, true>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct, n1, d1);
+    teno_nd<true, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct, n2, d2);
+    return sum_quotients(n1, d1, n2, d2);
+  } else {
+    double n1, d1, n2, d2;
+    teno_nd<false, false>(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], eps, ct, n1, d1);
+    teno_nd<false, false>(fm[5], fm[4], fm[3], fm[2], fm[1], fm[0], eps, ct, n2, d2);
+    return sum_quotients(n1, d1, n2, d2);
   }
 }
 
@@ -714,7 +814,7 @@ __device__ __forceinline__ void char_flux_w(const W& w, const Phys& ph, double f
     }
     double rc;
     if constexpr (CONV == CONV_WFILTER) rc = weno_filter_flux(fp, fm, cfa);
-    else rc = recon_pair<CONV>(fp, fm, ph.eps, ct);
+    else rc = recon_pair<CONV, W::STREAM>(fp, fm, ph.eps, ct);
     double r[NV];
     rcol<ND, D, K>(v, ph, r);
     sfor<0, NV>([&](auto mc) {
