@@ -526,6 +526,7 @@ static void launch_cv_d(const double* Qs, const double* A, const double* B, doub
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= g.n[e];
+  seg = fit_segment(lines, g.n[D], seg, 4);
   const long long total = lines * ((g.n[D] + seg - 1) / seg);
   const unsigned blocks = (unsigned)((total + 127) / 128);
   if (acc)

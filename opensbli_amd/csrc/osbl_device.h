@@ -39,6 +39,14 @@ struct Grid {
   __host__ __device__ __forceinline__ int stride(int d) const { return d == 0 ? 1 : (d == 1 ? pitch : plane); }
 };
 
+// Segment length of a sliding-window sweep: `seg` cells per thread, halved
+// (down to `lo`) while the launch would have fewer than 2^17 threads, so small
+// blocks (2-D, 1-D lines) still give the GPU a few thousand waves.
+__host__ __forceinline__ int fit_segment(long long lines, int n, int seg, int lo) {
+  while (seg > lo && lines * ((n + seg - 1) / seg) < (1LL << 17)) seg /= 2;
+  return seg;
+}
+
 struct Phys {
   double gama, gm1, Minf, Re, Pr, dt, eps, teno_ct;
   double inv_delta[3];

@@ -742,6 +742,7 @@ static void launch_sweep_t(const double* Q, double* R, const Grid& g, const Phys
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= g.n[e];
+  seg = fit_segment(lines, g.n[D], seg, 4);
   const long long nseg = (g.n[D] + seg - 1) / seg;
   const long long total = lines * nseg;
   const unsigned blocks = (unsigned)((total + 255) / 256);
@@ -755,6 +756,7 @@ template <int CONV, int AVG>
 static void launch_sweep_rk_t(int mode, const double* Q, double* R, const Grid& g, const Phys& ph, int seg,
                               const double* theta, const RkArgs& rk, hipStream_t s) {
   const long long lines = (long long)g.n[0] * g.n[1];
+  seg = fit_segment(lines, g.n[2], seg, 4);
   const long long nseg = (g.n[2] + seg - 1) / seg;
   const long long total = lines * nseg;
   const unsigned blocks = (unsigned)((total + 255) / 256);

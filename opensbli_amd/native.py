@@ -88,6 +88,15 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise OsblError(f"{LIB_PATH} is missing: build it with `python -m opensbli_amd.build` "
                         "or __graft_entry__.build(); there is no CPU fallback")
+    # torch's wheel bundles its own libamdhip64.so.7 under the same soname as
+    # the one this library's RUNPATH names; whichever is loaded first serves
+    # the whole process.  Loading torch first keeps one runtime, the one torch
+    # was built against, so torch tensors and this library share devices and
+    # streams however the caller orders its imports.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, dp, i64 = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
     L.osbl_abi_version.restype = ctypes.c_int
