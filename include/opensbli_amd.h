@@ -143,6 +143,19 @@ void osbl_destroy(osbl_ctx* ctx);
 int osbl_set_field(osbl_ctx* ctx, int var, const double* host);
 int osbl_get_field(osbl_ctx* ctx, int var, double* host);
 
+/*
+ * One conservative variable WITH the halo layers the boundary conditions
+ * fill, after applying them to the current state: what the dataset of an
+ * OPS dat holds when ops_fetch_dat_hdf5_file writes it at the end of a stage
+ * (io_hdf5.py:121-123; the boundary kernels close every stage,
+ * algorithm.py:440-442).  d_m[d] <= 0 and d_p[d] >= 0 receive the halo depths
+ * per direction (every layer of a periodic direction, the schemes' boundary
+ * halos on other faces; 0 beyond ndim); host receives
+ * prod(n[d] - d_m[d] + d_p[d]) doubles, x fastest.  host NULL only reports
+ * the depths.
+ */
+int osbl_get_field_halo(osbl_ctx* ctx, int var, int32_t* d_m, int32_t* d_p, double* host);
+
 /* Same, with a device pointer (e.g. a torch tensor's data_ptr()); enqueued on the context's stream. */
 int osbl_set_field_dev(osbl_ctx* ctx, int var, const double* dev);
 int osbl_get_field_dev(osbl_ctx* ctx, int var, double* dev);

@@ -145,7 +145,11 @@ class SimulationEquations(_EquationSet):
 
     @property
     def time_advance_arrays(self):
-        return ["rho", "rhou", "rhoE"]
+        """The conservative arrays (opensbliequations.py:426-438)."""
+        nd = getattr(self.equations[0], "ndim", None) if self.equations else None
+        if nd is None:
+            return [DataObject("rho"), DataObject("rhou"), DataObject("rhoE")]
+        return [DataObject("rho")] + [DataObject("rhou%d" % d) for d in range(nd)] + [DataObject("rhoE")]
 
 
 class ConstituentRelations(_EquationSet):
@@ -486,17 +490,140 @@ class MetricsEquation:
 # ---------------------------------------------------------------------------
 # I/O and documentation hooks of the reference, accepted and recorded
 # ---------------------------------------------------------------------------
-class iohdf5:
-    """HDF5 output placement (io_hdf5.py:19).  The HIPC target returns the
-    final arrays from HIPC.run(); it records the request and writes nothing."""
+class BeforeSimulationStarts:
+    """Placement before the time loop (common.py:5)."""
 
-    def __init__(self, save_every=None, **kwargs):
-        self.arrays = []
+
+class AfterSimulationEnds:
+    """Placement after the time loop (common.py:11)."""
+
+
+class InTheSimulation:
+    """Placement inside the time loop, every `frequency` iterations (common.py:21)."""
+
+    def __init__(self, frequency=False):
+        self.frequency = frequency
+
+
+class iohdf5:
+    """HDF5 input/output of block arrays (io_hdf5.py:19-138), in the OPS file
+    layout (opensbli_amd.hdf5).
+
+    iotype "write" (the default): after the time loop the arrays go to
+    "opensbli_output.h5" (or the `name` keyword); with save_every, also after
+    every iteration k with k % save_every == 0, to "opensbli_output_%06d.h5" % k.
+    iotype "read": before the loop the arrays are read from `filename`
+    (default "data.h5"): conservative arrays the initial condition does not set
+    become the initial state (a restart), other arrays are available to the
+    initial-condition equations (the KH app's random_nums).  Keyword values are
+    lower-cased, as the reference does (io_hdf5.py:26-29)."""
+
+    def __init__(self, arrays=None, save_every=None, **kwargs):
+        if kwargs:
+            self.kwargs = {key.lower(): str(val).lower() for key, val in kwargs.items()}
+        else:
+            self.kwargs = {"iotype": "write"}
         self.save_every = save_every
-        self.kwargs = kwargs
+        self.algorithm_place = []
+        if save_every:
+            self.algorithm_place.append(InTheSimulation(save_every))
+        if self.kwargs["iotype"] == "write":
+            self.algorithm_place.append(AfterSimulationEnds())
+        elif self.kwargs["iotype"] == "read":
+            self.algorithm_place = [BeforeSimulationStarts()]
+        else:
+            raise ValueError("")
+        self.arrays = []
+        if arrays:
+            self.add_arrays(arrays)
 
     def add_arrays(self, arrays):
         self.arrays.extend(flatten(list(arrays)) if isinstance(arrays, (list, tuple)) else [arrays])
+
+    @property
+    def iotype(self):
+        return self.kwargs["iotype"]
+
+    def file_name(self, iteration=None):
+        """Output file of a write, after the loop (iteration None) or after
+        `iteration` iterations inside it (io_hdf5.py:99-118)."""
+        if "name" in self.kwargs:
+            name = self.kwargs["name"]
+            if ".h5" in name:
+                fname = name
+            elif "." in name:
+                raise ValueError("")
+            else:
+                fname = name + ".h5"
+            if iteration is not None:
+                raise ValueError("dynamic fname not allowed ")
+            return fname
+        if iteration is None:
+            return "opensbli_output.h5"
+        return "opensbli_output_%06d.h5" % iteration
+
+    def read_file_name(self):
+        return self.kwargs.get("filename", "data.h5")
+
+    def check_datasets(self, defined):
+        """Refuse arrays the simulation does not define (io_hdf5.py:55-62)."""
+        missing = [x for x in _array_names(self.arrays) if x not in defined]
+        if missing:
+            raise ValueError("The dataset(s): '%s' added to the HDF5 class are not defined in the simulation code. "
+                             "Please check the HDF5 add_arrays input in the problem script." % str(", ".join(missing)))
+
+
+def _array_names(arrays, ndim=None):
+    """Array names of an iohdf5 list: symbols or strings, a trailing _B<n>
+    dropped, "rhou" expanded to its components when ndim is given."""
+    out = []
+    for a in arrays:
+        nm = re.sub(r"_B\d+$", "", str(a))
+        if nm == "rhou" and ndim:
+            out.extend("rhou%d" % d for d in range(ndim))
+        else:
+            out.append(nm)
+    return out
+
+
+def output_hdf5(array, array_name, halos, npoints, block, **kwargs):
+    """Write arrays (with their halo layers) to an HDF5 file a simulation reads
+    with iohdf5(iotype="Read") (helperfunctions.py:103-127), default "data.h5".
+
+    halos: one (low, high) pair per direction, e.g. [(-5, 5), (-5, 5)]; npoints
+    the interior points per direction; each array's axes in reverse direction
+    order (x last), of extent npoints + halos.  On a 2-D block the argument
+    checks are the reference's (helperfunctions.py:80-88)."""
+    from . import hdf5
+    if not isinstance(array, list):
+        array = [array]
+    if not isinstance(array_name, list):
+        array_name = [array_name]
+    assert len(array) == len(array_name)
+    nd = block.ndim
+    if nd == 2 or len(halos) == 2:
+        if len(halos) != 2:
+            raise ValueError("Two halos should be provided for each dimension.")
+        for h in halos:
+            if len(h) != nd:
+                raise ValueError("halos provided for hdf5 output should be of size %d" % nd)
+    elif len(halos) != nd or any(len(h) != 2 for h in halos):
+        raise ValueError("halos provided for hdf5 output should be one (low, high) pair per dimension (%d)" % nd)
+    d_m = [int(halos[d][0]) for d in range(nd)]
+    d_p = [int(halos[d][1]) for d in range(nd)]
+    fname = kwargs.get("filename", "data.h5")
+    hdf5.fetch_block_hdf5_file(fname, block.blockname, block.blocknumber, nd, truncate=True)
+    for ar, name in zip(array, array_name):
+        ar = np.asarray(ar, dtype=np.float64)
+        if ar.ndim != nd:
+            raise ValueError("array %s has %d dimensions, the block %d" % (name, ar.ndim, nd))
+        size = [ar.shape[nd - 1 - d] - abs(d_m[d]) - abs(d_p[d]) for d in range(nd)]
+        if list(size) != [int(v) for v in npoints]:
+            raise ValueError("array %s: extent %s minus halos %s is not npoints %s"
+                             % (name, ar.shape, halos, list(npoints)))
+        hdf5.fetch_dat_hdf5_file(fname, block.blockname, block.blocknumber, hdf5.dataset_name(name, block.blocknumber),
+                                 ar, [-abs(v) for v in d_m], [abs(v) for v in d_p])
+    return
 
 
 class LatexWriter:
@@ -875,14 +1002,92 @@ class HIPC:
         return p
 
     # ----------------------------------------------------------------------
-    def _env(self, problem):
+    def _env(self, problem, d_m=None, d_p=None):
+        """Symbols of the initial-condition equations over the interior grid,
+        or over the grid widened by halo depths d_m (<= 0) and d_p per
+        direction: constants, grid indices and the arrays read from HDF5."""
         env = {sympy.Symbol(k): val for k, val in self.values.items()}
         nd = problem["ndim"]
-        grids = np.meshgrid(*[np.arange(n) for n in reversed(problem["n"])], indexing="ij")
+        d_m = d_m or [0] * nd
+        d_p = d_p or [0] * nd
+        grids = np.meshgrid(*[np.arange(d_m[d], problem["n"][d] + d_p[d]) for d in reversed(range(nd))],
+                            indexing="ij")
         for d in range(nd):
             env[sympy.Symbol("idx%d" % d)] = grids[nd - 1 - d].astype(np.float64)
         env[sympy.pi] = math.pi
-        return env
+        for nm, (a, info) in getattr(self, "_inputs", {}).items():
+            sl = []
+            for d in reversed(range(nd)):
+                lo = d_m[d] - info["d_m"][d]
+                hi = lo + problem["n"][d] - d_m[d] + d_p[d]
+                if lo < 0 or hi > a.shape[nd - 1 - d]:
+                    raise ValueError("%s is read with halos %s/%s in direction %d; %s/%s are needed"
+                                     % (nm, info["d_m"][d], info["d_p"][d], d, d_m[d], d_p[d]))
+                sl.append(slice(lo, hi))
+            env[sympy.Symbol(nm)] = a[tuple(sl)]
+        return env, grids[0].shape
+
+    # ----------------------------------------------------------------------
+    # HDF5 input and output (iohdf5)
+    def _io(self, kind):
+        return [io for io in self.block.io if isinstance(io, iohdf5) and io.iotype == kind]
+
+    def _read_inputs(self, problem):
+        """Arrays of the iotype "read" objects, with their halos: {name: (array, info)}
+        (the datasets declared by ops_decl_dat_hdf5, opsc.py:702-705)."""
+        from . import hdf5
+        nd = problem["ndim"]
+        out = {}
+        for io in self._io("read"):
+            path = io.read_file_name()
+            for nm in _array_names(io.arrays, nd):
+                a, info = hdf5.decl_dat_hdf5(path, self.block.blockname, hdf5.dataset_name(nm, self.block.blocknumber))
+                if info["ndim"] != nd or list(info["size"]) != list(problem["n"]):
+                    raise ValueError("%s in %s has size %s; the block has %s"
+                                     % (nm, path, info["size"], list(problem["n"])))
+                out[nm] = (a, info)
+        return out
+
+    def _defined_arrays(self, nd):
+        """Arrays of the block an iohdf5 write may name: the conservative
+        arrays, the DataObjects of the initial condition and the arrays read."""
+        names = {"rho", "rhoE"} | {"rhou%d" % d for d in range(nd)}
+        names |= {k for k in self._fields[1] if _KINDS.get(k) == "data"}
+        return names | set(self._inputs)
+
+    def _init_equations(self):
+        katz = [e for e in self.block.equation_classes if isinstance(e, Initialise_Katzer)]
+        if katz:
+            return list(katz[0].coordinate_evaluations)
+        return list(flatten([e.equations for e in self.block.equation_classes
+                             if isinstance(e, GridBasedInitialisation)]))
+
+    def _write(self, io, fname, blk, problem):
+        """One iohdf5 write (ops_fetch_block_hdf5_file + ops_fetch_dat_hdf5_file,
+        io_hdf5.py:99-127): conservative arrays from the device with the halo
+        layers the boundary conditions fill, arrays of the initial condition
+        evaluated over the same widened grid (as the initialisation kernel
+        covers the halos), arrays read from HDF5 as read."""
+        from . import hdf5
+        nd = problem["ndim"]
+        cons = ["rho"] + ["rhou%d" % d for d in range(nd)] + ["rhoE"]
+        bn, bname = self.block.blocknumber, self.block.blockname
+        hdf5.fetch_block_hdf5_file(fname, bname, bn, nd)
+        widened = None
+        for nm in _array_names(io.arrays, nd):
+            if nm in cons:
+                a, dm, dp = blk.get_field_halo(cons.index(nm))
+            elif nm in self._fields[1]:
+                dm, dp = blk.halo_depths()
+                if widened is None:
+                    env, shape = self._env(problem, dm, dp)
+                    widened = {}
+                    self._evaluate(self._init_equations(), env, shape, widened)
+                a = widened[nm]
+            else:
+                a, info = self._inputs[nm]
+                dm, dp = info["d_m"], info["d_p"]
+            hdf5.fetch_dat_hdf5_file(fname, bname, bn, hdf5.dataset_name(nm, bn), a, dm, dp)
 
     @staticmethod
     def _evaluate(equations, env, shape, out, forbid=()):
@@ -907,7 +1112,8 @@ class HIPC:
         nd = problem["ndim"]
         shape = tuple(reversed(problem["n"]))
         names = ["rho"] + ["rhou%d" % d for d in range(nd)] + ["rhoE"]
-        env = self._env(problem)
+        self._inputs = self._read_inputs(problem)
+        env, _ = self._env(problem)
         out = {}
         katz = [e for e in self.block.equation_classes if isinstance(e, Initialise_Katzer)]
         inits = [e for e in self.block.equation_classes if isinstance(e, GridBasedInitialisation)]
@@ -929,9 +1135,12 @@ class HIPC:
             out["rhoE"] = rho * T / (gama * (gama - 1) * Minf ** 2) + 0.5 * (out["rhou0"] ** 2 + rv ** 2) / rho
         elif inits:
             self._evaluate(list(flatten([e.equations for e in inits])), env, shape, out)
-        else:
+        elif not all(nm in self._inputs for nm in names):
             raise ValueError("no GridBasedInitialisation on the block")
         for nm in names:
+            if nm not in out and nm in self._inputs:
+                # a restart: the conservative array read from HDF5
+                out[nm] = np.ascontiguousarray(env[sympy.Symbol(nm)])
             if nm not in out:
                 raise ValueError("initial condition does not define %s" % nm)
         self._fields = (env, out)
@@ -1002,14 +1211,40 @@ class HIPC:
         return problem
 
     def run(self, niter=None, return_block=False):
+        """Evaluate the initial condition (reading the iohdf5 inputs), run
+        niter iterations and write the iohdf5 outputs: after every iteration k
+        with k % save_every == 0 and after the loop (algorithm.py:443-465)."""
         problem = self.problem()
         q0 = self.initial_state(problem)
         problem = self._attach_boundary_data(problem, q0)
-        blk = native.Block(problem)
-        blk.set_state(q0)
+        nd = problem["ndim"]
+        writes = self._io("write")
+        defined = self._defined_arrays(nd)
+        for io in writes:
+            io.check_datasets(defined | {"rhou"})
+            io.file_name()
+            if io.save_every:
+                io.file_name(int(io.save_every))
         n = int(self.values.get("niter", 0) if niter is None else niter)
-        blk.run(n)
-        q = blk.get_state()
+        blk = native.Block(problem)
+        try:
+            blk.set_state(q0)
+            events = sorted({k for io in writes if io.save_every
+                             for k in range(int(io.save_every), n + 1, int(io.save_every))})
+            done = 0
+            for k in events:
+                blk.run(k - done)
+                done = k
+                for io in writes:
+                    if io.save_every and k % int(io.save_every) == 0:
+                        self._write(io, io.file_name(k), blk, problem)
+            blk.run(n - done)
+            for io in writes:
+                self._write(io, io.file_name(), blk, problem)
+            q = blk.get_state()
+        except BaseException:
+            blk.close()
+            raise
         names = ["rho"] + ["rhou%d" % d for d in range(problem["ndim"])] + ["rhoE"]
         result = {nm: q[i] for i, nm in enumerate(names)}
         if return_block:

@@ -8,7 +8,8 @@ viscous terms by Central(4).
 from opensbli_amd.api import (SFD, BinomialFilter, Central, ConstituentRelations, DataObject, Double, EinsteinEquation,
                               GridBasedInitialisation, GridVariable, HIPC, LLFTeno, LLFWeno, PeriodicBC, RoeAverage,
                               RungeKutta, RungeKuttaLS, SimulationBlock, SimulationDataType, SimulationEquations,
-                              TraditionalAlgorithmRK, WENOFilter, parse_expr, substitute_simulation_parameters)
+                              TraditionalAlgorithmRK, WENOFilter, iohdf5, parse_expr,
+                              substitute_simulation_parameters)
 
 NDIM = 3
 
@@ -29,9 +30,13 @@ def equations(scheme):
     ]
 
 
-def build(n=64, scheme="central", rk=None, niter=100, dt="0.003385", filters=()):
+def build(n=64, scheme="central", rk=None, niter=100, dt="0.003385", filters=(), save_every=None, restart=None):
     """filters: any of "weno" (WENOFilter(block, 5), Ducros sensor), "sfd"
-    (SFD(block)), "binomial" (BinomialFilter(block, 4) on the half x0 < pi)."""
+    (SFD(block)), "binomial" (BinomialFilter(block, 4) on the half x0 < pi).
+    save_every: write the conservative arrays to HDF5 every save_every
+    iterations and at the end, as the reference's app does
+    (taylor_green_vortex.py:127-130).  restart: start from the conservative
+    arrays of this HDF5 file instead of the initial condition."""
     stress = "Eq(tau_i_j, (1.0/Re)*(Der(u_i,x_j)+ Der(u_j,x_i)- (2/3)* KD(_i,_j)* Der(u_k,x_k)))"
     heat = "Eq(q_j, (1.0/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
     consts = ["Re", "Pr", "gama", "Minf", "mu"]
@@ -86,7 +91,17 @@ def build(n=64, scheme="central", rk=None, niter=100, dt="0.003385", filters=())
     if "binomial" in filters:
         half = block.grid_indexes[0] < n // 2
         extra += BinomialFilter(block, 4, grid_condition=half, sigma=0.05).equation_classes
-    block.set_equations([cr, sim, init] + extra)
+    io = []
+    if save_every:
+        h5 = iohdf5(save_every=save_every, **{"iotype": "Write"})
+        h5.add_arrays(sim.time_advance_arrays)
+        io.append(h5)
+    if restart:
+        rd = iohdf5(**{"iotype": "Read", "filename": restart})
+        rd.add_arrays(sim.time_advance_arrays)
+        io.append(rd)
+    block.setio(io)
+    block.set_equations([cr, sim] + ([] if restart else [init]) + extra)
     block.set_discretisation_schemes(schemes)
     block.discretise()
     alg = TraditionalAlgorithmRK(block)

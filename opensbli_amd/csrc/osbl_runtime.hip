@@ -5,6 +5,7 @@
 // sequence of stream-ordered launches per RK stage:
 //   boundary conditions -> [constituent relations] -> characteristic sweeps
 //   -> fused central/viscous residual + RK update -> boundary conditions.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -661,6 +662,45 @@ int osbl_get_field(osbl_ctx* c, int var, double* host) {
   int rc = osbl_get_field_dev(c, var, c->stage_buf);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(host, c->stage_buf, cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+static int check_ready(const osbl_ctx* c);
+
+// halo layers the boundary conditions keep filled on each face: every layer
+// of a periodic direction, the boundary halos of the schemes on the others
+static void filled_halos(const osbl_ctx* c, int32_t* d_m, int32_t* d_p) {
+  for (int d = 0; d < 3; ++d) {
+    d_m[d] = d_p[d] = 0;
+    if (d >= c->nd) continue;
+    if (c->cfg.bc[d][0] == OSBL_BC_PERIODIC) {
+      d_m[d] = -c->g.h[d];
+      d_p[d] = c->g.h[d];
+    } else {
+      d_m[d] = -std::min(bc_halos(c, 0), c->g.h[d]);
+      d_p[d] = std::min(bc_halos(c, 1), c->g.h[d]);
+    }
+  }
+}
+
+int osbl_get_field_halo(osbl_ctx* c, int var, int32_t* d_m, int32_t* d_p, double* host) {
+  if (!c || !d_m || !d_p) return fail(OSBL_E_ARG, "null argument");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable index %d out of range [0,%d)", var, c->nv);
+  filled_halos(c, d_m, d_p);
+  if (!host) return OSBL_OK;
+  if (int rc = check_ready(c)) return rc;
+  apply_bcs(c, Qcur(c));
+  const int m0 = -d_m[0], m1 = -d_m[1], m2 = -d_m[2];
+  const size_t W = size_t(c->g.n[0] + m0 + d_p[0]);
+  const size_t H = size_t(c->g.n[1] + m1 + d_p[1]);
+  const int K = c->g.n[2] + m2 + d_p[2];
+  const double* src = Qcur(c) + var * c->g.fs;
+  for (int k = 0; k < K; ++k) {
+    HIPCHK(hipMemcpy2DAsync(host + size_t(k) * H * W, W * sizeof(double), src + c->g.idx(-m0, -m1, k - m2),
+                            size_t(c->g.pitch) * sizeof(double), W * sizeof(double), H, hipMemcpyDeviceToHost,
+                            c->stream));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   return OSBL_OK;
 }

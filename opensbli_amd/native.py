@@ -30,7 +30,7 @@ EXPORTS = [
     "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
     "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
     "osbl_set_coordinates", "osbl_set_bc_plane", "osbl_set_filter_mask", "osbl_get_sfd_field",
-    "osbl_set_sfd_field",
+    "osbl_set_sfd_field", "osbl_get_field_halo",
 ]
 
 
@@ -108,6 +108,8 @@ def load():
     L.osbl_get_field.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_set_field_dev.argtypes = [vp, ctypes.c_int, vp]
     L.osbl_get_field_dev.argtypes = [vp, ctypes.c_int, vp]
+    L.osbl_get_field_halo.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32), dp]
     L.osbl_run.argtypes = [vp, i64]
     L.osbl_rhs.argtypes = [vp, dp]
     L.osbl_sync.argtypes = [vp]
@@ -285,6 +287,25 @@ class Block:
             check(load().osbl_get_field(self.h, m, _dptr(a)))
             out[m] = a
         return out
+
+    def halo_depths(self):
+        """(d_m, d_p) per direction of get_field_halo: d_m <= 0 low side, d_p >= 0 high side."""
+        dm, dp_ = (ctypes.c_int32 * 3)(), (ctypes.c_int32 * 3)()
+        check(load().osbl_get_field_halo(self.h, 0, dm, dp_, None))
+        nd = len(self.shape)
+        return list(dm)[:nd], list(dp_)[:nd]
+
+    def get_field_halo(self, m):
+        """Variable m with the halo layers the boundary conditions fill (applied
+        to the current state first): (array, d_m, d_p), the array's axes in
+        reverse direction order (x last)."""
+        dm, dp_ = self.halo_depths()
+        nd = len(self.shape)
+        shape = tuple(self.shape[k] - dm[nd - 1 - k] + dp_[nd - 1 - k] for k in range(nd))
+        a = np.empty(shape)
+        m_arr, p_arr = (ctypes.c_int32 * 3)(), (ctypes.c_int32 * 3)()
+        check(load().osbl_get_field_halo(self.h, int(m), m_arr, p_arr, _dptr(a)))
+        return a, dm, dp_
 
     def set_state_dev(self, ptrs):
         for m, ptr in enumerate(ptrs):

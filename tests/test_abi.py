@@ -10,9 +10,11 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(header="opensbli_amd.h"):
+    """Functions declared by one header of include/ (every header: None)."""
     names = []
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    pattern = "*.h" if header is None else header
+    for h in glob.glob(os.path.join(ROOT, "include", pattern)):
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(osbl_\w+)\s*\(", src, flags=re.M):
@@ -32,6 +34,23 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     assert lib.osbl_abi_version() == 1
     assert sorted(native.EXPORTS) == declared_functions()
+
+
+def test_every_header_is_covered():
+    """Each function of include/*.h belongs to one library's export list."""
+    from opensbli_amd import hdf5, native
+    assert sorted(native.EXPORTS + hdf5.EXPORTS) == declared_functions(None)
+
+
+def test_hdf5_library_exports_every_declared_symbol():
+    from opensbli_amd import build, hdf5
+    if not os.path.exists(hdf5.H5_LIB_PATH) and not build.hdf5_available():
+        pytest.skip("no HDF5 library in this image")
+    lib = hdf5.load()
+    missing = [n for n in declared_functions("opensbli_amd_h5.h") if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(hdf5.EXPORTS) == declared_functions("opensbli_amd_h5.h")
+    assert hdf5.library_version() >= (1, 8, 0)
 
 
 def test_config_struct_layout_matches_header(tmp_path):

@@ -1,0 +1,128 @@
+"""HDF5 output, restart and input of the problem scripts on the GPU: iohdf5
+writes (io_hdf5.py:99-127, every save_every iterations and after the loop,
+algorithm.py:443-465), iohdf5 reads (opsc.py:702-705) and output_hdf5
+(helperfunctions.py:103-127), through the C-ABI entry osbl_get_field_halo.
+
+Writing a file leaves the evolution unchanged (bitwise), a file holds the
+state with the halo layers the boundary conditions fill, and a restart from
+the file written after k iterations reproduces the uninterrupted run bitwise.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _need_h5():
+    from opensbli_amd import hdf5
+    if not os.path.exists(hdf5.H5_LIB_PATH):
+        pytest.skip("libosbl_h5.so is not built (no HDF5 library)")
+    return hdf5
+
+
+NAMES3 = ["rho", "rhou0", "rhou1", "rhou2", "rhoE"]
+
+
+def test_tgv_output_files_halos_and_restart(tmp_path, monkeypatch):
+    hdf5 = _need_h5()
+    monkeypatch.chdir(tmp_path)
+    from opensbli_amd.apps.taylor_green import build
+    _say("case TGV 16^3 central, 20 steps, save_every 10, restart from step 10")
+    res_io = build(16, "central", niter=20, save_every=10).run()
+    files = sorted(f for f in os.listdir(".") if f.endswith(".h5"))
+    assert files == ["opensbli_output.h5", "opensbli_output_000010.h5", "opensbli_output_000020.h5"], files
+    res = build(16, "central", niter=20).run()
+    for nm in NAMES3:
+        assert np.array_equal(res_io[nm], res[nm]), nm
+        a, info = hdf5.decl_dat_hdf5("opensbli_output.h5", "opensbliblock00", nm + "_B0")
+        assert info["size"] == [16, 16, 16]
+        h = -info["d_m"][0]
+        assert info["d_m"] == [-h] * 3 and info["d_p"] == [h] * 3 and h >= 2
+        # every halo layer of a periodic block is the periodic image
+        assert np.array_equal(a, np.pad(res[nm], h, mode="wrap")), nm
+        b, _ = hdf5.decl_dat_hdf5("opensbli_output_000020.h5", "opensbliblock00", nm + "_B0")
+        assert np.array_equal(a, b)
+    assert hdf5.list_names("opensbli_output.h5") == ["opensbliblock00"]
+    # restart from the file of iteration 10: ten more iterations give the 20-iteration state
+    res_rs = build(16, "central", niter=10, restart="opensbli_output_000010.h5").run()
+    for nm in NAMES3:
+        assert np.array_equal(res_rs[nm], res[nm]), nm
+
+
+def test_kh_app_with_hdf5_noise_against_oracle(tmp_path, monkeypatch, oracle):
+    """The KH app: noise written by output_hdf5, read by iohdf5 into the
+    initial condition; WENO5-Z, 20 steps, against the oracle; its output file
+    holds the state and the coordinates over the widened grid."""
+    hdf5 = _need_h5()
+    monkeypatch.chdir(tmp_path)
+    from opensbli_amd.apps.kelvin_helmholtz import build
+    n = 32
+    _say("case KH app 32^2 WENO-Z 20 steps, noise from data.h5")
+    prog = build(n, niter=20, save_every=10, seed=4)
+    res = prog.run()
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhoE")])
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(20)
+    f = oracle.RefSolver(p, variant="fma")
+    f.set_state(q0)
+    f.run(20)
+    err = rel_l2(q, r.get_state())
+    spread = rel_l2(f.get_state(), r.get_state())
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), (err, spread)
+    names = hdf5.list_names("opensbli_output.h5", "opensbliblock00")
+    assert sorted(names) == sorted(["rho_B0", "rhou0_B0", "rhou1_B0", "rhoE_B0", "x0_B0", "x1_B0"])
+    for m, nm in enumerate(("rho", "rhou0", "rhou1", "rhoE")):
+        assert np.array_equal(hdf5.read_interior("opensbli_output.h5", "opensbliblock00", nm + "_B0"), q[m])
+    x0, info = hdf5.decl_dat_hdf5("opensbli_output.h5", "opensbliblock00", "x0_B0")
+    h = -info["d_m"][0]
+    i = np.arange(-h, n + info["d_p"][0])
+    assert np.allclose(x0, np.broadcast_to(-0.5 + i / n, x0.shape), rtol=0, atol=1e-15)
+    assert os.path.exists("opensbli_output_000010.h5") and os.path.exists("opensbli_output_000020.h5")
+
+
+def test_walls_output_leaves_the_run_unchanged(tmp_path, monkeypatch):
+    """Flat plate 2-D (walls, closures, stretched y): output every 7 iterations
+    over 20; the state equals the uninterrupted run's with its boundary
+    conditions applied, which the output files hold."""
+    hdf5 = _need_h5()
+    monkeypatch.chdir(tmp_path)
+    from opensbli_amd.api import DataObject, iohdf5
+    from opensbli_amd.apps.flat_plate import build
+    _say("case flat plate 2-D 64x48, 20 steps, save_every 7")
+    prog = build(n=(64, 48), lengths=(400.0, 115.0), niter=20)
+    h5 = iohdf5(save_every=7, **{"iotype": "Write"})
+    h5.add_arrays([DataObject(nm) for nm in ("rho", "rhou0", "rhou1", "rhoE", "x0", "x1")])
+    prog.block.setio([h5])
+    res_io = prog.run()
+    prog2 = build(n=(64, 48), lengths=(400.0, 115.0), niter=20)
+    _, blk = prog2.run(return_block=True)
+    try:
+        for m, nm in enumerate(("rho", "rhou0", "rhou1", "rhoE")):
+            a, dm, dp = blk.get_field_halo(m)
+            inner = a[-dm[1]:a.shape[0] - dp[1], -dm[0]:a.shape[1] - dp[0]]
+            assert np.array_equal(res_io[nm], inner), nm
+            b, info = hdf5.decl_dat_hdf5("opensbli_output.h5", "opensbliblock00", nm + "_B0")
+            assert info["d_m"] == dm and info["d_p"] == dp
+            assert np.array_equal(a, b), nm
+    finally:
+        blk.close()
+    files = sorted(f for f in os.listdir(".") if f.endswith(".h5"))
+    assert files == ["opensbli_output.h5", "opensbli_output_000007.h5", "opensbli_output_000014.h5"], files
+    x1 = hdf5.read_interior("opensbli_output.h5", "opensbliblock00", "x1_B0")
+    assert np.all(np.diff(x1[:, 0]) > 0) and x1[0, 0] == 0.0
