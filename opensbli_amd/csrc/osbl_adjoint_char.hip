@@ -13,12 +13,15 @@
 //   * lambda_max = max_p |lambda(p)| to its first maximising point (the
 //     subgradient the forward's max picks),
 //   * the characteristic projections CS = L.Q, CF = u_D CS + p (L_{1+D} + u_D L_E),
-//   * the eigenvector rows and columns, and the Roe (or simple) average, with
-//     forward-mode dual numbers over the averaged state and the two states it
-//     is built from,
+//   * the eigenvector rows and columns and the Roe (or simple) average, by
+//     hand-written reverses (dual numbers over the averaged state for the
+//     1-D eigensystem),
 //   * the constituent relations u_D, p, a of every stencil point.
-// Contributions to the six cells of the window accumulate in registers; a
-// cell is written once no later interface of the line touches it.
+// Each interface is evaluated once: a thread owns a segment of interfaces,
+// accumulates their contributions to the cells in registers and writes
+// per-segment partial sums, which a second kernel adds per cell.
+#include <algorithm>
+
 #include "osbl_kernels.h"
 
 namespace osbl {
@@ -198,51 +201,183 @@ __device__ __forceinline__ void weno5z_grad(const double x[5], double gbar, doub
   g[4] = tb0 + sb0 - rb0 * S6;
 }
 
-// The averaged state (rho, a, u_0..u_{ND-1}) as dual numbers over the
-// conservative variables of the two states (slots 0..NV-1 left, NV..2NV-1
-// right), by the same formulas as interface_state.
-template <int ND, int AVG>
-__device__ __forceinline__ void average_dual(const double qL[ND + 2], const double qR[ND + 2], const Phys& ph,
-                                             Dual<2 * (ND + 2)>& rho, Dual<2 * (ND + 2)>& a,
-                                             Dual<2 * (ND + 2)> (&u)[ND]) {
-  constexpr int NV = ND + 2, N = 2 * NV;
-  Dual<N> L[NV], R[NV];
-#pragma unroll
-  for (int m = 0; m < NV; ++m) {
-    L[m] = mkvar<N>(qL[m], m);
-    R[m] = mkvar<N>(qR[m], NV + m);
-  }
-  Dual<N> uL[ND], uR[ND];
-  const Dual<N> irL = 1.0 / L[0], irR = 1.0 / R[0];
-  Dual<N> keL = mkc<N>(0.0), keR = mkc<N>(0.0);
-#pragma unroll
-  for (int m = 0; m < ND; ++m) {
-    uL[m] = L[1 + m] * irL;
-    uR[m] = R[1 + m] * irR;
-    keL = keL + uL[m] * uL[m];
-    keR = keR + uR[m] * uR[m];
-  }
-  const Dual<N> pL = ph.gm1 * (L[NV - 1] - 0.5 * L[0] * keL);
-  const Dual<N> pR = ph.gm1 * (R[NV - 1] - 0.5 * R[0] * keR);
-  if constexpr (AVG == OSBL_AVG_ROE) {
-    const Dual<N> sL = dsqrt(L[0]), sR = dsqrt(R[0]);
-    const Dual<N> inv = 1.0 / (sR + sL);
-    rho = sL * sR;
-    Dual<N> ke = mkc<N>(0.0);
+// Adjoints of the averaged state's symbols (AvgT's members).
+template <int ND>
+struct AvgBar {
+  double rho = 0.0, a = 0.0, u[ND] = {}, ia = 0.0, ir = 0.0, phi = 0.0, alpha = 0.0, bta = 0.0, Hh = 0.0;
+};
+
+// Reverse of lrow<ND, D, K> (ND >= 2): b += (d l / d v)^T nl.
+template <int ND, int D, int K>
+__device__ __forceinline__ void lrow_rev(const Avg<ND>& v, const Phys& ph, const double nl[ND + 2], AvgBar<ND>& b) {
+  constexpr int NV = ND + 2;
+  const double gm1 = ph.gm1;
+  constexpr bool dir_row = (ND == 3) ? (K == D) : (K == 0);
+  constexpr bool shear = (ND == 3) ? (K < 3 && K != D) : (K == 1);
+  if constexpr (dir_row) {
+    // l0 = 1 - phi ia^2, l_{1+m} = gm1 u_m ia^2, l_E = -gm1 ia^2
+    const double ia2 = v.ia * v.ia;
+    double ia2b = -v.phi * nl[0] - gm1 * nl[NV - 1];
 #pragma unroll
     for (int m = 0; m < ND; ++m) {
-      u[m] = (sL * uL[m] + sR * uR[m]) * inv;
-      ke = ke + u[m] * u[m];
+      ia2b += gm1 * v.u[m] * nl[1 + m];
+      b.u[m] += gm1 * ia2 * nl[1 + m];
     }
-    const Dual<N> hh = ((L[NV - 1] + pL) / sL + (R[NV - 1] + pR) / sR) * inv;
-    a = dsqrt(ph.gm1 * (hh - 0.5 * ke));
+    b.phi += -ia2 * nl[0];
+    b.ia += 2.0 * v.ia * ia2b;
+  } else if constexpr (shear) {
+    // l0 = -s u_c ir, l_{1+c} = s ir
+    constexpr int c = (ND == 3) ? lc3(D, K) : 1 - D;
+    constexpr double sgn_ = (ND == 3) ? (double)eps3(D, K, lc3(D, K)) : (D == 0 ? -1.0 : 1.0);
+    b.u[c] += -sgn_ * v.ir * nl[0];
+    b.ir += -sgn_ * v.u[c] * nl[0] + sgn_ * nl[1 + c];
   } else {
-    rho = 0.5 * (L[0] + R[0]);
+    // l0 = bta (phi - sg u_D a), l_{1+m} = -bta (gm1 u_m - [m == D] sg a), l_E = bta gm1
+    constexpr double sg = (K == NV - 2) ? 1.0 : -1.0;
+    double btab = nl[0] * (v.phi - sg * v.u[D] * v.a) + gm1 * nl[NV - 1];
 #pragma unroll
-    for (int m = 0; m < ND; ++m) u[m] = 0.5 * (uL[m] + uR[m]);
-    const Dual<N> aL = dsqrt(ph.gama * pL * irL), aR = dsqrt(ph.gama * pR * irR);
-    a = 0.5 * (aL + aR);
+    for (int m = 0; m < ND; ++m) {
+      btab -= nl[1 + m] * (gm1 * v.u[m] - (m == D ? sg * v.a : 0.0));
+      b.u[m] += -v.bta * gm1 * nl[1 + m];
+    }
+    b.bta += btab;
+    b.phi += v.bta * nl[0];
+    b.u[D] += -v.bta * sg * v.a * nl[0];
+    b.a += -v.bta * sg * v.u[D] * nl[0] + v.bta * sg * nl[1 + D];
   }
+}
+
+// Reverse of rcol<ND, D, K> (ND >= 2): b += (d r / d v)^T nr.
+template <int ND, int D, int K>
+__device__ __forceinline__ void rcol_rev(const Avg<ND>& v, const Phys& ph, const double nr[ND + 2], AvgBar<ND>& b) {
+  constexpr int NV = ND + 2;
+  const double igm1 = ph.igm1;
+  constexpr bool dir_row = (ND == 3) ? (K == D) : (K == 0);
+  constexpr bool shear = (ND == 3) ? (K < 3 && K != D) : (K == 1);
+  if constexpr (dir_row) {
+    // r0 = 1, r_{1+m} = u_m, r_E = phi / (gama - 1)
+#pragma unroll
+    for (int m = 0; m < ND; ++m) b.u[m] += nr[1 + m];
+    b.phi += igm1 * nr[NV - 1];
+  } else if constexpr (shear) {
+    // r_{1+c} = s rho, r_E = s rho u_c
+    constexpr int c = (ND == 3) ? lc3(D, K) : 1 - D;
+    constexpr double sgn_ = (ND == 3) ? (double)eps3(D, K, lc3(D, K)) : (D == 0 ? -1.0 : 1.0);
+    b.rho += sgn_ * nr[1 + c] + sgn_ * v.u[c] * nr[NV - 1];
+    b.u[c] += sgn_ * v.rho * nr[NV - 1];
+  } else {
+    // r0 = alpha, r_{1+m} = alpha (u_m + [m == D] sg a), r_E = alpha ((phi + a^2)/(gama-1) + sg u_D a)
+    constexpr double sg = (K == NV - 2) ? 1.0 : -1.0;
+    const double rE = (v.phi + v.a * v.a) * igm1 + sg * v.u[D] * v.a;
+    double alb = nr[0] + rE * nr[NV - 1];
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      alb += (v.u[m] + (m == D ? sg * v.a : 0.0)) * nr[1 + m];
+      b.u[m] += v.alpha * nr[1 + m];
+    }
+    b.alpha += alb;
+    b.a += v.alpha * sg * nr[1 + D] + v.alpha * (2.0 * v.a * igm1 + sg * v.u[D]) * nr[NV - 1];
+    b.phi += v.alpha * igm1 * nr[NV - 1];
+    b.u[D] += v.alpha * sg * v.a * nr[NV - 1];
+  }
+}
+
+// Reverse of finish_avg: folds the adjoints of ia, ir, phi, alpha, bta, Hh
+// into those of rho, a, u.
+template <int ND>
+__device__ __forceinline__ void finish_avg_rev(const Avg<ND>& v, const Phys& ph, AvgBar<ND>& b) {
+  constexpr double RS2 = 0.70710678118654752440;
+  b.rho += v.ia * RS2 * b.alpha;
+  b.ia += v.rho * RS2 * b.alpha + v.ir * RS2 * b.bta;
+  b.ir += v.ia * RS2 * b.bta;
+  b.a += 2.0 * v.a * ph.igm1 * b.Hh;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) b.u[m] += (ph.gm1 * b.phi + b.Hh) * v.u[m];
+  b.a += -v.ia * v.ia * b.ia;
+  b.rho += -v.ir * v.ir * b.ir;
+}
+
+// Reverse of interface_state's average: the adjoints (rho, a, u) of the
+// averaged state added to those of the two points' conservative variables.
+template <int ND, int AVG>
+__device__ __forceinline__ void average_rev(const Pt<ND>& L, const Pt<ND>& R, const Phys& ph, const AvgBar<ND>& b,
+                                            double (&gL)[ND + 2], double (&gR)[ND + 2]) {
+  constexpr int NV = ND + 2;
+  double pLb, pRb, rLb, rRb, uLb[ND], uRb[ND];
+  if constexpr (AVG == OSBL_AVG_ROE) {
+    const double sL = sqrt(L.q[0]), sR = sqrt(R.q[0]);
+    const double inv = 1.0 / (sL + sR);
+    const double HL = (L.q[NV - 1] + L.p) / sL, HR = (R.q[NV - 1] + R.p) / sR;
+    double ke = 0.0, u[ND];
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      u[m] = (sL * L.u[m] + sR * R.u[m]) * inv;
+      ke += u[m] * u[m];
+    }
+    const double hh = (HL + HR) * inv;
+    const double a = sqrt(ph.gm1 * (hh - 0.5 * ke));
+    // a = sqrt(gm1 (hh - ke/2))
+    const double tb = b.a * ph.gm1 / (2.0 * a);
+    double invb = tb * (HL + HR);
+    const double Hb = tb * inv;
+    double sLb = b.rho * sR, sRb = b.rho * sL;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      const double ub = b.u[m] - tb * u[m];
+      sLb += ub * L.u[m] * inv;
+      sRb += ub * R.u[m] * inv;
+      uLb[m] = ub * sL * inv;
+      uRb[m] = ub * sR * inv;
+      invb += ub * (sL * L.u[m] + sR * R.u[m]);
+    }
+    const double sb = -inv * inv * invb;
+    sLb += sb - Hb * HL / sL;
+    sRb += sb - Hb * HR / sR;
+    // H = (E + p) / s
+    pLb = Hb / sL;
+    pRb = Hb / sR;
+    gL[NV - 1] += Hb / sL;
+    gR[NV - 1] += Hb / sR;
+    rLb = sLb / (2.0 * sL);
+    rRb = sRb / (2.0 * sR);
+  } else {
+    // rho = (rho_L + rho_R)/2, u = (u_L + u_R)/2, a = (a_L + a_R)/2, a = sqrt(gama p / rho)
+    rLb = 0.5 * b.rho;
+    rRb = 0.5 * b.rho;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      uLb[m] = 0.5 * b.u[m];
+      uRb[m] = 0.5 * b.u[m];
+    }
+    const double aLb = 0.5 * b.a, aRb = 0.5 * b.a;
+    pLb = aLb * ph.gama / (2.0 * L.a * L.q[0]);
+    pRb = aRb * ph.gama / (2.0 * R.a * R.q[0]);
+    rLb += -aLb * L.a / (2.0 * L.q[0]);
+    rRb += -aRb * R.a / (2.0 * R.q[0]);
+  }
+  // p = gm1 (E - rho |u|^2 / 2), u_m = (rho u_m) / rho
+  double keL = 0.0, keR = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    keL += L.u[m] * L.u[m];
+    keR += R.u[m] * R.u[m];
+  }
+  gL[NV - 1] += ph.gm1 * pLb;
+  gR[NV - 1] += ph.gm1 * pRb;
+  rLb += -0.5 * ph.gm1 * keL * pLb;
+  rRb += -0.5 * ph.gm1 * keR * pRb;
+  const double irL = 1.0 / L.q[0], irR = 1.0 / R.q[0];
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    const double uLt = uLb[m] - ph.gm1 * L.q[0] * L.u[m] * pLb;
+    const double uRt = uRb[m] - ph.gm1 * R.q[0] * R.u[m] * pRb;
+    gL[1 + m] += uLt * irL;
+    gR[1 + m] += uRt * irR;
+    rLb += -uLt * L.u[m] * irL;
+    rRb += -uRt * R.u[m] * irR;
+  }
+  gL[0] += rLb;
+  gR[0] += rRb;
 }
 
 // |lambda_K| at a point and its derivative with respect to (u_D, a)
@@ -272,15 +407,20 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
   constexpr int NV = ND + 2, NS = ND + 2;  // averaged symbols: rho, a, u_0..u_{ND-1}
   Avg<ND> v;
   interface_state<ND, AVG>(v, w[2], w[3], ph);
-  AvgT<ND, Dual<NS>> vd;
-  vd.rho = mkvar<NS>(v.rho, 0);
-  vd.a = mkvar<NS>(v.a, 1);
+  // adjoints of the averaged state: by the hand-written reverse of the
+  // eigenvector entries (ND >= 2) or by dual numbers (the 1-D eigensystem)
+  AvgBar<ND> b;
+  [[maybe_unused]] AvgT<ND, Dual<NS>> vd;
+  [[maybe_unused]] double nvs[NS];
+  if constexpr (ND == 1) {
+    vd.rho = mkvar<NS>(v.rho, 0);
+    vd.a = mkvar<NS>(v.a, 1);
 #pragma unroll
-  for (int m = 0; m < ND; ++m) vd.u[m] = mkvar<NS>(v.u[m], 2 + m);
-  finish_avg<ND>(vd, ph);
-  double nvs[NS];
+    for (int m = 0; m < ND; ++m) vd.u[m] = mkvar<NS>(v.u[m], 2 + m);
+    finish_avg<ND>(vd, ph);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) nvs[s] = 0.0;
+    for (int q = 0; q < NS; ++q) nvs[q] = 0.0;
+  }
   double nud[6], npp[6], na[6];
 #pragma unroll
   for (int p = 0; p < 6; ++p) { nud[p] = 0.0; npp[p] = 0.0; na[p] = 0.0; }
@@ -289,14 +429,21 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
     constexpr int K = decltype(kc)::value;
     constexpr unsigned LM = lmask<ND, D, K>();
     constexpr unsigned RM = rmask<ND, D, K>();
-    Dual<NS> ld[NV], rd[NV];
+    double l[NV], r[NV];
 #pragma unroll
-    for (int m = 0; m < NV; ++m) { ld[m] = mkc<NS>(0.0); rd[m] = mkc<NS>(0.0); }
-    lrow<ND, D, K>(vd, ph, ld);
-    rcol<ND, D, K>(vd, ph, rd);
-    double l[NV];
+    for (int m = 0; m < NV; ++m) { l[m] = 0.0; r[m] = 0.0; }
+    [[maybe_unused]] Dual<NS> ld[NV], rd[NV];
+    if constexpr (ND == 1) {
 #pragma unroll
-    for (int m = 0; m < NV; ++m) l[m] = ld[m].v;
+      for (int m = 0; m < NV; ++m) { ld[m] = mkc<NS>(0.0); rd[m] = mkc<NS>(0.0); }
+      lrow<ND, D, K>(vd, ph, ld);
+      rcol<ND, D, K>(vd, ph, rd);
+#pragma unroll
+      for (int m = 0; m < NV; ++m) { l[m] = ld[m].v; r[m] = rd[m].v; }
+    } else {
+      lrow<ND, D, K>(v, ph, l);
+      rcol<ND, D, K>(v, ph, r);
+    }
     // lambda_max and its first maximising point
     double lam = 0.0, dlu = 0.0, dla = 0.0;
     int pst = -1;
@@ -330,13 +477,25 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
     const double rc = weno5z(fp[0], fp[1], fp[2], fp[3], fp[4]) + weno5z(fm[5], fm[4], fm[3], fm[2], fm[1]);
     // F += r rc
     double nrc = 0.0;
+    double nr[NV];
 #pragma unroll
-    for (int m = 0; m < NV; ++m)
+    for (int m = 0; m < NV; ++m) {
+      nr[m] = 0.0;
       if ((RM >> m) & 1u) {
-        nrc += rd[m].v * nu[m];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) nvs[s] += rd[m].d[s] * rc * nu[m];
+        nrc += r[m] * nu[m];
+        nr[m] = rc * nu[m];
       }
+    }
+    if constexpr (ND == 1) {
+#pragma unroll
+      for (int m = 0; m < NV; ++m)
+        if ((RM >> m) & 1u) {
+#pragma unroll
+          for (int q = 0; q < NS; ++q) nvs[q] += rd[m].d[q] * nr[m];
+        }
+    } else {
+      rcol_rev<ND, D, K>(v, ph, nr, b);
+    }
     // reconstructions
     double gp[5], gm[5];
     weno5z_grad(fp, nrc, gp);
@@ -377,33 +536,32 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
         nud[p] += dlu * nlam;
         na[p] += dla * nlam;
       }
+    if constexpr (ND == 1) {
 #pragma unroll
-    for (int m = 0; m < NV; ++m)
-      if ((LM >> m) & 1u) {
+      for (int m = 0; m < NV; ++m)
+        if ((LM >> m) & 1u) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) nvs[s] += ld[m].d[s] * nl[m];
-      }
+          for (int q = 0; q < NS; ++q) nvs[q] += ld[m].d[q] * nl[m];
+        }
+    } else {
+      lrow_rev<ND, D, K>(v, ph, nl, b);
+    }
   });
 
   // averaged state -> the two states
-  {
-    constexpr int N2 = 2 * NV;
-    Dual<N2> rho, a, u[ND];
-    average_dual<ND, AVG>(w[2].q, w[3].q, ph, rho, a, u);
-#pragma unroll
-    for (int k = 0; k < N2; ++k) {
-      double acc = nvs[0] * rho.d[k] + nvs[1] * a.d[k];
-#pragma unroll
-      for (int m = 0; m < ND; ++m) acc += nvs[2 + m] * u[m].d[k];
-      if (k < NV) g[2][k] += acc;
-      else g[3][k - NV] += acc;
-    }
+  if constexpr (ND == 1) {
+    b.rho = nvs[0];
+    b.a = nvs[1];
+    b.u[0] = nvs[2];
+  } else {
+    finish_avg_rev<ND>(v, ph, b);
   }
+  average_rev<ND, AVG>(w[2], w[3], ph, b, g[2], g[3]);
   // u_D, p, a of every point -> its conservative variables
 #pragma unroll
   for (int p = 0; p < 6; ++p) {
     const double rho = w[p].q[0], ir = 1.0 / rho;
-    const double a = w[p].a, pr = w[p].p;
+    const double a = w[p].a;
     double uu = 0.0;
 #pragma unroll
     for (int m = 0; m < ND; ++m) uu += w[p].u[m] * w[p].u[m];
@@ -419,147 +577,256 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
     g0 += -nud[p] * w[p].u[D] * ir;
     g[p][1 + D] += nud[p] * ir;
     g[p][0] += g0;
-    (void)pr;
   }
 }
 
 }  // namespace
 
 // One direction of the characteristic residual's VJP on packed periodic
-// arrays (n0 x n1 x n2, x fastest).  mu = alpha A + beta B (B optional) is
-// formed on the fly.  ACC = false: out1 = c (A + B) + V (and out2 = A + B when
-// given); ACC = true: out1 += V.  Each thread owns `seg` cells of one line.
-template <int ND, int D, int AVG, bool ACC>
-__global__ void __launch_bounds__(128) k_char_vjp(const double* __restrict__ Qs, const double* __restrict__ A,
-                                                  const double* __restrict__ B, double alpha, double beta, double c,
-                                                  double* __restrict__ out1, double* __restrict__ out2, int n0,
-                                                  int n1, int n2, Phys ph, int seg) {
+// arrays (n0 x n1 x n2, x fastest), in two passes.  mu = alpha A + beta B (B
+// optional) is formed on the fly.
+//
+// k_char_vjp_part: thread (line, segment s) evaluates the VJP of the
+// segment's own interfaces i = c0..c1-1 once each, sliding the window along
+// the line, and writes its partial sums for the cells c0-2 .. c1+2 the
+// interfaces touch (seg + 5 slots) to `part` (vjp_pidx).  A cell is written
+// as soon as no later interface of the segment reaches it.
+// k_char_vjp_gather: per cell, the sum of the (at most three) partials that
+// cover it, periodic images included, in a fixed order; ACC = false:
+// out1 = c (A + B) + V (and out2 = A + B when given); ACC = true: out1 += V.
+// origin and stride of grid line `line` along D (lines numbered with the
+// lower remaining direction fastest)
+template <int D>
+__device__ __forceinline__ void vjp_line(long long line, int n0, int n1, long long& base, long long& sd) {
+  int co[3];
+  if constexpr (D == 0) {
+    co[1] = (int)(line % n1);
+    co[2] = (int)(line / n1);
+  } else {
+    constexpr int O = (D == 1) ? 2 : 1;
+    co[0] = (int)(line % n0);
+    co[O] = (int)(line / n0);
+  }
+  co[D] = 0;
+  base = co[0] + (long long)n0 * (co[1] + (long long)n1 * co[2]);
+  sd = D == 0 ? 1 : (D == 1 ? (long long)n0 : (long long)n0 * n1);
+}
+
+// Index of a partial sum (variable m, segment s, slot k, line).  Along y and z
+// the lanes of a wave are adjacent lines, along x adjacent segments of a line;
+// either way they write adjacent doubles.
+template <int D>
+__device__ __forceinline__ long long vjp_pidx(int m, int s, int k, long long line, int nseg, int SP,
+                                              long long lines) {
+  if constexpr (D == 0) return ((long long)m * SP + k) * (lines * nseg) + line * nseg + s;
+  else return (((long long)m * nseg + s) * SP + k) * lines + line;
+}
+
+template <int ND, int D, int AVG>
+__global__ void __launch_bounds__(128) k_char_vjp_part(const double* __restrict__ Qs, const double* __restrict__ A,
+                                                       const double* __restrict__ B, double alpha, double beta,
+                                                       double* __restrict__ part, int n0, int n1, int n2, Phys ph,
+                                                       int seg) {
   constexpr int NV = ND + 2;
   const int nn[3] = {n0, n1, n2};
   const long long cells = (long long)n0 * n1 * n2;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int N = nn[D];
-  const long long nseg = (N + seg - 1) / seg;
+  const int nseg = (N + seg - 1) / seg;
+  const int SP = seg + 5;
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= nn[e];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= lines * nseg) return;
-  int co[3];
-  int s;
-  if constexpr (D == 0) {
-    co[1] = (int)(t % n1);
-    const long long r = t / n1;
-    co[2] = (int)(r % n2);
-    s = (int)(r / n2);
-  } else {
-    constexpr int O = (D == 1) ? 2 : 1;
-    co[0] = (int)(t % n0);
-    const long long r = t / n0;
-    co[O] = (int)(r % nn[O]);
-    s = (int)(r / nn[O]);
-  }
-  co[D] = 0;
-  const long long base = co[0] + (long long)n0 * (co[1] + (long long)n1 * co[2]);
-  const long long sd = D == 0 ? 1 : (D == 1 ? (long long)n0 : (long long)n0 * n1);
+  // x: lanes along a line's segments (their loads share cache lines); y, z: lanes along x
+  const long long line = D == 0 ? t / nseg : t % lines;
+  const int s = (int)(D == 0 ? t % nseg : t / lines);
+  long long base, sd;
+  vjp_line<D>(line, n0, n1, base, sd);
   const int c0 = s * seg, c1 = min(c0 + seg, N);
   auto at = [&](int i) { return base + (long long)(((i % N) + N) % N) * sd; };
-  auto load = [&](Pt<ND>& pt, double* mu, int i) {
+  auto load = [&](Pt<ND>& pt, int i) {
+    const long long y = at(i);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) pt.q[m] = Qs[m * cells + y];
+    finish_pt<ND>(pt, ph);
+  };
+  auto load_mu = [&](double* mu, int i) {
     const long long y = at(i);
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
-      pt.q[m] = Qs[m * cells + y];
       double v = alpha * A[m * cells + y];
       if (B) v += beta * B[m * cells + y];
       mu[m] = v;
     }
-    finish_pt<ND>(pt, ph);
+  };
+  // partial of slot k (cell c0 - 2 + k), variable m
+  auto put = [&](int k, const double* v) {
+#pragma unroll
+    for (int m = 0; m < NV; ++m) part[vjp_pidx<D>(m, s, k, line, nseg, SP, lines)] = v[m];
   };
   const double inv = ph.inv_delta[D];
   Pt<ND> w[6];
-  double mw[6][NV];
+  double mu2[NV], mu3[NV];  // weights of the interface's two cells
   double acc[6][NV];
 #pragma unroll
   for (int p = 0; p < 6; ++p)
 #pragma unroll
     for (int m = 0; m < NV; ++m) acc[p][m] = 0.0;
-  // window for interface i holds points i-2..i+3; the first interface is c0-3
+  // window for interface i holds points i-2..i+3; the first interface is c0
 #pragma unroll
-  for (int p = 0; p < 5; ++p) load(w[p], mw[p], c0 - 5 + p);
-  for (int i = c0 - 3; i < c1 + 2; ++i) {
-    load(w[5], mw[5], i + 3);
+  for (int p = 0; p < 5; ++p) load(w[p], c0 - 2 + p);
+  load_mu(mu2, c0);
+  for (int i = c0; i < c1; ++i) {
+    load(w[5], i + 3);
+    load_mu(mu3, i + 1);
     double nu[NV];
 #pragma unroll
-    for (int m = 0; m < NV; ++m) nu[m] = inv * (mw[3][m] - mw[2][m]);
+    for (int m = 0; m < NV; ++m) nu[m] = inv * (mu3[m] - mu2[m]);
     char_flux_vjp<ND, D, AVG>(w, nu, ph, acc);
-    const int cell = i - 2;
-    if (cell >= c0) {
-      const long long x = at(cell);
+    // cell i - 2 gets nothing from the segment's later interfaces
+    put(i - c0, acc[0]);
 #pragma unroll
-      for (int m = 0; m < NV; ++m) {
-        if constexpr (ACC) {
-          out1[m * cells + x] += acc[0][m];
-        } else {
-          const double ab = A[m * cells + x] + (B ? B[m * cells + x] : 0.0);
-          out1[m * cells + x] = c * ab + acc[0][m];
-          if (out2) out2[m * cells + x] = ab;
-        }
-      }
-    }
+    for (int m = 0; m < NV; ++m) mu2[m] = mu3[m];
 #pragma unroll
     for (int p = 0; p < 5; ++p) {
       w[p] = w[p + 1];
 #pragma unroll
-      for (int m = 0; m < NV; ++m) {
-        mw[p][m] = mw[p + 1][m];
-        acc[p][m] = acc[p + 1][m];
-      }
+      for (int m = 0; m < NV; ++m) acc[p][m] = acc[p + 1][m];
     }
 #pragma unroll
     for (int m = 0; m < NV; ++m) acc[5][m] = 0.0;
   }
+  // cells c1-2 .. c1+2
+#pragma unroll
+  for (int p = 0; p < 5; ++p) put(c1 - c0 + p, acc[p]);
+}
+
+template <int ND, int D, bool ACC>
+__global__ void __launch_bounds__(256) k_char_vjp_gather(const double* __restrict__ A, const double* __restrict__ B,
+                                                         double c, double* __restrict__ out1,
+                                                         double* __restrict__ out2, const double* __restrict__ part,
+                                                         int n0, int n1, int n2, int seg) {
+  constexpr int NV = ND + 2;
+  const int nn[3] = {n0, n1, n2};
+  const long long cells = (long long)n0 * n1 * n2;
+  const int N = nn[D];
+  const int nseg = (N + seg - 1) / seg;
+  const int SP = seg + 5;
+  long long lines = 1;
+  for (int e = 0; e < 3; ++e)
+    if (e != D) lines *= nn[e];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= lines * N) return;
+  // lanes along x in every direction (coalesced state and output)
+  const long long line = D == 0 ? t / N : t % lines;
+  const int j = (int)(D == 0 ? t % N : t / lines);
+  long long base, sd;
+  vjp_line<D>(line, n0, n1, base, sd);
+  double v[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) v[m] = 0.0;
+  // the cell at line position j, and its periodic images j - N, j + N, for
+  // every segment whose slot range [s seg - 2, min(s seg + seg, N) + 2] holds it
+#pragma unroll
+  for (int w = -1; w <= 1; ++w) {
+    const int jj = j + w * N;
+    if (jj < -2 || jj > N + 2) continue;
+    const int shi = min(nseg - 1, (jj + 2) / seg);
+    for (int s = max(0, (jj + 2) / seg - 2); s <= shi; ++s) {
+      const int c0 = s * seg, c1 = min(c0 + seg, N);
+      const int k = jj - (c0 - 2);
+      if (k < 0 || k > c1 - c0 + 4) continue;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) v[m] += part[vjp_pidx<D>(m, s, k, line, nseg, SP, lines)];
+    }
+  }
+  const long long x = base + (long long)j * sd;
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    if constexpr (ACC) {
+      out1[m * cells + x] += v[m];
+    } else {
+      const double ab = A[m * cells + x] + (B ? B[m * cells + x] : 0.0);
+      out1[m * cells + x] = c * ab + v[m];
+      if (out2) out2[m * cells + x] = ab;
+    }
+  }
+}
+
+// doubles of the partial-sum scratch one direction needs
+template <int ND, int D>
+static long long vjp_part_size(const Grid& g, int seg) {
+  long long lines = 1;
+  for (int e = 0; e < 3; ++e)
+    if (e != D) lines *= g.n[e];
+  return (long long)(ND + 2) * lines * ((g.n[D] + seg - 1) / seg) * (seg + 5);
 }
 
 template <int ND, int D, int AVG>
 static void launch_cv_d(const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
-                        double* out1, double* out2, const Grid& g, const Phys& ph, int seg, bool acc,
+                        double* out1, double* out2, double* part, const Grid& g, const Phys& ph, int seg, bool acc,
                         hipStream_t s) {
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= g.n[e];
-  seg = fit_segment(lines, g.n[D], seg, 4);
   const long long total = lines * ((g.n[D] + seg - 1) / seg);
-  const unsigned blocks = (unsigned)((total + 127) / 128);
+  k_char_vjp_part<ND, D, AVG><<<(unsigned)((total + 127) / 128), 128, 0, s>>>(Qs, A, B, alpha, beta, part, g.n[0],
+                                                                            g.n[1], g.n[2], ph, seg);
+  const long long ncell = lines * g.n[D];
+  const unsigned gb = (unsigned)((ncell + 255) / 256);
   if (acc)
-    k_char_vjp<ND, D, AVG, true><<<blocks, 128, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0], g.n[1],
-                                                         g.n[2], ph, seg);
+    k_char_vjp_gather<ND, D, true><<<gb, 256, 0, s>>>(A, B, c, out1, out2, part, g.n[0], g.n[1], g.n[2], seg);
   else
-    k_char_vjp<ND, D, AVG, false><<<blocks, 128, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0], g.n[1],
-                                                          g.n[2], ph, seg);
+    k_char_vjp_gather<ND, D, false><<<gb, 256, 0, s>>>(A, B, c, out1, out2, part, g.n[0], g.n[1], g.n[2], seg);
 }
 
 template <int ND, int AVG>
 static void launch_cv_nd(const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
-                         double* out1, double* out2, const Grid& g, const Phys& ph, int seg, bool acc,
+                         double* out1, double* out2, double* part, const Grid& g, const Phys& ph, int seg, bool acc,
                          hipStream_t s) {
-  launch_cv_d<ND, 0, AVG>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
-  if constexpr (ND >= 2) launch_cv_d<ND, 1, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, g, ph, seg, true, s);
-  if constexpr (ND >= 3) launch_cv_d<ND, 2, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, g, ph, seg, true, s);
+  launch_cv_d<ND, 0, AVG>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+  if constexpr (ND >= 2) launch_cv_d<ND, 1, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, part, g, ph, seg, true, s);
+  if constexpr (ND >= 3) launch_cv_d<ND, 2, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, part, g, ph, seg, true, s);
+}
+
+template <int ND>
+static long long vjp_part_nd(const Grid& g, int seg) {
+  long long n = vjp_part_size<ND, 0>(g, seg);
+  if constexpr (ND >= 2) n = std::max(n, vjp_part_size<ND, 1>(g, seg));
+  if constexpr (ND >= 3) n = std::max(n, vjp_part_size<ND, 2>(g, seg));
+  return n;
+}
+
+// Segment length of the VJP sweeps: the preferred one, halved down to 4 while
+// a launch would have fewer than 2^17 threads (a launch along any direction
+// has about cells / seg threads).
+int adj_char_segment(const Grid& g, int seg) {
+  return fit_segment((long long)g.n[1] * g.n[2], g.n[0], seg, 4);
+}
+
+// doubles of the scratch launch_adj_stage_char needs for segment length seg
+long long adj_char_scratch(const Grid& g, int seg) {
+  if (g.nd == 3) return vjp_part_nd<3>(g, seg);
+  if (g.nd == 2) return vjp_part_nd<2>(g, seg);
+  return vjp_part_nd<1>(g, seg);
 }
 
 // out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B) for the WENO5-Z
 // characteristic residual, out2 = A + B (optional); with acc, out1 += the
-// product instead.  Packed periodic arrays.
+// product instead.  Packed periodic arrays; part holds adj_char_scratch(g, seg)
+// doubles.
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
-                           double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, int seg,
-                           bool acc, hipStream_t s) {
+                           double beta, double c, double* out1, double* out2, double* part, const Grid& g,
+                           const Phys& ph, int seg, bool acc, hipStream_t s) {
   if (avg == OSBL_AVG_ROE) {
-    if (nd == 3) launch_cv_nd<3, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
-    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
-    else launch_cv_nd<1, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
+    if (nd == 3) launch_cv_nd<3, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+    else launch_cv_nd<1, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
   } else {
-    if (nd == 3) launch_cv_nd<3, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
-    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
-    else launch_cv_nd<1, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, seg, acc, s);
+    if (nd == 3) launch_cv_nd<3, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+    else launch_cv_nd<1, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
   }
 }
 

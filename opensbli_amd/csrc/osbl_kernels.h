@@ -36,8 +36,10 @@ void launch_adj_stage(int nd, const double* Qs, const double* A, const double* B
                       double* out1, double* out2, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_pack_all(const double* F, double* out, const Grid& g, int nv, hipStream_t s);
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
-                           double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, int seg,
-                           bool acc, hipStream_t s);
+                           double beta, double c, double* out1, double* out2, double* part, const Grid& g,
+                           const Phys& ph, int seg, bool acc, hipStream_t s);
+int adj_char_segment(const Grid& g, int seg);
+long long adj_char_scratch(const Grid& g, int seg);
 void launch_weno_filter_apply(int nd, double* Q, const double* R, double* theta, const Grid& g, const Phys& ph,
                               bool ducros, bool mach, double control, int walls, hipStream_t s);
 void launch_sfd(int nd, double* Q, double* F, const Grid& g, double c, double E, hipStream_t s);

@@ -192,7 +192,7 @@ struct osbl_ctx {
   double A[5]{}, B[5]{};
   int seg = 32;
   int vsz = 32;             // z segment of the tiled viscous kernels
-  int vjp_seg = 8;          // cells per thread of the characteristic VJP sweeps
+  int vjp_seg = 8;          // preferred interfaces per thread of the characteristic VJP sweeps
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
   bool tiled_gen = false;   // the same on the general path (closures/metric/mu(T) in x and y only)
@@ -217,6 +217,8 @@ struct osbl_ctx {
   double* Abuf = nullptr;  // stage state with halos
   double* Mbuf = nullptr;  // adjoint weights with halos
   double* pk = nullptr;    // 3 packed arrays: mu, l, lo
+  double* vpart = nullptr; // per-segment partial sums of the characteristic VJP
+  int vseg = 0;            // its segment length
 };
 
 namespace {
@@ -617,6 +619,7 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->Abuf);
   (void)hipFree(c->Mbuf);
   (void)hipFree(c->pk);
+  (void)hipFree(c->vpart);
   (void)hipFree(c->cw_dev);
   (void)hipFree(c->Theta);
   (void)hipFree(c->QF);
@@ -1072,6 +1075,11 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
     HIPCHK(hipMemsetAsync(c->Abuf, 0, bundle, c->stream));
     HIPCHK(hipMemsetAsync(c->Mbuf, 0, bundle, c->stream));
   }
+  if (c->cfg.conv == OSBL_CONV_WENO_Z && !c->vpart) {
+    c->vseg = adj_char_segment(c->g, c->vjp_seg);
+    const size_t bytes = (size_t)adj_char_scratch(c->g, c->vseg) * sizeof(double);
+    if (hipMalloc((void**)&c->vpart, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint scratch");
+  }
   return OSBL_OK;
 }
 
@@ -1090,8 +1098,8 @@ static void vjp(osbl_ctx* c, const double* q, const double* mu, double* out) {
 static void adj_stage(osbl_ctx* c, const double* Qs, const double* A, const double* B, double alpha, double beta,
                       double cc, double* out1, double* out2) {
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
-    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->vjp_seg,
-                          false, c->stream);
+    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, c->vpart, c->g, c->ph,
+                          c->vseg, false, c->stream);
   else
     launch_adj_stage(c->nd, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->stream);
 }
@@ -1101,8 +1109,8 @@ int osbl_rhs_vjp_dev(osbl_ctx* c, const double* q_dev, const double* mu_dev, dou
   int rc = adjoint_ready(c, false);
   if (rc) return rc;
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
-    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, c->g,
-                          c->ph, c->vjp_seg, true, c->stream);
+    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, c->vpart,
+                          c->g, c->ph, c->vseg, true, c->stream);
   else vjp(c, q_dev, mu_dev, out_dev);
   HIPCHK(hipGetLastError());
   return OSBL_OK;
