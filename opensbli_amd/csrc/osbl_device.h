@@ -47,6 +47,20 @@ __host__ __forceinline__ int fit_segment(long long lines, int n, int seg, int lo
   return seg;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id.  The
+// tile of a workgroup: linear ids are renumbered so that each XCD gets one
+// contiguous range of tiles (x fastest, then y, then z) and tiles sharing
+// halo rows sit behind one L2.  A bijection when the tile count is a
+// multiple of 8, the identity otherwise.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+  const int nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+  const int lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const int logical = (total % 8 == 0) ? (lin % 8) * (total / 8) + lin / 8 : lin;
+  bx = logical % nx;
+  by = (logical / nx) % ny;
+  bz = logical / (nx * ny);
+}
+
 struct Phys {
   double gama, gm1, Minf, Re, Pr, dt, eps, teno_ct;
   double inv_delta[3];

@@ -541,8 +541,10 @@ __global__ void __launch_bounds__(VTX * VTY, 1) k_visc_update3d(const double* __
   __shared__ double sp[3][4][VPL];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * VTX + tx;
-  const int x0 = blockIdx.x * VTX, y0 = blockIdx.y * VTY;
-  const int z0 = blockIdx.z * sz;
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int x0 = bx * VTX, y0 = by * VTY;
+  const int z0 = bz * sz;
   const int z1 = min(z0 + sz, g.n[2]);
   const int x = x0 + tx, y = y0 + ty;
   const bool inside = x < g.n[0] && y < g.n[1];
