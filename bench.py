@@ -96,6 +96,7 @@ def main():
     if args.workload == "adjoint":
         out = adjoint_ratio()
         out["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
+        out["central_viscous"] = adjoint_ratio(viscous=True, reps=1)
         print(json.dumps(out), flush=True)
         return
 
@@ -167,6 +168,7 @@ def main():
         _log("bench: adjoint/forward ratio (2-D Kelvin-Helmholtz 512^2, 200 steps, checkpoints every 20)")
         adj = adjoint_ratio()
         adj["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
+        adj["central_viscous"] = adjoint_ratio(viscous=True, reps=1)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -198,10 +200,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew"):
+def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew", viscous=False):
     """Backward/forward time of the differentiable 2-D Kelvin-Helmholtz run:
     forward = the checkpointed forward integration, backward = recomputation
-    of every segment plus the hand-written adjoint steps (median of reps)."""
+    of every segment plus the hand-written adjoint steps (median of reps).
+    viscous: with constant-viscosity terms (Re 1000, Pr 0.72, Minf 0.5)."""
     import numpy as np
     import torch
     from opensbli_amd.adjoint import integrate, kh_initial_state, kinetic_energy
@@ -209,6 +212,8 @@ def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew"):
     from opensbli_amd.problems import kelvin_helmholtz
     amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
     problem, _ = kelvin_helmholtz(n, amps, conv=conv)
+    if viscous:
+        problem.update(viscous="const", Re=1000.0, Pr=0.72, Minf=0.5)
     b = Block(problem)
     fw, bw = [], []
     for rep in range(reps + 1):
@@ -227,7 +232,8 @@ def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew"):
             bw.append(t2 - t1)
     b.close()
     f, w = float(np.median(fw)), float(np.median(bw))
-    return {"workload": f"kelvin_helmholtz_{n}^2_{conv}_rk3_{steps}_steps_checkpoint_{every}",
+    tag = "_viscous" if viscous else ""
+    return {"workload": f"kelvin_helmholtz_{n}^2_{conv}{tag}_rk3_{steps}_steps_checkpoint_{every}",
             "forward_ms": f * 1e3, "backward_ms": w * 1e3, "ratio": w / f}
 
 

@@ -39,6 +39,9 @@ void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, c
                            double beta, double c, double* out1, double* out2, double* part, const Grid& g,
                            const Phys& ph, int seg, bool acc, hipStream_t s);
 int adj_char_segment(const Grid& g, int seg);
+int visc_vjp_fields(int nd);
+void launch_visc_vjp(int nd, const double* Qpad, const double* P, const double* A, const double* B, double alpha,
+                     double beta, double* Mpad, double* W, double* out, const Grid& g, const Phys& ph, hipStream_t s);
 long long adj_char_scratch(const Grid& g, int seg);
 void launch_weno_filter_apply(int nd, double* Q, const double* R, double* theta, const Grid& g, const Phys& ph,
                               bool ducros, bool mach, double control, int walls, hipStream_t s);

@@ -218,6 +218,7 @@ struct osbl_ctx {
   double* Mbuf = nullptr;  // adjoint weights with halos
   double* pk = nullptr;    // 3 packed arrays: mu, l, lo
   double* vpart = nullptr; // per-segment partial sums of the characteristic VJP
+  double* vw = nullptr;    // viscous VJP fields (padded)
   int vseg = 0;            // its segment length
 };
 
@@ -620,6 +621,7 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->Mbuf);
   (void)hipFree(c->pk);
   (void)hipFree(c->vpart);
+  (void)hipFree(c->vw);
   (void)hipFree(c->cw_dev);
   (void)hipFree(c->Theta);
   (void)hipFree(c->QF);
@@ -1060,9 +1062,13 @@ int osbl_num_stages(osbl_ctx* c) { return c ? c->nstages : fail(OSBL_E_ARG, "nul
 // discrete adjoint (inviscid central skew or LLF WENO5-Z, periodic, RK3)
 // ---------------------------------------------------------------------------
 static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
-  if ((c->cfg.conv != OSBL_CONV_CENTRAL_SKEW && c->cfg.conv != OSBL_CONV_WENO_Z) || c->cfg.viscous != OSBL_VISC_NONE)
-    return fail(OSBL_E_ARG, "the adjoint covers the inviscid central skew-symmetric and WENO5-Z schemes "
+  if (c->cfg.conv != OSBL_CONV_CENTRAL_SKEW && c->cfg.conv != OSBL_CONV_WENO_Z)
+    return fail(OSBL_E_ARG, "the adjoint covers the central skew-symmetric and WENO5-Z schemes "
                             "(TENO's cut-off is not differentiable)");
+  if (c->cfg.viscous == OSBL_VISC_SUTHERLAND)
+    return fail(OSBL_E_ARG, "the adjoint covers constant viscosity (not Sutherland's law)");
+  if (c->general || c->cfg.filters)
+    return fail(OSBL_E_ARG, "the adjoint covers periodic blocks without metric or filters");
   for (int d = 0; d < c->nd; ++d)
     if (c->cfg.bc[d][0] != OSBL_BC_PERIODIC) return fail(OSBL_E_ARG, "the adjoint needs periodic boundaries");
   if (need_rk3 && c->cfg.rk != OSBL_RK3) return fail(OSBL_E_ARG, "the adjoint step covers RungeKutta(3)");
@@ -1074,6 +1080,11 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
       return fail(OSBL_E_NOMEM, "adjoint buffers");
     HIPCHK(hipMemsetAsync(c->Abuf, 0, bundle, c->stream));
     HIPCHK(hipMemsetAsync(c->Mbuf, 0, bundle, c->stream));
+  }
+  if (c->cfg.viscous == OSBL_VISC_CONST && !c->vw) {
+    const size_t bytes = (size_t)c->g.fs * visc_vjp_fields(c->nd) * sizeof(double);
+    if (hipMalloc((void**)&c->vw, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint viscous scratch");
+    HIPCHK(hipMemsetAsync(c->vw, 0, bytes, c->stream));
   }
   if (c->cfg.conv == OSBL_CONV_WENO_Z && !c->vpart) {
     c->vseg = adj_char_segment(c->g, c->vjp_seg);
@@ -1094,6 +1105,16 @@ static void vjp(osbl_ctx* c, const double* q, const double* mu, double* out) {
   launch_rhs_vjp(c->nd, c->Abuf, c->P, c->Mbuf, out, c->g, c->ph, c->stream);
 }
 
+// out (packed) += (dV/dQ at q)^T (alpha A + beta B) for the viscous terms V
+static void visc_vjp(osbl_ctx* c, const double* q, const double* A, const double* B, double alpha, double beta,
+                     double* out) {
+  const long long n = cells(c);
+  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, c->Abuf + m * c->g.fs, c->g, c->stream);
+  launch_halo_periodic(c->Abuf, c->nv, c->g, c->stream);
+  launch_prims(c->nd, c->Abuf, c->P, c->g, c->ph, c->stream);
+  launch_visc_vjp(c->nd, c->Abuf, c->P, A, B, alpha, beta, c->Mbuf, c->vw, out, c->g, c->ph, c->stream);
+}
+
 // out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B), out2 = A + B (optional)
 static void adj_stage(osbl_ctx* c, const double* Qs, const double* A, const double* B, double alpha, double beta,
                       double cc, double* out1, double* out2) {
@@ -1102,6 +1123,7 @@ static void adj_stage(osbl_ctx* c, const double* Qs, const double* A, const doub
                           c->vseg, false, c->stream);
   else
     launch_adj_stage(c->nd, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->stream);
+  if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, Qs, A, B, alpha, beta, out1);
 }
 
 int osbl_rhs_vjp_dev(osbl_ctx* c, const double* q_dev, const double* mu_dev, double* out_dev) {
@@ -1112,6 +1134,7 @@ int osbl_rhs_vjp_dev(osbl_ctx* c, const double* q_dev, const double* mu_dev, dou
     launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, c->vpart,
                           c->g, c->ph, c->vseg, true, c->stream);
   else vjp(c, q_dev, mu_dev, out_dev);
+  if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, q_dev, mu_dev, nullptr, 1.0, 0.0, out_dev);
   HIPCHK(hipGetLastError());
   return OSBL_OK;
 }

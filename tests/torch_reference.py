@@ -1,11 +1,12 @@
-"""Plain PyTorch FP64 statement of the inviscid central skew-symmetric scheme
-with RungeKutta(3) on a periodic 2-D grid (test infrastructure): the autograd
+"""Plain PyTorch FP64 statement of the central skew-symmetric scheme, the
+LLF characteristic WENO5-Z residual and the constant-viscosity terms with
+RungeKutta(3) on periodic grids (test infrastructure): the autograd
 reference for the hand-written adjoint.
 
 The same discretisation as the reference's TGV-style Skew form
-(parsing.py:75-111, scheme.py:187-252) without viscous terms: 4th-order
-central first derivatives with periodic wrap, 3-register RK3
-(rk_sbli.py:58-60,117-133).
+(parsing.py:75-111, scheme.py:187-252): 4th-order central first derivatives
+with periodic wrap, 3-register RK3 (rk_sbli.py:58-60,117-133); the WENO5-Z
+and viscous parts are described where they start.
 """
 import torch
 
@@ -154,4 +155,73 @@ def rk3_step_weno(q, dt, delta, gama):
 def integrate_weno(q, nsteps, dt, delta, gama):
     for _ in range(nsteps):
         q = rk3_step_weno(q, dt, delta, gama)
+    return q
+
+
+# ---------------------------------------------------------------------------
+# Viscous terms with constant viscosity (taylor_green_vortex.py:14-19): the
+# stress tensor and heat flux differentiated by the product rule as
+# Der.differentiate does (parsing.py:141-186), second derivatives with the
+# 4th-order D2 weights, mixed derivatives as D_b(D_a u) for a < b
+# (scheme.py:81-85,360-387); any dimension, periodic.
+# ---------------------------------------------------------------------------
+def d2(f, axis, inv):
+    """4th-order central second derivative along a tensor axis (periodic)."""
+    return inv * inv * ((-1.0 / 12.0) * (torch.roll(f, 2, axis) + torch.roll(f, -2, axis))
+                        + (4.0 / 3.0) * (torch.roll(f, 1, axis) + torch.roll(f, -1, axis)) - 2.5 * f)
+
+
+def viscous(q, delta, gama, Minf, Re, Pr):
+    nd = q.dim() - 1
+    nv = nd + 2
+    rho = q[0]
+    u = [q[1 + d] / rho for d in range(nd)]
+    p = (gama - 1.0) * (q[nv - 1] - 0.5 * rho * sum(ui * ui for ui in u))
+    T = p * gama * Minf * Minf / rho
+    inv_re = 1.0 / Re
+    cq = 1.0 / ((gama - 1.0) * Minf * Minf * Pr * Re)
+    ax = [nd - 1 - j for j in range(nd)]
+    inv = [1.0 / delta[j] for j in range(nd)]
+    du = [[d1(u[m], ax[j], inv[j]) for j in range(nd)] for m in range(nd)]
+    dd = [[[None] * nd for _ in range(nd)] for _ in range(nd)]
+    for m in range(nd):
+        for a in range(nd):
+            dd[m][a][a] = d2(u[m], ax[a], inv[a])
+            for b in range(a + 1, nd):
+                v = d1(d1(u[m], ax[a], inv[a]), ax[b], inv[b])
+                dd[m][a][b] = v
+                dd[m][b][a] = v
+    divu = sum(du[k][k] for k in range(nd))
+    energy = sum(cq * d2(T, ax[j], inv[j]) for j in range(nd))
+    out = [torch.zeros_like(rho)]
+    for m in range(nd):
+        sk = sum(dd[k][m][k] for k in range(nd))
+        visc = torch.zeros_like(rho)
+        for j in range(nd):
+            tt = dd[m][j][j] + dd[j][m][j] - ((2.0 / 3.0) * sk if j == m else 0.0)
+            dtau = inv_re * tt
+            tau = inv_re * (du[m][j] + du[j][m] - ((2.0 / 3.0) * divu if j == m else 0.0))
+            visc = visc + dtau
+            energy = energy + du[m][j] * tau + u[m] * dtau
+        out.append(visc)
+    out.append(energy)
+    return torch.stack(out)
+
+
+def rhs_ns(q, delta, gama, Minf, Re, Pr, conv="central_skew"):
+    """Central skew (any dimension) or WENO5-Z (2-D) convective residual plus the viscous terms."""
+    r = rhs(q, delta, gama) if conv == "central_skew" else rhs_weno_z(q, delta, gama)
+    return r + viscous(q, delta, gama, Minf, Re, Pr)
+
+
+def integrate_ns(q, nsteps, dt, delta, gama, Minf, Re, Pr, conv="central_skew"):
+    rknew = [2.0 / 3.0, 5.0 / 12.0, 3.0 / 5.0]
+    rkold = [1.0 / 4.0, 3.0 / 20.0, 3.0 / 5.0]
+    for _ in range(nsteps):
+        qold, cur = q, q
+        for s in range(3):
+            r = rhs_ns(cur, delta, gama, Minf, Re, Pr, conv)
+            cur = qold + dt * rknew[s] * r
+            qold = qold + dt * rkold[s] * r
+        q = cur
     return q
