@@ -240,9 +240,11 @@ int osbl_kernel_work(osbl_ctx* ctx, int index, double* bytes, double* flops);
 int osbl_num_stages(osbl_ctx* ctx);
 
 /*
- * Discrete adjoint of the inviscid central skew-symmetric scheme and of the
- * LLF characteristic WENO5-Z scheme on periodic blocks (no counterpart in the
- * reference, which generates forward code only; TENO's cut-off is excluded).  All arrays are device pointers to packed interior fields
+ * Discrete adjoint of the central skew-symmetric scheme and of the LLF
+ * characteristic WENO5-Z scheme, each with or without constant-viscosity
+ * terms, on periodic blocks without metric or filters (no counterpart in the
+ * reference, which generates forward code only; TENO's cut-off and
+ * Sutherland's law are excluded).  All arrays are device pointers to packed interior fields
  * (nvar blocks of n[0]*n[1]*n[2] doubles), enqueued on the context's stream.
  *  osbl_rhs_vjp_dev:      out += (dR/dQ at q)^T mu
  *  osbl_step_stages_dev:  advance one RK step from the current state and store

@@ -9,9 +9,9 @@ Forward: the block is advanced with the GPU kernels; the state is kept every
 `every` steps.  Backward: for each segment, from last to first, the segment
 is recomputed from its checkpoint while the input states of every RK stage
 are stored, then the adjoint of each step (osbl_adjoint_step_dev) is applied
-from the last step back.  Covered: the inviscid central skew-symmetric scheme
-and the LLF characteristic WENO5-Z scheme on periodic blocks with
-RungeKutta(3) (TENO's cut-off is not differentiable).
+from the last step back.  Covered: the central skew-symmetric scheme and the
+LLF characteristic WENO5-Z scheme, with or without constant-viscosity terms,
+on periodic blocks with RungeKutta(3) (TENO's cut-off is not differentiable).
 """
 import math
 
