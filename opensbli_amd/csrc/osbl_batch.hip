@@ -58,6 +58,15 @@ __device__ __forceinline__ void stage_segment(const double* __restrict__ Q, doub
     finish_pt<ND>(w[5], ph);
     if (f + 1 < c1) fetch_q<ND>(nxt, Q, fs, off + 6);
     double fh[NV];
+    // the RK register of this iteration's cell is fetched before the flux so
+    // that the load overlaps its arithmetic; the cell's state is window point 2
+    double qoin[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) qoin[m] = 0.0;
+    if (f >= c0 && !(MODE == 0 && first)) {
+#pragma unroll
+      for (int m = 0; m < NV; ++m) qoin[m] = Qo[m * fs + off + 2];
+    }
     char_flux<ND, 0, CONV, AVG>(w, ph, fh, ph.teno_ct);
     if (f >= c0) {
       const int x = off + 2;
@@ -65,13 +74,13 @@ __device__ __forceinline__ void stage_segment(const double* __restrict__ Q, doub
       for (int m = 0; m < NV; ++m) {
         const double r = -(fh[m] - fprev[m]) * in.inv_dx;
         if constexpr (MODE == 0) {
-          const double qo = first ? Q[m * fs + x] : Qo[m * fs + x];
+          const double qo = first ? w[2].q[m] : qoin[m];
           Qn[m * fs + x] = qo + in.dt * cA * r;
           Qo[m * fs + x] = qo + in.dt * cB * r;
         } else {
-          const double tmp = (cA == 0.0 ? 0.0 : cA * Qo[m * fs + x]) + in.dt * r;
+          const double tmp = (cA == 0.0 ? 0.0 : cA * qoin[m]) + in.dt * r;
           Qo[m * fs + x] = tmp;
-          Qn[m * fs + x] = Q[m * fs + x] + cB * tmp;
+          Qn[m * fs + x] = w[2].q[m] + cB * tmp;
         }
       }
     }

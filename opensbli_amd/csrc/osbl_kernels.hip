@@ -171,6 +171,20 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
     finish_pt<ND>(w[5], ph);
     if (f + 1 < c1) fetch_q<ND>(nxt, Q, g.fs, off + 6 * st);
     double fh[NV];
+    // the residual (and RK register) of this iteration's cell are fetched
+    // before the flux, so that the loads overlap its arithmetic instead of
+    // stalling the update after it
+    [[maybe_unused]] double rin[NV], qoin[NV];
+    if constexpr (RKM >= 0 || ACC) {
+      if (f >= c0) {
+        const int x = off + 2 * st;
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          rin[m] = R[m * g.fs + x];
+          if constexpr (RKM >= 0) qoin[m] = (RKM == 0 && rk.first) ? 0.0 : rk.Qo[m * g.fs + x];
+        }
+      }
+    }
     // interface between window points 2 and 3: its left point is at off + 2 st
     const double ct = theta ? adaptive_ct(theta[off + 2 * st], ph) : ph.teno_ct;
     if constexpr (AVG == OSBL_AVG_ROE) roe_factors<ND>(w[3], rf.sR, rf.hR);
@@ -187,17 +201,17 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
       for (int m = 0; m < NV; ++m) {
         const double v = met ? -(gm * ((fh[m] - fprev[m]) * inv)) : -(fh[m] - fprev[m]) * inv;
         if constexpr (RKM == 0) {
-          const double r = R[m * g.fs + x] + v;
-          const double qo = rk.first ? w[2].q[m] : rk.Qo[m * g.fs + x];
+          const double r = rin[m] + v;
+          const double qo = rk.first ? w[2].q[m] : qoin[m];
           rk.Qn[m * g.fs + x] = qo + ph.dt * rk.cA * r;
           rk.Qo[m * g.fs + x] = qo + ph.dt * rk.cB * r;
         } else if constexpr (RKM == 1) {
-          const double r = R[m * g.fs + x] + v;
-          const double tmp = (rk.cA == 0.0 ? 0.0 : rk.cA * rk.Qo[m * g.fs + x]) + ph.dt * r;
+          const double r = rin[m] + v;
+          const double tmp = (rk.cA == 0.0 ? 0.0 : rk.cA * qoin[m]) + ph.dt * r;
           rk.Qo[m * g.fs + x] = tmp;
           rk.Qn[m * g.fs + x] = w[2].q[m] + rk.cB * tmp;
         } else if constexpr (ACC) {
-          R[m * g.fs + x] += v;
+          R[m * g.fs + x] = rin[m] + v;
         } else {
           R[m * g.fs + x] = v;
         }
