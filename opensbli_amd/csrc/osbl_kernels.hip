@@ -173,9 +173,11 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
     double fh[NV];
     // the residual (and RK register) of this iteration's cell are fetched
     // before the flux, so that the loads overlap its arithmetic instead of
-    // stalling the update after it
+    // stalling the update after it -- except for TENO6, whose sweep would
+    // then need more registers than two waves per SIMD leave it
+    constexpr bool EARLY = CONV != OSBL_CONV_TENO6;
     [[maybe_unused]] double rin[NV], qoin[NV];
-    if constexpr (RKM >= 0 || ACC) {
+    if constexpr ((RKM >= 0 || ACC) && EARLY) {
       if (f >= c0) {
         const int x = off + 2 * st;
 #pragma unroll
@@ -201,17 +203,17 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
       for (int m = 0; m < NV; ++m) {
         const double v = met ? -(gm * ((fh[m] - fprev[m]) * inv)) : -(fh[m] - fprev[m]) * inv;
         if constexpr (RKM == 0) {
-          const double r = rin[m] + v;
-          const double qo = rk.first ? w[2].q[m] : qoin[m];
+          const double r = (EARLY ? rin[m] : R[m * g.fs + x]) + v;
+          const double qo = rk.first ? w[2].q[m] : (EARLY ? qoin[m] : rk.Qo[m * g.fs + x]);
           rk.Qn[m * g.fs + x] = qo + ph.dt * rk.cA * r;
           rk.Qo[m * g.fs + x] = qo + ph.dt * rk.cB * r;
         } else if constexpr (RKM == 1) {
-          const double r = rin[m] + v;
-          const double tmp = (rk.cA == 0.0 ? 0.0 : rk.cA * qoin[m]) + ph.dt * r;
+          const double r = (EARLY ? rin[m] : R[m * g.fs + x]) + v;
+          const double tmp = (rk.cA == 0.0 ? 0.0 : rk.cA * (EARLY ? qoin[m] : rk.Qo[m * g.fs + x])) + ph.dt * r;
           rk.Qo[m * g.fs + x] = tmp;
           rk.Qn[m * g.fs + x] = w[2].q[m] + rk.cB * tmp;
         } else if constexpr (ACC) {
-          R[m * g.fs + x] = rin[m] + v;
+          R[m * g.fs + x] = (EARLY ? rin[m] : R[m * g.fs + x]) + v;
         } else {
           R[m * g.fs + x] = v;
         }
