@@ -301,46 +301,142 @@ __global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restr
       else R[m * g.fs + base + x] = v;
     }
   };
-  double prev[NV];  // flux left of the chunk's first cell (valid in every lane)
-  double first[NV];
-  if (!periodic) {
-    // flux at interface -1/2 (points -3..2): lane 0 of a chunk staged at x0 = -1
-    stage_chunk(-1);
-    double fh[NV];
-    flux(fh, -1 + lane);
+  if constexpr (CONV == OSBL_CONV_WENO_Z || CONV == OSBL_CONV_WENO_JS) {
+    // Pipelined walk (the WENO sweeps have the registers for it): slot s
+    // holds point x0 - 7 + s, lane l evaluates the flux at interface
+    // I = x0 - 5 + l (points I-2 .. I+3 = slots l .. l+5) and finishes cell
+    // I.  Slots 0..4 of a chunk are slots 64..68 of the one before, so each
+    // chunk stages one new point per lane, whose raw values are fetched while
+    // the previous chunk evaluates its fluxes.  Periodic lines start at I = 0
+    // (the flux left of cell 0 is the line's last one, used after the walk),
+    // others at I = -1.
+    auto clampx = [&](int x) { return max(-g.h[0], min(x, xmax)); };
+    auto stage_q = [&](int slot, const double (&q)[NV]) {
+      Pt<ND> p;
 #pragma unroll
-    for (int m = 0; m < NV; ++m) prev[m] = __shfl(fh[m], 0);
-  }
-  for (int x0 = 0; x0 < N; x0 += 64) {
-    stage_chunk(x0);
-    double fh[NV];
-    flux(fh, x0 + lane);
-    const int x = x0 + lane;
-    double left[NV];
+      for (int m = 0; m < NV; ++m) p.q[m] = q[m];
+      finish_pt<ND>(p, ph);
+#pragma unroll
+      for (int m = 0; m < NV; ++m) S[m][slot] = p.q[m];
+#pragma unroll
+      for (int m = 0; m < ND; ++m) S[NV + m][slot] = p.u[m];
+      S[NV + ND][slot] = p.p;
+      S[NV + ND + 1][slot] = p.a;
+      if constexpr (ROE) roe_factors<ND>(p, S[NV + ND + 2][slot], S[NV + ND + 3][slot]);
+    };
+    auto wave_sync = [&] {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto flux_at = [&](double fh[NV]) {
+      const LdsWindow<ND> w{&S[0][lane], XP};
+      [[maybe_unused]] RoeF rf{};
+      if constexpr (ROE) {
+        rf.sL = S[NV + ND + 2][lane + 2];
+        rf.hL = S[NV + ND + 3][lane + 2];
+        rf.sR = S[NV + ND + 2][lane + 3];
+        rf.hR = S[NV + ND + 3][lane + 3];
+      }
+      char_flux_w<ND, 0, CONV, AVG, true>(w, ph, fh, ph.teno_ct, rf);
+    };
+    const int x0s = periodic ? 5 : 4;
+    double nq[NV];
+    if (lane < 5) {
+      double q0[NV];
+      fetch_q<ND>(q0, Q, g.fs, base + clampx(x0s - 7 + lane));
+      stage_q(lane, q0);
+    }
+    fetch_q<ND>(nq, Q, g.fs, base + clampx(x0s - 2 + lane));
+    double prev[NV], first[NV];
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
-      const double up = __shfl_up(fh[m], 1);
-      left[m] = lane == 0 ? prev[m] : up;
+      prev[m] = 0.0;
+      first[m] = 0.0;
     }
-    const int last = min(63, N - 1 - x0);  // lane holding the chunk's last valid interface
+    for (int x0 = x0s; x0 - 5 <= N - 1; x0 += 64) {
+      stage_q(5 + lane, nq);
+      wave_sync();
+      // the next chunk's point, in flight while this chunk's fluxes are evaluated
+      fetch_q<ND>(nq, Q, g.fs, base + clampx(x0 + 62 + lane));
+      double fh[NV];
+      const int I = x0 - 5 + lane;
+      flux_at(fh);
+      double left[NV];
 #pragma unroll
-    for (int m = 0; m < NV; ++m) {
-      if (x0 == 0) first[m] = fh[m];
-      prev[m] = __shfl(fh[m], last);
+      for (int m = 0; m < NV; ++m) {
+        const double up = __shfl_up(fh[m], 1);
+        left[m] = lane == 0 ? prev[m] : up;
+      }
+      const int last = min(63, N - 1 - (x0 - 5));  // lane holding the line's last interface of the chunk
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        if (x0 == x0s) first[m] = __shfl(fh[m], 0);
+        prev[m] = __shfl(fh[m], last);
+      }
+      if (I >= 0 && I < N && !(periodic && I == 0)) {
+        double r[NV];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) r[m] = fh[m] - left[m];
+        emit(I, r);
+      }
+      // slots 64..68 become slots 0..4 of the next chunk
+      wave_sync();
+      if (lane < 5) {
+#pragma unroll
+        for (int e = 0; e < NF; ++e) S[e][lane] = S[e][64 + lane];
+      }
+      wave_sync();
     }
-    if (x < N && !(periodic && x == 0)) {
+    if (periodic && lane == 0) {
+      // the line's last interface flux is the flux left of cell 0
       double r[NV];
 #pragma unroll
-      for (int m = 0; m < NV; ++m) r[m] = fh[m] - left[m];
-      emit(x, r);
+      for (int m = 0; m < NV; ++m) r[m] = first[m] - prev[m];
+      emit(0, r);
     }
-  }
-  if (periodic && lane == 0) {
-    // the line's last interface flux is the flux left of cell 0
-    double r[NV];
-#pragma unroll
-    for (int m = 0; m < NV; ++m) r[m] = first[m] - prev[m];
-    emit(0, r);
+  } else {
+    double prev[NV];  // flux left of the chunk's first cell (valid in every lane)
+    double first[NV];
+    if (!periodic) {
+      // flux at interface -1/2 (points -3..2): lane 0 of a chunk staged at x0 = -1
+      stage_chunk(-1);
+      double fh[NV];
+      flux(fh, -1 + lane);
+  #pragma unroll
+      for (int m = 0; m < NV; ++m) prev[m] = __shfl(fh[m], 0);
+    }
+    for (int x0 = 0; x0 < N; x0 += 64) {
+      stage_chunk(x0);
+      double fh[NV];
+      flux(fh, x0 + lane);
+      const int x = x0 + lane;
+      double left[NV];
+  #pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        const double up = __shfl_up(fh[m], 1);
+        left[m] = lane == 0 ? prev[m] : up;
+      }
+      const int last = min(63, N - 1 - x0);  // lane holding the chunk's last valid interface
+  #pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        if (x0 == 0) first[m] = fh[m];
+        prev[m] = __shfl(fh[m], last);
+      }
+      if (x < N && !(periodic && x == 0)) {
+        double r[NV];
+  #pragma unroll
+        for (int m = 0; m < NV; ++m) r[m] = fh[m] - left[m];
+        emit(x, r);
+      }
+    }
+    if (periodic && lane == 0) {
+      // the line's last interface flux is the flux left of cell 0
+      double r[NV];
+  #pragma unroll
+      for (int m = 0; m < NV; ++m) r[m] = first[m] - prev[m];
+      emit(0, r);
+    }
   }
 }
 
