@@ -291,13 +291,31 @@ __global__ void __launch_bounds__(128) k_update_gen(const double* __restrict__ Q
 // metric (checked by the caller).  The derivative and assembly formulas are
 // those of viscous_gen.
 // ---------------------------------------------------------------------------
+// u_0..u_{ND-1}, p, T (and mu(T) by Sutherland's law) of the point at x --
+// the constituent relations k_prims / k_prims_mu store, formed in place
+template <int ND, bool MU>
+__device__ __forceinline__ void prims_at(const double* __restrict__ Q, long long fs, int x, const Phys& ph,
+                                         double (&u)[ND], double& p, double& T, double& mu) {
+  const double rho = Q[x];
+  const double ir = 1.0 / rho;
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    u[m] = Q[(1 + m) * fs + x] * ir;
+    ke += u[m] * u[m];
+  }
+  p = ph.gm1 * (Q[(ND + 1) * fs + x] - 0.5 * rho * ke);
+  T = p * ph.gama * ph.Minf * ph.Minf * ir;
+  if constexpr (MU) mu = pow(T, 1.5) * (1.0 + ph.suth) / (T + ph.suth);
+}
+
 constexpr int GTX = 64, GTY = 8, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
 
 // closure rows of the first (Carpenter 4, ReducedAccess 2) and second (2) derivatives
 __host__ __device__ __forceinline__ int closure_rows(int mod) { return mod == 0 ? 0 : (mod == 1 ? 4 : 2); }
 
 template <bool SUTH>
-__global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __restrict__ P, double* __restrict__ R,
+__global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __restrict__ Q, double* __restrict__ R,
                                                             Grid g, Phys ph, Bnd b, int sz) {
   constexpr int NF = SUTH ? 5 : 4;  // u0, u1, u2, T (, mu)
   constexpr int NQ = NF + 4;        // + D_x u0, D_x u2, D_y u1, D_y u2
@@ -316,17 +334,19 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
   const double G0 = (b.dm[0] && inside) ? b.dm[0][x] : 1.0, S0 = (b.sd[0] && inside) ? b.sd[0][x] : 0.0;
   const double G1 = (b.dm[1] && inside) ? b.dm[1][y] : 1.0, S1 = (b.sd[1] && inside) ? b.sd[1][y] : 0.0;
 
+  // the constituent relations of every staged point, formed from Q
   auto stage = [&](int zp, int slot) {
-    for (int p = tid; p < GPL; p += GTX * GTY) {
-      const int xx = p % GPX, yy = p / GPX;
+    for (int pt = tid; pt < GPL; pt += GTX * GTY) {
+      const int xx = pt % GPX, yy = pt / GPX;
       const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
       if (gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1]) {
-        const int o = g.idx(gx, gy, zp);
-        sp[slot][0][p] = P[o];
-        sp[slot][1][p] = P[g.fs + o];
-        sp[slot][2][p] = P[2 * g.fs + o];
-        sp[slot][3][p] = P[4 * g.fs + o];
-        if constexpr (SUTH) sp[slot][4][p] = P[5 * g.fs + o];
+        double u[3], pr, T, mu = 0.0;
+        prims_at<3, SUTH>(Q, g.fs, g.idx(gx, gy, zp), ph, u, pr, T, mu);
+        sp[slot][0][pt] = u[0];
+        sp[slot][1][pt] = u[1];
+        sp[slot][2][pt] = u[2];
+        sp[slot][3][pt] = T;
+        if constexpr (SUTH) sp[slot][4][pt] = mu;
       }
     }
   };
@@ -488,12 +508,12 @@ __global__ void __launch_bounds__(128) k_visc_shell(const double* __restrict__ P
   for (int m = 0; m < 5; ++m) R[m * g.fs + x] = r[m];
 }
 
-void launch_visc_gen3d(const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b, int suth, int sz,
-                       hipStream_t s) {
+void launch_visc_gen3d(const double* Q, const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b,
+                       int suth, int sz, hipStream_t s) {
   dim3 grid((g.n[0] + GTX - 1) / GTX, (g.n[1] + GTY - 1) / GTY, (g.n[2] + sz - 1) / sz);
   dim3 block(GTX, GTY);
-  if (suth) k_visc_gen3d<true><<<grid, block, 0, s>>>(P, R, g, ph, b, sz);
-  else k_visc_gen3d<false><<<grid, block, 0, s>>>(P, R, g, ph, b, sz);
+  if (suth) k_visc_gen3d<true><<<grid, block, 0, s>>>(Q, R, g, ph, b, sz);
+  else k_visc_gen3d<false><<<grid, block, 0, s>>>(Q, R, g, ph, b, sz);
   // the closure rows and columns (distinct indices; a face's rows may overlap
   // the other face's on tiny blocks, hence the de-duplication)
   ShellRows sr{};
@@ -520,20 +540,36 @@ __global__ void __launch_bounds__(256) k_prims_mu(const double* __restrict__ Q, 
   const int k = blockIdx.z - g.h[2];
   if (i >= g.n[0] + g.h[0]) return;
   const int x = g.idx(i, j, k);
-  const double rho = Q[x];
-  const double ir = 1.0 / rho;
-  double ke = 0.0;
+  double u[ND], p, T, mu;
+  prims_at<ND, true>(Q, g.fs, x, ph, u, p, T, mu);
 #pragma unroll
-  for (int m = 0; m < ND; ++m) {
-    const double u = Q[(1 + m) * g.fs + x] * ir;
-    P[m * g.fs + x] = u;
-    ke += u * u;
-  }
-  const double p = ph.gm1 * (Q[(ND + 1) * g.fs + x] - 0.5 * rho * ke);
-  const double T = p * ph.gama * ph.Minf * ph.Minf * ir;
+  for (int m = 0; m < ND; ++m) P[m * g.fs + x] = u[m];
   P[ND * g.fs + x] = p;
   P[(ND + 1) * g.fs + x] = T;
-  P[(ND + 2) * g.fs + x] = pow(T, 1.5) * (1.0 + ph.suth) / (T + ph.suth);
+  P[(ND + 2) * g.fs + x] = mu;
+}
+
+// The same over the slabs within W points of the x and y faces only (every
+// z): what k_visc_shell reads when the tiled kernel forms the constituent
+// relations itself.  Closure rows (at most 4) plus the reach of their
+// one-sided and mixed stencils (at most 6) stay inside W = 10.
+constexpr int PRIM_SLAB = 10;
+template <int ND, bool MU>
+__global__ void __launch_bounds__(256) k_prims_slab(const double* __restrict__ Q, double* __restrict__ P, Grid g,
+                                                    Phys ph) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - g.h[0];
+  const int j = blockIdx.y - g.h[1];
+  const int k = blockIdx.z - g.h[2];
+  if (i >= g.n[0] + g.h[0]) return;
+  if (!(i < PRIM_SLAB || i >= g.n[0] - PRIM_SLAB || j < PRIM_SLAB || j >= g.n[1] - PRIM_SLAB)) return;
+  const int x = g.idx(i, j, k);
+  double u[ND], p, T, mu = 0.0;
+  prims_at<ND, MU>(Q, g.fs, x, ph, u, p, T, mu);
+#pragma unroll
+  for (int m = 0; m < ND; ++m) P[m * g.fs + x] = u[m];
+  P[ND * g.fs + x] = p;
+  P[(ND + 1) * g.fs + x] = T;
+  if constexpr (MU) P[(ND + 2) * g.fs + x] = mu;
 }
 
 // Modified Ducros sensor (shock_sensors.py:12-49) as a constituent relation:
@@ -621,6 +657,13 @@ void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, c
   if (nd == 3) launch_gen_nd<3>(conv, visc, Q, P, R, Qo, Qn, g, ph, b, cA, cB, first, mode, suth, s);
   else if (nd == 2) launch_gen_nd<2>(conv, visc, Q, P, R, Qo, Qn, g, ph, b, cA, cB, first, mode, suth, s);
   else launch_gen_nd<1>(conv, visc, Q, P, R, Qo, Qn, g, ph, b, cA, cB, first, mode, suth, s);
+}
+
+void launch_prims_slab(const double* Q, double* P, const Grid& g, const Phys& ph, int suth, hipStream_t s) {
+  const int PX = g.n[0] + 2 * g.h[0];
+  dim3 grid((PX + 255) / 256, g.n[1] + 2 * g.h[1], g.n[2] + 2 * g.h[2]);
+  if (suth) k_prims_slab<3, true><<<grid, 256, 0, s>>>(Q, P, g, ph);
+  else k_prims_slab<3, false><<<grid, 256, 0, s>>>(Q, P, g, ph);
 }
 
 void launch_prims_mu(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s) {

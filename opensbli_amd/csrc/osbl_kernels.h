@@ -20,8 +20,9 @@ void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, doub
 void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R,
                        double* Qo, double* Qn, const Grid& g, const Phys& ph, const Bnd& b, double cA, double cB,
                        int first, int suth, hipStream_t s);
-void launch_visc_gen3d(const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b, int suth, int sz,
-                       hipStream_t s);
+void launch_visc_gen3d(const double* Q, const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b,
+                       int suth, int sz, hipStream_t s);
+void launch_prims_slab(const double* Q, double* P, const Grid& g, const Phys& ph, int suth, hipStream_t s);
 void launch_prims_mu(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_bc_face(int nd, double* Q, double* Q2, const Grid& g, const Phys& ph, int d, int side, int type, int nh,
                     double twall, const double* plane, hipStream_t s);

@@ -312,10 +312,19 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
   double* Qn = c->Qbuf[1 - c->cur];
   const int conv = c->cfg.conv;
   if (c->need_prims) {
-    timed(c, F_PRIMS, [&] {
-      if (c->suth) launch_prims_mu(c->nd, Q, c->P, c->g, c->ph, c->stream);
-      else launch_prims(c->nd, Q, c->P, c->g, c->ph, c->stream);
-    });
+    // the tiled general schedule forms the constituent relations inside its
+    // viscous kernel; only the near-face rows it leaves to k_visc_shell read P
+    const bool slab = c->tiled_gen && mode != 2;
+    bool shell = false;
+    for (int d = 0; d < 2; ++d)
+      for (int sd = 0; sd < 2; ++sd) shell |= c->bnd.mod[d][sd] != 0;
+    if (!slab || shell) {
+      timed(c, F_PRIMS, [&] {
+        if (slab) launch_prims_slab(Q, c->P, c->g, c->ph, c->suth ? 1 : 0, c->stream);
+        else if (c->suth) launch_prims_mu(c->nd, Q, c->P, c->g, c->ph, c->stream);
+        else launch_prims(c->nd, Q, c->P, c->g, c->ph, c->stream);
+      });
+    }
   }
   if (c->Theta) timed(c, F_PRIMS, [&] { launch_sensor(c->nd, Q, c->Theta, c->g, c->ph, c->bnd, c->stream); });
   if (c->tiled_visc && mode != 2) {
@@ -345,7 +354,7 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
                    c->stream);
     });
     timed(c, F_UPDATE, [&] {
-      launch_visc_gen3d(c->P, c->R, c->g, c->ph, c->bnd, c->suth ? 1 : 0, c->vsz, c->stream);
+      launch_visc_gen3d(Q, c->P, c->R, c->g, c->ph, c->bnd, c->suth ? 1 : 0, c->vsz, c->stream);
     });
     timed(c, F_SWEEP_Y, [&] {
       launch_sweep(c->nd, 1, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[1], c->Theta, true,
