@@ -105,7 +105,12 @@ typedef struct osbl_config {
   double twall;          /* Twall of the isothermal wall */
   int32_t sensor;        /* 1: modified Ducros sensor with the adaptive TENO cut-off
                             (shock_sensors.py:12-49, teno.py:430-443, LLFTeno(formulation='adaptive')) */
-  int32_t reserved2;
+  int32_t curvilinear;   /* 1: general curvilinear metric D_ij = d xi_i/d x_j and detJ from the grid
+                            coordinates (osbl_set_grid_coordinates): the strong conservation form
+                            -Conservative(detJ F_j, xi_j)/detJ with the characteristic system about the
+                            metric normal (MetricsEquation with curvilinear directions, metric.py:72-147;
+                            euler_eigensystem.py:18-40; apps/euler_wave_curvilinear/euler_wave.py);
+                            periodic Euler blocks (2-D, 3-D) with a WENO/TENO scheme */
   double teno_a1, teno_a2, sensor_eps;  /* teno_a1, teno_a2, epsilon of the sensor */
   int32_t binomial_order;  /* even order of the binomial filter, 2..8 (binomial_filter.py:16-24) */
   int32_t wf_sensor;       /* OSBL_WF_*: dissipation_sensor of the WENO filter */
@@ -186,6 +191,22 @@ int osbl_use_graphs(osbl_ctx* ctx, int enable);
    SD_ddd = CD(D_dd, xi_d) with the faces' stencil closures.  Both faces of d
    must modify the central stencils. */
 int osbl_set_coordinates(osbl_ctx* ctx, int d, const double* x_host);
+
+/* Grid coordinates of a curvilinear block: ndim arrays x_c (c = 0..ndim-1),
+   each over the interior extended by `halo` >= 2 points on both sides of every
+   direction, x fastest ((n[0]+2 halo)(n[1]+2 halo)(n[2]+2 halo) doubles, 1 for
+   absent directions).  Replaces the metric kernels of MetricsEquation
+   (metric.py:72-147, 241-273): J_ab = CD(x_a, xi_b) with the 4th-order central
+   derivative, detJ = det J and D = adj(J)/detJ over the interior, then the
+   periodic exchange of D and detJ (metric.py:315-341).  The coordinate arrays
+   are what the reference's initial condition writes into x0..x2 over the
+   interior and its halos (gridbasedinit.py:46-57). */
+int osbl_set_grid_coordinates(osbl_ctx* ctx, int halo, const double* coords);
+
+/* Interior values (x fastest) of a curvilinear metric array: idx = a*ndim + b
+   for D_ab, ndim*ndim for detJ -- the datasets D00.. and detJ the reference
+   can write with iohdf5 (apps/euler_wave_curvilinear/euler_wave.py:137). */
+int osbl_get_metric_field(osbl_ctx* ctx, int idx, double* host);
 
 /* Per-face plane of conservative values, [nvar][n_e2][n_e1] over the interior
    extents of the two other directions (e1 < e2): the values a DirichletBC face

@@ -7,7 +7,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libosbl_hip.so")
 SOURCES = ["osbl_kernels.hip", "osbl_runtime.hip", "osbl_batch.hip", "osbl_adjoint.hip", "osbl_general.hip",
-           "osbl_boundary.hip", "osbl_adjoint_char.hip", "osbl_filters.hip", "osbl_adjoint_visc.hip"]
+           "osbl_boundary.hip", "osbl_adjoint_char.hip", "osbl_filters.hip", "osbl_adjoint_visc.hip",
+           "osbl_curvilinear.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # HDF5 C library of the image (headers and libhdf5.so); OSBL_HDF5_PREFIX overrides
 HDF5_PREFIX = os.environ.get("OSBL_HDF5_PREFIX", "/opt/conda")

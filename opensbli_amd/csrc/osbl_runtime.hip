@@ -202,6 +202,10 @@ struct osbl_ctx {
   double* cw_dev = nullptr;                 // closure rows
   double* met_dev[3] = {nullptr, nullptr, nullptr};  // D_dd then SD_ddd, n[d] each
   double* plane_dev[3][2] = {};             // per-face planes
+  bool curv = false;                        // general curvilinear metric
+  double* cmet = nullptr;                   // its D_ab (a*nd + b) and detJ, padded fields
+  double* cflux = nullptr;                  // interface fluxes of one direction (nv padded fields)
+  bool cmet_ready = false;                  // osbl_set_grid_coordinates has run
   int nplane[3][2] = {};
   // profiling
   bool prof = false;
@@ -327,6 +331,22 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
     }
   }
   if (c->Theta) timed(c, F_PRIMS, [&] { launch_sensor(c->nd, Q, c->Theta, c->g, c->ph, c->bnd, c->stream); });
+  if (c->curv) {
+    // strong conservation form: per direction the interface fluxes about the
+    // metric normal and their difference, divided by detJ after the last one
+    for (int d = 0; d < c->nd; ++d) {
+      timed(c, F_SWEEP_X + d, [&] {
+        launch_curv_direction(c->nd, d, conv, c->cfg.averaging, Q, c->cmet, c->cflux, c->R, c->g, c->ph,
+                              c->cfg.delta[d], c->stream);
+      });
+    }
+    if (mode != 2) {
+      timed(c, F_UPDATE, [&] {
+        launch_update(c->nd, conv, false, mode, Q, c->P, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->stream);
+      });
+    }
+    return;
+  }
   if (c->tiled_visc && mode != 2) {
     // 3-D characteristic scheme with viscous terms: the x sweep assigns R, the
     // viscous kernel and the y sweep accumulate, and the z sweep adds the last
@@ -432,6 +452,18 @@ int check_config(const osbl_config* cfg) {
                               "central stencils on both faces", d);
   }
   if (cfg->gama <= 1.0) return fail(OSBL_E_ARG, "gama must exceed 1");
+  if (cfg->curvilinear) {
+    if (cfg->ndim < 2) return fail(OSBL_E_ARG, "a curvilinear metric needs a 2-D or 3-D block");
+    if (is_central(cfg->conv))
+      return fail(OSBL_E_ARG, "curvilinear blocks are implemented for the WENO/TENO schemes");
+    if (cfg->viscous != OSBL_VISC_NONE || cfg->filters || cfg->sensor)
+      return fail(OSBL_E_ARG, "curvilinear blocks are implemented for the Euler equations without filters or "
+                              "sensor");
+    for (int d = 0; d < cfg->ndim; ++d)
+      if (cfg->bc[d][0] != OSBL_BC_PERIODIC || cfg->stretch[d])
+        return fail(OSBL_E_ARG, "curvilinear blocks are implemented with periodic boundaries in every direction "
+                                "(direction %d is not)", d);
+  }
   if (cfg->sensor) {
     if (cfg->conv != OSBL_CONV_TENO5 && cfg->conv != OSBL_CONV_TENO6)
       return fail(OSBL_E_ARG, "the adaptive cut-off (Ducros sensor) applies to the TENO schemes");
@@ -565,6 +597,12 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
     osbl_destroy(c);
     return rc;
   }
+  c->curv = cfg->curvilinear != 0;
+  if (c->curv && ((rc = alloc(&c->cmet, (size_t)fs * (c->nd * c->nd + 1) * sizeof(double))) ||
+                  (rc = alloc(&c->cflux, bundle)))) {
+    osbl_destroy(c);
+    return rc;
+  }
   const bool wf_ducros = (cfg->filters & OSBL_FILTER_WENO) && cfg->wf_sensor == OSBL_WF_DUCROS;
   if ((cfg->sensor || wf_ducros) && (rc = alloc(&c->Theta, (size_t)fs * sizeof(double)))) {
     osbl_destroy(c);
@@ -631,6 +669,8 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->pk);
   (void)hipFree(c->vpart);
   (void)hipFree(c->vw);
+  (void)hipFree(c->cmet);
+  (void)hipFree(c->cflux);
   (void)hipFree(c->cw_dev);
   (void)hipFree(c->Theta);
   (void)hipFree(c->QF);
@@ -721,6 +761,8 @@ int osbl_get_field_halo(osbl_ctx* c, int var, int32_t* d_m, int32_t* d_p, double
 
 // metric and face planes the configuration asks for must be set before stepping
 static int check_ready(const osbl_ctx* c) {
+  if (c->curv && !c->cmet_ready)
+    return fail(OSBL_E_STATE, "curvilinear block: call osbl_set_grid_coordinates first");
   for (int d = 0; d < c->nd; ++d) {
     if (c->cfg.stretch[d] && !c->bnd.dm[d])
       return fail(OSBL_E_STATE, "direction %d is stretched: call osbl_set_coordinates first", d);
@@ -862,6 +904,52 @@ int osbl_set_coordinates(osbl_ctx* c, int d, const double* x) {
       (void)hipGraphExecDestroy(c->graph2[k]);
       c->graph2[k] = nullptr;
     }
+  return OSBL_OK;
+}
+
+int osbl_set_grid_coordinates(osbl_ctx* c, int halo, const double* coords) {
+  if (!c || !coords) return fail(OSBL_E_ARG, "null argument");
+  if (!c->curv) return fail(OSBL_E_STATE, "the block is not curvilinear");
+  if (halo < 2 || halo > 64) return fail(OSBL_E_ARG, "coordinate halo %d: need 2..64 points", halo);
+  const int nd = c->nd;
+  // the central stencil reads 2 points beyond the interior: keep those
+  int e[3], in[3];
+  for (int d = 0; d < 3; ++d) {
+    e[d] = d < nd ? c->g.n[d] + 4 : 1;
+    in[d] = d < nd ? c->g.n[d] + 2 * halo : 1;
+  }
+  const int off = halo - 2;
+  const long long per = (long long)e[0] * e[1] * e[2], per_in = (long long)in[0] * in[1] * in[2];
+  std::vector<double> sub((size_t)(per * nd));
+  for (int a = 0; a < nd; ++a)
+    for (int k = 0; k < e[2]; ++k)
+      for (int j = 0; j < e[1]; ++j)
+        for (int i = 0; i < e[0]; ++i) {
+          const long long src = (i + off) + (long long)in[0] * ((j + (nd > 1 ? off : 0)) +
+                                                                 (long long)in[1] * (k + (nd > 2 ? off : 0)));
+          sub[(size_t)(a * per + i + (long long)e[0] * (j + (long long)e[1] * k))] = coords[a * per_in + src];
+        }
+  double* X = nullptr;
+  if (hipMalloc((void**)&X, sub.size() * sizeof(double)) != hipSuccess)
+    return fail(OSBL_E_NOMEM, "hipMalloc of the coordinates failed");
+  int rc = OSBL_OK;
+  if (hipMemcpyAsync(X, sub.data(), sub.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = fail(OSBL_E_HIP, "coordinate upload failed");
+  if (!rc) launch_curv_metric(nd, X, c->cmet, c->g, c->ph, e, c->stream);
+  if (!rc && (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = fail(OSBL_E_HIP, "metric kernels failed");
+  (void)hipFree(X);
+  if (!rc) c->cmet_ready = true;
+  return rc;
+}
+
+int osbl_get_metric_field(osbl_ctx* c, int idx, double* host) {
+  if (!c || !host) return fail(OSBL_E_ARG, "null argument");
+  if (!c->curv || !c->cmet_ready) return fail(OSBL_E_STATE, "no curvilinear metric on this block");
+  if (idx < 0 || idx > c->nd * c->nd) return fail(OSBL_E_ARG, "metric index %d out of range", idx);
+  launch_pack(c->cmet + idx * c->g.fs, c->stage_buf, c->g, c->stream);
+  HIPCHK(hipMemcpyAsync(host, c->stage_buf, cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return OSBL_OK;
 }
 
@@ -1076,7 +1164,7 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
                             "(TENO's cut-off is not differentiable)");
   if (c->cfg.viscous == OSBL_VISC_SUTHERLAND)
     return fail(OSBL_E_ARG, "the adjoint covers constant viscosity (not Sutherland's law)");
-  if (c->general || c->cfg.filters)
+  if (c->general || c->curv || c->cfg.filters)
     return fail(OSBL_E_ARG, "the adjoint covers periodic blocks without metric or filters");
   for (int d = 0; d < c->nd; ++d)
     if (c->cfg.bc[d][0] != OSBL_BC_PERIODIC) return fail(OSBL_E_ARG, "the adjoint needs periodic boundaries");

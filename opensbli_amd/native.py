@@ -30,7 +30,7 @@ EXPORTS = [
     "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
     "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
     "osbl_set_coordinates", "osbl_set_bc_plane", "osbl_set_filter_mask", "osbl_get_sfd_field",
-    "osbl_set_sfd_field", "osbl_get_field_halo",
+    "osbl_set_sfd_field", "osbl_get_field_halo", "osbl_set_grid_coordinates", "osbl_get_metric_field",
 ]
 
 
@@ -58,7 +58,7 @@ class OsblConfig(ctypes.Structure):
         ("refT", ctypes.c_double),
         ("twall", ctypes.c_double),
         ("sensor", ctypes.c_int32),
-        ("reserved2", ctypes.c_int32),
+        ("curvilinear", ctypes.c_int32),
         ("teno_a1", ctypes.c_double),
         ("teno_a2", ctypes.c_double),
         ("sensor_eps", ctypes.c_double),
@@ -140,6 +140,8 @@ def load():
     L.osbl_set_filter_mask.argtypes = [vp, vp]
     L.osbl_get_sfd_field.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_set_sfd_field.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_set_grid_coordinates.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_get_metric_field.argtypes = [vp, ctypes.c_int, dp]
     _lib = L
     return L
 
@@ -182,6 +184,7 @@ def make_config(p):
     c.refT = p.get("RefT", 288.0)
     c.twall = p.get("Twall", 1.0)
     c.sensor = 1 if p.get("sensor") else 0
+    c.curvilinear = 1 if p.get("curvilinear") else 0
     c.teno_a1 = p.get("teno_a1", 10.5)
     c.teno_a2 = p.get("teno_a2", 4.5)
     c.sensor_eps = p.get("sensor_eps", 1.0e-30)
@@ -229,12 +232,29 @@ class Block:
         for d, x in problem.get("coords", {}).items():
             a = np.ascontiguousarray(x, dtype=np.float64)
             check(L.osbl_set_coordinates(self.h, int(d), _dptr(a)))
+        if problem.get("grid_coords") is not None:
+            self.set_grid_coordinates(problem["grid_coords"], problem.get("grid_halo", 2))
         for (d, side), pl in problem.get("planes", {}).items():
             a = np.ascontiguousarray(pl, dtype=np.float64)
             check(L.osbl_set_bc_plane(self.h, int(d), int(side), _dptr(a)))
         mask = ((problem.get("filters") or {}).get("binomial") or {}).get("mask")
         if mask is not None:
             self.set_filter_mask(mask)
+
+    def set_grid_coordinates(self, coords, halo=2):
+        """Coordinates of a curvilinear block: (ndim,) + the interior extended by
+        `halo` points on every side, axes in (z, y, x) order."""
+        a = np.ascontiguousarray(coords, dtype=np.float64)
+        want = (self.nd,) + tuple(n + 2 * halo for n in self.shape)
+        if a.shape != want:
+            raise ValueError(f"coordinates have shape {a.shape}, expected {want}")
+        check(load().osbl_set_grid_coordinates(self.h, int(halo), _dptr(a)))
+
+    def metric_field(self, idx):
+        """Interior values of D_ab (idx = a*ndim + b) or detJ (idx = ndim*ndim)."""
+        out = np.empty(self.shape)
+        check(load().osbl_get_metric_field(self.h, int(idx), _dptr(out)))
+        return out
 
     def set_filter_mask(self, mask):
         """Grid condition of the binomial filter (interior cells, nonzero where it applies); None clears it."""

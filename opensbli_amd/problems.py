@@ -205,3 +205,47 @@ def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950
     if sensor:
         problem.update(sensor=1, teno_a1=teno_a1, teno_a2=teno_a2, sensor_eps=sensor_eps)
     return problem, q0
+
+
+def wavy_grid(n, delta, amp=0.04, waves=6.0, omega=0.25, length=2.0, halo=2):
+    """Coordinates of the wavy curvilinear mesh of the curvilinear Euler wave
+    (apps/euler_wave_curvilinear/euler_wave.py:97-107):
+        x0 = i dx + A sin(2 pi omega) sin(S pi j dy / L)
+        x1 = j dy + A sin(2 pi omega) sin(S pi i dx / L)
+    over the interior extended by `halo` points on every side (the initial
+    condition covers the halos, gridbasedinit.py:46-57).  Returns an array of
+    shape (2, n1 + 2 halo, n0 + 2 halo) (x fastest)."""
+    n0, n1 = n
+    dx, dy = delta
+    i = np.arange(-halo, n0 + halo) * 1.0
+    j = np.arange(-halo, n1 + halo) * 1.0
+    jj, ii = np.meshgrid(j, i, indexing="ij")
+    s = amp * math.sin(2.0 * math.pi * omega)
+    x0 = ii * dx + s * np.sin(waves * math.pi * jj * dy / length)
+    x1 = jj * dy + s * np.sin(waves * math.pi * ii * dx / length)
+    return np.stack([x0, x1])
+
+
+def euler_wave_curvilinear(n, conv="weno_z", averaging="roe", rk="ls3", gama=1.4, dt=0.0005, amp=0.04,
+                           coords=None, halo=2):
+    """2-D Euler density wave on a curvilinear mesh of [0, 2)^2, periodic
+    (apps/euler_wave_curvilinear/euler_wave.py): rho = 1 + 0.2 sin(pi (x0 + x1))
+    at the mesh points, u0 = 1, u1 = -0.5, p = 1; LLF WENO5-Z with the Roe
+    average and RK3 low storage, Delta = 2/n.  `coords` overrides the mesh
+    (shape (2, n + 2 halo, n + 2 halo)); the default is the wavy mesh with
+    amplitude `amp`.  Returns (problem, q0), q0 of shape (4, n, n)."""
+    d = 2.0 / n
+    if coords is None:
+        coords = wavy_grid((n, n), (d, d), amp=amp, halo=halo)
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    if coords.shape != (2, n + 2 * halo, n + 2 * halo):
+        raise ValueError("coords must have shape (2, n + 2 halo, n + 2 halo)")
+    problem = dict(ndim=2, n=[n, n], delta=[d, d], conv=conv, averaging=averaging, viscous="none", rk=rk,
+                   bc=[["periodic", "periodic"]] * 2, gama=gama, dt=dt, curvilinear=1, grid_coords=coords,
+                   grid_halo=halo)
+    x0 = coords[0, halo:halo + n, halo:halo + n]
+    x1 = coords[1, halo:halo + n, halo:halo + n]
+    rho = 1.0 + 0.2 * np.sin(np.pi * (x0 + x1))
+    u0, u1, p = 1.0, -0.5, 1.0
+    q = np.stack([rho, rho * u0, rho * u1, p / (gama - 1) + 0.5 * rho * (u0 * u0 + u1 * u1)])
+    return problem, q

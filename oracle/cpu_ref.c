@@ -53,6 +53,8 @@ struct ref_state {
   double* msens;      /* Yee Mach sensor of the WENO filter */
   double* tmp[MAXV];  /* binomial filter output */
   unsigned char* fmask; /* binomial filter grid condition (interior cells) or NULL */
+  double* met[10];    /* curvilinear: D_ab at a*nd + b and detJ at nd*nd, whole padded arrays */
+  double* U[3];       /* curvilinear: contravariant velocity U_a = D_ab u_b (a constituent relation) */
 };
 
 static inline ptrdiff_t IX(const ref_state* s, int i, int j, int k) {
@@ -262,6 +264,16 @@ static void constituent_relations(ref_state* s) {
     s->p[x] = pp;
     s->T[x] = pp * g * M2 / rho[x];
     s->a[x] = sqrt(g * pp / rho[x]);
+  }
+  if (s->c.curvilinear) { /* Eq(U_i, D_i_j*u_j) (apps/euler_wave_curvilinear/euler_wave.py:46) */
+    LOOP(R) {
+      ptrdiff_t x = IX(s, i, j, k);
+      for (int a = 0; a < nd; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < nd; ++b) v += s->met[a * nd + b][x] * s->u[b][x];
+        s->U[a][x] = v;
+      }
+    }
   }
   if (s->c.sensor) ducros_sensor(s);
   if (s->c.viscous == REF_VISC_SUTHERLAND) { /* Eq(mu, T**1.5*(1+SuthT/RefT)/(T+SuthT/RefT)), katzer_SBLI.py:26 */
@@ -769,10 +781,8 @@ double ref_weno_filter_flux(const double* fp6, const double* fm6, const double* 
 /* ------------------------------------------------------------------------- */
 void ref_eigensystem(int nd, int d, double g, double rho, const double* u, double a, double* L, double* R,
                      double* lam) {
-  const double ia = 1.0 / a, ir = 1.0 / rho, gm1 = g - 1.0;
-  const double s2 = sqrt(2.0);
-  const double alpha = rho / (a * s2), bta = ir * ia / s2;
   if (nd == 1) {
+    const double ia = 1.0 / a, gm1 = g - 1.0;
     double u0 = u[0];
     double Hh = a * a / gm1 + u0 * u0 / 2.0;
     double den = 2.0 * Hh - u0 * u0;
@@ -791,13 +801,31 @@ void ref_eigensystem(int nd, int d, double g, double rho, const double* u, doubl
     memcpy(L, Lm, sizeof(Lm));
     return;
   }
+  double kv[3] = {0, 0, 0};
+  kv[d] = 1.0;
+  ref_eigensystem_k(nd, g, rho, u, a, kv, L, R, lam);
+}
+
+/* About the metric direction k = (D_d0, .., D_d(nd-1)) of a curvilinear grid
+   (euler_eigensystem.py:18-40): k_t = k/|k| (|k| replaced by its inverse
+   inv_AVG_met_fact, shock_capturing.py:319-331), theta_t = k_t.u in L and R,
+   eigenvalues U, U(, U), U + a|k|, U - a|k| with U = k.u.  k = e_d gives the
+   Cartesian system exactly. */
+void ref_eigensystem_k(int nd, double g, double rho, const double* u, double a, const double* k, double* L,
+                       double* R, double* lam) {
+  const double ia = 1.0 / a, ir = 1.0 / rho, gm1 = g - 1.0;
+  const double s2 = sqrt(2.0);
+  const double alpha = rho / (a * s2), bta = ir * ia / s2;
+  double kk = 0.0, U = 0.0;
+  for (int m = 0; m < nd; ++m) { kk += k[m] * k[m]; U += k[m] * u[m]; }
+  const double kmag = sqrt(kk), ikm = 1.0 / kmag;
   double kt[3] = {0, 0, 0};
-  kt[d] = 1.0;
+  for (int m = 0; m < nd; ++m) kt[m] = k[m] * ikm;
   if (nd == 2) {
     double u0 = u[0], u1 = u[1];
     double th = kt[0] * u0 + kt[1] * u1;
     double phi = gm1 * ((u0 * u0 + u1 * u1) / 2.0);
-    lam[0] = th; lam[1] = th; lam[2] = th + a; lam[3] = th - a;
+    lam[0] = U; lam[1] = U; lam[2] = U + a * kmag; lam[3] = U - a * kmag;
     double Rm[16] = {1, 0, alpha, alpha,
                      u0, kt[1] * rho, alpha * (u0 + kt[0] * a), alpha * (u0 - kt[0] * a),
                      u1, -kt[0] * rho, alpha * (u1 + kt[1] * a), alpha * (u1 - kt[1] * a),
@@ -815,7 +843,7 @@ void ref_eigensystem(int nd, int d, double g, double rho, const double* u, doubl
   double th = kt[0] * u0 + kt[1] * u1 + kt[2] * u2;
   double phi = gm1 * ((u0 * u0 + u1 * u1 + u2 * u2) / 2.0);
   double ia2 = ia * ia;
-  lam[0] = th; lam[1] = th; lam[2] = th; lam[3] = th + a; lam[4] = th - a;
+  lam[0] = U; lam[1] = U; lam[2] = U; lam[3] = U + a * kmag; lam[4] = U - a * kmag;
   double Rm[25] = {
       kt[0], kt[1], kt[2], alpha, alpha,
       kt[0] * u0, kt[1] * u0 - kt[2] * rho, kt[2] * u0 + kt[1] * rho, alpha * (u0 + kt[0] * a), alpha * (u0 - kt[0] * a),
@@ -899,17 +927,42 @@ static void reconstruction_direction(ref_state* s, int d, int filter) {
       avg[1] = (s->a[x] + s->a[x + st]) / 2.0;
     }
     double L[25], Rv[25], lam[5];
-    ref_eigensystem(nd, d, g, avg[0], ua, avg[1], L, Rv, lam);
+    if (s->c.curvilinear) {
+      /* metric terms of the direction, simple-averaged (averaging.py:49-57,104-113) */
+      double kv[3];
+      for (int m = 0; m < nd; ++m) kv[m] = (s->met[d * nd + m][x] + s->met[d * nd + m][x + st]) / 2.0;
+      ref_eigensystem_k(nd, g, avg[0], ua, avg[1], kv, L, Rv, lam);
+    } else {
+      ref_eigensystem(nd, d, g, avg[0], ua, avg[1], L, Rv, lam);
+    }
     /* window p = -2..3 */
     double CF[5][6], CS[5][6];
     double lmax[5] = {0, 0, 0, 0, 0};
     for (int pp = 0; pp < 6; ++pp) {
       ptrdiff_t y = x + (pp - 2) * st;
       double Fv[5], Qv[5];
-      double udp = s->u[d][y], pr = s->p[y];
-      Fv[0] = s->q[1 + d][y];
-      for (int m = 0; m < nd; ++m) Fv[1 + m] = s->q[1 + m][y] * udp + (m == d ? pr : 0.0);
-      Fv[nd + 1] = (pr + s->q[nd + 1][y]) * udp;
+      double udp = s->u[d][y], pr = s->p[y], kmag = 1.0;
+      if (s->c.curvilinear) {
+        /* fluxes detJ rho U_d, detJ (rhou_m U_d + p D_dm), detJ (p + rhoE) U_d
+           (apps/euler_wave_curvilinear/euler_wave.py:20-25) */
+        const double dj = s->met[nd * nd][y], Ud = s->U[d][y];
+        Fv[0] = dj * s->q[0][y] * Ud;
+        for (int m = 0; m < nd; ++m) Fv[1 + m] = dj * (s->q[1 + m][y] * Ud + pr * s->met[d * nd + m][y]);
+        Fv[nd + 1] = dj * (pr + s->q[nd + 1][y]) * Ud;
+        /* the point's eigenvalues use its own metric: U = k.u, a|k| */
+        double kk = 0.0;
+        udp = 0.0;
+        for (int m = 0; m < nd; ++m) {
+          const double km = s->met[d * nd + m][y];
+          udp += km * s->u[m][y];
+          kk += km * km;
+        }
+        kmag = sqrt(kk);
+      } else {
+        Fv[0] = s->q[1 + d][y];
+        for (int m = 0; m < nd; ++m) Fv[1 + m] = s->q[1 + m][y] * udp + (m == d ? pr : 0.0);
+        Fv[nd + 1] = (pr + s->q[nd + 1][y]) * udp;
+      }
       for (int m = 0; m < nv; ++m) Qv[m] = s->q[m][y];
       for (int kk = 0; kk < nv; ++kk) {
         double cf = 0.0, cs = 0.0;
@@ -918,7 +971,7 @@ static void reconstruction_direction(ref_state* s, int d, int filter) {
         CS[kk][pp] = cs;
       }
       /* pointwise eigenvalues with the stencil point's u_d and a (shock_capturing.py:512-536) */
-      double ap = s->a[y];
+      double ap = s->c.curvilinear ? s->a[y] * kmag : s->a[y];
       double lp[5];
       if (nd == 1) { lp[0] = udp - ap; lp[1] = udp; lp[2] = udp + ap; }
       else if (nd == 2) { lp[0] = udp; lp[1] = udp; lp[2] = udp + ap; lp[3] = udp - ap; }
@@ -957,6 +1010,15 @@ static void shock_capturing(ref_state* s, int filter) {
     ptrdiff_t x = IX(s, i, j, k);
     for (int m = 0; m < nv; ++m) {
       double r = 0.0;
+      if (s->c.curvilinear) {
+        /* -Conservative(detJ F_j, xi_j)/detJ (apps/euler_wave_curvilinear/euler_wave.py:20-25) */
+        for (int d = 0; d < nd; ++d) {
+          const double* w = s->wk[5 * d + m];
+          r += (w[x] - w[x - s->st[d]]) / s->c.delta[d];
+        }
+        s->res[m][x] = -r / s->met[nd * nd][x];
+        continue;
+      }
       for (int d = 0; d < nd; ++d) {
         const double* w = s->wk[5 * d + m];
         /* WD(F, x_d) -> D_dd WD(F, xi_d) on a stretched direction (opensblifunctions.py:607-616) */
@@ -1384,6 +1446,13 @@ int ref_create(const ref_config* cfg, ref_state** out) {
   if ((cfg->filters & REF_FILTER_BINOMIAL) &&
       (cfg->binomial_order < 2 || cfg->binomial_order > 10 || cfg->binomial_order % 2)) return -1;
   if ((cfg->filters & REF_FILTER_SFD) && !(cfg->omega_filt != 0.0)) return -1;
+  if (cfg->curvilinear) {
+    /* periodic Euler blocks with a characteristic scheme (the curvilinear app) */
+    if (cfg->ndim < 2 || cfg->viscous != REF_VISC_NONE || cfg->filters || cfg->sensor) return -1;
+    if (cfg->conv == REF_CONV_CENTRAL_SKEW || cfg->conv == REF_CONV_CENTRAL_CONS) return -1;
+    for (int d = 0; d < cfg->ndim; ++d)
+      if (cfg->stretch[d] || cfg->bc[d][0] != REF_BC_PERIODIC || cfg->bc[d][1] != REF_BC_PERIODIC) return -1;
+  }
   ref_state* s = (ref_state*)calloc(1, sizeof(ref_state));
   s->c = *cfg;
   s->nd = cfg->ndim;
@@ -1417,6 +1486,14 @@ int ref_create(const ref_config* cfg, ref_state** out) {
     if (cfg->filters & REF_FILTER_SFD) s->qf[m] = (double*)calloc(1, bytes);
     if (cfg->filters & REF_FILTER_BINOMIAL) s->tmp[m] = (double*)calloc(1, bytes);
   }
+  if (cfg->curvilinear) {
+    for (int m = 0; m <= s->nd * s->nd; ++m) s->met[m] = (double*)calloc(1, bytes);
+    for (int m = 0; m < s->nd; ++m) s->U[m] = (double*)calloc(1, bytes);
+    /* until coordinates are set: the identity map x_d = i_d Delta_d */
+    for (int a = 0; a < s->nd; ++a)
+      for (ptrdiff_t x = 0; x < s->ntot; ++x) s->met[a * s->nd + a][x] = 1.0;
+    for (ptrdiff_t x = 0; x < s->ntot; ++x) s->met[s->nd * s->nd][x] = 1.0;
+  }
   carpenter_init();
   for (int d = 0; d < s->nd; ++d)
     for (int side = 0; side < 2; ++side) {
@@ -1442,6 +1519,8 @@ void ref_destroy(ref_state* s) {
   free(s->kappa); free(s->msens); free(s->fmask);
   for (int m = 0; m < MAXV; ++m) { free(s->qf[m]); free(s->tmp[m]); }
   for (int w = 0; w < NWK; ++w) free(s->wk[w]);
+  for (int m = 0; m < 10; ++m) free(s->met[m]);
+  for (int m = 0; m < 3; ++m) free(s->U[m]);
   for (int d = 0; d < 3; ++d) {
     free(s->dm[d]); free(s->sd[d]);
     free(s->plane[d][0]); free(s->plane[d][1]);
@@ -1484,6 +1563,71 @@ int ref_set_coordinates(ref_state* s, int d, const double* xc) {
   s->sd[d] = (double*)calloc((size_t)N, sizeof(double));
   for (int c = 0; c < N; ++c) s->dm[d][c] = 1.0 / (inv * d1_gen(xc, c, 1, N, s->mod[d][0], s->mod[d][1], c));
   for (int c = 0; c < N; ++c) s->sd[d][c] = inv * d1_gen(s->dm[d], c, 1, N, s->mod[d][0], s->mod[d][1], c);
+  return 0;
+}
+
+/* Metric of a curvilinear block (metric.py:72-147): the Jacobian
+   J_ab = CD(x_a, xi_b) of the grid coordinates by the 4th-order central first
+   derivative over the interior, detJ = det J and D = adj(J)/detJ
+   (generate_fd_metrics_equations, metric.py:136-147), then the periodic
+   exchange of D and detJ into every halo layer (metric_boundary_condition,
+   metric.py:315-341; curvilinear blocks are periodic in every direction). */
+int ref_set_grid_coordinates(ref_state* s, int halo, const double* xc) {
+  if (!s->c.curvilinear) return -1;
+  if (halo < 2 || halo > H) return -2;
+  const int nd = s->nd;
+  int e[3], o[3];
+  for (int d = 0; d < 3; ++d) {
+    o[d] = d < nd ? halo : 0;
+    e[d] = s->n[d] + 2 * o[d];
+  }
+  const ptrdiff_t per = (ptrdiff_t)e[0] * e[1] * e[2];
+  double* x[3] = {NULL, NULL, NULL};
+  for (int c = 0; c < nd; ++c) {
+    x[c] = (double*)calloc((size_t)s->ntot, sizeof(double));
+    for (int kk = 0; kk < e[2]; ++kk)
+      for (int jj = 0; jj < e[1]; ++jj)
+        for (int ii = 0; ii < e[0]; ++ii)
+          x[c][IX(s, ii - o[0], jj - o[1], kk - o[2])] = xc[c * per + ii + (ptrdiff_t)e[0] * (jj + (ptrdiff_t)e[1] * kk)];
+  }
+  double inv[3];
+  for (int d = 0; d < nd; ++d) inv[d] = 1.0 / s->c.delta[d];
+  range3 I = rng_all(s, 0);
+  LOOP(I) {
+    ptrdiff_t p = IX(s, i, j, k);
+    double J[3][3] = {{0}};
+    for (int a = 0; a < nd; ++a)
+      for (int b = 0; b < nd; ++b) J[a][b] = inv[b] * d1_gen(x[a], p, s->st[b], s->n[b], 0, 0, 0);
+    double adj[3][3], det;
+    if (nd == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      adj[0][0] = J[1][1]; adj[0][1] = -J[0][1];
+      adj[1][0] = -J[1][0]; adj[1][1] = J[0][0];
+    } else {
+      /* adj(J)_ab = cofactor C_ba */
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const int r0 = (b + 1) % 3, r1 = (b + 2) % 3, c0 = (a + 1) % 3, c1 = (a + 2) % 3;
+          adj[a][b] = J[r0][c0] * J[r1][c1] - J[r0][c1] * J[r1][c0];
+        }
+      det = J[0][0] * adj[0][0] + J[0][1] * adj[1][0] + J[0][2] * adj[2][0];
+    }
+    s->met[nd * nd][p] = det;
+    for (int a = 0; a < nd; ++a)
+      for (int b = 0; b < nd; ++b) s->met[a * nd + b][p] = adj[a][b] / det;
+  }
+  for (int c = 0; c < nd; ++c) free(x[c]);
+  for (int d = 0; d < nd; ++d)
+    for (int side = 0; side < 2; ++side) periodic_exchange(s, s->met, nd * nd + 1, d, side);
+  return 0;
+}
+
+int ref_get_metric_field(const ref_state* s, int idx, double* v) {
+  if (!s->c.curvilinear || idx < 0 || idx > s->nd * s->nd) return -1;
+  ptrdiff_t c = 0;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) v[c++] = s->met[idx][IX(s, i, j, k)];
   return 0;
 }
 

@@ -61,7 +61,8 @@ typedef struct ref_config {
   double suthT, refT;    /* Sutherland constants SuthT, RefT */
   double twall;          /* Twall of the isothermal wall */
   int32_t sensor;        /* 1: Ducros sensor theta with the adaptive TENO cut-off */
-  int32_t pad2_;
+  int32_t curvilinear;   /* 1: general metric D_ij = d xi_i / d x_j and detJ from grid coordinates
+                            (MetricsEquation with curvilinear directions, metric.py:72-147) */
   double teno_a1, teno_a2, sensor_eps;  /* teno_a1, teno_a2, epsilon of the sensor */
   int32_t binomial_order;  /* even order N of the binomial filter, 2..10 */
   int32_t wf_sensor;       /* REF_WF_*: dissipation sensor of the WENO filter */
@@ -82,6 +83,15 @@ int ref_get_field(const ref_state* s, int var, double* interior);
 /* coordinates x_d (n[d] values) of a stretched direction; computes the
    metric D_dd = 1/CD(x_d, xi_d) and SD_ddd = CD(D_dd, xi_d) */
 int ref_set_coordinates(ref_state* s, int d, const double* x);
+/* grid coordinates of a curvilinear block: ndim arrays x_c (c = 0..ndim-1),
+   each over the interior extended by `halo` (2..5) points on both sides of
+   every active direction, x fastest; computes J_ab = CD(x_a, xi_b), detJ =
+   det J and D = adj(J)/detJ over the interior, then the periodic exchange of
+   D and detJ (metric.py:72-147, 315-341) */
+int ref_set_grid_coordinates(ref_state* s, int halo, const double* coords);
+/* interior values of a curvilinear metric array: idx = a*ndim + b for D_ab,
+   ndim*ndim for detJ */
+int ref_get_metric_field(const ref_state* s, int idx, double* interior);
 /* per-face plane of conservative values, [nvar][n_e2][n_e1] over the interior
    extents of the other two directions (e1 < e2): the Dirichlet values, or the
    inflow state that an inlet face keeps in its halos */
@@ -100,6 +110,11 @@ int ref_smoothness(int conv, int side, const double* f6, double* beta);
    L (nvar*nvar row-major), R (nvar*nvar), lambda (nvar) */
 void ref_eigensystem(int ndim, int d, double gama, double rho, const double* u, double a,
                      double* L, double* R, double* lam);
+/* the same about a metric direction k (ndim >= 2, euler_eigensystem.py:18-40,
+   88-135): unit normal k/|k| in L and R, lambda = (U.., U + a|k|, U - a|k|)
+   with U = k.u */
+void ref_eigensystem_k(int ndim, double gama, double rho, const double* u, double a, const double* k,
+                       double* L, double* R, double* lam);
 /* Roe average between two states; out = [rho, u0..u(ndim-1), a] */
 void ref_roe_average(int ndim, double gama, const double* qL, const double* qR, double* out);
 /* central weights on points -2..2 of the first and second derivative */

@@ -61,7 +61,7 @@ class RefConfig(ctypes.Structure):
         ("refT", ctypes.c_double),
         ("twall", ctypes.c_double),
         ("sensor", ctypes.c_int32),
-        ("pad2_", ctypes.c_int32),
+        ("curvilinear", ctypes.c_int32),
         ("teno_a1", ctypes.c_double),
         ("teno_a2", ctypes.c_double),
         ("sensor_eps", ctypes.c_double),
@@ -117,6 +117,10 @@ def lib(variant="plain"):
         L.ref_set_sfd_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         L.ref_weno_filter_flux.argtypes = [dp, dp, dp, dp]
         L.ref_weno_filter_flux.restype = ctypes.c_double
+        L.ref_eigensystem_k.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, ctypes.c_double, dp,
+                                        dp, dp, dp]
+        L.ref_set_grid_coordinates.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.ref_get_metric_field.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -155,6 +159,7 @@ def make_config(p):
     c.refT = p.get("RefT", 288.0)
     c.twall = p.get("Twall", 1.0)
     c.sensor = 1 if p.get("sensor") else 0
+    c.curvilinear = 1 if p.get("curvilinear") else 0
     c.teno_a1 = p.get("teno_a1", 10.5)
     c.teno_a2 = p.get("teno_a2", 4.5)
     c.sensor_eps = p.get("sensor_eps", 1.0e-30)
@@ -169,6 +174,16 @@ def weno_filter_flux(fp, fm, cf):
     rj = np.zeros(1)
     v = lib().ref_weno_filter_flux(_ptr(a), _ptr(b), _ptr(c), _ptr(rj))
     return v, rj[0]
+
+
+def eigensystem_k(gama, rho, u, a, k):
+    """(L, R, lambda) of the Euler eigensystem about the metric direction k."""
+    nd = len(u)
+    nv = nd + 2
+    uu, kk = (np.ascontiguousarray(v, dtype=np.float64) for v in (u, k))
+    L, R, lam = np.zeros(nv * nv), np.zeros(nv * nv), np.zeros(nv)
+    lib().ref_eigensystem_k(nd, gama, rho, _ptr(uu), a, _ptr(kk), _ptr(L), _ptr(R), _ptr(lam))
+    return L.reshape(nv, nv), R.reshape(nv, nv), lam
 
 
 def closure_weights(scheme, order):
@@ -196,6 +211,10 @@ class RefSolver:
             a = np.ascontiguousarray(x, dtype=np.float64)
             if self.L.ref_set_coordinates(self.h, int(d), _ptr(a)) != 0:
                 raise RuntimeError(f"ref_set_coordinates({d}) failed")
+        if problem.get("grid_coords") is not None:
+            a = np.ascontiguousarray(problem["grid_coords"], dtype=np.float64)
+            if self.L.ref_set_grid_coordinates(self.h, int(problem.get("grid_halo", 2)), _ptr(a)) != 0:
+                raise RuntimeError("ref_set_grid_coordinates failed")
         for (d, side), pl in problem.get("planes", {}).items():
             a = np.ascontiguousarray(pl, dtype=np.float64)
             if self.L.ref_set_bc_plane(self.h, int(d), int(side), _ptr(a)) != 0:
@@ -231,6 +250,13 @@ class RefSolver:
         if self.L.ref_get_metric(self.h, d, _ptr(dm), _ptr(sd)) != 0:
             raise RuntimeError("no metric for direction %d" % d)
         return dm, sd
+
+    def metric_field(self, idx):
+        """Interior values of D_ab (idx = a*ndim + b) or detJ (idx = ndim*ndim)."""
+        out = np.empty(self.shape)
+        if self.L.ref_get_metric_field(self.h, int(idx), _ptr(out)) != 0:
+            raise RuntimeError("no curvilinear metric array %d" % idx)
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

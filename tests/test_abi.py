@@ -89,3 +89,20 @@ def test_invalid_config_rejected():
     problem["n"] = [4, 16, 16]
     with pytest.raises(native.OsblError, match="at least 8"):
         native.Block(problem)
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(conv="central_cons"), "WENO/TENO"),
+    (dict(bc=[["periodic", "periodic"], ["extrap0", "extrap0"]]), "periodic boundaries"),
+    (dict(ndim=1, n=[16], delta=[0.125], bc=[["periodic", "periodic"]]), "2-D or 3-D"),
+])
+def test_curvilinear_config_rejected(change, msg):
+    """Curvilinear blocks are periodic Euler blocks with a WENO/TENO scheme;
+    other configurations are refused before any device call."""
+    from opensbli_amd import native
+    from opensbli_amd.problems import euler_wave_curvilinear
+    problem, _ = euler_wave_curvilinear(16)
+    problem = {k: v for k, v in problem.items() if k != "grid_coords"}
+    problem.update(change)
+    with pytest.raises(native.OsblError, match=msg):
+        native.Block(problem)

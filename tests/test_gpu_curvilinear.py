@@ -1,0 +1,188 @@
+"""Curvilinear blocks on the GPU (through the C-ABI) against the oracle: the
+metric from grid coordinates (metric.py:72-147), the residual of the strong
+conservation form with the characteristic system about the metric normal
+(apps/euler_wave_curvilinear/euler_wave.py), and the time loop.
+
+Bar as in the other parity tests: relative L2 per conservative variable below
+max(1e-10, 4 x the spread between the oracle's plain and FMA builds).  The
+rotated-mesh case is also checked against the GPU's own Cartesian path (the
+2-D characteristic fields are frame-invariant, so the two agree to rounding).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(problem, q0, oracle):
+    from opensbli_amd.native import Block
+    g = Block(problem)
+    g.set_state(q0)
+    r = oracle.RefSolver(problem)
+    r.set_state(q0)
+    return g, r
+
+
+def _spread(problem, q0, oracle, ref, steps=None):
+    f = oracle.RefSolver(problem, variant="fma")
+    f.set_state(q0)
+    if steps is None:
+        return rel_l2(f.rhs(), ref)
+    f.run(steps)
+    return rel_l2(f.get_state(), ref)
+
+
+def _mesh3d(n, h=2):
+    """A smooth periodic-metric 3-D mesh: an affine map plus waves along each direction."""
+    d = 2.0 / n
+    i = np.arange(-h, n + h) * d
+    kk, jj, ii = np.meshgrid(i, i, i, indexing="ij")
+    A = np.array([[1.0, 0.15, -0.1], [0.1, 0.9, 0.2], [-0.05, 0.1, 1.1]])
+    xi = [ii, jj, kk]
+    x = [sum(A[a, b] * xi[b] for b in range(3)) for a in range(3)]
+    x[0] = x[0] + 0.03 * np.sin(np.pi * jj) * np.cos(np.pi * kk)
+    x[1] = x[1] + 0.03 * np.sin(np.pi * kk + 0.3) * np.sin(np.pi * ii)
+    x[2] = x[2] + 0.02 * np.cos(np.pi * ii) * np.sin(np.pi * jj)
+    return np.stack(x), d
+
+
+def _problem3d(n, conv="weno_z", averaging="roe"):
+    coords, d = _mesh3d(n)
+    problem = dict(ndim=3, n=[n] * 3, delta=[d] * 3, conv=conv, averaging=averaging, viscous="none", rk="ls3",
+                   bc=[["periodic", "periodic"]] * 3, gama=1.4, dt=2e-3, curvilinear=1, grid_coords=coords,
+                   grid_halo=2)
+    c = coords[:, 2:2 + n, 2:2 + n, 2:2 + n]
+    rho = 1.0 + 0.2 * np.sin(np.pi * (c[0] + c[1] + c[2]))
+    u = [0.6, -0.3, 0.25]
+    p = 1.0 + 0.1 * np.cos(np.pi * (c[0] - c[2]))
+    q = np.stack([rho] + [rho * ui for ui in u] + [p / 0.4 + 0.5 * rho * sum(ui * ui for ui in u)])
+    return problem, q
+
+
+def test_metric_matches_oracle():
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import euler_wave_curvilinear
+    from oracle import ref
+    _say("case curvilinear metric: wavy 2-D 48^2 and skewed wavy 3-D 16^3")
+    for problem, q in (euler_wave_curvilinear(48), _problem3d(16)):
+        nd = problem["ndim"]
+        g = Block(problem)
+        r = ref.RefSolver(problem)
+        for idx in range(nd * nd + 1):
+            a, b = g.metric_field(idx), r.metric_field(idx)
+            assert np.allclose(a, b, rtol=1e-14, atol=1e-14), (nd, idx, np.abs(a - b).max())
+        g.close()
+
+
+@pytest.mark.parametrize("conv,avg", [("weno_z", "roe"), ("weno_js", "simple"), ("teno6", "roe"), ("teno5", "simple")])
+def test_rhs_wavy_2d(conv, avg, oracle):
+    from opensbli_amd.problems import euler_wave_curvilinear
+    _say(f"case curvilinear rhs 2-D wavy 64^2 conv={conv} avg={avg}")
+    problem, q0 = euler_wave_curvilinear(64, conv=conv, averaging=avg)
+    g, r = _pair(problem, q0, oracle)
+    rr = r.rhs()
+    err = rel_l2(g.rhs(), rr)
+    spread = _spread(problem, q0, oracle, rr)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+
+
+def test_euler_wave_curvilinear_100_steps(oracle):
+    """The reference app's configuration (WENO5-Z, Roe, RK3 low storage, dt =
+    5e-4, wavy mesh) at 64^2 for 100 steps."""
+    from opensbli_amd.problems import euler_wave_curvilinear
+    _say("case curvilinear euler wave 64^2, 100 steps")
+    problem, q0 = euler_wave_curvilinear(64)
+    g, r = _pair(problem, q0, oracle)
+    g.run(100)
+    r.run(100)
+    rs = r.get_state()
+    err = rel_l2(g.get_state(), rs)
+    spread = _spread(problem, q0, oracle, rs, steps=100)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+
+
+def test_rotated_mesh_equals_the_cartesian_path(oracle):
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import euler_wave_curvilinear
+    _say("case curvilinear rotated mesh 32^2 vs Cartesian, rhs and 20 steps")
+    n, h, theta = 32, 2, 0.7
+    d = 2.0 / n
+    c, s = math.cos(theta), math.sin(theta)
+    Rot = np.array([[c, -s], [s, c]])
+    i = np.arange(-h, n + h) * d
+    jj, ii = np.meshgrid(i, i, indexing="ij")
+    coords = np.stack([Rot[0, 0] * ii + Rot[0, 1] * jj, Rot[1, 0] * ii + Rot[1, 1] * jj])
+    problem, _ = euler_wave_curvilinear(n, coords=coords, dt=2e-3)
+    cart = {k: v for k, v in problem.items() if k not in ("curvilinear", "grid_coords", "grid_halo")}
+    x = np.arange(n) * d
+    y, xx = np.meshgrid(x, x, indexing="ij")
+    rho = 1.0 + 0.2 * np.sin(np.pi * xx) * np.cos(np.pi * y)
+    ug = np.stack([0.4 + 0.3 * np.cos(np.pi * y), -0.2 + 0.25 * np.sin(np.pi * (xx + y))])
+    E = (1.0 + 0.1 * np.cos(np.pi * (xx - y))) / 0.4 + 0.5 * rho * (ug ** 2).sum(0)
+    qg = np.stack([rho, rho * ug[0], rho * ug[1], E])
+    up = np.einsum("ab,bij->aij", Rot, ug)
+    qp = np.stack([rho, rho * up[0], rho * up[1], E])
+
+    def rotate(q):
+        m = np.einsum("ab,bij->aij", Rot, q[1:3])
+        return np.stack([q[0], m[0], m[1], q[3]])
+
+    gc, gp = Block(cart), Block(problem)
+    gc.set_state(qg)
+    gp.set_state(qp)
+    err = rel_l2(gp.rhs(), rotate(gc.rhs()))
+    _say("rhs rel L2", err)
+    assert np.all(err < 1e-12), err
+    gc.run(20)
+    gp.run(20)
+    err = rel_l2(gp.get_state(), rotate(gc.get_state()))
+    _say("20 steps rel L2", err)
+    assert np.all(err < 1e-12), err
+    gc.close()
+    gp.close()
+
+
+@pytest.mark.parametrize("conv", ["weno_z", "teno6"])
+def test_skewed_wavy_3d(conv, oracle):
+    _say(f"case curvilinear 3-D 16^3 conv={conv}: rhs and 20 steps")
+    problem, q0 = _problem3d(16, conv=conv)
+    g, r = _pair(problem, q0, oracle)
+    rr = r.rhs()
+    err = rel_l2(g.rhs(), rr)
+    spread = _spread(problem, q0, oracle, rr)
+    _say("rhs rel L2", err, "spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+    g.run(20)
+    r.run(20)
+    rs = r.get_state()
+    err = rel_l2(g.get_state(), rs)
+    spread = _spread(problem, q0, oracle, rs, steps=20)
+    _say("20 steps rel L2", err, "spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+
+
+def test_run_needs_coordinates():
+    from opensbli_amd.native import Block, OsblError
+    from opensbli_amd.problems import euler_wave_curvilinear
+    _say("case curvilinear block without coordinates refuses to run")
+    problem, q0 = euler_wave_curvilinear(16)
+    problem = {k: v for k, v in problem.items() if k != "grid_coords"}
+    g = Block(problem)
+    g.set_state(q0)
+    with pytest.raises(OsblError, match="osbl_set_grid_coordinates"):
+        g.run(1)
+    g.close()
