@@ -20,6 +20,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -79,7 +81,8 @@ def main():
     ap.add_argument("--conv", default="weno_z", choices=["weno_z", "weno_js", "teno6", "teno5", "central_skew"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble", "flat_plate", "adjoint"])
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble", "flat_plate", "adjoint",
+                                                           "curvilinear"])
     ap.add_argument("--no-adjoint", action="store_true", help="tgv: skip the adjoint/forward ratio measurement")
     ap.add_argument("--plain-stride", type=int, default=16,
                     help="ensemble: time the per-instance baseline on every k-th instance and scale by work")
@@ -93,6 +96,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "flat_plate":
         return bench_flat_plate(args)
+    if args.workload == "curvilinear":
+        return bench_curvilinear(args)
     if args.workload == "adjoint":
         out = adjoint_ratio()
         out["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
@@ -273,6 +278,43 @@ def bench_flat_plate(args):
         "config": {"workload": "flat_plate_480x256x128_weno_z_sutherland_metric_ls3", "grid": [480, 256, 128]},
         "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items() if v[1]},
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_curvilinear(args):
+    """Curvilinear blocks (general metric, strong conservation form, WENO5-Z +
+    Roe, RungeKuttaLS(3)): the Euler wave on the wavy mesh at 1024^2 and on the
+    skewed wavy 3-D mesh at 128^3; Mcell-updates/s per RK stage and per-kernel
+    times."""
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import euler_wave_curvilinear, euler_wave_curvilinear_3d
+    out = {"metric": "Mcell-updates/s per RK stage (FP64, curvilinear Euler wave)", "unit": "Mcell-updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (analytic density wave on analytic meshes)", "cases": {}}
+    for name, (problem, q0) in (("wavy_2d_1024^2", euler_wave_curvilinear(1024)),
+                                ("skewed_wavy_3d_128^3", euler_wave_curvilinear_3d(128))):
+        _log(f"bench: curvilinear {name}")
+        blk = Block(problem)
+        blk.set_state(q0)
+        stages = blk.num_stages()
+        blk.run(args.warmup)
+        blk.sync()
+        blk.profile(True)
+        blk.sync()
+        t0 = time.perf_counter()
+        blk.run(args.steps)
+        blk.sync()
+        t1 = time.perf_counter()
+        stats = blk.kernel_stats()
+        blk.profile(False)
+        blk.close()
+        cells = int(np.prod(problem["n"]))
+        el = t1 - t0
+        out["cases"][name] = {
+            "value": cells * stages * args.steps / el / 1e6,
+            "ms_per_step": el * 1e3 / args.steps,
+            "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items() if v[1]},
+        }
     print(json.dumps(out), flush=True)
 
 

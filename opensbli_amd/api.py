@@ -468,12 +468,17 @@ class MetricsEquation:
     """Coordinate transformation of the derivatives (metric.py:20).  The HIPC
     target implements directions stretched along themselves, parameters
     (stretched, False): the metric is D_dd = 1/CD(x_d, xi_d) with
-    SD_ddd = CD(D_dd, xi_d), computed from the block's coordinates."""
+    SD_ddd = CD(D_dd, xi_d), computed from the block's coordinates; and the
+    general curvilinear metric, every direction curvilinear: D = adj(J)/detJ
+    with J_ab = CD(x_a, xi_b) (metric.py:72-147), used by equations written in
+    strong conservation form in xi (apps/euler_wave_curvilinear/euler_wave.py)
+    on periodic blocks."""
 
     def __init__(self, **kwargs):
         self.stretching_metric = None
         self.curvilinear_metric = None
         self.ndim = None
+        self.metric_subs = {}
 
     def generate_transformations(self, ndim, coordinate_symbol, parameters, max_order):
         if len(flatten(list(parameters))) != ndim * 2:
@@ -481,8 +486,8 @@ class MetricsEquation:
         self.ndim = ndim
         self.stretching_metric = [bool(p[0]) for p in parameters]
         self.curvilinear_metric = [bool(p[1]) for p in parameters]
-        if any(self.curvilinear_metric):
-            raise NotImplementedError("curvilinear (non-diagonal) metrics are not implemented by the HIPC target")
+        if any(self.curvilinear_metric) and not all(self.curvilinear_metric):
+            raise NotImplementedError("the HIPC target implements curvilinear metrics in every direction or in none")
         self.coordinate_symbol = coordinate_symbol
         return
 
@@ -772,6 +777,16 @@ def _cons(tag):
     }
 
 
+# strong conservation form in curvilinear coordinates xi
+# (apps/euler_wave_curvilinear/euler_wave.py:17-25)
+def _curv(tag):
+    return {
+        "rho": {(-1, "Conservative(detJ*rho*U_j,xi_j%s)/detJ" % tag)},
+        "rhou_i": {(-1, "Conservative(detJ*(rhou_i*U_j+p*D_j_i),xi_j%s)/detJ" % tag)},
+        "rhoE": {(-1, "Conservative(detJ*(p+rhoE)*U_j,xi_j%s)/detJ" % tag)},
+    }
+
+
 VISCOUS = {"rhou_i": {(1, "Der(tau_i_j,x_j)")}, "rhoE": {(1, "Der(u_i*tau_i_j,x_j)")}}
 STRESS = "Eq(tau_i_j,(1.0/Re)*(Der(u_i,x_j)+Der(u_j,x_i)-(2/3)*KD(_i,_j)*Der(u_k,x_k)))"
 HEAT_POS = "Eq(q_j,(1.0/((gama-1)*Minf*Minf*Pr*Re))*Der(T,x_j))"
@@ -786,6 +801,7 @@ CONSTITUENT = {
     "p": "(gama-1)*(rhoE-rho*(1/2)*(KD(_i,_j)*u_i*u_j))",
     "a": "(gama*p/rho)**0.5",
     "T": "p*gama*Minf*Minf/(rho)",
+    "U_i": "D_i_j*u_j",
 }
 
 
@@ -826,7 +842,8 @@ def _recognise(block):
                 heat = sign
     conv = None
     averaging = "roe"
-    for name, want in [("skew", SKEW), ("weno", _cons("@Weno")), ("teno", _cons("@Teno")), ("central", _cons(""))]:
+    for name, want in [("skew", SKEW), ("weno", _cons("@Weno")), ("teno", _cons("@Teno")), ("central", _cons("")),
+                       ("weno_curv", _curv("@Weno")), ("teno_curv", _curv("@Teno"))]:
         ok = True
         for var in ("rho", "rhou_i", "rhoE"):
             got = set(terms[var])
@@ -841,6 +858,19 @@ def _recognise(block):
             break
     if conv is None:
         raise NotImplementedError("equation system not recognised by the HIPC target: %s" % terms)
+    curvilinear = conv.endswith("_curv")
+    if curvilinear:
+        conv = conv[:-5]
+        if has_visc:
+            raise NotImplementedError("viscous terms on curvilinear blocks are not implemented by the HIPC target")
+        metrics = [e for e in block.equation_classes if isinstance(e, MetricsEquation)]
+        if not metrics or not metrics[0].curvilinear_metric or not all(metrics[0].curvilinear_metric):
+            raise ValueError("equations in curvilinear coordinates need a MetricsEquation with curvilinear "
+                             "directions on the block")
+        if metrics[0].ndim != ndim:
+            raise ValueError("the metric was generated for %d dimensions on a %d-D block" % (metrics[0].ndim, ndim))
+        if not [s for s in spatial if isinstance(s, Central)]:
+            raise ValueError("A Central scheme should be instantiated for metrics.")
     if conv in ("skew", "central"):
         cen = [s for s in spatial if isinstance(s, Central)]
         if not cen:
@@ -877,7 +907,8 @@ def _recognise(block):
             lhs, rhs = _parse_eq(e.text)
             have[lhs] = rhs
     need = ["u_i", "p"] + (["a"] if conv not in ("central_skew", "central_cons") else []) + \
-        (["T"] if viscous != "none" else []) + (["mu"] if viscous == "sutherland" else [])
+        (["T"] if viscous != "none" else []) + (["mu"] if viscous == "sutherland" else []) + \
+        (["U_i"] if curvilinear else [])
     for k in need:
         if k not in have:
             raise ValueError("Constituent relation is not found for %s" % k)
@@ -900,13 +931,21 @@ def _recognise(block):
             raise ValueError("A Central scheme should be instantiated for metrics.")
         for d in range(ndim):
             stretch[d] = int(metric.stretching_metric[d])
-    return dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch, sensor=sensor)
+    out = dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch, sensor=sensor)
+    if curvilinear:
+        out["curvilinear"] = 1
+    return out
 
 
 # ---------------------------------------------------------------------------
 # the target
 # ---------------------------------------------------------------------------
 _CURRENT = {}
+
+
+def _metric_names(nd):
+    """Datasets of the curvilinear metric, in the order of osbl_get_metric_field."""
+    return ["D%d%d" % (a, b) for a in range(nd) for b in range(nd)] + ["detJ"]
 
 
 def _c_eval(expr, env):
@@ -1053,6 +1092,8 @@ class HIPC:
         arrays, the DataObjects of the initial condition and the arrays read."""
         names = {"rho", "rhoE"} | {"rhou%d" % d for d in range(nd)}
         names |= {k for k in self._fields[1] if _KINDS.get(k) == "data"}
+        if self.scheme.get("curvilinear"):
+            names |= set(_metric_names(nd))
         return names | set(self._inputs)
 
     def _init_equations(self):
@@ -1074,9 +1115,16 @@ class HIPC:
         bn, bname = self.block.blocknumber, self.block.blockname
         hdf5.fetch_block_hdf5_file(fname, bname, bn, nd)
         widened = None
+        metric = _metric_names(nd) if self.scheme.get("curvilinear") else []
         for nm in _array_names(io.arrays, nd):
             if nm in cons:
                 a, dm, dp = blk.get_field_halo(cons.index(nm))
+            elif nm in metric:
+                # the metric arrays hold the periodic images in their halos
+                # (metric_boundary_condition, metric.py:315-341)
+                dm, dp = blk.halo_depths()
+                a = np.pad(blk.metric_field(metric.index(nm)),
+                           [(-dm[d], dp[d]) for d in reversed(range(nd))], mode="wrap")
             elif nm in self._fields[1]:
                 dm, dp = blk.halo_depths()
                 if widened is None:
@@ -1163,6 +1211,20 @@ class HIPC:
                 coords[d] = np.ascontiguousarray(line)
         if coords:
             problem["coords"] = coords
+        if problem.get("curvilinear"):
+            # the coordinates over the interior and the 2 points beyond it the
+            # metric's central derivatives read (the initial condition covers
+            # the halos, gridbasedinit.py:46-57)
+            h = 2
+            env2, shape2 = self._env(problem, [-h] * nd, [h] * nd)
+            wide = {}
+            self._evaluate(self._init_equations(), env2, shape2, wide)
+            missing = ["x%d" % d for d in range(nd) if "x%d" % d not in wide]
+            if missing:
+                raise ValueError("a curvilinear block needs its coordinate DataObjects %s in the initialisation"
+                                 % missing)
+            problem["grid_coords"] = np.ascontiguousarray(np.stack([wide["x%d" % d] for d in range(nd)]))
+            problem["grid_halo"] = h
         planes = {}
         conserved = {"rho", "rhoE"} | {"rhou%d" % d for d in range(nd)}
         for d in range(nd):

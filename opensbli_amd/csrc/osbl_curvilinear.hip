@@ -14,8 +14,10 @@
 // The characteristic fluxes are L (detJ F_d) and the characteristic state L Q.
 //
 // Layout: the padded fields of the block (Grid); the metric holds nd*nd + 1
-// padded fields, D_ab at a*nd + b and detJ last.  One thread per interface;
-// the interface fluxes of a direction go to a scratch bundle, then a
+// padded fields, D_ab at a*nd + b and detJ last.  Per direction: a point
+// kernel forms detJ F_d, U and a|k| once per point; one thread per interface
+// then projects the six stencil points onto the characteristic fields and
+// reconstructs, writing the interface fluxes to a scratch bundle; a
 // difference kernel accumulates -(F[i] - F[i-1])/Delta_d and the last
 // direction divides by detJ.
 #include "osbl_kernels.h"
@@ -113,6 +115,18 @@ __device__ __forceinline__ void curv_rcol(const Avg<ND>& v, const double kt[ND],
   }
 }
 
+// thread t of a 1-D launch over the box e[0] x e[1] x e[2] (x fastest);
+// false past its end
+__device__ __forceinline__ bool box_index(const int e[3], int c[3]) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)e[0] * e[1] * e[2]) return false;
+  c[0] = (int)(t % e[0]);
+  const long long r = t / e[0];
+  c[1] = (int)(r % e[1]);
+  c[2] = (int)(r / e[1]);
+  return true;
+}
+
 // metric of point y along direction d, the contravariant velocity U = k.u and |k|
 template <int ND>
 __device__ __forceinline__ void point_metric(const double* __restrict__ M, long long fs, int d, int y,
@@ -128,13 +142,41 @@ __device__ __forceinline__ void point_metric(const double* __restrict__ M, long 
   km = sqrt(kk);
 }
 
+// per point along direction d (d index in [-3, n_d + 3), the others
+// interior): the fluxes detJ F_d (nv fields), U = k.u and a|k| with the
+// point's own metric k = (D_d0, ..)
+template <int ND>
+__global__ void __launch_bounds__(256) k_curv_point(const double* __restrict__ Q, const double* __restrict__ M,
+                                                    double* __restrict__ PF, Grid g, Phys ph, int d) {
+  constexpr int NV = ND + 2;
+  const int e[3] = {g.n[0] + (d == 0 ? 6 : 0), g.n[1] + (d == 1 ? 6 : 0), g.n[2] + (d == 2 ? 6 : 0)};
+  int c[3];
+  if (!box_index(e, c)) return;
+  c[d] -= 3;
+  const int y = g.idx(c[0], c[1], c[2]);
+  const long long fs = g.fs;
+  Pt<ND> s;
+  load_pt<ND>(s, Q, fs, y, ph);
+  double k[ND], U, km;
+  point_metric<ND>(M, fs, d, y, s, k, U, km);
+  const double dj = M[(ND * ND) * fs + y];
+  PF[y] = dj * s.q[0] * U;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) PF[(1 + m) * fs + y] = dj * (s.q[1 + m] * U + s.p * k[m]);
+  PF[(NV - 1) * fs + y] = dj * (s.p + s.q[NV - 1]) * U;
+  PF[NV * fs + y] = U;
+  PF[(NV + 1) * fs + y] = s.a * km;
+}
+
 // one interface i+1/2 along d per thread, i in [-1, n_d); the other indices interior
 template <int ND, int CONV, int AVG>
 __global__ void __launch_bounds__(128) k_curv_flux(const double* __restrict__ Q, const double* __restrict__ M,
-                                                   double* __restrict__ F, Grid g, Phys ph, int d) {
+                                                   const double* __restrict__ PF, double* __restrict__ F, Grid g,
+                                                   Phys ph, int d) {
   constexpr int NV = ND + 2;
-  int c[3] = {(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)blockIdx.y, (int)blockIdx.z};
-  if (c[0] >= g.n[0] + (d == 0 ? 1 : 0)) return;
+  const int e[3] = {g.n[0] + (d == 0 ? 1 : 0), g.n[1] + (d == 1 ? 1 : 0), g.n[2] + (d == 2 ? 1 : 0)};
+  int c[3];
+  if (!box_index(e, c)) return;
   c[d] -= 1;
   const int x = g.idx(c[0], c[1], c[2]);
   const int st = g.stride(d);
@@ -158,54 +200,52 @@ __global__ void __launch_bounds__(128) k_curv_flux(const double* __restrict__ Q,
     kt[m] *= ikm;
     th += kt[m] * v.u[m];
   }
-  // local wave speeds: maximum over the six stencil points
-  double lmax[NV];
+  double L[NV][NV];
+#pragma unroll
+  for (int f = 0; f < NV; ++f) curv_lrow<ND>(v, kt, th, ph, f, L[f]);
+  // characteristic fluxes and states on the six stencil points, and the
+  // local wave speeds (maximum over the points)
+  double cf[NV][6], cs[NV][6], lmax[NV];
 #pragma unroll
   for (int f = 0; f < NV; ++f) lmax[f] = 0.0;
 #pragma unroll
   for (int p = 0; p < 6; ++p) {
-    Pt<ND> s;
     const int y = x + (p - 2) * st;
-    load_pt<ND>(s, Q, fs, y, ph);
-    double k[ND], U, km;
-    point_metric<ND>(M, fs, d, y, s, k, U, km);
-    const double ak = s.a * km;
+    double Fv[NV], qv[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      Fv[m] = PF[m * fs + y];
+      qv[m] = Q[m * fs + y];
+    }
+    const double U = PF[NV * fs + y], ak = PF[(NV + 1) * fs + y];
 #pragma unroll
     for (int f = 0; f < ND; ++f) lmax[f] = fmax(lmax[f], fabs(U));
     lmax[ND] = fmax(lmax[ND], fabs(U + ak));
     lmax[ND + 1] = fmax(lmax[ND + 1], fabs(U - ak));
+#pragma unroll
+    for (int f = 0; f < NV; ++f) {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        a += L[f][m] * Fv[m];
+        b += L[f][m] * qv[m];
+      }
+      cf[f][p] = a;
+      cs[f][p] = b;
+    }
   }
-  // per characteristic field: split fluxes on the six points, reconstruction,
-  // and its contribution R[:, f] rec_f to the interface flux
+  // per characteristic field: split fluxes, reconstruction, and its
+  // contribution R[:, f] rec_f to the interface flux
   double out[NV];
 #pragma unroll
   for (int m = 0; m < NV; ++m) out[m] = 0.0;
 #pragma unroll
   for (int f = 0; f < NV; ++f) {
-    double l[NV];
-    curv_lrow<ND>(v, kt, th, ph, f, l);
     double fp[6], fm[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-      Pt<ND> s;
-      const int y = x + (p - 2) * st;
-      load_pt<ND>(s, Q, fs, y, ph);
-      double k[ND], U, km;
-      point_metric<ND>(M, fs, d, y, s, k, U, km);
-      const double dj = M[(ND * ND) * fs + y];
-      double Fv[NV];
-      Fv[0] = dj * s.q[0] * U;
-#pragma unroll
-      for (int m = 0; m < ND; ++m) Fv[1 + m] = dj * (s.q[1 + m] * U + s.p * k[m]);
-      Fv[NV - 1] = dj * (s.p + s.q[NV - 1]) * U;
-      double cf = 0.0, cs = 0.0;
-#pragma unroll
-      for (int m = 0; m < NV; ++m) {
-        cf += l[m] * Fv[m];
-        cs += l[m] * s.q[m];
-      }
-      fp[p] = 0.5 * (cf + lmax[f] * cs);
-      fm[p] = 0.5 * (cf - lmax[f] * cs);
+      fp[p] = 0.5 * (cf[f][p] + lmax[f] * cs[f][p]);
+      fm[p] = 0.5 * (cf[f][p] - lmax[f] * cs[f][p]);
     }
     const double rec = recon_pair<CONV>(fp, fm, ph.eps, ph.teno_ct);
     double r[NV];
@@ -223,9 +263,9 @@ template <int ND>
 __global__ void __launch_bounds__(256) k_curv_diff(const double* __restrict__ F, const double* __restrict__ M,
                                                    double* __restrict__ R, Grid g, double delta, int d, int first,
                                                    int last) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n[0]) return;
-  const int x = g.idx(i, blockIdx.y, blockIdx.z);
+  int c[3];
+  if (!box_index(g.n, c)) return;
+  const int x = g.idx(c[0], c[1], c[2]);
   const int st = g.stride(d);
   const long long fs = g.fs;
   const double dj = last ? M[(ND * ND) * fs + x] : 1.0;
@@ -287,35 +327,39 @@ __global__ void __launch_bounds__(256) k_curv_metric(const double* __restrict__ 
 }
 
 template <int ND, int CONV>
-void launch_flux_c(int avg, const double* Q, const double* M, double* F, const Grid& g, const Phys& ph, int d,
-                   hipStream_t s) {
-  dim3 grid((g.n[0] + (d == 0 ? 1 : 0) + 127) / 128, g.n[1] + (d == 1 ? 1 : 0), g.n[2] + (d == 2 ? 1 : 0));
-  if (avg == OSBL_AVG_ROE) k_curv_flux<ND, CONV, OSBL_AVG_ROE><<<grid, 128, 0, s>>>(Q, M, F, g, ph, d);
-  else k_curv_flux<ND, CONV, OSBL_AVG_SIMPLE><<<grid, 128, 0, s>>>(Q, M, F, g, ph, d);
+void launch_flux_c(int avg, const double* Q, const double* M, const double* PF, double* F, const Grid& g,
+                   const Phys& ph, int d, hipStream_t s) {
+  const long long total = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
+  const unsigned grid = (unsigned)((total + 127) / 128);
+  if (avg == OSBL_AVG_ROE) k_curv_flux<ND, CONV, OSBL_AVG_ROE><<<grid, 128, 0, s>>>(Q, M, PF, F, g, ph, d);
+  else k_curv_flux<ND, CONV, OSBL_AVG_SIMPLE><<<grid, 128, 0, s>>>(Q, M, PF, F, g, ph, d);
 }
 
 template <int ND>
-void launch_flux_nd(int conv, int avg, const double* Q, const double* M, double* F, const Grid& g, const Phys& ph,
-                    int d, hipStream_t s) {
+void launch_flux_nd(int conv, int avg, const double* Q, const double* M, double* PF, double* F, const Grid& g,
+                    const Phys& ph, int d, hipStream_t s) {
+  const long long pts = (long long)(g.n[0] + (d == 0 ? 6 : 0)) * (g.n[1] + (d == 1 ? 6 : 0)) *
+                        (g.n[2] + (d == 2 ? 6 : 0));
+  k_curv_point<ND><<<(unsigned)((pts + 255) / 256), 256, 0, s>>>(Q, M, PF, g, ph, d);
   switch (conv) {
-    case OSBL_CONV_WENO_JS: launch_flux_c<ND, OSBL_CONV_WENO_JS>(avg, Q, M, F, g, ph, d, s); break;
-    case OSBL_CONV_WENO_Z: launch_flux_c<ND, OSBL_CONV_WENO_Z>(avg, Q, M, F, g, ph, d, s); break;
-    case OSBL_CONV_TENO5: launch_flux_c<ND, OSBL_CONV_TENO5>(avg, Q, M, F, g, ph, d, s); break;
-    default: launch_flux_c<ND, OSBL_CONV_TENO6>(avg, Q, M, F, g, ph, d, s); break;
+    case OSBL_CONV_WENO_JS: launch_flux_c<ND, OSBL_CONV_WENO_JS>(avg, Q, M, PF, F, g, ph, d, s); break;
+    case OSBL_CONV_WENO_Z: launch_flux_c<ND, OSBL_CONV_WENO_Z>(avg, Q, M, PF, F, g, ph, d, s); break;
+    case OSBL_CONV_TENO5: launch_flux_c<ND, OSBL_CONV_TENO5>(avg, Q, M, PF, F, g, ph, d, s); break;
+    default: launch_flux_c<ND, OSBL_CONV_TENO6>(avg, Q, M, PF, F, g, ph, d, s); break;
   }
 }
 
 }  // namespace
 
 // -(1/detJ) sum_d (F_d[i] - F_d[i-1])/Delta_d of direction d into R (assigned
-// at d = 0, divided by detJ at the last direction); F is a scratch bundle of
-// nd + 2 padded fields.  Q needs filled halos (4 points), M the metric with
+// at d = 0, divided by detJ at the last direction); PF (nd + 4 padded fields)
+// and F (nd + 2) are scratch bundles.  Q needs filled halos (4 points), M the metric with
 // its periodic halos.
-void launch_curv_direction(int nd, int d, int conv, int avg, const double* Q, const double* M, double* F, double* R,
-                           const Grid& g, const Phys& ph, double delta, hipStream_t s) {
-  if (nd == 3) launch_flux_nd<3>(conv, avg, Q, M, F, g, ph, d, s);
-  else launch_flux_nd<2>(conv, avg, Q, M, F, g, ph, d, s);
-  dim3 grid((g.n[0] + 255) / 256, g.n[1], g.n[2]);
+void launch_curv_direction(int nd, int d, int conv, int avg, const double* Q, const double* M, double* PF, double* F,
+                           double* R, const Grid& g, const Phys& ph, double delta, hipStream_t s) {
+  if (nd == 3) launch_flux_nd<3>(conv, avg, Q, M, PF, F, g, ph, d, s);
+  else launch_flux_nd<2>(conv, avg, Q, M, PF, F, g, ph, d, s);
+  const unsigned grid = (unsigned)(((long long)g.n[0] * g.n[1] * g.n[2] + 255) / 256);
   const int first = d == 0, last = d == nd - 1;
   if (nd == 3) k_curv_diff<3><<<grid, 256, 0, s>>>(F, M, R, g, delta, d, first, last);
   else k_curv_diff<2><<<grid, 256, 0, s>>>(F, M, R, g, delta, d, first, last);

@@ -51,8 +51,8 @@ void launch_binomial(int nd, const double* Q, double* Qn, const Grid& g, const d
                      const unsigned char* mask, hipStream_t s);
 void launch_lincomb(double* out, double a, const double* x, double b, const double* y, long long n, hipStream_t s);
 
-void launch_curv_direction(int nd, int d, int conv, int avg, const double* Q, const double* M, double* F, double* R,
-                           const Grid& g, const Phys& ph, double delta, hipStream_t s);
+void launch_curv_direction(int nd, int d, int conv, int avg, const double* Q, const double* M, double* PF, double* F,
+                           double* R, const Grid& g, const Phys& ph, double delta, hipStream_t s);
 void launch_curv_metric(int nd, const double* X, double* M, const Grid& g, const Phys& ph, const int e[3],
                         hipStream_t s);
 

@@ -205,6 +205,7 @@ struct osbl_ctx {
   bool curv = false;                        // general curvilinear metric
   double* cmet = nullptr;                   // its D_ab (a*nd + b) and detJ, padded fields
   double* cflux = nullptr;                  // interface fluxes of one direction (nv padded fields)
+  double* cpt = nullptr;                    // per-point detJ F_d, U, a|k| of one direction (nv + 2 fields)
   bool cmet_ready = false;                  // osbl_set_grid_coordinates has run
   int nplane[3][2] = {};
   // profiling
@@ -336,7 +337,7 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
     // metric normal and their difference, divided by detJ after the last one
     for (int d = 0; d < c->nd; ++d) {
       timed(c, F_SWEEP_X + d, [&] {
-        launch_curv_direction(c->nd, d, conv, c->cfg.averaging, Q, c->cmet, c->cflux, c->R, c->g, c->ph,
+        launch_curv_direction(c->nd, d, conv, c->cfg.averaging, Q, c->cmet, c->cpt, c->cflux, c->R, c->g, c->ph,
                               c->cfg.delta[d], c->stream);
       });
     }
@@ -599,7 +600,8 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   }
   c->curv = cfg->curvilinear != 0;
   if (c->curv && ((rc = alloc(&c->cmet, (size_t)fs * (c->nd * c->nd + 1) * sizeof(double))) ||
-                  (rc = alloc(&c->cflux, bundle)))) {
+                  (rc = alloc(&c->cflux, bundle)) ||
+                  (rc = alloc(&c->cpt, (size_t)fs * (c->nv + 2) * sizeof(double))))) {
     osbl_destroy(c);
     return rc;
   }
@@ -671,6 +673,7 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->vw);
   (void)hipFree(c->cmet);
   (void)hipFree(c->cflux);
+  (void)hipFree(c->cpt);
   (void)hipFree(c->cw_dev);
   (void)hipFree(c->Theta);
   (void)hipFree(c->QF);

@@ -249,3 +249,35 @@ def euler_wave_curvilinear(n, conv="weno_z", averaging="roe", rk="ls3", gama=1.4
     u0, u1, p = 1.0, -0.5, 1.0
     q = np.stack([rho, rho * u0, rho * u1, p / (gama - 1) + 0.5 * rho * (u0 * u0 + u1 * u1)])
     return problem, q
+
+
+def skewed_wavy_grid3d(n, halo=2):
+    """A 3-D curvilinear mesh of the cube [0, 2)^3 in index space: a skewing
+    affine map plus waves across each direction (periodic metric).  Returns
+    (coords of shape (3, n + 2 halo, n + 2 halo, n + 2 halo), Delta)."""
+    d = 2.0 / n
+    i = np.arange(-halo, n + halo) * d
+    kk, jj, ii = np.meshgrid(i, i, i, indexing="ij")
+    A = np.array([[1.0, 0.15, -0.1], [0.1, 0.9, 0.2], [-0.05, 0.1, 1.1]])
+    xi = [ii, jj, kk]
+    x = [sum(A[a, b] * xi[b] for b in range(3)) for a in range(3)]
+    x[0] = x[0] + 0.03 * np.sin(np.pi * jj) * np.cos(np.pi * kk)
+    x[1] = x[1] + 0.03 * np.sin(np.pi * kk + 0.3) * np.sin(np.pi * ii)
+    x[2] = x[2] + 0.02 * np.cos(np.pi * ii) * np.sin(np.pi * jj)
+    return np.stack(x), d
+
+
+def euler_wave_curvilinear_3d(n, conv="weno_z", averaging="roe", rk="ls3", gama=1.4, dt=2e-3, halo=2):
+    """3-D Euler density wave on the skewed wavy mesh (periodic): rho = 1 +
+    0.2 sin(pi (x0 + x1 + x2)), u = (0.6, -0.3, 0.25), p = 1 + 0.1 cos(pi (x0 - x2)).
+    Returns (problem, q0), q0 of shape (5, n, n, n)."""
+    coords, d = skewed_wavy_grid3d(n, halo)
+    problem = dict(ndim=3, n=[n] * 3, delta=[d] * 3, conv=conv, averaging=averaging, viscous="none", rk=rk,
+                   bc=[["periodic", "periodic"]] * 3, gama=gama, dt=dt, curvilinear=1, grid_coords=coords,
+                   grid_halo=halo)
+    c = coords[:, halo:halo + n, halo:halo + n, halo:halo + n]
+    rho = 1.0 + 0.2 * np.sin(np.pi * (c[0] + c[1] + c[2]))
+    u = [0.6, -0.3, 0.25]
+    p = 1.0 + 0.1 * np.cos(np.pi * (c[0] - c[2]))
+    q = np.stack([rho] + [rho * ui for ui in u] + [p / (gama - 1) + 0.5 * rho * sum(ui * ui for ui in u)])
+    return problem, q

@@ -92,7 +92,7 @@ def test_facade_rejects_unsupported_boundaries():
         api.ExtrapolationBC(0, 0, order=2)
     m = api.MetricsEquation()
     with pytest.raises(NotImplementedError):
-        m.generate_transformations(2, "x", [(True, True), (True, True)], 2)
+        m.generate_transformations(2, "x", [(True, True), (True, False)], 2)
     with pytest.raises(ValueError):
         m.generate_transformations(2, "x", [(False, False)], 2)
 
@@ -143,3 +143,87 @@ def test_filter_constructors_follow_the_reference_checks():
         api.WENOFilter(b2, 5, dissipation_sensor="Yee")
     w = api.WENOFilter(b2, 5, dissipation_sensor="Constant", Mach_correction=True)
     assert b2.shock_filter and w.sensor == "constant" and w.mach_correction
+
+
+def test_curvilinear_app_matches_problem():
+    """The curvilinear Euler wave script (apps/euler_wave_curvilinear) gives the
+    problem and initial state of problems.euler_wave_curvilinear, with the
+    coordinates evaluated over the interior and 2 points beyond it."""
+    from opensbli_amd.apps.euler_wave_curvilinear import build
+    from opensbli_amd.problems import euler_wave_curvilinear
+    prog = build(32, niter=10)
+    p = prog.problem()
+    ref, q_ref = euler_wave_curvilinear(32, dt=0.0005)
+    for k in ["ndim", "n", "conv", "averaging", "viscous", "rk", "bc", "curvilinear"]:
+        assert p[k] == ref[k], k
+    assert np.allclose(p["delta"], ref["delta"], rtol=1e-15, atol=0)
+    q = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q)
+    assert p["grid_halo"] == 2
+    assert np.array_equal(p["grid_coords"], ref["grid_coords"])
+    assert np.array_equal(q, q_ref)
+
+
+def test_curvilinear_rhs_through_the_facade_matches_oracle_problem(oracle):
+    """The facade's problem runs in the oracle exactly like the helper's."""
+    from opensbli_amd.apps.euler_wave_curvilinear import build
+    from opensbli_amd.problems import euler_wave_curvilinear
+    prog = build(16, niter=2)
+    p = prog.problem()
+    q = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q)
+    ref, q_ref = euler_wave_curvilinear(16, dt=0.0005)
+    a = oracle.RefSolver(p)
+    a.set_state(q)
+    b = oracle.RefSolver(ref)
+    b.set_state(q_ref)
+    assert np.array_equal(a.rhs(), b.rhs())
+
+
+def _curvilinear_block(metric_params, sim_eqs, with_metric=True):
+    from opensbli_amd import api
+    ee = api.EinsteinEquation()
+    sim = api.SimulationEquations()
+    for t in sim_eqs:
+        sim.add_equations(ee.expand(t, 2, "xi", [], ["gama"]))
+    cr = api.ConstituentRelations()
+    for t in ["Eq(u_i, rhou_i/rho)", "Eq(p, (gama-1)*(rhoE-rho*(1/2)*(KD(_i,_j)*u_i*u_j)))",
+              "Eq(a, (gama*p/rho)**0.5)", "Eq(U_i, D_i_j*u_j)"]:
+        cr.add_equations(ee.expand(t, 2, "xi", [], ["gama"]))
+    block = api.SimulationBlock(2)
+    block.set_block_boundaries([api.PeriodicBC(d, s) for d in range(2) for s in (0, 1)])
+    classes = [cr, sim]
+    if with_metric:
+        m = api.MetricsEquation()
+        m.generate_transformations(2, "x", metric_params, 2)
+        classes.append(m)
+    block.set_equations(classes)
+    w = api.LLFWeno(5, formulation="Z", averaging=api.RoeAverage([0, 1]))
+    c = api.Central(4)
+    r = api.RungeKuttaLS(3)
+    block.set_discretisation_schemes({w.name: w, c.name: c, r.name: r})
+    block.discretise()
+    return api.HIPC(api.TraditionalAlgorithmRK(block))
+
+
+CURV_EQS = ["Eq(Der(rho,t), - Conservative(detJ * rho*U_j,xi_j,**{'scheme':'Weno'})/detJ)",
+            "Eq(Der(rhou_i,t) , -  Conservative(detJ * (rhou_i*U_j + p*D_j_i), xi_j , **{'scheme':'Weno'})/detJ)",
+            "Eq(Der(rhoE,t), - Conservative(detJ * (p+rhoE)*U_j,xi_j, **{'scheme':'Weno'})/detJ)"]
+
+
+def test_curvilinear_facade_checks():
+    from opensbli_amd import api
+    prog = _curvilinear_block([(True, True), (True, True)], CURV_EQS)
+    assert prog.scheme["curvilinear"] == 1 and prog.scheme["conv"] == "weno_z"
+    # metric in one direction only
+    m = api.MetricsEquation()
+    with pytest.raises(NotImplementedError, match="every direction"):
+        m.generate_transformations(2, "x", [(True, True), (False, False)], 2)
+    # strong-conservation equations without a curvilinear metric on the block
+    with pytest.raises(ValueError, match="MetricsEquation"):
+        _curvilinear_block(None, CURV_EQS, with_metric=False)
+    # a wrong momentum flux (the transposed metric) is not recognised
+    bad = list(CURV_EQS)
+    bad[1] = bad[1].replace("p*D_j_i", "p*D_i_j")
+    with pytest.raises(NotImplementedError, match="not recognised"):
+        _curvilinear_block([(True, True), (True, True)], bad)

@@ -44,31 +44,9 @@ def _spread(problem, q0, oracle, ref, steps=None):
     return rel_l2(f.get_state(), ref)
 
 
-def _mesh3d(n, h=2):
-    """A smooth periodic-metric 3-D mesh: an affine map plus waves along each direction."""
-    d = 2.0 / n
-    i = np.arange(-h, n + h) * d
-    kk, jj, ii = np.meshgrid(i, i, i, indexing="ij")
-    A = np.array([[1.0, 0.15, -0.1], [0.1, 0.9, 0.2], [-0.05, 0.1, 1.1]])
-    xi = [ii, jj, kk]
-    x = [sum(A[a, b] * xi[b] for b in range(3)) for a in range(3)]
-    x[0] = x[0] + 0.03 * np.sin(np.pi * jj) * np.cos(np.pi * kk)
-    x[1] = x[1] + 0.03 * np.sin(np.pi * kk + 0.3) * np.sin(np.pi * ii)
-    x[2] = x[2] + 0.02 * np.cos(np.pi * ii) * np.sin(np.pi * jj)
-    return np.stack(x), d
-
-
 def _problem3d(n, conv="weno_z", averaging="roe"):
-    coords, d = _mesh3d(n)
-    problem = dict(ndim=3, n=[n] * 3, delta=[d] * 3, conv=conv, averaging=averaging, viscous="none", rk="ls3",
-                   bc=[["periodic", "periodic"]] * 3, gama=1.4, dt=2e-3, curvilinear=1, grid_coords=coords,
-                   grid_halo=2)
-    c = coords[:, 2:2 + n, 2:2 + n, 2:2 + n]
-    rho = 1.0 + 0.2 * np.sin(np.pi * (c[0] + c[1] + c[2]))
-    u = [0.6, -0.3, 0.25]
-    p = 1.0 + 0.1 * np.cos(np.pi * (c[0] - c[2]))
-    q = np.stack([rho] + [rho * ui for ui in u] + [p / 0.4 + 0.5 * rho * sum(ui * ui for ui in u)])
-    return problem, q
+    from opensbli_amd.problems import euler_wave_curvilinear_3d
+    return euler_wave_curvilinear_3d(n, conv=conv, averaging=averaging)
 
 
 def test_metric_matches_oracle():

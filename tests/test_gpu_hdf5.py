@@ -126,3 +126,39 @@ def test_walls_output_leaves_the_run_unchanged(tmp_path, monkeypatch):
     assert files == ["opensbli_output.h5", "opensbli_output_000007.h5", "opensbli_output_000014.h5"], files
     x1 = hdf5.read_interior("opensbli_output.h5", "opensbliblock00", "x1_B0")
     assert np.all(np.diff(x1[:, 0]) > 0) and x1[0, 0] == 0.0
+
+
+def test_curvilinear_app_output_and_parity(tmp_path, monkeypatch, oracle):
+    """The curvilinear Euler wave app (apps/euler_wave_curvilinear): 20 steps
+    against the oracle; its output file holds the metric D00..D11 with the
+    periodic images in the halos and the coordinates over the widened grid."""
+    hdf5 = _need_h5()
+    monkeypatch.chdir(tmp_path)
+    from opensbli_amd.apps.euler_wave_curvilinear import build
+    from opensbli_amd.problems import euler_wave_curvilinear, wavy_grid
+    n = 32
+    _say("case curvilinear euler wave app 32^2, 20 steps, save_every 10")
+    res = build(n, niter=20, save_every=10).run()
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhoE")])
+    p, q0 = euler_wave_curvilinear(n, dt=0.0005)
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(20)
+    f = oracle.RefSolver(p, variant="fma")
+    f.set_state(q0)
+    f.run(20)
+    err = rel_l2(q, r.get_state())
+    spread = rel_l2(f.get_state(), r.get_state())
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), (err, spread)
+    names = hdf5.list_names("opensbli_output.h5", "opensbliblock00")
+    want = ["rho", "rhou0", "rhou1", "rhoE", "x0", "x1", "D00", "D01", "D10", "D11"]
+    assert sorted(names) == sorted(nm + "_B0" for nm in want)
+    for idx, nm in enumerate(["D00", "D01", "D10", "D11"]):
+        a, info = hdf5.decl_dat_hdf5("opensbli_output.h5", "opensbliblock00", nm + "_B0")
+        h = -info["d_m"][0]
+        assert np.allclose(a, np.pad(r.metric_field(idx), h, mode="wrap"), rtol=1e-14, atol=1e-14), nm
+    x0, info = hdf5.decl_dat_hdf5("opensbli_output.h5", "opensbliblock00", "x0_B0")
+    h = -info["d_m"][0]
+    assert np.allclose(x0, wavy_grid((n, n), (2.0 / n, 2.0 / n), halo=h)[0], rtol=0, atol=1e-15)
+    assert os.path.exists("opensbli_output_000010.h5") and os.path.exists("opensbli_output_000020.h5")
