@@ -203,10 +203,20 @@ int osbl_set_coordinates(osbl_ctx* ctx, int d, const double* x_host);
    interior and its halos (gridbasedinit.py:46-57). */
 int osbl_set_grid_coordinates(osbl_ctx* ctx, int halo, const double* coords);
 
-/* Interior values (x fastest) of a curvilinear metric array: idx = a*ndim + b
-   for D_ab, ndim*ndim for detJ -- the datasets D00.. and detJ the reference
-   can write with iohdf5 (apps/euler_wave_curvilinear/euler_wave.py:137). */
+/* Interior values (x fastest) of a metric array: on a curvilinear block
+   idx = a*ndim + b for D_ab and ndim*ndim for detJ; on a block with stretched
+   directions idx = d*ndim + d for D_dd of a stretched direction d (a function
+   of its own index).  The datasets D00.., detJ the reference writes with
+   iohdf5 (apps/euler_wave_curvilinear/euler_wave.py:135,
+   apps/katzer_SBLI/katzer_SBLI.py:135). */
 int osbl_get_metric_field(osbl_ctx* ctx, int idx, double* host);
+
+/* Interior values (x fastest) of the Ducros shock sensor theta of the last
+   stage evaluated (shock_sensors.py:12-49), on blocks with the adaptive TENO
+   or the WENO filter's Ducros sensor.  The adaptive cut-off the reference
+   stores as the TENO dataset (teno.py:430-443, store_sensor) is
+   10^-floor(teno_a1 - teno_a2 (1 - (1 - theta)^4 (1 + 4 theta))) of it. */
+int osbl_get_sensor(osbl_ctx* ctx, double* host);
 
 /* Per-face plane of conservative values, [nvar][n_e2][n_e1] over the interior
    extents of the two other directions (e1 < e2): the values a DirichletBC face

@@ -943,6 +943,12 @@ def _recognise(block):
 _CURRENT = {}
 
 
+def _adaptive_ct(theta, a1, a2):
+    """TENO_CT = 1/10^floor(a1 - a2 (1 - (1 - theta)^4 (1 + 4 theta))) (teno.py:430-438)."""
+    beta = a1 - a2 * (1.0 - (1.0 - theta) ** 4 * (1.0 + 4.0 * theta))
+    return 1.0 / 10.0 ** np.floor(beta)
+
+
 def _metric_names(nd):
     """Datasets of the curvilinear metric, in the order of osbl_get_metric_field."""
     return ["D%d%d" % (a, b) for a in range(nd) for b in range(nd)] + ["detJ"]
@@ -1094,7 +1100,16 @@ class HIPC:
         names |= {k for k in self._fields[1] if _KINDS.get(k) == "data"}
         if self.scheme.get("curvilinear"):
             names |= set(_metric_names(nd))
+        names |= {"D%d%d" % (d, d) for d in range(nd) if self.scheme["stretch"][d]}
+        if self._stores_teno():
+            names.add("TENO")
         return names | set(self._inputs)
+
+    def _stores_teno(self):
+        """LLFTeno(formulation='adaptive', store_sensor=True) keeps its cut-off
+        in the TENO dataset (teno.py:439-442)."""
+        tenos = [s for s in self.block.discretisation_schemes.values() if isinstance(s, LLFTeno)]
+        return bool(self.scheme.get("sensor") and tenos and tenos[0].store_sensor)
 
     def _init_equations(self):
         katz = [e for e in self.block.equation_classes if isinstance(e, Initialise_Katzer)]
@@ -1125,6 +1140,24 @@ class HIPC:
                 dm, dp = blk.halo_depths()
                 a = np.pad(blk.metric_field(metric.index(nm)),
                            [(-dm[d], dp[d]) for d in reversed(range(nd))], mode="wrap")
+            elif re.fullmatch(r"D(\d)\1", nm) and int(nm[1]) < nd and problem["stretch"][int(nm[1])]:
+                # D_dd of a stretched direction: mirrored across its faces by the
+                # metric boundary kernel (D[-t] = D[t], metric.py:359-379), a
+                # function of its own index elsewhere
+                s_ = int(nm[1])
+                dm, dp = blk.halo_depths()
+                a = blk.metric_field(s_ * nd + s_)
+                for d in range(nd):
+                    ax = nd - 1 - d
+                    width = [(0, 0)] * nd
+                    width[ax] = (-dm[d], dp[d])
+                    a = np.pad(a, width, mode="reflect" if d == s_ else "edge")
+            elif nm == "TENO":
+                # the adaptive cut-off of the last stage at every interior point
+                # (teno.py:430-443); the halo layers are left at zero
+                dm, dp = blk.halo_depths()
+                a = np.pad(_adaptive_ct(blk.sensor(), problem["teno_a1"], problem["teno_a2"]),
+                           [(-dm[d], dp[d]) for d in reversed(range(nd))])
             elif nm in self._fields[1]:
                 dm, dp = blk.halo_depths()
                 if widened is None:

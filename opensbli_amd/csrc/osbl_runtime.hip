@@ -948,9 +948,34 @@ int osbl_set_grid_coordinates(osbl_ctx* c, int halo, const double* coords) {
 
 int osbl_get_metric_field(osbl_ctx* c, int idx, double* host) {
   if (!c || !host) return fail(OSBL_E_ARG, "null argument");
-  if (!c->curv || !c->cmet_ready) return fail(OSBL_E_STATE, "no curvilinear metric on this block");
   if (idx < 0 || idx > c->nd * c->nd) return fail(OSBL_E_ARG, "metric index %d out of range", idx);
+  if (!c->curv) {
+    // a direction stretched along itself: D_dd is a function of its own index
+    const int d = idx / c->nd;
+    if (idx == c->nd * c->nd || idx % c->nd != d || !c->cfg.stretch[d])
+      return fail(OSBL_E_ARG, "metric index %d: not an array of this block (D_dd of its stretched directions)", idx);
+    if (!c->bnd.dm[d]) return fail(OSBL_E_STATE, "direction %d is stretched: call osbl_set_coordinates first", d);
+    std::vector<double> line((size_t)c->g.n[d]);
+    HIPCHK(hipMemcpyAsync(line.data(), c->met_dev[d], line.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long long o = 0;
+    for (int k = 0; k < c->g.n[2]; ++k)
+      for (int j = 0; j < c->g.n[1]; ++j)
+        for (int i = 0; i < c->g.n[0]; ++i) host[o++] = line[d == 0 ? i : (d == 1 ? j : k)];
+    return OSBL_OK;
+  }
+  if (!c->cmet_ready) return fail(OSBL_E_STATE, "curvilinear block: call osbl_set_grid_coordinates first");
   launch_pack(c->cmet + idx * c->g.fs, c->stage_buf, c->g, c->stream);
+  HIPCHK(hipMemcpyAsync(host, c->stage_buf, cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
+int osbl_get_sensor(osbl_ctx* c, double* host) {
+  if (!c || !host) return fail(OSBL_E_ARG, "null argument");
+  if (!c->Theta) return fail(OSBL_E_STATE, "the block has no shock sensor");
+  launch_pack(c->Theta, c->stage_buf, c->g, c->stream);
   HIPCHK(hipMemcpyAsync(host, c->stage_buf, cells(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return OSBL_OK;

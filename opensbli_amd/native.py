@@ -31,6 +31,7 @@ EXPORTS = [
     "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
     "osbl_set_coordinates", "osbl_set_bc_plane", "osbl_set_filter_mask", "osbl_get_sfd_field",
     "osbl_set_sfd_field", "osbl_get_field_halo", "osbl_set_grid_coordinates", "osbl_get_metric_field",
+    "osbl_get_sensor",
 ]
 
 
@@ -142,6 +143,7 @@ def load():
     L.osbl_set_sfd_field.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_set_grid_coordinates.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_get_metric_field.argtypes = [vp, ctypes.c_int, dp]
+    L.osbl_get_sensor.argtypes = [vp, dp]
     _lib = L
     return L
 
@@ -250,8 +252,15 @@ class Block:
             raise ValueError(f"coordinates have shape {a.shape}, expected {want}")
         check(load().osbl_set_grid_coordinates(self.h, int(halo), _dptr(a)))
 
+    def sensor(self):
+        """The Ducros sensor theta of the last stage evaluated (interior)."""
+        out = np.empty(self.shape)
+        check(load().osbl_get_sensor(self.h, _dptr(out)))
+        return out
+
     def metric_field(self, idx):
-        """Interior values of D_ab (idx = a*ndim + b) or detJ (idx = ndim*ndim)."""
+        """Interior values of D_ab (idx = a*ndim + b) or detJ (idx = ndim*ndim) of
+        a curvilinear block; D_dd (idx = d*ndim + d) of a stretched direction."""
         out = np.empty(self.shape)
         check(load().osbl_get_metric_field(self.h, int(idx), _dptr(out)))
         return out

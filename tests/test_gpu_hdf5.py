@@ -162,3 +162,43 @@ def test_curvilinear_app_output_and_parity(tmp_path, monkeypatch, oracle):
     h = -info["d_m"][0]
     assert np.allclose(x0, wavy_grid((n, n), (2.0 / n, 2.0 / n), halo=h)[0], rtol=0, atol=1e-15)
     assert os.path.exists("opensbli_output_000010.h5") and os.path.exists("opensbli_output_000020.h5")
+
+
+def test_flat_plate_writes_metric_and_teno_cutoff(tmp_path, monkeypatch, oracle):
+    """The flat plate with the adaptive TENO writes D11 and TENO as
+    katzer_SBLI.py:135 asks: D11 of the stretched wall-normal direction
+    mirrored across its faces (metric.py:359-379) and the adaptive cut-off of
+    the last stage (teno.py:430-443), both against the oracle."""
+    hdf5 = _need_h5()
+    monkeypatch.chdir(tmp_path)
+    from opensbli_amd.api import DataObject, _adaptive_ct, iohdf5
+    from opensbli_amd.apps.flat_plate import build
+    _say("case flat plate 2-D 64x48 adaptive TENO5, 10 steps, output D11 and TENO")
+    prog = build(n=(64, 48), lengths=(400.0, 115.0), niter=10, scheme="teno5")
+    h5 = iohdf5(**{"iotype": "Write"})
+    h5.add_arrays([DataObject(nm) for nm in ("rho", "rhou0", "rhou1", "rhoE", "x0", "x1", "D11", "TENO")])
+    prog.block.setio([h5])
+    res = prog.run()
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(10)
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhoE")])
+    err = rel_l2(q, r.get_state())
+    _say("state rel L2", err)
+    assert np.all(err < TOL), err
+    d11, info = hdf5.decl_dat_hdf5("opensbli_output.h5", "opensbliblock00", "D11_B0")
+    dm, dp = info["d_m"], info["d_p"]
+    line = r.metric(1)[0]
+    hj = -dm[1]
+    want = np.pad(line, (hj, dp[1]), mode="reflect")  # D[-t] = D[t], D[N-1+t] = D[N-1-t]
+    assert d11.shape == (48 + hj + dp[1], 64 - dm[0] + dp[0])
+    # the library forms D11 on the host in its own evaluation order: ulp-level differences
+    assert np.allclose(d11, want[:, None], rtol=1e-13, atol=0), np.abs(d11 - want[:, None]).max()
+    teno = hdf5.read_interior("opensbli_output.h5", "opensbliblock00", "TENO_B0")
+    ct = _adaptive_ct(r.sensor(), p["teno_a1"], p["teno_a2"])
+    same = np.mean(teno == ct)
+    _say("TENO cut-off equal to the oracle's at", same, "of the points; range", teno.min(), teno.max())
+    assert same > 0.999 and teno.min() > 0
