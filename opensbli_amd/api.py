@@ -929,6 +929,10 @@ def _recognise(block):
             raise ValueError("the metric was generated for %d dimensions on a %d-D block" % (metric.ndim, ndim))
         if not [s for s in spatial if isinstance(s, Central)]:
             raise ValueError("A Central scheme should be instantiated for metrics.")
+        if any(metric.curvilinear_metric):
+            raise NotImplementedError("equations in x transformed by a curvilinear metric (apply_metrics) are not "
+                                      "implemented by the HIPC target; write them in strong conservation form in "
+                                      "xi as apps/euler_wave_curvilinear does")
         for d in range(ndim):
             stretch[d] = int(metric.stretching_metric[d])
     out = dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch, sensor=sensor)
