@@ -66,6 +66,8 @@ extern "C" {
 #define OSBL_BC_ISOTHERMAL_WALL 5 /* IsothermalWallBC with rhoE = rho Twall/(gama(gama-1)Minf^2) (isothermal_wall.py:33-88) */
 #define OSBL_BC_DIRICHLET 6       /* DirichletBC, values from osbl_set_bc_plane (dirichlet.py:28-41) */
 #define OSBL_BC_INLET_PEXTRAP 7   /* InletPressureExtrapolateBC, side 0 (inlet_pressure_extrapolate.py:32-66) */
+#define OSBL_BC_SYMMETRY 8        /* SymmetryBC: mirrored halos with the normal momentum negated; the central
+                                     stencils are kept (symmetry.py:24-48) */
 
 /* stencil closure at faces whose BC is a ModifyCentralDerivative (all but
    periodic and zero-gradient; bc_core.py:15-16, opensblifunctions.py:523-534) */

@@ -54,9 +54,16 @@ def _mk(name, kind):
     return s
 
 
+class _GridSymbol(sympy.Symbol):
+    """A kernel-local variable: a symbol distinct from the dataset of the same
+    name (Eq(DataObject(rho), GridVariable(rho)) is an assignment)."""
+
+
 def GridVariable(name):
     """Kernel-local variable of an initial condition (grid.py:214)."""
-    return _mk(name, "grid")
+    name = str(name)
+    _KINDS.setdefault(name, "grid")
+    return _GridSymbol(name)
 
 
 def DataObject(name):
@@ -424,6 +431,13 @@ class ZeroGradientOutletBC(_BC):
     """q[boundary] = q[boundary - 1], halos mirrored (zero_gradient_outlet.py:5);
     leaves the central stencils unmodified."""
     kind = "zero_gradient"
+
+
+class SymmetryBC(_BC):
+    """Symmetry plane (symmetry.py:6): halos mirrored with the normal momentum
+    negated; not a ModifyCentralDerivative, so the central stencils read the
+    halos (a `scheme` argument is accepted and unused, as in the reference)."""
+    kind = "symmetry"
 
 
 class AdiabaticWallBC(_BC):
@@ -906,6 +920,11 @@ def _recognise(block):
         for e in c.equations:
             lhs, rhs = _parse_eq(e.text)
             have[lhs] = rhs
+    # mu/Re with mu a declared constant rather than a constituent relation:
+    # constant viscosity at the effective Reynolds number Re/mu
+    mu_constant = viscous == "sutherland" and "mu" not in have and any("mu" in e.constants for e in eqs)
+    if mu_constant:
+        viscous = "const"
     need = ["u_i", "p"] + (["a"] if conv not in ("central_skew", "central_cons") else []) + \
         (["T"] if viscous != "none" else []) + (["mu"] if viscous == "sutherland" else []) + \
         (["U_i"] if curvilinear else [])
@@ -936,6 +955,8 @@ def _recognise(block):
         for d in range(ndim):
             stretch[d] = int(metric.stretching_metric[d])
     out = dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch, sensor=sensor)
+    if mu_constant:
+        out["mu_constant"] = 1
     if curvilinear:
         out["curvilinear"] = 1
     return out
@@ -945,6 +966,17 @@ def _recognise(block):
 # the target
 # ---------------------------------------------------------------------------
 _CURRENT = {}
+
+
+def _lookup(env, sym):
+    """The value of a symbol of an initial-condition expression: the symbol
+    itself, else a kernel-local variable of the same name (a bare name in the
+    generated C refers to the local variable; datasets print as name_B0)."""
+    if sym in env:
+        return env[sym]
+    if not isinstance(sym, _GridSymbol):
+        return env.get(_GridSymbol(sym.name))
+    return None
 
 
 def _adaptive_ct(theta, a1, a2):
@@ -1017,6 +1049,10 @@ class HIPC:
         for k in ("Minf", "Re", "Pr", "eps"):
             if k in v:
                 p[k] = float(v[k])
+        if p.pop("mu_constant", None):
+            if "mu" not in v or "Re" not in v:
+                raise ValueError("simulation parameters mu and Re need values")
+            p["Re"] = float(v["Re"]) / float(v["mu"])
         if "TENO_CT" in v:
             p["teno_ct"] = float(v["TENO_CT"])
         if p["viscous"] == "sutherland":
@@ -1183,13 +1219,15 @@ class HIPC:
             if bad:
                 raise NotImplementedError("boundary values depending on the solution (%s) are not implemented "
                                           "by the HIPC target" % bad)
-            missing = [s for s in syms if s not in env]
+            vals = [_lookup(env, s) for s in syms]
+            missing = [s for s, v in zip(syms, vals) if v is None]
             if missing:
                 raise ValueError("equation uses undefined symbols %s" % missing)
-            f = sympy.lambdify(syms, rhs, modules="numpy")
-            val = np.broadcast_to(np.asarray(f(*[env[s] for s in syms]), dtype=np.float64), shape).copy()
+            f = sympy.lambdify(syms, rhs, modules="numpy", dummify=True)
+            val = np.broadcast_to(np.asarray(f(*vals), dtype=np.float64), shape).copy()
             env[eq.lhs] = val
-            out[str(eq.lhs)] = val
+            if not isinstance(eq.lhs, _GridSymbol):
+                out[str(eq.lhs)] = val
 
     def initial_state(self, problem):
         """Evaluate the initial condition over the interior grid: the
@@ -1301,11 +1339,12 @@ class HIPC:
             syms = sorted(cond.free_symbols, key=lambda z: z.name)
             look = dict(env)
             look.update({sympy.Symbol(k): val for k, val in fields.items()})
-            missing = [z for z in syms if z not in look]
+            vals = [_lookup(look, z) for z in syms]
+            missing = [z for z, v in zip(syms, vals) if v is None]
             if missing:
                 raise ValueError("grid_condition uses undefined symbols %s" % missing)
-            f = sympy.lambdify(syms, cond, modules="numpy")
-            mask = np.broadcast_to(np.asarray(f(*[look[z] for z in syms]), dtype=bool), shape)
+            f = sympy.lambdify(syms, cond, modules="numpy", dummify=True)
+            mask = np.broadcast_to(np.asarray(f(*vals), dtype=bool), shape)
             problem["filters"]["binomial"]["mask"] = np.ascontiguousarray(mask, dtype=np.uint8)
         return problem
 

@@ -47,6 +47,14 @@ __global__ void __launch_bounds__(256) k_bc_face(double* __restrict__ Q, double*
         for (int hh = 1; hh <= nh; ++hh)
           Q[v * fs + x - hh * st] = 2.0 * Q[v * fs + x - (hh - 1) * st] - Q[v * fs + x - (hh - 2) * st];
       break;
+    case OSBL_BC_SYMMETRY:  // symmetry.py:24-48: mirror images, normal momentum negated; the boundary point stays
+      for (int hh = 1; hh <= nh; ++hh)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const double s = Q[v * fs + x + hh * st];
+          Q[v * fs + x - hh * st] = (v == 1 + d) ? -s : s;
+        }
+      break;
     case OSBL_BC_ZERO_GRADIENT:  // q[0] = q[1]; q[-i] = q[i] (zero_gradient_outlet.py:16-23)
       for (int v = 0; v < NV; ++v) {
         Q[v * fs + x] = Q[v * fs + x + st];

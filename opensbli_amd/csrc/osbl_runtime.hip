@@ -164,7 +164,9 @@ double d1_line(const double* f, int N, int lo, int hi, int c, const double* cw) 
   return (1.0 / 12.0) * (f[c - 2] - f[c + 2]) + (2.0 / 3.0) * (f[c + 1] - f[c - 1]);
 }
 
-bool modifies_stencil(int bc) { return bc != OSBL_BC_PERIODIC && bc != OSBL_BC_ZERO_GRADIENT; }
+bool modifies_stencil(int bc) {
+  return bc != OSBL_BC_PERIODIC && bc != OSBL_BC_ZERO_GRADIENT && bc != OSBL_BC_SYMMETRY;
+}
 
 }  // namespace
 
@@ -435,7 +437,7 @@ int check_config(const osbl_config* cfg) {
   for (int d = 0; d < cfg->ndim; ++d) {
     for (int s = 0; s < 2; ++s) {
       const int bc = cfg->bc[d][s];
-      if (bc < OSBL_BC_PERIODIC || bc > OSBL_BC_INLET_PEXTRAP)
+      if (bc < OSBL_BC_PERIODIC || bc > OSBL_BC_SYMMETRY)
         return fail(OSBL_E_ARG, "unknown boundary condition %d on direction %d side %d", bc, d, s);
       if (bc == OSBL_BC_INLET_PEXTRAP && s != 0)
         return fail(OSBL_E_ARG, "direction %d: the pressure-extrapolating inlet is implemented for side 0 only", d);

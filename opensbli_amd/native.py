@@ -16,7 +16,7 @@ AVG = {"simple": 0, "roe": 1}
 VISC = {"none": 0, "const": 1, "sutherland": 2}
 RK = {"rk3": 0, "ls3": 1, "ls3_ssp": 2, "ls4": 3}
 BC = {"periodic": 0, "extrap0": 1, "extrap1": 2, "zero_gradient": 3, "adiabatic_wall": 4,
-      "isothermal_wall": 5, "dirichlet": 6, "inlet_pextrap": 7}
+      "isothermal_wall": 5, "dirichlet": 6, "inlet_pextrap": 7, "symmetry": 8}
 BSCHEME = {"carpenter": 0, "reduced_access": 1}
 FILTERS = {"weno": 1, "sfd": 2, "binomial": 4}
 WF_SENSOR = {"ducros": 0, "constant": 1}

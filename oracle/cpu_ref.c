@@ -1124,6 +1124,11 @@ static void bc_point(ref_state* s, int d, int side, ptrdiff_t x, ptrdiff_t st, i
       for (int v = 0; v < nv; ++v)
         for (int hh = 1; hh <= nh; ++hh) q[v][x - hh * st] = 2.0 * q[v][x - (hh - 1) * st] - q[v][x - (hh - 2) * st];
       break;
+    case REF_BC_SYMMETRY: /* symmetry.py:24-48: q[-i] = q[i] with the normal momentum v - 2 (v.n) n; the
+                             boundary point is left as it is */
+      for (int hh = 1; hh <= nh; ++hh)
+        for (int v = 0; v < nv; ++v) q[v][x - hh * st] = (v == 1 + d) ? -q[v][x + hh * st] : q[v][x + hh * st];
+      break;
     case REF_BC_ZERO_GRADIENT: /* q[0] = q[1]; q[-i] = q[i] (zero_gradient_outlet.py:16-23) */
       for (int v = 0; v < nv; ++v) {
         q[v][x] = q[v][x + st];
@@ -1498,8 +1503,9 @@ int ref_create(const ref_config* cfg, ref_state** out) {
   for (int d = 0; d < s->nd; ++d)
     for (int side = 0; side < 2; ++side) {
       const int bc = cfg->bc[d][side];
-      /* every BC but periodic and ZeroGradientOutlet is a ModifyCentralDerivative */
-      if (bc != REF_BC_PERIODIC && bc != REF_BC_ZERO_GRADIENT) s->mod[d][side] = 1 + cfg->bscheme[d][side];
+      /* every BC but periodic, ZeroGradientOutlet and Symmetry is a ModifyCentralDerivative */
+      if (bc != REF_BC_PERIODIC && bc != REF_BC_ZERO_GRADIENT && bc != REF_BC_SYMMETRY)
+        s->mod[d][side] = 1 + cfg->bscheme[d][side];
     }
   /* the residual kernels consume every convective work array before the
      viscous kernels run, so the two groups share one pool */

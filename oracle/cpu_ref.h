@@ -36,7 +36,8 @@ enum {
   REF_BC_ADIABATIC_WALL = 4,  /* AdiabaticWallBC, adiabatic_wall.py:28-69 */
   REF_BC_ISOTHERMAL_WALL = 5, /* IsothermalWallBC, isothermal_wall.py:33-88 */
   REF_BC_DIRICHLET = 6,       /* DirichletBC, dirichlet.py:28-41 */
-  REF_BC_INLET_PEXTRAP = 7    /* InletPressureExtrapolateBC (side 0), inlet_pressure_extrapolate.py:32-66 */
+  REF_BC_INLET_PEXTRAP = 7,   /* InletPressureExtrapolateBC (side 0), inlet_pressure_extrapolate.py:32-66 */
+  REF_BC_SYMMETRY = 8         /* SymmetryBC, symmetry.py:24-48 */
 };
 /* near-boundary replacement of the central stencils (bc_core.py:15-16,
    opensblifunctions.py:523-534) at faces whose BC modifies them */

@@ -220,3 +220,89 @@ def test_general_3d_viscous_schedules(case, oracle):
     if stretch:
         p["coords"] = {1: stretched_coordinate(n[1], 1.0, stretch, "sinh")}
     _compare(p, q0, oracle, 100)
+
+
+@pytest.mark.parametrize("conv", ["weno_z", "teno5"])
+def test_symmetry_plane(conv, oracle):
+    """SymmetryBC on the top face: 50 steps against the oracle, and equal to the
+    mirror-symmetric block of twice the height run on the GPU."""
+    from opensbli_amd.native import Block
+    from test_oracle_walls import _mirrored_state, _symmetry_problem
+    _say(f"case symmetry plane 24x16 ({conv}) vs oracle and vs mirrored 24x31, 50 steps")
+    ny, nf = 16, 31
+    q = _mirrored_state(24, nf, (ny - 1) * 0.03)
+    qh = np.ascontiguousarray(q[:, :ny])
+    ph = _symmetry_problem(ny, "symmetry", conv)
+    g = Block(ph)
+    g.set_state(qh)
+    g.run(50)
+    r = oracle.RefSolver(ph)
+    r.set_state(qh)
+    r.run(50)
+    err = rel_l2(g.get_state(), r.get_state())
+    _say("vs oracle rel L2", err)
+    assert np.all(err < TOL), err
+    full = Block(_symmetry_problem(nf, "adiabatic_wall", conv))
+    full.set_state(q)
+    full.run(50)
+    err = rel_l2(g.get_state(), full.get_state()[:, :ny])
+    _say("vs mirrored block rel L2", err)
+    assert np.all(err < 1e-13), err
+    g.close()
+    full.close()
+
+
+def test_viscous_shock_tube_app_against_oracle(oracle):
+    """The viscous shock tube app (apps/viscous_shock_tube): TENO5, constant
+    mu, adiabatic walls with ReducedAccess closures and the symmetry plane,
+    at 100 x 50 for 200 steps (dt 2.5e-5) against the oracle."""
+    from opensbli_amd.apps.viscous_shock_tube import build
+    _say("case viscous shock tube app 100x50, 200 steps, vs oracle")
+    prog = build((100, 50), niter=200, dt="0.000025", output=False)
+    res = prog.run()
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(200)
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhoE")])
+    err = rel_l2(q, r.get_state())
+    _say("rel L2", err)
+    assert np.all(err < TOL), err
+
+
+def test_viscous_shock_tube_wall_density_matches_reference_data():
+    """The app as the reference runs it (500 x 250, dt = 5e-6, 200000
+    iterations to t = 1): the bottom-wall density for x >= 0.3 against the
+    grid-converged solution the reference ships (reference_Re200.txt, Zhou et
+    al. 2018), the comparison of apps/viscous_shock_tube/plot.py:74-84.  A
+    500 x 250 solution is not grid-converged; the bound is a shape check."""
+    import os
+    import time
+    from opensbli_amd.apps.viscous_shock_tube import build
+    from opensbli_amd.native import Block
+    _say("case viscous shock tube 500x250 to t = 1 (200000 steps) vs reference wall density")
+    prog = build((500, 250), output=False)
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    g = Block(p)
+    g.set_state(q0)
+    t0 = time.time()
+    for k in range(10):
+        g.run(20000)
+        g.sync()
+        _say("  %d steps, %.1f s" % (20000 * (k + 1), time.time() - t0))
+    rho = g.get_state()[0]
+    g.close()
+    x = np.arange(500) / 499.0
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "viscous_shock_tube_wall_density_Re200.txt"))
+    sel = x >= 0.3
+    want = np.interp(x[sel], ref[:, 0], ref[:, 1])
+    got = rho[0, sel]
+    l1 = np.abs(got - want).mean() / np.abs(want).mean()
+    _say("bottom-wall density, x >= 0.3: relative L1 %.4f, max |diff| %.3f (range %.2f..%.2f)"
+         % (l1, np.abs(got - want).max(), want.min(), want.max()))
+    assert np.all(np.isfinite(rho))
+    assert l1 < 0.05, l1

@@ -12,6 +12,7 @@ convergence rate.
 import numpy as np
 import pytest
 
+from conftest import rel_l2
 from oracle import ref
 from oracle.ref import RefSolver
 from opensbli_amd.boundary_layer import SimilarityProfile
@@ -277,3 +278,47 @@ def test_flat_plate_oracle_runs():
     q = s.get_state()
     assert np.isfinite(q).all()
     assert np.all(q[1:4, :, 0, :] == 0.0)  # no-slip wall row
+
+
+def _symmetry_problem(ny, top, conv):
+    return dict(ndim=2, n=[24, ny], delta=[0.04, 0.03], conv=conv, averaging="roe", viscous="const", rk="ls3",
+                bc=[["adiabatic_wall", "adiabatic_wall"], ["adiabatic_wall", top]],
+                bscheme=[["reduced_access"] * 2] * 2, gama=1.4, Minf=1.0, Re=200.0, Pr=0.73, dt=2e-4, eps=1e-15,
+                teno_ct=1e-6)
+
+
+def _mirrored_state(nx, nf, yc):
+    x = np.arange(nx) * 0.04
+    y = np.arange(nf) * 0.03
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    rho = 1 + 0.3 * np.exp(-((X - 0.5) ** 2 + (Y - yc) ** 2) / 0.05) + 0.1 * np.cos(3 * (Y - yc))
+    u = 0.2 * np.sin(2 * X) * np.cos(2 * (Y - yc))
+    v = 0.3 * np.sin(3 * X) * np.sin(2 * (Y - yc))  # odd about the plane
+    p = 1 / 1.4 + 0.1 * np.cos(2 * X) * np.cos(Y - yc)
+    return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+
+
+@pytest.mark.parametrize("conv", ["weno_z", "teno5"])
+def test_symmetry_plane_is_the_mirrored_domain(oracle, conv):
+    """SymmetryBC (symmetry.py:24-48) on the top face of an n-point block
+    evolves exactly like the mirror-symmetric 2n-1-point block with walls at
+    both ends: its halos are the mirror images with the normal momentum
+    negated, the boundary point is left to the scheme, and the central
+    stencils are not closed there.  A BC that kept the normal momentum, or
+    closed the stencils, fails this."""
+    ny = 16
+    nf = 2 * ny - 1
+    q = _mirrored_state(24, nf, (ny - 1) * 0.03)
+    full = oracle.RefSolver(_symmetry_problem(nf, "adiabatic_wall", conv))
+    full.set_state(q)
+    half = oracle.RefSolver(_symmetry_problem(ny, "symmetry", conv))
+    half.set_state(np.ascontiguousarray(q[:, :ny]))
+    full.run(20)
+    half.run(20)
+    err = rel_l2(half.get_state(), full.get_state()[:, :ny])
+    assert np.all(err < 1e-14), err
+    # the planted mistake: a top face that closes the stencils (a wall) differs
+    wall = oracle.RefSolver(_symmetry_problem(ny, "zero_gradient", conv))
+    wall.set_state(np.ascontiguousarray(q[:, :ny]))
+    wall.run(20)
+    assert np.max(rel_l2(wall.get_state(), full.get_state()[:, :ny])) > 1e-6
