@@ -306,3 +306,86 @@ def test_viscous_shock_tube_wall_density_matches_reference_data():
          % (l1, np.abs(got - want).max(), want.min(), want.max()))
     assert np.all(np.isfinite(rho))
     assert l1 < 0.05, l1
+
+
+def test_shock_reflection_app_against_oracle(oracle):
+    """The inviscid shock reflection app (apps/inviscid_shock_reflection):
+    Dirichlet inflow and top face (two states split at x = 40), zeroth-order
+    outflow, symmetry plane at the bottom; 92 x 52, dt 0.5, 100 steps against
+    the oracle."""
+    from opensbli_amd.apps.inviscid_shock_reflection import build
+    _say("case shock reflection app 92x52, 100 steps, vs oracle")
+    prog = build((92, 52), niter=100, dt="0.5", output=False)
+    res = prog.run()
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(100)
+    f = oracle.RefSolver(p, variant="fma")
+    f.set_state(q0)
+    f.run(100)
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhoE")])
+    err = rel_l2(q, r.get_state())
+    spread = rel_l2(f.get_state(), r.get_state())
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+
+
+def _oblique_shock_p3():
+    """Pressure behind the regular reflection of the app's incident shock:
+    the flow behind it (inviscid_shock.py:108-121) turned back parallel to the
+    wall by the weak oblique shock of the theta-beta-M relation."""
+    import math
+    from scipy.optimize import brentq
+    from opensbli_amd.apps.inviscid_shock_reflection import POST_SHOCK
+    g = 1.4
+    p2 = POST_SHOCK["p_ratio"] / (1.4 * 4.0)
+    r2, u2, v2 = POST_SHOCK["rho"], POST_SHOCK["u0"], POST_SHOCK["u1"]
+    M = math.hypot(u2, v2) / math.sqrt(g * p2 / r2)
+    th = math.atan(-v2 / u2)
+    f = lambda b: math.atan(2.0 / math.tan(b) * (M * M * math.sin(b) ** 2 - 1.0) /  # noqa: E731
+                            (M * M * (g + math.cos(2.0 * b)) + 2.0)) - th
+    lo = math.asin(1.0 / M) + 1e-9
+    b = lo
+    while f(b) * f(b + 1e-3) > 0:
+        b += 1e-3
+    beta = brentq(f, b, b + 1e-3)
+    return p2, p2 * (1.0 + 2.0 * g / (g + 1.0) * (M * M * math.sin(beta) ** 2 - 1.0))
+
+
+def test_shock_reflection_steady_state_matches_oblique_shock_theory():
+    """The app as the reference runs it (457 x 255, dt = 0.1, 10000
+    iterations, three flow-throughs): the pressure behind the incident shock
+    and behind its reflection off the symmetry plane equal the oblique-shock
+    values (p2/p1 = 1.18647 imposed on top, p3/p1 = 1.3994 from the
+    theta-beta-M relation) within 0.1 % (measured: 6 digits)."""
+    import time
+    from opensbli_amd.apps.inviscid_shock_reflection import build
+    from opensbli_amd.native import Block
+    _say("case shock reflection 457x255, 10000 steps vs oblique-shock theory")
+    prog = build(output=False)
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    g = Block(p)
+    g.set_state(q0)
+    t0 = time.time()
+    for k in range(5):
+        g.run(2000)
+        g.sync()
+        _say("  %d steps, %.1f s" % (2000 * (k + 1), time.time() - t0))
+    q = g.get_state()
+    g.close()
+    pr = 0.4 * (q[3] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0])
+    x = np.arange(457) * 350.0 / 456
+    y = np.arange(255) * 115.0 / 254
+    Y, X = np.meshgrid(y, x, indexing="ij")
+    p2, p3 = _oblique_shock_p3()
+    reg2 = (X >= 120) & (X <= 160) & (Y >= 90) & (Y <= 110)
+    reg3 = (X >= 250) & (X <= 330) & (Y <= 8)
+    m2, m3 = pr[reg2].mean(), pr[reg3].mean()
+    _say("region 2: %.6f (theory %.6f); region 3: %.6f (theory %.6f), spread %.2e"
+         % (m2, p2, m3, p3, pr[reg3].std()))
+    assert abs(m2 / p2 - 1.0) < 1e-3 and abs(m3 / p3 - 1.0) < 1e-3
