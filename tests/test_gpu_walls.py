@@ -389,3 +389,76 @@ def test_shock_reflection_steady_state_matches_oblique_shock_theory():
     _say("region 2: %.6f (theory %.6f); region 3: %.6f (theory %.6f), spread %.2e"
          % (m2, p2, m3, p3, pr[reg3].std()))
     assert abs(m2 / p2 - 1.0) < 1e-3 and abs(m3 / p3 - 1.0) < 1e-3
+
+
+def test_katzer_sbli_app_against_oracle(oracle):
+    """The Katzer SBLI app (apps/katzer_SBLI): adaptive TENO5 with the Ducros
+    sensor, Sutherland viscosity, stretched wall-normal grid, inlet,
+    outlet, isothermal wall and the shock-generator top; 100 x 50, 200 steps
+    against the oracle."""
+    from opensbli_amd.apps.katzer_sbli import build
+    _say("case katzer SBLI app 100x50, 200 steps, vs oracle")
+    prog = build((100, 50), niter=200, output=False)
+    res = prog.run()
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    r = oracle.RefSolver(p)
+    r.set_state(q0)
+    r.run(200)
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhou1", "rhoE")])
+    err = rel_l2(q, r.get_state())
+    _say("rel L2", err)
+    assert np.all(err < TOL), err
+
+
+def test_katzer_sbli_skin_friction_and_wall_pressure_match_reference_data():
+    """The app as the reference runs it (500 x 250, dt = 0.04, 250000
+    iterations): skin friction and wall pressure formed as its plot script
+    does (apps/katzer_SBLI/plot.py:95-116) against the data it ships
+    (reference_data.txt)."""
+    import os
+    import time
+    from opensbli_amd.apps.katzer_sbli import build
+    from opensbli_amd.native import Block
+    _say("case katzer SBLI 500x250, 250000 steps vs reference skin friction and wall pressure")
+    prog = build(output=False)
+    p = prog.problem()
+    q0 = prog.initial_state(p)
+    p = prog._attach_boundary_data(p, q0)
+    g = Block(p)
+    g.set_state(q0)
+    t0 = time.time()
+    for k in range(10):
+        g.run(25000)
+        g.sync()
+        _say("  %d steps, %.1f s" % (25000 * (k + 1), time.time() - t0))
+    q = g.get_state()
+    d11 = g.metric_field(3)
+    g.close()
+    assert np.all(np.isfinite(q))
+    rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+    pr = 0.4 * (q[3] - 0.5 * (u * u + v * v) * rho)
+    T = 1.4 * 4.0 * pr / rho
+    sr = 110.4 / 288.0
+    mu_w = T[0] ** 1.5 * (1.0 + sr) / (T[0] + sr)
+    coeffs = np.array([-1.83333333333334, 3.00000000000002, -1.50000000000003, 0.333333333333356,
+                       -8.34617916606957e-15, 1.06910884386911e-15]).reshape(6, 1)
+    dudy = (d11[0:6] * u[0:6] * coeffs).sum(0) / (115.0 / 249.0)
+    cf = mu_w * dudy / (0.5 * 950.0)
+    pw = pr[0] / pr[0, 0]
+    x = np.arange(500) * 400.0 / 499
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "katzer_sbli_skin_friction_wall_pressure.txt"))
+    sel = (x >= 10.0) & (x <= 390.0)
+    cf_ref = np.interp(x[sel], ref[:, 0], ref[:, 1])
+    pw_ref = np.interp(x[sel], ref[:, 0], ref[:, 2])
+    cf_err = np.abs(cf[sel] - cf_ref).mean() / np.abs(cf_ref).mean()
+    pw_err = np.abs(pw[sel] - pw_ref).mean() / np.abs(pw_ref).mean()
+    sep = x[sel][cf[sel] < 0]
+    sep_ref = x[sel][cf_ref < 0]
+    _say("skin friction rel L1 %.4f, wall pressure rel L1 %.4f; separated x in [%.1f, %.1f], reference [%.1f, %.1f]"
+         % (cf_err, pw_err, sep.min() if sep.size else -1, sep.max() if sep.size else -1,
+            sep_ref.min() if sep_ref.size else -1, sep_ref.max() if sep_ref.size else -1))
+    # measured: 1.8 % and 0.08 %, separation [140.3, 287.8] against [138.7, 287.8]
+    assert cf_err < 0.04 and pw_err < 0.005, (cf_err, pw_err)
+    assert sep.size and abs(sep.min() - sep_ref.min()) < 5.0 and abs(sep.max() - sep_ref.max()) < 5.0
