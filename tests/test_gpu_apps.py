@@ -71,3 +71,19 @@ def test_euler_wave_app_against_oracle_and_exact(oracle):
     e = res["rho"] - exact
     _say("L1 %.3e, Linf %.3e" % (np.abs(e).mean(), np.abs(e).max()))
     assert np.abs(e).max() < 1e-9  # measured 1.6e-10
+
+
+def test_shu_osher_app_against_oracle(oracle):
+    """apps/shu_osher at 400 points, dt 2e-3, 900 steps to t = 1.8 (the
+    reference runs 3200 points): WENO5-Z with Dirichlet states from the
+    initial condition at x = -5 and 5, against the oracle."""
+    from opensbli_amd.apps.shu_osher import build
+    _say("case Shu-Osher app 400 points, 900 steps")
+    prog = build(400, dt="0.002", niter=900, output=False)
+    res = prog.run()
+    q = np.stack([res[k] for k in ("rho", "rhou0", "rhoE")])
+    ref = _oracle_run(oracle, prog, 900)
+    spread = rel_l2(_oracle_run(oracle, prog, 900, "fma"), ref)
+    err = rel_l2(q, ref)
+    _say("rel L2 vs oracle", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
