@@ -121,6 +121,9 @@ typedef struct osbl_config {
   double sigma_filt;       /* sigma_filt: blending weight of the binomial filter */
   double chi_filt, omega_filt;  /* chi_filt, omega_filt of SFD */
   double shock_filter_control;  /* shock_filter_control: global scale of the WENO filter */
+  double body_force[3];    /* c_j of a constant body force: -KD(_i,_j)*c_j in the momentum and
+                              -Dot(c_j, u_j) in the energy equation (apps/channel_flow); blocks with
+                              closures, a metric or Sutherland viscosity (the general update kernels) */
 } osbl_config;
 
 typedef struct osbl_ctx osbl_ctx;

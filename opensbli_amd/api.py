@@ -847,6 +847,14 @@ def _recognise(block):
     if len(temporal) != 1:
         raise ValueError("Found more than one temporal scheme on the block" if temporal else
                          "no temporal scheme on the block")
+    # a constant body force (apps/channel_flow): -KD(_i,_j)*c_j and -Dot(c_j, u_j)
+    fm, fe = (-1, "KD(_i,_j)*c_j"), (-1, "Dot(c_j,u_j)")
+    force = fm in terms["rhou_i"] and fe in terms["rhoE"]
+    if force:
+        terms["rhou_i"].discard(fm)
+        terms["rhoE"].discard(fe)
+    elif fm in terms["rhou_i"] or fe in terms["rhoE"]:
+        raise NotImplementedError("a body force needs both -KD(_i,_j)*c_j and -Dot(c_j, u_j)")
     viscous_terms = VISCOUS
     has_visc = any(t in terms[k] for k, v in viscous_terms.items() for t in v)
     heat = None
@@ -957,6 +965,8 @@ def _recognise(block):
     out = dict(conv=conv, averaging=averaging, viscous=viscous, rk=temporal[0].rk, stretch=stretch, sensor=sensor)
     if mu_constant:
         out["mu_constant"] = 1
+    if force:
+        out["force"] = 1
     if curvilinear:
         out["curvilinear"] = 1
     return out
@@ -1049,6 +1059,11 @@ class HIPC:
         for k in ("Minf", "Re", "Pr", "eps"):
             if k in v:
                 p[k] = float(v[k])
+        if p.pop("force", None):
+            missing = ["c%d" % d for d in range(nd) if "c%d" % d not in v]
+            if missing:
+                raise ValueError("simulation parameters %s need values (the body force c_j)" % missing)
+            p["body_force"] = [float(v["c%d" % d]) for d in range(nd)]
         if p.pop("mu_constant", None):
             if "mu" not in v or "Re" not in v:
                 raise ValueError("simulation parameters mu and Re need values")

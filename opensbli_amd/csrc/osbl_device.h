@@ -68,6 +68,8 @@ struct Phys {
   double suth;        // SuthT/RefT of the Sutherland law
   double teno_a1, teno_a2, sensor_eps;  // adaptive TENO cut-off and Ducros sensor constants
   double igm1;        // 1/(gama-1)
+  double force[3];    // c_j of the constant body force (zero without one)
+  int has_force;
 };
 
 // 10^n for an integer n by repeated products (exact for |n| <= 22)

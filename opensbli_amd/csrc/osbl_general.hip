@@ -239,6 +239,34 @@ __device__ __forceinline__ void central_gen(const double* __restrict__ Q, const 
 
 // Residual + RK update of one interior cell (modes as k_update: 0 3-register
 // RK3, 1 low-storage, 2 residual only into R).
+// the constant body force: - KD(_i,_j)*c_j and - Dot(c_j, u_j)
+// (apps/channel_flow/compressible_TCF_TENO/turbulent_channel.py:15-16)
+template <int ND>
+__device__ __forceinline__ void add_body_force(const double* __restrict__ Q, long long fs, int x, const Phys& ph,
+                                               double* r) {
+  double cu = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    r[1 + m] -= ph.force[m];
+    cu += ph.force[m] * (Q[(1 + m) * fs + x] / Q[x]);
+  }
+  r[ND + 1] -= cu;
+}
+
+// R += the body force over the interior (the tiled general schedule)
+__global__ void __launch_bounds__(256) k_body_force3d(const double* __restrict__ Q, double* __restrict__ R, Grid g,
+                                                      Phys ph) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n[0]) return;
+  const int x = g.idx(i, blockIdx.y, blockIdx.z);
+  double r[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) r[m] = R[m * g.fs + x];
+  add_body_force<3>(Q, g.fs, x, ph, r);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) R[m * g.fs + x] = r[m];
+}
+
 template <int ND, int CONV, bool VISC>
 __global__ void __launch_bounds__(128) k_update_gen(const double* __restrict__ Q, const double* __restrict__ P,
                                                     double* __restrict__ R, double* __restrict__ Qo,
@@ -259,6 +287,7 @@ __global__ void __launch_bounds__(128) k_update_gen(const double* __restrict__ Q
     for (int m = 0; m < NV; ++m) r[m] = R[m * g.fs + x];
   }
   if constexpr (VISC) viscous_gen<ND>(P, g.fs, x, st, ci, g, b, ph, suth != 0, r);
+  if (ph.has_force) add_body_force<ND>(Q, g.fs, x, ph, r);
   if (mode == 2) {
 #pragma unroll
     for (int m = 0; m < NV; ++m) R[m * g.fs + x] = r[m];
@@ -657,6 +686,11 @@ void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, c
   if (nd == 3) launch_gen_nd<3>(conv, visc, Q, P, R, Qo, Qn, g, ph, b, cA, cB, first, mode, suth, s);
   else if (nd == 2) launch_gen_nd<2>(conv, visc, Q, P, R, Qo, Qn, g, ph, b, cA, cB, first, mode, suth, s);
   else launch_gen_nd<1>(conv, visc, Q, P, R, Qo, Qn, g, ph, b, cA, cB, first, mode, suth, s);
+}
+
+void launch_body_force3d(const double* Q, double* R, const Grid& g, const Phys& ph, hipStream_t s) {
+  dim3 grid((g.n[0] + 255) / 256, g.n[1], g.n[2]);
+  k_body_force3d<<<grid, 256, 0, s>>>(Q, R, g, ph);
 }
 
 void launch_prims_slab(const double* Q, double* P, const Grid& g, const Phys& ph, int suth, hipStream_t s) {

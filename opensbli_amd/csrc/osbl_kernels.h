@@ -22,6 +22,7 @@ void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, c
                        int first, int suth, hipStream_t s);
 void launch_visc_gen3d(const double* Q, const double* P, double* R, const Grid& g, const Phys& ph, const Bnd& b,
                        int suth, int sz, hipStream_t s);
+void launch_body_force3d(const double* Q, double* R, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_prims_slab(const double* Q, double* P, const Grid& g, const Phys& ph, int suth, hipStream_t s);
 void launch_prims_mu(int nd, const double* Q, double* P, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_bc_face(int nd, double* Q, double* Q2, const Grid& g, const Phys& ph, int d, int side, int type, int nh,

@@ -378,6 +378,7 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
     });
     timed(c, F_UPDATE, [&] {
       launch_visc_gen3d(Q, c->P, c->R, c->g, c->ph, c->bnd, c->suth ? 1 : 0, c->vsz, c->stream);
+      if (c->ph.has_force) launch_body_force3d(Q, c->R, c->g, c->ph, c->stream);
     });
     timed(c, F_SWEEP_Y, [&] {
       launch_sweep(c->nd, 1, conv, c->cfg.averaging, Q, c->R, c->g, c->ph, c->seg, c->bnd.dm[1], c->Theta, true,
@@ -455,6 +456,18 @@ int check_config(const osbl_config* cfg) {
                               "central stencils on both faces", d);
   }
   if (cfg->gama <= 1.0) return fail(OSBL_E_ARG, "gama must exceed 1");
+  {
+    // the body force lives in the general update kernels (blocks with
+    // closures, a metric or Sutherland viscosity: the channel flows)
+    bool force = false, general = cfg->viscous == OSBL_VISC_SUTHERLAND;
+    for (int d = 0; d < cfg->ndim; ++d) {
+      force |= cfg->body_force[d] != 0.0;
+      general |= cfg->stretch[d] != 0 || modifies_stencil(cfg->bc[d][0]) || modifies_stencil(cfg->bc[d][1]);
+    }
+    if (force && (!general || cfg->curvilinear))
+      return fail(OSBL_E_ARG, "the body force is implemented on blocks with closures, a metric or Sutherland "
+                              "viscosity");
+  }
   if (cfg->curvilinear) {
     if (cfg->ndim < 2) return fail(OSBL_E_ARG, "a curvilinear metric needs a 2-D or 3-D block");
     if (is_central(cfg->conv))
@@ -567,6 +580,10 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   ph.teno_a1 = cfg->teno_a1;
   ph.teno_a2 = cfg->teno_a2;
   ph.sensor_eps = cfg->sensor_eps;
+  for (int d = 0; d < 3; ++d) {
+    ph.force[d] = d < c->nd ? cfg->body_force[d] : 0.0;
+    ph.has_force |= ph.force[d] != 0.0;
+  }
   c->nstages = rk_coefficients(cfg->rk, c->A, c->B);
   c->suth = cfg->viscous == OSBL_VISC_SUTHERLAND;
   bool closures = false, stretched = false;

@@ -71,6 +71,7 @@ class OsblConfig(ctypes.Structure):
         ("chi_filt", ctypes.c_double),
         ("omega_filt", ctypes.c_double),
         ("shock_filter_control", ctypes.c_double),
+        ("body_force", ctypes.c_double * 3),
     ]
 
 
@@ -207,6 +208,8 @@ def make_config(p):
     c.wf_sensor = WF_SENSOR[w.get("sensor", "ducros")]
     c.wf_mach = 1 if w.get("mach_correction") else 0
     c.shock_filter_control = float(w.get("control", 1.0))
+    for d in range(3):
+        c.body_force[d] = float(p.get("body_force", [0.0, 0.0, 0.0])[d]) if d < nd else 0.0
     return c
 
 

@@ -281,3 +281,34 @@ def euler_wave_curvilinear_3d(n, conv="weno_z", averaging="roe", rk="ls3", gama=
     p = 1.0 + 0.1 * np.cos(np.pi * (c[0] - c[2]))
     q = np.stack([rho] + [rho * ui for ui in u] + [p / (gama - 1) + 0.5 * rho * sum(ui * ui for ui in u)])
     return problem, q
+
+
+def channel(n=(24, 33, 12), conv="teno5", c=(-1.0, 0.0, 0.0), Re=190.71, Minf=0.0955, Pr=0.7, dt=2e-4, by=1.7,
+            seed=3):
+    """Compressible channel driven by a body force, the structure of the
+    reference's TENO channel (apps/channel_flow/compressible_TCF_TENO):
+    conservative Navier-Stokes with a characteristic scheme, Sutherland
+    viscosity, the body force c_j, periodic x0 (and x2), isothermal walls at
+    x1 = -1 and 1 on a wall-clustered grid (tanh map from both walls),
+    T_wall = 1.  Initial state: a laminar-like profile u0 = 1.5 (1 - x1^2)
+    plus seeded noise of amplitude 0.02, rho = 1, T = 1.  Returns (problem, q0)."""
+    nd = len(n)
+    d = [4.0 * math.pi / n[0], 2.0 / (n[1] - 1)] + ([4.0 * math.pi / 3.0 / n[2]] if nd == 3 else [])
+    eta = np.arange(n[1]) / (n[1] - 1.0)
+    y = -np.tanh(by * (1.0 - 2.0 * eta)) / np.tanh(by)
+    bc = [["periodic", "periodic"], ["isothermal_wall", "isothermal_wall"]] + ([["periodic", "periodic"]] if nd == 3
+                                                                              else [])
+    problem = dict(ndim=nd, n=list(n), delta=d, conv=conv, averaging="roe", viscous="sutherland", rk="ls3", bc=bc,
+                   bscheme=[["carpenter", "carpenter"]] * nd, gama=1.4, Minf=Minf, Re=Re, Pr=Pr, dt=dt, eps=1e-15,
+                   teno_ct=1e-7, SuthT=110.4, RefT=273.0, Twall=1.0, stretch=[0, 1, 0][:nd] + [0] * (3 - nd),
+                   coords={1: y}, body_force=list(c[:nd]))
+    shape = tuple(reversed(n))
+    rs = np.random.RandomState(seed)
+    Y = y.reshape((1,) * (nd - 2) + (n[1], 1)) * np.ones(shape)
+    u0 = 1.5 * (1.0 - Y ** 2) + 0.02 * rs.standard_normal(shape) * (1.0 - Y ** 2)
+    u1 = 0.02 * rs.standard_normal(shape) * (1.0 - Y ** 2)
+    vel = [u0, u1] + ([0.02 * rs.standard_normal(shape) * (1.0 - Y ** 2)] if nd == 3 else [])
+    rho = np.ones(shape)
+    p = 1.0 / (1.4 * Minf ** 2)
+    q = np.stack([rho] + [rho * v for v in vel] + [p / 0.4 + 0.5 * rho * sum(v * v for v in vel)])
+    return problem, q

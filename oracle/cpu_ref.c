@@ -1237,6 +1237,21 @@ static void evaluate_rhs(ref_state* s) {
     default: shock_capturing(s, 0); break;
   }
   if (s->c.viscous != REF_VISC_NONE) viscous(s);
+  const double* c = s->c.body_force;
+  if (c[0] != 0.0 || c[1] != 0.0 || c[2] != 0.0) {
+    /* - KD(_i,_j)*c_j and - Dot(c_j, u_j) (apps/channel_flow/compressible_TCF_TENO/turbulent_channel.py:15-16) */
+    const int nd = s->nd;
+    range3 I = rng_all(s, 0);
+    LOOP(I) {
+      ptrdiff_t x = IX(s, i, j, k);
+      double cu = 0.0;
+      for (int m = 0; m < nd; ++m) {
+        s->res[1 + m][x] -= c[m];
+        cu += c[m] * s->u[m][x];
+      }
+      s->res[nd + 1][x] -= cu;
+    }
+  }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1681,13 +1696,7 @@ int ref_set_bc_plane(ref_state* s, int d, int side, const double* plane) {
 
 int ref_rhs(ref_state* s, double* residual) {
   apply_bcs(s);
-  constituent_relations(s);
-  switch (s->c.conv) {
-    case REF_CONV_CENTRAL_SKEW: central_skew(s); break;
-    case REF_CONV_CENTRAL_CONS: central_conservative(s); break;
-    default: shock_capturing(s, 0); break;
-  }
-  if (s->c.viscous != REF_VISC_NONE) viscous(s);
+  evaluate_rhs(s);
   ptrdiff_t c = 0;
   for (int m = 0; m < s->nvar; ++m)
     for (int k = 0; k < s->n[2]; ++k)

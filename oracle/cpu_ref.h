@@ -72,6 +72,7 @@ typedef struct ref_config {
   double sigma_filt;       /* binomial blending weight sigma_filt */
   double chi_filt, omega_filt;   /* SFD damping coefficient and cut frequency */
   double shock_filter_control;   /* global scale of the WENO filter */
+  double body_force[3];          /* c_j: -c_i in the momentum, -c_j u_j in the energy (apps/channel_flow) */
 } ref_config;
 
 typedef struct ref_state ref_state;

@@ -73,6 +73,7 @@ class RefConfig(ctypes.Structure):
         ("chi_filt", ctypes.c_double),
         ("omega_filt", ctypes.c_double),
         ("shock_filter_control", ctypes.c_double),
+        ("body_force", ctypes.c_double * 3),
     ]
 
 
@@ -165,6 +166,8 @@ def make_config(p):
     c.sensor_eps = p.get("sensor_eps", 1.0e-30)
     (c.filters, c.binomial_order, c.wf_sensor, c.wf_mach, c.sigma_filt, c.chi_filt, c.omega_filt,
      c.shock_filter_control) = filter_fields(p)
+    for d in range(3):
+        c.body_force[d] = float(p.get("body_force", [0.0, 0.0, 0.0])[d]) if d < p["ndim"] else 0.0
     return c
 
 

@@ -106,3 +106,14 @@ def test_curvilinear_config_rejected(change, msg):
     problem.update(change)
     with pytest.raises(native.OsblError, match=msg):
         native.Block(problem)
+
+
+def test_body_force_needs_the_general_kernels():
+    """The body force is refused on blocks without closures, metric or
+    Sutherland viscosity (checked before any device call)."""
+    from opensbli_amd import native
+    from opensbli_amd.problems import taylor_green
+    problem, _ = taylor_green(16, conv="weno_z")
+    problem["body_force"] = [-1.0, 0.0, 0.0]
+    with pytest.raises(native.OsblError, match="body force"):
+        native.Block(problem)

@@ -462,3 +462,29 @@ def test_katzer_sbli_skin_friction_and_wall_pressure_match_reference_data():
     # measured: 1.8 % and 0.08 %, separation [140.3, 287.8] against [138.7, 287.8]
     assert cf_err < 0.04 and pw_err < 0.005, (cf_err, pw_err)
     assert sep.size and abs(sep.min() - sep_ref.min()) < 5.0 and abs(sep.max() - sep_ref.max()) < 5.0
+
+
+@pytest.mark.parametrize("n,steps", [((24, 33, 12), 50), ((32, 33), 100)])
+def test_channel_with_body_force(n, steps, oracle):
+    """The TENO channel structure (apps/channel_flow/compressible_TCF_TENO):
+    body force, Sutherland viscosity, isothermal walls on a wall-clustered
+    grid, periodic x; 3-D through the tiled general schedule (the force as its
+    own pass), 2-D through the per-point general kernel; against the oracle."""
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import channel
+    _say(f"case channel with body force {n}, {steps} steps")
+    problem, q0 = channel(n)
+    g = Block(problem)
+    g.set_state(q0)
+    g.run(steps)
+    r = oracle.RefSolver(problem)
+    r.set_state(q0)
+    r.run(steps)
+    f = oracle.RefSolver(problem, variant="fma")
+    f.set_state(q0)
+    f.run(steps)
+    rs = r.get_state()
+    err = rel_l2(g.get_state(), rs)
+    spread = rel_l2(f.get_state(), rs)
+    _say("rel L2", err, "oracle self-spread", spread)
+    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err

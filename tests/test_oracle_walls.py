@@ -322,3 +322,24 @@ def test_symmetry_plane_is_the_mirrored_domain(oracle, conv):
     wall.set_state(np.ascontiguousarray(q[:, :ny]))
     wall.run(20)
     assert np.max(rel_l2(wall.get_state(), full.get_state()[:, :ny])) > 1e-6
+
+
+def test_body_force_adds_its_terms_pointwise(oracle):
+    """The body force -KD(_i,_j)*c_j, -Dot(c_j, u_j) (apps/channel_flow) changes
+    the right-hand side at every interior point by exactly (0, -c, -c.u)."""
+    from opensbli_amd.problems import channel
+    for n in ((16, 17), (12, 17, 8)):
+        problem, q = channel(n)
+        nd = len(n)
+        base = dict(problem, body_force=[0.0] * nd)
+        a = oracle.RefSolver(problem)
+        a.set_state(q)
+        b = oracle.RefSolver(base)
+        b.set_state(q)
+        dR = a.rhs() - b.rhs()
+        c = problem["body_force"]
+        u = [q[1 + m] / q[0] for m in range(nd)]
+        assert np.abs(dR[0]).max() == 0.0
+        for m in range(nd):
+            assert np.allclose(dR[1 + m], -c[m], rtol=0, atol=1e-12)
+        assert np.allclose(dR[nd + 1], -sum(c[m] * u[m] for m in range(nd)), rtol=0, atol=1e-12)
