@@ -8,10 +8,19 @@ RK3 iteration (three stages).  The state is resident in HBM before timing.
 
 Prints one JSON line:
   value = Mcell-updates/s per RK stage = cells * stages * steps / t / 1e6
-  roofline = the dominant kernel's algorithmic bytes per launch / its mean
-             launch time (HIP events on the library's stream) vs 8 TB/s
+  roofline = the dominant kernel against its binding roofline, FP64 issue:
+             its FP64 operations per launch (counted from the kernel
+             formulas) / its mean launch time (HIP events on the library's
+             stream) vs 78.6 TFLOP/s; roofline_hbm gives the same kernel's
+             algorithmic bytes per launch against 8 TB/s
+  ensemble = the batched-ensemble speed-up (512 uneven 1-D WENO5-Z shock
+             tubes: one persistent launch vs per-instance launches)
+  flat_plate = the 480x256x128 flat plate on the tanh (factor 5) wall-normal
+             mesh, Mcell-updates/s per RK stage
+  adjoint = backward/forward time of the checkpointed 512^2 Kelvin-Helmholtz run
   cpu_baseline = the oracle (C restatement of the generated OPS kernel
-             sequence) on the host's cores, one step of the same workload.
+             sequence) on the host's cores, one warm-up step then the median
+             of three timed steps of the same workload.
 """
 import argparse
 import json
@@ -51,24 +60,56 @@ def measured_traffic(family, n, conv):
     return None if fam is None else fam.get("bytes_per_launch")
 
 
+def host_cpu():
+    """Model, sockets, cores and the cores this process may use (lscpu)."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core"}
+    for line in txt.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            info[keys[k.strip()]] = v.strip()
+    return info
+
+
+def _median_timed(solver, steps, reps=3):
+    """One untimed warm-up run of `steps` (first touch of the work arrays),
+    then the median wall time of `reps` timed runs."""
+    solver.run(steps)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        solver.run(steps)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
 def cpu_baseline(n, conv, steps=1):
-    """Time the oracle on this host, the same problem, `steps` RK3 steps."""
+    """Time the oracle on this host, the same problem: a warm-up of `steps`
+    RK3 steps, then the median of three timed runs of `steps` steps."""
     from opensbli_amd.problems import taylor_green
     from oracle.ref import RefSolver
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     problem, q0 = taylor_green(n, conv=conv)
     s = RefSolver(problem)
     s.set_state(q0)
-    t0 = time.perf_counter()
-    s.run(steps)
-    dt = time.perf_counter() - t0
+    dt, ts = _median_timed(s, steps)
     return {
         "value": n ** 3 * 3 * steps / dt / 1e6,
         "unit": "Mcell-updates/s per RK stage",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} RK3 step(s) of the same {n}^3 TGV {conv} problem "
-                  f"({dt:.1f} s, OMP_NUM_THREADS={threads})",
+        "sample": f"{steps} RK3 step(s) of the same {n}^3 TGV {conv} problem, median of 3 after 1 warm-up "
+                  f"({', '.join('%.2f' % t for t in ts)} s; OMP_NUM_THREADS={threads})",
+        "host": host_cpu(),
     }
 
 
@@ -84,6 +125,7 @@ def main():
     ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble", "flat_plate", "adjoint",
                                                            "curvilinear"])
     ap.add_argument("--no-adjoint", action="store_true", help="tgv: skip the adjoint/forward ratio measurement")
+    ap.add_argument("--no-sub", action="store_true", help="tgv: skip the ensemble and flat-plate sub-results")
     ap.add_argument("--plain-stride", type=int, default=16,
                     help="ensemble: time the per-instance baseline on every k-th instance and scale by work")
     args = ap.parse_args()
@@ -141,23 +183,30 @@ def main():
     tot_ms, launches = stats[fam]
     avg_s = tot_ms / 1e3 / max(launches, 1)
     kb, kf = work[fam]
+    traffic = measured_traffic(fam, n, args.conv)
+    # the characteristic sweeps do ~20 FP64 operations per algorithmic byte
+    # against a machine balance of 78.6e12 / 8e12 ~ 9.8: FP64 issue binds
     roofline = {
+        "kernel": fam,
+        "bound": "fp64",
+        "achieved": kf / avg_s / 1e12,
+        "peak": FP64_PEAK_TFS,
+        "unit": "TFLOP/s",
+        "frac": kf / avg_s / 1e12 / FP64_PEAK_TFS,
+        "traffic": traffic,
+        "flops_per_launch": kf,
+        "avg_launch_ms": avg_s * 1e3,
+        "launches": launches,
+    }
+    roofline_hbm = {
         "kernel": fam,
         "bound": "hbm",
         "achieved": kb / avg_s / 1e9,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": kb / avg_s / 1e9 / HBM_PEAK_GBS,
-        "traffic": measured_traffic(fam, n, args.conv),
-        "avg_launch_ms": avg_s * 1e3,
-        "launches": launches,
-    }
-    roofline_fp64 = {
-        "kernel": fam,
-        "achieved": kf / avg_s / 1e12,
-        "peak": FP64_PEAK_TFS,
-        "unit": "TFLOP/s",
-        "frac": kf / avg_s / 1e12 / FP64_PEAK_TFS,
+        "traffic": traffic,
+        "bytes_per_launch": kb,
     }
     stage_roofline = {
         "hbm_frac": stage_bytes * stage_evals / elapsed / 1e9 / HBM_PEAK_GBS,
@@ -174,6 +223,13 @@ def main():
         adj = adjoint_ratio()
         adj["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
         adj["central_viscous"] = adjoint_ratio(viscous=True, reps=1)
+
+    ens = fp = None
+    if not args.no_sub:
+        _log("bench: batched ensemble (512 uneven 1-D WENO5-Z shock tubes, seed 3)")
+        ens = ensemble_result(args.plain_stride)
+        _log("bench: flat plate 480x256x128 (tanh wall-normal mesh, factor 5)")
+        fp = flat_plate_result(args.steps, args.warmup)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -196,9 +252,11 @@ def main():
         "config": {"workload": f"taylor_green_vortex_{n}^3_{args.conv}_roe_viscous_rk3", "grid": [n, n, n],
                    "scheme": args.conv, "rk_stages": stages, "Re": 1600.0, "Minf": 0.1},
         "roofline": roofline,
-        "roofline_fp64": roofline_fp64,
+        "roofline_hbm": roofline_hbm,
         "stage_roofline": stage_roofline,
         "kernels": kernels,
+        "ensemble": ens,
+        "flat_plate": fp,
         "adjoint": adj,
         "cpu_baseline": cpu,
     }
@@ -242,42 +300,49 @@ def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew", visco
             "forward_ms": f * 1e3, "backward_ms": w * 1e3, "ratio": w / f}
 
 
-def bench_flat_plate(args):
-    """The 480x256x128 flat plate (WENO5-Z + Sutherland viscous + metric,
-    RungeKuttaLS(3)): Mcell-updates/s per RK stage and per-kernel times."""
+def flat_plate_result(steps, warmup):
+    """The 480x256x128 flat plate (WENO5-Z + Sutherland viscous terms + the
+    metric of the tanh-stretched wall-normal direction, RungeKuttaLS(3)):
+    Mcell-updates/s per RK stage over `steps` steps and per-kernel times."""
     from opensbli_amd.native import Block
     from opensbli_amd.problems import flat_plate
     problem, q0 = flat_plate()
     blk = Block(problem)
     blk.set_state(q0)
+    del q0
     stages = blk.num_stages()
-    blk.run(args.warmup)
+    blk.run(warmup)
     blk.sync()
     blk.profile(True)
     blk.sync()
     t0 = time.perf_counter()
-    blk.run(args.steps)
+    blk.run(steps)
     blk.sync()
     t1 = time.perf_counter()
     stats = blk.kernel_stats()
     blk.profile(False)
+    finite = bool(np.isfinite(blk.get_state()[0]).all())
     blk.close()
-    cells = 480 * 256 * 128
+    n = problem["n"]
+    cells = int(np.prod(n))
     el = t1 - t0
-    out = {
+    return {
         "metric": "Mcell-updates/s per RK stage (FP64, 480x256x128 flat plate)",
-        "value": cells * stages * args.steps / el / 1e6,
+        "value": cells * stages * steps / el / 1e6,
         "unit": "Mcell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / args.steps,
-        "higher_is_better": True,
-        "dtype": "f64",
+        "steps": steps,
+        "ms_per_step": el * 1e3 / steps,
+        "state_finite": finite,
         "data": "synthetic (similarity boundary-layer profile + N(0,1e-4) velocity noise, seed 5)",
-        "config": {"workload": "flat_plate_480x256x128_weno_z_sutherland_metric_ls3", "grid": [480, 256, 128]},
+        "config": {"workload": f"flat_plate_{n[0]}x{n[1]}x{n[2]}_weno_z_sutherland_{problem['stretch_kind']}"
+                               f"{problem['stretch_factor']:g}_metric_ls3", "grid": n, "dt": problem["dt"]},
         "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in stats.items() if v[1]},
     }
+
+
+def bench_flat_plate(args):
+    out = flat_plate_result(args.steps, args.warmup)
+    out.update(n_gpus=1, warmup=args.warmup, higher_is_better=True, dtype="f64")
     print(json.dumps(out), flush=True)
 
 
@@ -318,10 +383,11 @@ def bench_curvilinear(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_ensemble(args):
+def ensemble_result(plain_stride=16):
     """512 uneven 1-D shock tubes (WENO5-Z, RK3, CFL 0.5, t = 0.2): one
-    persistent launch vs a graph-staged launch sequence vs per-instance runs."""
-    import numpy as np
+    persistent launch vs a graph-staged launch sequence vs per-instance runs
+    (timed on every plain_stride-th instance and scaled by its share of the
+    cell updates)."""
     import torch
     from opensbli_amd.ensemble import Ensemble, autotune, ensemble_parameters, plain_runs
 
@@ -342,18 +408,15 @@ def bench_ensemble(args):
     ms_s = e.run("staged")
     cu = e.cell_updates()
     e.close()
-    idx = list(range(0, 512, args.plain_stride))
+    idx = list(range(0, 512, plain_stride))
     t_plain_sample = plain_runs(par, indices=idx)
     work = par["n"].astype(float) * par["nsteps"]
     frac = work[idx].sum() / work.sum()
     t_plain = t_plain_sample / frac
-    out = {
+    return {
         "metric": "batched ensemble speed-up over per-instance launches (512 1-D WENO5-Z shock tubes)",
         "value": t_plain / (ms_p / 1e3),
         "unit": "x",
-        "n_gpus": 1,
-        "higher_is_better": True,
-        "dtype": "f64",
         "data": "synthetic (seeded shock tubes, numpy RandomState(3))",
         "config": {"workload": "ensemble_512_shock_tubes_weno_z_rk3", "instances": 512,
                    "n_range": [int(par["n"].min()), int(par["n"].max())], "t_end": 0.2, "cfl": 0.5},
@@ -361,13 +424,18 @@ def bench_ensemble(args):
         "persistent_wall_s": t_p,
         "staged_graph_ms": ms_s,
         "plain_s_estimate": t_plain,
-        "plain_sample": f"every {args.plain_stride}th instance ({len(idx)} runs, {frac:.3f} of the work) "
+        "plain_sample": f"every {plain_stride}th instance ({len(idx)} runs, {frac:.3f} of the work) "
                         f"took {t_plain_sample:.2f} s",
         "cell_updates": cu,
         "mcell_updates_per_s_persistent": cu / ms_p / 1e3,
         "mcell_updates_per_s_staged": cu / ms_s / 1e3,
         "autotune": {"max_n": max_n, "threads": threads, "seconds": tune_s},
     }
+
+
+def bench_ensemble(args):
+    out = ensemble_result(args.plain_stride)
+    out.update(n_gpus=1, higher_is_better=True, dtype="f64")
     print(json.dumps(out), flush=True)
 
 

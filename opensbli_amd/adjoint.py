@@ -9,7 +9,8 @@ Forward: the block is advanced with the GPU kernels; the state is kept every
 `every` steps.  Backward: for each segment, from last to first, the segment
 is recomputed from its checkpoint while the input states of every RK stage
 are stored, then the adjoint of each step (osbl_adjoint_step_dev) is applied
-from the last step back.  Covered: the central skew-symmetric scheme and the
+from the last step back; the block is then left at the forward output.
+Covered: the central skew-symmetric scheme and the
 LLF characteristic WENO5-Z scheme, with or without constant-viscosity terms,
 on periodic blocks with RungeKutta(3) (TENO's cut-off is not differentiable).
 """
@@ -66,6 +67,12 @@ class _Integrate(torch.autograd.Function):
             for s in range(k - 1, -1, -1):
                 block.adjoint_step_dev(stages[s].data_ptr(), lam.data_ptr())
             block.sync()
+        # the recomputation moved the block's state: leave it at the forward
+        # output, where the forward pass left it, so a caller can go on
+        # integrating the same block
+        qN = ctx.checkpoints[-1]
+        block.set_state_dev([qN[m].data_ptr() for m in range(nv)])
+        block.sync()
         return lam, None, None, None
 
 

@@ -4,8 +4,10 @@ The structure of the reference's SBLI scripts (katzer_SBLI.py,
 transitional_SBLI.py): conservative Navier-Stokes with a characteristic
 shock-capturing scheme, Sutherland viscosity as a constituent relation,
 StoreSome(4) viscous terms, a wall-normal stretched grid through
-MetricsEquation, InletPressureExtrapolateBC / ExtrapolationBC in x0, a no-slip
-adiabatic wall and a free-stream Dirichlet top in x1 (ReducedAccess closures),
+MetricsEquation (tanh map with factor 5 by default, the reference apps' sinh
+map as an option), InletPressureExtrapolateBC / ExtrapolationBC in x0, a
+no-slip adiabatic wall and a free-stream Dirichlet top in x1 (ReducedAccess
+closures),
 periodic x2, the Initialise_Katzer similarity profile, RungeKuttaLS(3).
 With OPSC(alg) replaced by HIPC(alg).
 """
@@ -17,10 +19,19 @@ from opensbli_amd.api import (AdiabaticWallBC, Central, ConstituentRelations, Co
                               substitute_simulation_parameters)
 
 
-def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025", scheme="Z", top=None):
+def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt=None, scheme="Z", top=None, stretch="tanh",
+          by=5.0):
     """scheme: "Z" or "JS" (LLFWeno), or "teno5"/"teno6" (adaptive LLFTeno with
-    the Ducros sensor, as katzer_SBLI.py:55-62)."""
+    the Ducros sensor, as katzer_SBLI.py:55-62).  stretch: the wall-normal
+    map x1(j), "tanh" Lx1 (1 - tanh(by (1 - eta))/tanh(by)) (the one-sided
+    form of the channel apps' map, turbulent_channel.py:154) or "sinh"
+    Lx1 sinh(by eta)/sinh(by) (katzer_SBLI.py:128), eta = j/(block0np1-1),
+    stretching factor by.  dt: a string for substitute_simulation_parameters;
+    None takes the stable step of the similarity profile on this grid."""
     ndim = len(n)
+    if dt is None:
+        from opensbli_amd.problems import flat_plate_dt
+        dt = repr(flat_plate_dt(n=n, lengths=lengths, stretch_factor=by, stretch=stretch))
     teno = scheme.startswith("teno")
     sc1 = "**{'scheme':'%s'}" % ("Teno" if teno else "Weno")
     mass = "Eq(Der(rho,t), - Conservative(rhou_j,x_j,%s))" % sc1
@@ -92,8 +103,14 @@ def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025",
     for con in ("Lx1", "by"):
         local_dict[con] = ConstantObject(con)
     gridx0 = parse_expr("Eq(DataObject(x0), block.deltas[0]*block.grid_indexes[0])", local_dict=local_dict)
-    gridx1 = parse_expr("Eq(DataObject(x1), Lx1*sinh(by*block.deltas[1]*block.grid_indexes[1]/Lx1)/sinh(by))",
-                        local_dict=local_dict)
+    if stretch == "sinh":
+        gridx1 = parse_expr("Eq(DataObject(x1), Lx1*sinh(by*block.deltas[1]*block.grid_indexes[1]/Lx1)/sinh(by))",
+                            local_dict=local_dict)
+    elif stretch == "tanh":
+        gridx1 = parse_expr("Eq(DataObject(x1), Lx1*(1.0 - tanh(by*(1.0 - block.deltas[1]*block.grid_indexes[1]/Lx1))"
+                            "/tanh(by)))", local_dict=local_dict)
+    else:
+        raise ValueError("unknown wall-normal stretching %r" % stretch)
     coordinate_evaluation = [gridx0, gridx1]
     polynomial_directions = [(False, DataObject("x0")), (True, DataObject("x1"))]
     if ndim == 3:
@@ -112,7 +129,7 @@ def build(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), niter=50, dt="0.025",
              "SuthT", "RefT", "eps", "Lx1", "by", "teno_a1", "teno_a2", "epsilon", "TENO_CT"]
     vals = ["1.4", "2.0", "0.72", "950.0", dt, str(niter), str(n[0]), str(n[1]),
             "%r/(block0np0-1)" % lengths[0], "%r/(block0np1-1)" % lengths[1], "110.4", "288.0", "1e-15",
-            repr(lengths[1]), "5.0", "10.5", "4.5", "1.0e-30", "1e-6"]
+            repr(lengths[1]), repr(float(by)), "10.5", "4.5", "1.0e-30", "1e-6"]
     if ndim == 3:
         names += ["block0np2", "Delta2block0"]
         vals += [str(n[2]), "%r/block0np2" % lengths[2]]

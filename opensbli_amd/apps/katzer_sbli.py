@@ -7,7 +7,8 @@ stretched normal to the wall (MetricsEquation), InletPressureExtrapolateBC /
 zeroth-order extrapolation in x0, an isothermal wall (Twall = 1.67619431) and
 on top a Dirichlet shock generator (the free stream for x0 <= 40, the state
 behind an oblique shock beyond), ReducedAccess closures, the Initialise_Katzer
-similarity profile, RungeKuttaLS(3).  The reference runs 500 x 250 points,
+similarity profile, RungeKuttaLS(3, formulation='SSP') (katzer_SBLI.py:72).
+The reference runs 500 x 250 points,
 dt = 0.04, 250000 iterations, and compares the skin friction and the wall
 pressure with reference_data.txt (apps/katzer_SBLI/plot.py).
 """
@@ -50,7 +51,7 @@ def build(n=(500, 250), niter=250000, dt="0.04", output=True):
     cr.add_equations(shock_sensor)
     llf = LLFTeno(5, formulation="adaptive", averaging=RoeAverage([0, 1]), sensor=sensor_array, store_sensor=True)
     cent = StoreSome(4, "u0 u1 u2 T")
-    rk = RungeKuttaLS(3)
+    rk = RungeKuttaLS(3, formulation="SSP")
     block.set_discretisation_schemes({llf.name: llf, cent.name: cent, rk.name: rk})
 
     env = {"block": block, "GridVariable": GridVariable, "DataObject": DataObject}

@@ -12,8 +12,9 @@
 //    through all its steps and stages, with block barriers between phases.
 //    Threads per workgroup come from a per-size table the autotuner fills.
 //  * staged: one launch per RK stage covering every instance still running;
-//    the stages of a chunk of steps are captured once into a hipGraph and
-//    replayed.
+//    the stages of a chunk of steps are captured once into a hipGraph (two,
+//    when a chunk ends on the other ping-pong buffer) and replayed for every
+//    chunk, the step number coming from a device counter.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -141,15 +142,21 @@ __global__ void __launch_bounds__(512) k_batch_persistent(const Inst* __restrict
   if (t < 2) extrap_ends(QA, fs, in, t);
 }
 
-// staged mode: one thread per (instance, segment); inactive instances skip
+// Staged mode.  The step a launch belongs to is *step0 + rel: step0 is a
+// device counter the chunk's last kernel advances, so one captured chunk of
+// steps serves every chunk of the run.
+//
+// one thread per (instance, segment); inactive instances skip
 template <int CONV, int AVG, int MODE>
 __global__ void __launch_bounds__(256) k_batch_stage(const Inst* __restrict__ insts, const int* __restrict__ seg_inst,
                                                       const int* __restrict__ seg_c0, int nseg, const double* __restrict__ Q,
                                                       double* __restrict__ Qo, double* __restrict__ Qn, long long fs,
-                                                      Phys ph, double cA, double cB, int first, int step, int seglen) {
+                                                      Phys ph, double cA, double cB, int first,
+                                                      const int* __restrict__ step0, int rel, int seglen) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nseg) return;
   const Inst in = insts[seg_inst[t]];
+  const int step = *step0 + rel;
   if (step >= in.nsteps) return;
   const int c0 = seg_c0[t];
   const int c1 = min(c0 + seglen, in.n);
@@ -157,24 +164,33 @@ __global__ void __launch_bounds__(256) k_batch_stage(const Inst* __restrict__ in
 }
 
 __global__ void __launch_bounds__(256) k_batch_bc(const Inst* __restrict__ insts, int ninst, double* __restrict__ Q,
-                                                   long long fs, int step) {
+                                                   long long fs, const int* __restrict__ step0, int rel) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 2 * ninst) return;
   const Inst in = insts[t >> 1];
-  if (step > in.nsteps) return;
+  if (*step0 + rel > in.nsteps) return;
   extrap_ends(Q, fs, in, t & 1);
 }
 
 // inactive instances keep their state: copy Q -> Qn so the ping-pong stays consistent
 __global__ void __launch_bounds__(256) k_batch_carry(const Inst* __restrict__ insts, const int* __restrict__ pt_inst,
                                                       const int* __restrict__ pt_i, int npts, const double* __restrict__ Q,
-                                                      double* __restrict__ Qn, long long fs, int step) {
+                                                      double* __restrict__ Qn, long long fs, const int* __restrict__ step0,
+                                                      int rel) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npts) return;
   const Inst in = insts[pt_inst[t]];
-  if (step < in.nsteps) return;
+  if (*step0 + rel < in.nsteps) return;
   const int x = in.off + pt_i[t];
   for (int m = 0; m < 3; ++m) Qn[m * fs + x] = Q[m * fs + x];
+}
+
+// the staged mode's step counter: set before a run, advanced after each chunk
+__global__ void k_batch_step_set(int* __restrict__ step0, int value) {
+  if (threadIdx.x == 0) step0[0] = value;
+}
+__global__ void k_batch_step_add(int* __restrict__ step0, int add) {
+  if (threadIdx.x == 0) step0[0] += add;
 }
 
 int bfail(int code, const std::string& msg) { return osbl::set_error(code, msg.c_str()); }
@@ -211,6 +227,9 @@ struct osbl_batch {
   int* d_pt_i = nullptr;
   int npts = 0;
   int graph_steps = 8;
+  int* d_step0 = nullptr;
+  // captured chunk of graph_steps steps, by the buffer it starts from (0: QA, 1: QB)
+  hipGraphExec_t chunk[2] = {nullptr, nullptr};
   // per-size thread table: threads for n <= bucket_n[k]
   std::vector<int> bucket_n{256, 512, 1024, 2048, 4096, 1 << 30};
   std::vector<int> bucket_t{64, 64, 128, 256, 256, 512};
@@ -244,16 +263,16 @@ void launch_persistent_t(osbl_batch* b) {
 }
 
 template <int CONV, int AVG>
-void launch_stage_t(osbl_batch* b, const double* Q, double* Qn, int s, int step) {
+void launch_stage_t(osbl_batch* b, const double* Q, double* Qn, int s, int rel) {
   const unsigned blocks = (unsigned)((b->nseg + 255) / 256);
   if (b->mode_rk == 0)
     k_batch_stage<CONV, AVG, 0><<<blocks, 256, 0, b->stream>>>(b->d_inst, b->d_seg_inst, b->d_seg_c0, b->nseg, Q, b->Qo,
-                                                               Qn, b->fs, b->ph, b->rk.A[s], b->rk.B[s], s == 0, step,
-                                                               b->seglen);
+                                                               Qn, b->fs, b->ph, b->rk.A[s], b->rk.B[s], s == 0,
+                                                               b->d_step0, rel, b->seglen);
   else
     k_batch_stage<CONV, AVG, 1><<<blocks, 256, 0, b->stream>>>(b->d_inst, b->d_seg_inst, b->d_seg_c0, b->nseg, Q, b->Qo,
-                                                               Qn, b->fs, b->ph, b->rk.A[s], b->rk.B[s], s == 0, step,
-                                                               b->seglen);
+                                                               Qn, b->fs, b->ph, b->rk.A[s], b->rk.B[s], s == 0,
+                                                               b->d_step0, rel, b->seglen);
 }
 
 template <class F>
@@ -279,39 +298,64 @@ int run_persistent(osbl_batch* b) {
   return OSBL_OK;
 }
 
+void drop_chunks(osbl_batch* b) {
+  for (auto& g : b->chunk)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// capture one chunk of graph_steps steps starting from buffer `start`
+int capture_chunk(osbl_batch* b, int start) {
+  const unsigned bcb = (unsigned)((2 * b->ninst + 255) / 256);
+  const unsigned cpb = (unsigned)((b->npts + 255) / 256);
+  double* Q = start ? b->QB : b->QA;
+  double* Qn = start ? b->QA : b->QB;
+  hipGraph_t graph;
+  BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int rel = 0; rel < b->graph_steps; ++rel) {
+    for (int s = 0; s < b->rk.ns; ++s) {
+      k_batch_bc<<<bcb, 256, 0, b->stream>>>(b->d_inst, b->ninst, Q, b->fs, b->d_step0, rel);
+      dispatch(b->base.conv, b->base.averaging, [&](auto c, auto a) {
+        launch_stage_t<decltype(c)::value, decltype(a)::value>(b, Q, Qn, s, rel);
+      });
+      k_batch_carry<<<cpb, 256, 0, b->stream>>>(b->d_inst, b->d_pt_inst, b->d_pt_i, b->npts, Q, Qn, b->fs,
+                                                b->d_step0, rel);
+      std::swap(Q, Qn);
+    }
+  }
+  k_batch_step_add<<<1, 64, 0, b->stream>>>(b->d_step0, b->graph_steps);
+  BCHK(hipStreamEndCapture(b->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&b->chunk[start], graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    b->chunk[start] = nullptr;
+    return bfail(OSBL_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return OSBL_OK;
+}
+
 int run_staged(osbl_batch* b) {
   int maxsteps = 0;
   for (auto& in : b->h_inst) maxsteps = std::max(maxsteps, in.nsteps);
   const unsigned bcb = (unsigned)((2 * b->ninst + 255) / 256);
-  const unsigned cpb = (unsigned)((b->npts + 255) / 256);
-  double* Q = b->QA;
-  double* Qn = b->QB;
-  // capture `graph_steps` steps at a time; the step index is a kernel argument,
-  // so a graph is built per chunk (updates of the arguments would also do).
+  // every chunk replays the same graph (at most one per starting buffer);
+  // steps past an instance's nsteps leave it unchanged, so the last chunk may
+  // run beyond maxsteps
+  const int swaps = (b->rk.ns * b->graph_steps) & 1;
+  int cur = 0;
+  k_batch_step_set<<<1, 64, 0, b->stream>>>(b->d_step0, 0);
+  BCHK(hipGetLastError());
   for (int step0 = 0; step0 < maxsteps; step0 += b->graph_steps) {
-    const int step1 = std::min(maxsteps, step0 + b->graph_steps);
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int step = step0; step < step1; ++step) {
-      for (int s = 0; s < b->rk.ns; ++s) {
-        k_batch_bc<<<bcb, 256, 0, b->stream>>>(b->d_inst, b->ninst, Q, b->fs, step);
-        dispatch(b->base.conv, b->base.averaging, [&](auto c, auto a) {
-          launch_stage_t<decltype(c)::value, decltype(a)::value>(b, Q, Qn, s, step);
-        });
-        k_batch_carry<<<cpb, 256, 0, b->stream>>>(b->d_inst, b->d_pt_inst, b->d_pt_i, b->npts, Q, Qn, b->fs, step);
-        std::swap(Q, Qn);
-      }
-    }
-    BCHK(hipStreamEndCapture(b->stream, &graph));
-    BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    BCHK(hipGraphLaunch(exec, b->stream));
-    BCHK(hipStreamSynchronize(b->stream));
-    BCHK(hipGraphExecDestroy(exec));
-    BCHK(hipGraphDestroy(graph));
+    if (!b->chunk[cur])
+      if (int rc = capture_chunk(b, cur)) return rc;
+    BCHK(hipGraphLaunch(b->chunk[cur], b->stream));
+    cur ^= swaps;
   }
-  if (Q != b->QA) BCHK(hipMemcpyAsync(b->QA, Q, sizeof(double) * 3 * b->fs, hipMemcpyDeviceToDevice, b->stream));
-  k_batch_bc<<<bcb, 256, 0, b->stream>>>(b->d_inst, b->ninst, b->QA, b->fs, 0);
+  if (cur) BCHK(hipMemcpyAsync(b->QA, b->QB, sizeof(double) * 3 * b->fs, hipMemcpyDeviceToDevice, b->stream));
+  k_batch_step_set<<<1, 64, 0, b->stream>>>(b->d_step0, 0);
+  k_batch_bc<<<bcb, 256, 0, b->stream>>>(b->d_inst, b->ninst, b->QA, b->fs, b->d_step0, 0);
   BCHK(hipGetLastError());
   return OSBL_OK;
 }
@@ -390,7 +434,8 @@ int osbl_batch_create(const osbl_config* base, int ninst, const int32_t* n, cons
       hipMalloc((void**)&b->d_seg_inst, sizeof(int) * b->nseg) != hipSuccess ||
       hipMalloc((void**)&b->d_seg_c0, sizeof(int) * b->nseg) != hipSuccess ||
       hipMalloc((void**)&b->d_pt_inst, sizeof(int) * b->npts) != hipSuccess ||
-      hipMalloc((void**)&b->d_pt_i, sizeof(int) * b->npts) != hipSuccess)
+      hipMalloc((void**)&b->d_pt_i, sizeof(int) * b->npts) != hipSuccess ||
+      hipMalloc((void**)&b->d_step0, sizeof(int)) != hipSuccess)
     return fail_free(bfail(OSBL_E_NOMEM, "hipMalloc failed"));
   if (hipMemset(b->QA, 0, bytes) != hipSuccess || hipMemset(b->QB, 0, bytes) != hipSuccess ||
       hipMemset(b->Qo, 0, bytes) != hipSuccess ||
@@ -409,8 +454,10 @@ int osbl_batch_create(const osbl_config* base, int ninst, const int32_t* n, cons
 void osbl_batch_destroy(osbl_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_chunks(b);
   for (void* p : {(void*)b->QA, (void*)b->QB, (void*)b->Qo, (void*)b->stage_buf, (void*)b->d_inst, (void*)b->d_order,
-                  (void*)b->d_seg_inst, (void*)b->d_seg_c0, (void*)b->d_pt_inst, (void*)b->d_pt_i})
+                  (void*)b->d_seg_inst, (void*)b->d_seg_c0, (void*)b->d_pt_inst, (void*)b->d_pt_i,
+                  (void*)b->d_step0})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -476,6 +523,10 @@ int osbl_batch_set_threads(osbl_batch* b, int nbuckets, const int32_t* max_n, co
 
 int osbl_batch_set_graph_steps(osbl_batch* b, int steps) {
   if (!b || steps < 1) return bfail(OSBL_E_ARG, "invalid argument");
+  if (steps != b->graph_steps) {
+    BCHK(hipStreamSynchronize(b->stream));
+    drop_chunks(b);
+  }
   b->graph_steps = steps;
   return OSBL_OK;
 }

@@ -672,6 +672,16 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   return OSBL_OK;
 }
 
+// forget the captured iterations (their launch arguments no longer hold);
+// the next osbl_run captures them again
+static void drop_graphs(osbl_ctx* c) {
+  for (int k = 0; k < 2; ++k)
+    if (c->graph2[k]) {
+      (void)hipGraphExecDestroy(c->graph2[k]);
+      c->graph2[k] = nullptr;
+    }
+}
+
 void osbl_destroy(osbl_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -921,11 +931,7 @@ int osbl_set_coordinates(osbl_ctx* c, int d, const double* x) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->bnd.dm[d] = c->met_dev[d];
   c->bnd.sd[d] = c->met_dev[d] + N;
-  for (int k = 0; k < 2; ++k)  // launch arguments changed
-    if (c->graph2[k]) {
-      (void)hipGraphExecDestroy(c->graph2[k]);
-      c->graph2[k] = nullptr;
-    }
+  drop_graphs(c);  // launch arguments changed
   return OSBL_OK;
 }
 
@@ -1004,6 +1010,9 @@ int osbl_set_filter_mask(osbl_ctx* c, const uint8_t* mask) {
   if (!c) return fail(OSBL_E_ARG, "null context");
   if (!(c->cfg.filters & OSBL_FILTER_BINOMIAL)) return fail(OSBL_E_STATE, "the block has no binomial filter");
   HIPCHK(hipStreamSynchronize(c->stream));
+  // the captured iterations hold the mask pointer (or its absence) as a kernel
+  // argument: both branches below change it, so they are captured again
+  drop_graphs(c);
   if (!mask) {
     HIPCHK(hipFree(c->fmask));
     c->fmask = nullptr;
@@ -1015,12 +1024,6 @@ int osbl_set_filter_mask(osbl_ctx* c, const uint8_t* mask) {
     return fail(OSBL_E_NOMEM, "hipMalloc of %zu bytes failed", n);
   }
   HIPCHK(hipMemcpy(c->fmask, mask, n, hipMemcpyHostToDevice));
-  // graphs hold the mask pointer as a kernel argument: capture again
-  for (int k = 0; k < 2; ++k)
-    if (c->graph2[k]) {
-      (void)hipGraphExecDestroy(c->graph2[k]);
-      c->graph2[k] = nullptr;
-    }
   return OSBL_OK;
 }
 
@@ -1062,11 +1065,7 @@ int osbl_set_bc_plane(osbl_ctx* c, int d, int side, const double* plane) {
     for (int k = 0; k < 2; ++k) launch_fill_face_halo(c->Qbuf[k], c->nv, c->g, d, side, c->plane_dev[d][side], c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 2; ++k)
-    if (c->graph2[k]) {
-      (void)hipGraphExecDestroy(c->graph2[k]);
-      c->graph2[k] = nullptr;
-    }
+  drop_graphs(c);
   return OSBL_OK;
 }
 

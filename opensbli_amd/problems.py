@@ -141,11 +141,51 @@ def stretched_coordinate(n, length, factor, kind="sinh"):
     raise ValueError("unknown stretching %r" % kind)
 
 
+def stable_dt(y, rho, u, T, deltas, Minf, Re, Pr=0.72, gama=1.4, SuthT=110.4, Tinf=288.0, cfl=0.8, vnn=0.3):
+    """Explicit time step of a wall-normal profile (rho, u, T at the points y)
+    on a grid with uniform spacings `deltas` in the other directions:
+    min(cfl / max(sum_d (|u_d| + a) / dx_d), vnn / max((gama/Pr) nu sum_d 1/dx_d^2)),
+    a = sqrt(T)/Minf, nu = mu(T)/(rho Re) with Sutherland's mu.  cfl = 0.8 and
+    vnn = 0.3 keep RK3's stability region (imaginary-axis reach sqrt(3),
+    negative-real reach 2.51) clear of the 4th-order operators' largest
+    eigenvalues (advection 1.37/dx, diffusion 16/3 nu/dx^2)."""
+    dy = np.diff(y)
+    dyc = np.minimum(np.r_[dy[0], dy], np.r_[dy, dy[-1]])        # smaller neighbouring spacing
+    a = np.sqrt(T) / Minf
+    s = SuthT / Tinf
+    mu = T ** 1.5 * (1.0 + s) / (T + s)
+    nu = mu / (rho * Re)
+    inv2 = 1.0 / dyc ** 2 + sum(1.0 / d ** 2 for d in deltas)
+    lam_a = (np.abs(u) + a) / deltas[0] + a / dyc + sum(a / d for d in deltas[1:])
+    lam_v = gama / Pr * nu * inv2
+    return float(min(cfl / lam_a.max(), vnn / lam_v.max()))
+
+
+def flat_plate_dt(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950.0, Pr=0.72, gama=1.4,
+                  Tinf=288.0, SuthT=110.4, stretch_factor=5.0, stretch="tanh", wall="adiabatic_wall", Twall=None):
+    """The time step flat_plate takes by default for these arguments (stable_dt
+    of the similarity profile on the grid)."""
+    from .boundary_layer import SimilarityProfile
+    nd = len(n)
+    delta = [lengths[0] / (n[0] - 1)] + ([lengths[2] / n[2]] if nd == 3 else [])
+    bl = SimilarityProfile(Minf, Re, Pr=Pr, gama=gama, Tw=-1.0 if wall == "adiabatic_wall" else Twall,
+                           Tinf=Tinf, suth=SuthT)
+    y = stretched_coordinate(n[1], lengths[1], stretch_factor, stretch)
+    rho, u, _, T = bl.evaluate(y)
+    return stable_dt(y, rho, u, T, delta, Minf, Re, Pr, gama, SuthT, Tinf)
+
+
 def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950.0, Pr=0.72, gama=1.4,
-               Tinf=288.0, SuthT=110.4, stretch_factor=5.0, stretch="sinh", dt=0.025, conv="weno_z", rk="ls3",
+               Tinf=288.0, SuthT=110.4, stretch_factor=5.0, stretch="tanh", dt=None, conv="weno_z", rk="ls3",
                noise=1e-4, seed=5, bscheme="reduced_access", wall="adiabatic_wall", Twall=None, ndim=None,
                sensor=False, teno_a1=10.5, teno_a2=4.5, sensor_eps=1.0e-30):
     """Laminar flat-plate boundary layer, wall-normal stretched grid.
+
+    The wall-normal map is `stretch` with `stretch_factor` (stretched_coordinate):
+    by default tanh with factor 5, clustered at the wall; "sinh" is the map of
+    the reference's Katzer app.  dt = None takes stable_dt of the similarity
+    profile on this grid (the tanh map's first cell is 75 times thinner than
+    the sinh map's, so its explicit step is set by the viscous terms there).
 
     x0 streamwise (inlet with pressure extrapolation, zeroth-order outlet),
     x1 wall-normal (stretched; no-slip wall below, free-stream Dirichlet
@@ -171,6 +211,9 @@ def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950
                            Tinf=Tinf, suth=SuthT)
     y = stretched_coordinate(n[1], lengths[1], stretch_factor, stretch)
     rho, u, rv, T = bl.evaluate(y)
+    if dt is None:
+        dt = stable_dt(y, rho, u, T, [delta[0]] + delta[2:], Minf, Re, Pr, gama, SuthT, Tinf)
+    dt = float(dt)
     # profile as (nvar, n1) columns
     cons = [rho, rho * u, rv] + ([np.zeros_like(rho)] if nd == 3 else [])
     ke = 0.5 * (cons[1] ** 2 + cons[2] ** 2) / rho
@@ -201,7 +244,8 @@ def flat_plate(n=(480, 256, 128), lengths=(400.0, 115.0, 35.0), Minf=2.0, Re=950
     problem = dict(ndim=nd, n=n, delta=delta, conv=conv, averaging="roe", viscous="sutherland", rk=rk, bc=bc,
                    bscheme=bs, stretch=[0, 1, 0][:nd] + [0] * (3 - nd), gama=gama, Minf=Minf, Re=Re, Pr=Pr, dt=dt,
                    eps=1e-16, teno_ct=1e-6, SuthT=SuthT, RefT=Tinf, Twall=bl.Twall if Twall is None else Twall,
-                   coords={1: y}, planes={(0, 0): inlet, (1, 1): top})
+                   coords={1: y}, planes={(0, 0): inlet, (1, 1): top}, stretch_kind=stretch,
+                   stretch_factor=float(stretch_factor))
     if sensor:
         problem.update(sensor=1, teno_a1=teno_a1, teno_a2=teno_a2, sensor_eps=sensor_eps)
     return problem, q0
@@ -285,13 +329,18 @@ def euler_wave_curvilinear_3d(n, conv="weno_z", averaging="roe", rk="ls3", gama=
 
 def channel(n=(24, 33, 12), conv="teno5", c=(-1.0, 0.0, 0.0), Re=190.71, Minf=0.0955, Pr=0.7, dt=2e-4, by=1.7,
             seed=3):
-    """Compressible channel driven by a body force, the structure of the
-    reference's TENO channel (apps/channel_flow/compressible_TCF_TENO):
-    conservative Navier-Stokes with a characteristic scheme, Sutherland
-    viscosity, the body force c_j, periodic x0 (and x2), isothermal walls at
-    x1 = -1 and 1 on a wall-clustered grid (tanh map from both walls),
-    T_wall = 1.  Initial state: a laminar-like profile u0 = 1.5 (1 - x1^2)
-    plus seeded noise of amplitude 0.02, rho = 1, T = 1.  Returns (problem, q0)."""
+    """Compressible channel driven by a constant body force c_j: conservative
+    Navier-Stokes with a characteristic scheme, Sutherland viscosity,
+    periodic x0 (and x2), isothermal walls at x1 = -1 and 1 on a
+    wall-clustered grid (tanh map from both walls), T_wall = 1.  The body
+    force terms are those of the reference's channel apps
+    (apps/channel_flow/compressible_TCF_TENO/turbulent_channel.py:15-16);
+    those apps themselves use the power law mu = T**0.7 (TCF_TENO, TCF_Central)
+    or the Feiereisen split form (turbulent_3D), which this build does not
+    recognise, so none of them runs here: this block exercises the body-force
+    terms with Sutherland viscosity instead.  Initial state: a laminar-like
+    profile u0 = 1.5 (1 - x1^2) plus seeded noise of amplitude 0.02, rho = 1,
+    T = 1.  Returns (problem, q0)."""
     nd = len(n)
     d = [4.0 * math.pi / n[0], 2.0 / (n[1] - 1)] + ([4.0 * math.pi / 3.0 / n[2]] if nd == 3 else [])
     eta = np.arange(n[1]) / (n[1] - 1.0)

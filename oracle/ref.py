@@ -22,6 +22,37 @@ FILTERS = {"weno": 1, "sfd": 2, "binomial": 4}
 WF_SENSOR = {"ducros": 0, "constant": 1}
 
 
+# parity bar: relative L2 difference per conservative variable (FP64)
+TOL = 1e-10
+# a momentum component whose norm is below this fraction of the momentum
+# magnitude (e.g. the Taylor-Green vortex's rho*w, zero at t = 0) is measured
+# against the momentum magnitude instead of its own norm
+SMALL_COMPONENT = 0.1
+
+
+def parity_errors(q, qr):
+    """Relative L2 difference of q against the reference qr per conservative
+    variable (leading axis: rho, rho*u_0.., rho*E; nvar = ndim + 2).  A
+    momentum component whose reference norm is below SMALL_COMPONENT times the
+    momentum magnitude sqrt(sum_i |rho*u_i|^2) is divided by that magnitude:
+    rounding in such a component scales with the whole momentum field, not
+    with the component's own (small or zero) size."""
+    q = np.asarray(q)
+    qr = np.asarray(qr)
+    nv = qr.shape[0]
+    err = np.empty(nv)
+    for m in range(nv):
+        den = np.linalg.norm(qr[m])
+        num = np.linalg.norm(q[m] - qr[m])
+        err[m] = num / den if den > 0 else num
+    if nv >= 3:
+        mom = np.sqrt(sum(np.linalg.norm(qr[m]) ** 2 for m in range(1, nv - 1)))
+        for m in range(1, nv - 1):
+            if mom > 0 and np.linalg.norm(qr[m]) < SMALL_COMPONENT * mom:
+                err[m] = np.linalg.norm(q[m] - qr[m]) / mom
+    return err
+
+
 def filter_fields(p):
     """Filter settings of a problem dict: (bitmask, binomial order, WENO-filter
     sensor, Mach correction, sigma, chi, omega, control)."""
@@ -227,6 +258,19 @@ class RefSolver:
             self._mask = np.ascontiguousarray(mask, dtype=np.uint8)
             assert self._mask.shape == self.shape
             self.L.ref_set_filter_mask(self.h, self._mask.ctypes.data)
+
+    def set_filter_mask(self, mask):
+        """Grid condition of the binomial filter (one byte per interior cell);
+        None applies the filter everywhere."""
+        if mask is None:
+            self._mask = None
+            ok = self.L.ref_set_filter_mask(self.h, None)
+        else:
+            self._mask = np.ascontiguousarray(mask, dtype=np.uint8)
+            assert self._mask.shape == self.shape
+            ok = self.L.ref_set_filter_mask(self.h, self._mask.ctypes.data)
+        if ok != 0:
+            raise RuntimeError("no binomial filter on this block")
 
     def sfd_field(self):
         out = np.empty((self.nvar,) + self.shape)

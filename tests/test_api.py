@@ -77,6 +77,7 @@ def test_flat_plate_script_matches_problem():
               "SuthT", "RefT"):
         assert p[k] == ref_p[k], k
     assert np.allclose(p["delta"], ref_p["delta"], rtol=1e-15)
+    assert p["dt"] == ref_p["dt"]
     assert p["bscheme"][:2] == ref_p["bscheme"][:2]
     assert np.abs(p["coords"][1] - ref_p["coords"][1]).max() < 1e-12
     for f in ((0, 0), (1, 1)):

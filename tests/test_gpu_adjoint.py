@@ -62,6 +62,8 @@ def _grad_adjoint(n, steps, every):
     qN = integrate(q, b, steps, every)
     J = kinetic_energy(qN, problem)
     J.backward()
+    # backward leaves the block at the forward output
+    assert np.array_equal(b.get_state(), qN.detach().cpu().numpy())
     return problem, amps, float(J), a.grad.cpu().numpy(), b
 
 

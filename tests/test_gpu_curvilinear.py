@@ -3,9 +3,8 @@ metric from grid coordinates (metric.py:72-147), the residual of the strong
 conservation form with the characteristic system about the metric normal
 (apps/euler_wave_curvilinear/euler_wave.py), and the time loop.
 
-Bar as in the other parity tests: relative L2 per conservative variable below
-max(1e-10, 4 x the spread between the oracle's plain and FMA builds).  The
-rotated-mesh case is also checked against the GPU's own Cartesian path (the
+Bar as in the other parity tests: relative L2 per conservative variable
+<= 1e-10 (conftest.parity_errors).  The rotated-mesh case is also checked against the GPU's own Cartesian path (the
 2-D characteristic fields are frame-invariant, so the two agree to rounding).
 """
 import math
@@ -14,11 +13,9 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import check_parity, rel_l2
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-10
 
 
 def _say(*a):
@@ -33,15 +30,6 @@ def _pair(problem, q0, oracle):
     r = oracle.RefSolver(problem)
     r.set_state(q0)
     return g, r
-
-
-def _spread(problem, q0, oracle, ref, steps=None):
-    f = oracle.RefSolver(problem, variant="fma")
-    f.set_state(q0)
-    if steps is None:
-        return rel_l2(f.rhs(), ref)
-    f.run(steps)
-    return rel_l2(f.get_state(), ref)
 
 
 def _problem3d(n, conv="weno_z", averaging="roe"):
@@ -70,11 +58,7 @@ def test_rhs_wavy_2d(conv, avg, oracle):
     _say(f"case curvilinear rhs 2-D wavy 64^2 conv={conv} avg={avg}")
     problem, q0 = euler_wave_curvilinear(64, conv=conv, averaging=avg)
     g, r = _pair(problem, q0, oracle)
-    rr = r.rhs()
-    err = rel_l2(g.rhs(), rr)
-    spread = _spread(problem, q0, oracle, rr)
-    _say("rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+    check_parity(g.rhs(), r.rhs(), f"curvilinear 2-D rhs {conv} {avg}")
 
 
 def test_euler_wave_curvilinear_100_steps(oracle):
@@ -86,11 +70,7 @@ def test_euler_wave_curvilinear_100_steps(oracle):
     g, r = _pair(problem, q0, oracle)
     g.run(100)
     r.run(100)
-    rs = r.get_state()
-    err = rel_l2(g.get_state(), rs)
-    spread = _spread(problem, q0, oracle, rs, steps=100)
-    _say("rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+    check_parity(g.get_state(), r.get_state(), "curvilinear euler wave 100 steps")
 
 
 def test_rotated_mesh_equals_the_cartesian_path(oracle):
@@ -139,18 +119,10 @@ def test_skewed_wavy_3d(conv, oracle):
     _say(f"case curvilinear 3-D 16^3 conv={conv}: rhs and 20 steps")
     problem, q0 = _problem3d(16, conv=conv)
     g, r = _pair(problem, q0, oracle)
-    rr = r.rhs()
-    err = rel_l2(g.rhs(), rr)
-    spread = _spread(problem, q0, oracle, rr)
-    _say("rhs rel L2", err, "spread", spread)
-    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+    check_parity(g.rhs(), r.rhs(), f"curvilinear 3-D rhs {conv}")
     g.run(20)
     r.run(20)
-    rs = r.get_state()
-    err = rel_l2(g.get_state(), rs)
-    spread = _spread(problem, q0, oracle, rs, steps=20)
-    _say("20 steps rel L2", err, "spread", spread)
-    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+    check_parity(g.get_state(), r.get_state(), f"curvilinear 3-D {conv} 20 steps")
 
 
 def test_run_needs_coordinates():

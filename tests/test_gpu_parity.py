@@ -2,21 +2,19 @@
 
 Bar: relative L2 <= 1e-10 per conservative variable (FP64), after the step
 counts of the benchmark configurations (100 steps) at sizes the oracle runs
-in seconds.  WENO5-Z's hard-coded eps = 1e-14 (weno.py:286) makes some
-components rounding-sensitive: two builds of the oracle itself (with and
-without FMA contraction) differ there.  For those schemes the bar of a
-variable is max(1e-10, 4 x that oracle self-spread), measured in the test.
+in seconds (conftest.parity_errors: a momentum component that is small
+against the momentum magnitude, such as the Taylor-Green vortex's rho*w which
+starts at zero, is measured against that magnitude).  Every test prints its
+per-variable errors.
 """
 import sys
 
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import check_parity, kinetic_energy_imbalance, uniform_rho_p_state
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-10
 
 
 def _say(*a):
@@ -33,28 +31,37 @@ def _pair(problem, q0, oracle):
     return g, r
 
 
-def _bar(problem, q0, oracle, steps, ref_state):
-    """Per-variable bar: max(TOL, 4 x spread between the two oracle builds)."""
-    f = oracle.RefSolver(problem, variant="fma")
-    f.set_state(q0)
-    f.run(steps)
-    spread = rel_l2(f.get_state(), ref_state)
-    return np.maximum(TOL, 4.0 * spread), spread
-
-
 @pytest.mark.parametrize("conv", ["central_skew", "central_cons", "weno_z", "weno_js", "teno6", "teno5"])
 def test_rhs_tgv3d(conv, oracle):
     from opensbli_amd.problems import taylor_green
     _say(f"case rhs tgv3d conv={conv} n=24")
     problem, q0 = taylor_green(24, conv=conv)
     g, r = _pair(problem, q0, oracle)
-    rr = r.rhs()
-    err = rel_l2(g.rhs(), rr)
-    f = oracle.RefSolver(problem, variant="fma")
-    f.set_state(q0)
-    spread = rel_l2(f.rhs(), rr)
-    _say("rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < np.maximum(TOL, 4.0 * spread)), err
+    check_parity(g.rhs(), r.rhs(), f"rhs {conv}")
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_skew_form_kinetic_energy_identity(nd, oracle):
+    """The skew-symmetric form's residual at uniform density and pressure
+    produces no kinetic energy (sum u.R_rhou - |u|^2 R_rho / 2 = 0 to
+    rounding on a periodic grid); the conservative form's does."""
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import taylor_green
+    n = 32
+    _say(f"case kinetic-energy identity nd={nd} n={n}")
+    q = uniform_rho_p_state(n, nd, seed=nd)
+    imb = {}
+    for conv in ("central_skew", "central_cons"):
+        problem, _ = taylor_green(n, conv=conv, viscous="none")
+        if nd == 2:
+            problem.update(ndim=2, n=[n, n], delta=problem["delta"][:2], bc=problem["bc"][:2])
+        g = Block(problem)
+        g.set_state(q)
+        imb[conv] = kinetic_energy_imbalance(q, g.rhs())
+        g.close()
+    _say("relative kinetic-energy production", imb)
+    assert imb["central_skew"] < 1e-12, imb
+    assert imb["central_cons"] > 1e-5, imb
 
 
 @pytest.mark.parametrize("conv", ["central_skew", "weno_z", "teno6", "weno_js", "teno5"])
@@ -66,40 +73,33 @@ def test_run_tgv3d_100_steps(conv, oracle):
     g, r = _pair(problem, q0, oracle)
     g.run(steps)
     r.run(steps)
-    qr = r.get_state()
-    err = rel_l2(g.get_state(), qr)
-    bar, spread = _bar(problem, q0, oracle, steps, qr)
-    _say("rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < bar), (err, bar)
+    check_parity(g.get_state(), r.get_state(), f"{conv} {steps} steps")
 
 
+@pytest.mark.parametrize("conv", ["teno6", "central_skew", "weno_z"])
 @pytest.mark.parametrize("rk", ["ls3", "ls3_ssp", "ls4"])
-def test_run_tgv3d_lowstorage(rk, oracle):
+def test_run_tgv3d_lowstorage(rk, conv, oracle):
     from opensbli_amd.problems import taylor_green
     n, steps = 16, 20
-    _say(f"case run tgv3d rk={rk} n={n} steps={steps}")
-    problem, q0 = taylor_green(n, conv="teno6", rk=rk)
+    _say(f"case run tgv3d conv={conv} rk={rk} n={n} steps={steps}")
+    problem, q0 = taylor_green(n, conv=conv, rk=rk)
     g, r = _pair(problem, q0, oracle)
     g.run(steps)
     r.run(steps)
-    qr = r.get_state()
-    err = rel_l2(g.get_state(), qr)
-    bar, spread = _bar(problem, q0, oracle, steps, qr)
-    _say("rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < bar), (err, bar)
+    check_parity(g.get_state(), r.get_state(), f"{conv} {rk} {steps} steps")
 
 
-def test_c1_tgv64_central_100_steps(oracle):
-    """Configuration 1 at full size: 64^3, central skew, RK3, 100 steps."""
+@pytest.mark.parametrize("rk", ["rk3", "ls3"])
+def test_c1_tgv64_central_100_steps(rk, oracle):
+    """Configuration 1 at full size: 64^3, central skew, RK3 (the 3-register
+    RungeKutta(3) and the low-storage RungeKuttaLS(3)), 100 steps."""
     from opensbli_amd.problems import taylor_green
-    _say("case C1 tgv64 central 100 steps")
-    problem, q0 = taylor_green(64, conv="central_skew")
+    _say(f"case C1 tgv64 central {rk} 100 steps")
+    problem, q0 = taylor_green(64, conv="central_skew", rk=rk)
     g, r = _pair(problem, q0, oracle)
     g.run(100)
     r.run(100)
-    err = rel_l2(g.get_state(), r.get_state())
-    _say("rel L2", err)
-    assert np.all(err < TOL), err
+    check_parity(g.get_state(), r.get_state(), f"C1 {rk}")
 
 
 @pytest.mark.parametrize("conv", ["weno_z", "teno6", "central_skew", "central_cons"])
@@ -109,16 +109,10 @@ def test_2d_kh(conv, oracle):
     amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
     problem, q0 = kelvin_helmholtz(64, amps, conv=conv)
     g, r = _pair(problem, q0, oracle)
-    err = rel_l2(g.rhs(), r.rhs())
-    _say("rhs rel L2", err)
-    assert np.all(err < TOL), err
+    check_parity(g.rhs(), r.rhs(), f"kh {conv} rhs")
     g.run(100)
     r.run(100)
-    qr = r.get_state()
-    err = rel_l2(g.get_state(), qr)
-    bar, spread = _bar(problem, q0, oracle, 100, qr)
-    _say("state rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < bar), (err, bar)
+    check_parity(g.get_state(), r.get_state(), f"kh {conv} 100 steps")
 
 
 @pytest.mark.parametrize("conv", ["weno_z", "teno5", "teno6", "weno_js"])
@@ -131,11 +125,7 @@ def test_1d_shock_tube(conv, n, oracle):
     nsteps = 100
     g.run(nsteps)
     r.run(nsteps)
-    qr = r.get_state()
-    err = rel_l2(g.get_state(), qr)
-    bar, spread = _bar(problem, q0, oracle, nsteps, qr)
-    _say("state rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < bar), (err, bar)
+    check_parity(g.get_state(), r.get_state(), f"sod {conv} n={n}")
 
 
 def test_tgv_symmetry_256_weno_z():
@@ -176,7 +166,4 @@ def test_api_script_tgv_weno_z(oracle):
     r.run(20)
     qr = r.get_state()
     q = np.stack([out[k] for k in ["rho", "rhou0", "rhou1", "rhou2", "rhoE"]])
-    err = rel_l2(q, qr)
-    bar, spread = _bar(problem, q0, oracle, 20, qr)
-    _say("rel L2", err, "oracle self-spread", spread)
-    assert np.all(err < bar), (err, bar)
+    check_parity(q, qr, "api tgv weno_z")

@@ -4,11 +4,9 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import check_parity
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-10
 
 
 def _say(*a):
@@ -22,19 +20,14 @@ def _compare(problem, q0, oracle, steps):
     g.set_state(q0)
     r = oracle.RefSolver(problem)
     r.set_state(q0)
-    f = oracle.RefSolver(problem, variant="fma")
-    f.set_state(q0)
     if steps == 0:
-        a, b, c = g.rhs(), r.rhs(), f.rhs()
+        a, b = g.rhs(), r.rhs()
     else:
         g.run(steps)
         r.run(steps)
-        f.run(steps)
-        a, b, c = g.get_state(), r.get_state(), f.get_state()
-    err, spread = rel_l2(a, b), rel_l2(c, b)
-    _say("rel L2", err, "oracle self-spread", spread)
+        a, b = g.get_state(), r.get_state()
     assert np.isfinite(a).all()
-    assert np.all(err < np.maximum(TOL, 4.0 * spread)), (err, spread)
+    check_parity(a, b, "rhs" if steps == 0 else f"{steps} steps")
 
 
 @pytest.mark.parametrize("conv", ["teno5", "teno6"])

@@ -336,6 +336,29 @@ def test_discrete_conservation(oracle, conv):
         assert abs(R[m].sum()) < 1e-11 * np.abs(R[m]).sum()
 
 
+@pytest.mark.parametrize("nd", [2, 3])
+def test_skew_form_conserves_kinetic_energy(oracle, nd):
+    """Blaisdell's skew-symmetric convective form (parsing.py:75-111) at
+    uniform density and pressure on a periodic grid: the residual produces no
+    kinetic energy, sum(u.R_rhou - |u|^2 R_rho / 2) = 0 to rounding, while the
+    conservative form of the same fluxes does (which separates the two forms:
+    conservation and free-stream checks hold for both)."""
+    from conftest import kinetic_energy_imbalance, uniform_rho_p_state
+    from opensbli_amd.problems import taylor_green
+    n = 16
+    q = uniform_rho_p_state(n, nd, seed=nd)
+    imb = {}
+    for conv in ("central_skew", "central_cons"):
+        problem, _ = taylor_green(n, conv=conv, viscous="none")
+        if nd == 2:
+            problem.update(ndim=2, n=[n, n], delta=problem["delta"][:2], bc=problem["bc"][:2])
+        s = oracle.RefSolver(problem)
+        s.set_state(q)
+        imb[conv] = kinetic_energy_imbalance(q, s.rhs())
+    assert imb["central_skew"] < 1e-12, imb
+    assert imb["central_cons"] > 1e-5, imb
+
+
 def euler_wave_problem(n, conv):
     """apps/euler_wave: rho = 1 + 0.2 sin(pi(x+y)), u = 1, v = -0.5, p = 1 on [0,2)^2."""
     g = 1.4
