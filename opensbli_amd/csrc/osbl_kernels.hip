@@ -636,21 +636,29 @@ __global__ void __launch_bounds__(256) k_update(const double* __restrict__ Q, co
 
 // ---------------------------------------------------------------------------
 // 3-D viscous terms + RK update for the characteristic schemes, tiled 2.5-D.
-// A 64 x 4 block of (x, y) columns marches along z.  Each z plane of the
+// A VTX x VTY block of (x, y) columns marches along z.  Each z plane of the
 // constituent relations u_0..u_2, T (computed here from Q, with a 2-point
-// x/y halo) is staged in one of three rotating LDS slots; the x/y stencils
+// x/y halo) is staged in one of four rotating LDS slots; the x/y stencils
 // read LDS, the z stencils read a per-thread register queue of the five
 // planes z-2..z+2 holding u, T and the inner derivatives D_x u_0, D_x u_2,
 // D_y u_1, D_y u_2 of the mixed terms D_z(D_x u), D_z(D_y u).  The formulas
 // are those of viscous_terms (scheme.py:299-387).
+//
+// Software pipeline: while plane z is evaluated, the raw Q of plane z+3 and
+// the cell's residual / RK registers are already in flight (loaded at the
+// top of the iteration into registers); plane z+3 then goes into the slot
+// plane z-1 vacated, and one barrier per plane separates those writes from
+// the next plane's reads.
 // ---------------------------------------------------------------------------
-constexpr int VTX = 64, VTY = 8, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
+constexpr int VTX = 64, VTY = 4, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
+constexpr int VNT = VTX * VTY;                 // threads per block
+constexpr int VPP = (VPL + VNT - 1) / VNT;     // staged points per thread and plane
 
 template <int MODE>
-__global__ void __launch_bounds__(VTX * VTY, 1) k_visc_update3d(const double* __restrict__ Q, double* __restrict__ R,
-                                                           double* __restrict__ Qo, double* __restrict__ Qn, Grid g,
-                                                           Phys ph, double cA, double cB, int first, int sz) {
-  __shared__ double sp[3][4][VPL];
+__global__ void __launch_bounds__(VNT, 2) k_visc_update3d(const double* __restrict__ Q, double* __restrict__ R,
+                                                       double* __restrict__ Qo, double* __restrict__ Qn, Grid g,
+                                                       Phys ph, double cA, double cB, int first, int sz) {
+  __shared__ double sp[4][4][VPL];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * VTX + tx;
   int bx, by, bz;
@@ -662,23 +670,42 @@ __global__ void __launch_bounds__(VTX * VTY, 1) k_visc_update3d(const double* __
   const bool inside = x < g.n[0] && y < g.n[1];
   const double ix = ph.inv_delta[0], iy = ph.inv_delta[1], iz = ph.inv_delta[2];
 
-  // stage plane zp (u0,u1,u2,T with a 2-point x/y halo) into LDS slot `slot`
-  auto stage = [&](int zp, int slot) {
-    for (int p = tid; p < VPL; p += VTX * VTY) {
-      const int xx = p % VPX, yy = p / VPX;
-      const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
-      if (gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1]) {
-        const int o = g.idx(gx, gy, zp);
-        const double rho = Q[o];
+  // this thread's staged points: slot index and in-plane offset (plane z = 0)
+  int sidx[VPP], soff[VPP];
+  bool sval[VPP];
+#pragma unroll
+  for (int k = 0; k < VPP; ++k) {
+    const int p = tid + k * VNT;
+    const int xx = p % VPX, yy = p / VPX;
+    const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
+    sidx[k] = p;
+    sval[k] = p < VPL && gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1];
+    soff[k] = sval[k] ? g.idx(gx, gy, 0) : 0;
+  }
+  double pq[VPP][5];  // raw Q of the plane in flight
+  auto fetch = [&](int zp) {
+    const int zo = zp * g.plane;
+#pragma unroll
+    for (int k = 0; k < VPP; ++k)
+      if (sval[k]) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) pq[k][m] = Q[m * g.fs + soff[k] + zo];
+      }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < VPP; ++k)
+      if (sval[k]) {
+        const double rho = pq[k][0];
         const double ir = 1.0 / rho;
-        const double u0 = Q[g.fs + o] * ir, u1 = Q[2 * g.fs + o] * ir, u2 = Q[3 * g.fs + o] * ir;
-        const double pr = ph.gm1 * (Q[4 * g.fs + o] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
+        const double u0 = pq[k][1] * ir, u1 = pq[k][2] * ir, u2 = pq[k][3] * ir;
+        const double pr = ph.gm1 * (pq[k][4] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
+        const int p = sidx[k];
         sp[slot][0][p] = u0;
         sp[slot][1][p] = u1;
         sp[slot][2][p] = u2;
         sp[slot][3][p] = pr * ph.gama * ph.Minf * ph.Minf * ir;
       }
-    }
   };
   const int c = (ty + 2) * VPX + tx + 2;  // this thread's point in a slot
   auto dx1 = [&](const double* f, int o) {
@@ -707,109 +734,135 @@ __global__ void __launch_bounds__(VTX * VTY, 1) k_visc_update3d(const double* __
     cq[6] = iy * dy1(u1, c);
     cq[7] = iy * dy1(u2, c);
   };
+  auto slot_of = [](int zp) { return (zp + 8) & 3; };
 
   double zq[5][8];
-  // prime the pipeline with planes z0-2 .. z0+1
+  // prologue: planes z0-2 .. z0+1 into the four slots, their columns into
+  // queue entries 0..3, then plane z0+2 into the slot of plane z0-2
   for (int k = 0; k < 4; ++k) {
-    const int zp = z0 - 2 + k;
-    __syncthreads();
-    stage(zp, (zp + 3) % 3);
-    __syncthreads();
-    double cq[8];
-    column((zp + 3) % 3, cq);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) zq[k + 1][e] = cq[e];
+    fetch(z0 - 2 + k);
+    store(slot_of(z0 - 2 + k));
   }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) column(slot_of(z0 - 2 + k), zq[k]);
+  fetch(z0 + 2);
+  __syncthreads();
+  store(slot_of(z0 + 2));
+  __syncthreads();
+  column(slot_of(z0 + 2), zq[4]);
+
   for (int z = z0; z < z1; ++z) {
-    __syncthreads();
-    stage(z + 2, (z + 2) % 3);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) zq[k][e] = zq[k + 1][e];
-    column((z + 2) % 3, zq[4]);
-    if (!inside) continue;
-    const int sl = z % 3;
-    const double* U[3] = {sp[sl][0], sp[sl][1], sp[sl][2]};
-    const double* T = sp[sl][3];
-    double du[3][3], dd[3][3][3];
-    // x and y first / second derivatives from LDS
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      du[m][0] = ix * dx1(U[m], c);
-      du[m][1] = iy * dy1(U[m], c);
-      dd[m][0][0] = ix * ix * dx2(U[m], c);
-      dd[m][1][1] = iy * iy * dy2(U[m], c);
-    }
-    // z derivatives from the queue (planes z-2..z+2 at indices 0..4)
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      du[m][2] = iz * ((1.0 / 12.0) * (zq[0][m] - zq[4][m]) + (2.0 / 3.0) * (zq[3][m] - zq[1][m]));
-      dd[m][2][2] = iz * iz * ((-1.0 / 12.0) * (zq[0][m] + zq[4][m]) + (4.0 / 3.0) * (zq[1][m] + zq[3][m]) - 2.5 * zq[2][m]);
-    }
-    // mixed D_y(D_x u_m), m = 0, 1, from LDS
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const double* f = U[m];
-      const double a2 = ix * dx1(f, c - 2 * VPX), a1 = ix * dx1(f, c - VPX);
-      const double b1 = ix * dx1(f, c + VPX), b2 = ix * dx1(f, c + 2 * VPX);
-      const double v = iy * ((1.0 / 12.0) * (a2 - b2) + (2.0 / 3.0) * (b1 - a1));
-      dd[m][0][1] = v;
-      dd[m][1][0] = v;
-    }
-    // mixed D_z(D_x u_m), m = 0 (queue 4), 2 (queue 5); D_z(D_y u_m), m = 1 (6), 2 (7)
-    auto dzq = [&](int e) {
-      return iz * ((1.0 / 12.0) * (zq[0][e] - zq[4][e]) + (2.0 / 3.0) * (zq[3][e] - zq[1][e]));
-    };
-    dd[0][0][2] = dd[0][2][0] = dzq(4);
-    dd[2][0][2] = dd[2][2][0] = dzq(5);
-    dd[1][1][2] = dd[1][2][1] = dzq(6);
-    dd[2][1][2] = dd[2][2][1] = dzq(7);
-    const double divu = du[0][0] + du[1][1] + du[2][2];
-    double r[5];
+    const bool more = z + 1 < z1;  // plane z+3 feeds the next plane's queue entry
+    if (more) fetch(z + 3);
     const int o = g.idx(x, y, z);
+    // the cell's residual and RK registers, in flight during the arithmetic
+    double rin[5], qa[5], qb[5];
+    if (inside) {
 #pragma unroll
-    for (int m = 0; m < 5; ++m) r[m] = MODE == 3 ? 0.0 : R[m * g.fs + o];
-    double energy = ph.cq * (ix * ix * dx2(T, c) + iy * iy * dy2(T, c) +
-                             iz * iz * ((-1.0 / 12.0) * (zq[0][3] + zq[4][3]) + (4.0 / 3.0) * (zq[1][3] + zq[3][3]) -
-                                        2.5 * zq[2][3]));
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double um = zq[2][m];
-      double sk = 0.0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) sk += dd[k][m][k];
-      double visc = 0.0;
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double tt = dd[m][j][j] + dd[j][m][j];
-        if (j == m) tt -= (2.0 / 3.0) * sk;
-        const double dtau = ph.inv_re * tt;
-        const double tau = ph.inv_re * (du[m][j] + du[j][m] - (j == m ? (2.0 / 3.0) * divu : 0.0));
-        visc += dtau;
-        energy += du[m][j] * tau + um * dtau;
+      for (int m = 0; m < 5; ++m) {
+        if constexpr (MODE == 2) rin[m] = R[m * g.fs + o];
+        if constexpr (MODE == 0 || MODE == 1) rin[m] = R[m * g.fs + o];
+        if constexpr (MODE == 0) qa[m] = first ? Q[m * g.fs + o] : Qo[m * g.fs + o];
+        if constexpr (MODE == 1) {
+          qa[m] = cA == 0.0 ? 0.0 : Qo[m * g.fs + o];
+          qb[m] = Q[m * g.fs + o];
+        }
       }
-      r[1 + m] += visc;
+      const int sl = slot_of(z);
+      const double* U[3] = {sp[sl][0], sp[sl][1], sp[sl][2]};
+      const double* T = sp[sl][3];
+      double du[3][3], dd[3][3][3];
+      // x and y first / second derivatives from LDS
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        du[m][0] = ix * dx1(U[m], c);
+        du[m][1] = iy * dy1(U[m], c);
+        dd[m][0][0] = ix * ix * dx2(U[m], c);
+        dd[m][1][1] = iy * iy * dy2(U[m], c);
+      }
+      // z derivatives from the queue (planes z-2..z+2 at indices 0..4)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        du[m][2] = iz * ((1.0 / 12.0) * (zq[0][m] - zq[4][m]) + (2.0 / 3.0) * (zq[3][m] - zq[1][m]));
+        dd[m][2][2] =
+            iz * iz * ((-1.0 / 12.0) * (zq[0][m] + zq[4][m]) + (4.0 / 3.0) * (zq[1][m] + zq[3][m]) - 2.5 * zq[2][m]);
+      }
+      // mixed D_y(D_x u_m), m = 0, 1, from LDS
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const double* f = U[m];
+        const double a2 = ix * dx1(f, c - 2 * VPX), a1 = ix * dx1(f, c - VPX);
+        const double b1 = ix * dx1(f, c + VPX), b2 = ix * dx1(f, c + 2 * VPX);
+        const double v = iy * ((1.0 / 12.0) * (a2 - b2) + (2.0 / 3.0) * (b1 - a1));
+        dd[m][0][1] = v;
+        dd[m][1][0] = v;
+      }
+      // mixed D_z(D_x u_m), m = 0 (queue 4), 2 (queue 5); D_z(D_y u_m), m = 1 (6), 2 (7)
+      auto dzq = [&](int e) {
+        return iz * ((1.0 / 12.0) * (zq[0][e] - zq[4][e]) + (2.0 / 3.0) * (zq[3][e] - zq[1][e]));
+      };
+      dd[0][0][2] = dd[0][2][0] = dzq(4);
+      dd[2][0][2] = dd[2][2][0] = dzq(5);
+      dd[1][1][2] = dd[1][2][1] = dzq(6);
+      dd[2][1][2] = dd[2][2][1] = dzq(7);
+      const double divu = du[0][0] + du[1][1] + du[2][2];
+      double r[5];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) r[m] = 0.0;
+      double energy = ph.cq * (ix * ix * dx2(T, c) + iy * iy * dy2(T, c) +
+                               iz * iz * ((-1.0 / 12.0) * (zq[0][3] + zq[4][3]) +
+                                          (4.0 / 3.0) * (zq[1][3] + zq[3][3]) - 2.5 * zq[2][3]));
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double um = zq[2][m];
+        double sk = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sk += dd[k][m][k];
+        double visc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double tt = dd[m][j][j] + dd[j][m][j];
+          if (j == m) tt -= (2.0 / 3.0) * sk;
+          const double dtau = ph.inv_re * tt;
+          const double tau = ph.inv_re * (du[m][j] + du[j][m] - (j == m ? (2.0 / 3.0) * divu : 0.0));
+          visc += dtau;
+          energy += du[m][j] * tau + um * dtau;
+        }
+        r[1 + m] = visc;
+      }
+      r[4] = energy;
+      if constexpr (MODE != 3) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) r[m] = rin[m] + r[m];
+      }
+      if constexpr (MODE == 2 || MODE == 3) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) R[m * g.fs + o] = r[m];
+      } else if constexpr (MODE == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          Qn[m * g.fs + o] = qa[m] + ph.dt * cA * r[m];
+          Qo[m * g.fs + o] = qa[m] + ph.dt * cB * r[m];
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          const double tmp = (cA == 0.0 ? 0.0 : cA * qa[m]) + ph.dt * r[m];
+          Qo[m * g.fs + o] = tmp;
+          Qn[m * g.fs + o] = qb[m] + cB * tmp;
+        }
+      }
     }
-    r[4] += energy;
-    if constexpr (MODE == 2 || MODE == 3) {
+    if (more) {
+      // plane z+3 replaces plane z-1 (last read before the previous barrier)
+      store(slot_of(z + 3));
+      __syncthreads();
 #pragma unroll
-      for (int m = 0; m < 5; ++m) R[m * g.fs + o] = r[m];
-    } else if constexpr (MODE == 0) {
+      for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        const double qo = first ? Q[m * g.fs + o] : Qo[m * g.fs + o];
-        Qn[m * g.fs + o] = qo + ph.dt * cA * r[m];
-        Qo[m * g.fs + o] = qo + ph.dt * cB * r[m];
-      }
-    } else {
-#pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        const double tmp = (cA == 0.0 ? 0.0 : cA * Qo[m * g.fs + o]) + ph.dt * r[m];
-        Qo[m * g.fs + o] = tmp;
-        Qn[m * g.fs + o] = Q[m * g.fs + o] + cB * tmp;
-      }
+        for (int e = 0; e < 8; ++e) zq[k][e] = zq[k + 1][e];
+      column(slot_of(z + 3), zq[4]);
     }
   }
 }

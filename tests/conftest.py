@@ -65,6 +65,32 @@ def uniform_rho_p_state(n, nd, gama=1.4, Minf=0.1, seed=0, kmax=2):
     return np.stack([rho] + [rho * v for v in vel] + [E])
 
 
+def generic_state(n, nd, gama=1.4, seed=0):
+    """Periodic state on [0, 2pi)^nd (interior arrays, x fastest) in which
+    every variable varies at order one: density, pressure and each velocity
+    component are sums of six seeded Fourier modes (|k_d| <= 2) about 1,
+    1/gama and 0, Mach number of order 0.3.  No term of the residual is a
+    small difference of large ones (unlike a low-Mach, divergence-free
+    field such as the Taylor-Green vortex's initial state)."""
+    rs = np.random.RandomState(seed)
+    x = np.arange(n) * (2.0 * np.pi / n)
+    grids = np.meshgrid(*([x] * nd), indexing="ij")[::-1]
+
+    def field(mean, amp):
+        f = np.full((n,) * nd, float(mean))
+        for _ in range(6):
+            k = rs.randint(-2, 3, nd)
+            f = f + amp * rs.uniform(0.3, 1.0) * np.cos(sum(k[d] * grids[d] for d in range(nd)) +
+                                                         rs.uniform(0.0, 2.0 * np.pi))
+        return f
+
+    rho = field(1.0, 0.08)
+    p = field(1.0 / gama, 0.05)
+    vel = [field(0.0, 0.25) for _ in range(nd)]
+    E = p / (gama - 1.0) + 0.5 * rho * sum(v * v for v in vel)
+    return np.stack([rho] + [rho * v for v in vel] + [E])
+
+
 def kinetic_energy_imbalance(q, R):
     """Relative discrete kinetic-energy production of the residual R at state
     q: sum over cells of u.R_rhou - |u|^2 R_rho / 2, divided by the sum of the

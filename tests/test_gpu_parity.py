@@ -12,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import check_parity, kinetic_energy_imbalance, uniform_rho_p_state
+from conftest import TOL, check_parity, generic_state, kinetic_energy_imbalance, parity_errors, uniform_rho_p_state
 
 pytestmark = pytest.mark.gpu
 
@@ -32,12 +32,48 @@ def _pair(problem, q0, oracle):
 
 
 @pytest.mark.parametrize("conv", ["central_skew", "central_cons", "weno_z", "weno_js", "teno6", "teno5"])
+def test_rhs_generic3d(conv, oracle):
+    """The residual (convective + viscous terms) at a state in which every
+    variable varies at order one, so that no component of it is a
+    cancellation: literal bar on every variable."""
+    from opensbli_amd.problems import taylor_green
+    _say(f"case rhs generic state conv={conv} n=24")
+    problem, _ = taylor_green(24, conv=conv)
+    problem["Minf"] = 1.0
+    q0 = generic_state(24, 3, seed=1)
+    g, r = _pair(problem, q0, oracle)
+    check_parity(g.rhs(), r.rhs(), f"rhs generic {conv}")
+
+
+@pytest.mark.parametrize("conv", ["central_skew", "central_cons", "weno_z", "weno_js", "teno6", "teno5"])
 def test_rhs_tgv3d(conv, oracle):
+    """The residual at the Taylor-Green vortex's initial state.  Its density
+    and energy components are small differences of large flux derivatives
+    (low Mach, divergence-free velocity), and with WENO5-Z the Z weights of
+    the nearly constant characteristic fields are rounding-sensitive there:
+    the oracle built with FMA contraction differs from itself by ~5e-10
+    (rho) and ~8e-10 (rho*E).  Those two components of WENO5-Z are held to
+    twice the oracle's own spread, measured here; every other component and
+    scheme to the literal bar.  The bar that applies to the Taylor-Green
+    workload, the state after 100 steps, is literal for every scheme
+    (test_run_tgv3d_100_steps)."""
     from opensbli_amd.problems import taylor_green
     _say(f"case rhs tgv3d conv={conv} n=24")
     problem, q0 = taylor_green(24, conv=conv)
     g, r = _pair(problem, q0, oracle)
-    check_parity(g.rhs(), r.rhs(), f"rhs {conv}")
+    rr = r.rhs()
+    err = parity_errors(g.rhs(), rr)
+    bar = np.full(len(err), TOL)
+    if conv == "weno_z":
+        f = oracle.RefSolver(problem, variant="fma")
+        f.set_state(q0)
+        spread = parity_errors(f.rhs(), rr)
+        _say("oracle FMA-contraction spread per variable: " + " ".join("%.2e" % e for e in spread))
+        bar[0] = max(TOL, 2.0 * spread[0])
+        bar[-1] = max(TOL, 2.0 * spread[-1])
+    _say(f"rhs tgv {conv} rel L2 per variable: " + " ".join("%.2e" % e for e in err) +
+         " (bars " + " ".join("%.1e" % b for b in bar) + ")")
+    assert np.all(err <= bar), (err, bar)
 
 
 @pytest.mark.parametrize("nd", [2, 3])
