@@ -294,6 +294,22 @@ int osbl_step_stages_dev(osbl_ctx* ctx, double* stages);
 int osbl_adjoint_step_dev(osbl_ctx* ctx, const double* stages, double* lambda);
 
 /*
+ * osbl_adjoint_step_dev_noalloc: osbl_adjoint_step_dev without any device
+ * allocation (no counterpart in the reference).  For callers that cannot
+ * allocate: code under HIP graph capture, tight backward loops, harnesses
+ * that forbid hipMalloc.  Its work buffers (the three stage adjoints, the
+ * characteristic VJP's partial sums, the viscous VJP's fields) come from a
+ * fixed device array of OSBL_NOALLOC_SCRATCH_MIB MiB that each device running
+ * the library holds until the process ends; a block needing more returns
+ * OSBL_E_ARG.  Every call on one device shares that array (its contents are
+ * the previous call's; what must start at zero is cleared by a kernel on the
+ * context's stream): calls that overlap on one device race.  Same arguments,
+ * kernels and results as osbl_adjoint_step_dev.
+ */
+#define OSBL_NOALLOC_SCRATCH_MIB 128
+int osbl_adjoint_step_dev_noalloc(osbl_ctx* ctx, const double* stages, double* lambda);
+
+/*
  * Ensembles of independent 1-D problems of uneven size (characteristic
  * WENO/TENO fluxes, zeroth-order extrapolation ends, RungeKutta(3) or
  * RungeKuttaLS(3)).  No counterpart in the reference, which would build and

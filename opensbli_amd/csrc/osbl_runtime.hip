@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "osbl_kernels.h"
@@ -1201,10 +1202,35 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
 
 int osbl_num_stages(osbl_ctx* c) { return c ? c->nstages : fail(OSBL_E_ARG, "null context"); }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------------------
-// discrete adjoint (inviscid central skew or LLF WENO5-Z, periodic, RK3)
+// Fixed device scratch of the allocation-free adjoint step.  One array per
+// device (each device that runs the library's kernels holds its own copy
+// until the process ends); host code addresses it only through the pointer
+// hipGetSymbolAddress returns for that device.  Every call on a device
+// shares that device's array: calls that overlap on one device race.
 // ---------------------------------------------------------------------------
-static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
+constexpr size_t kScratchBytes = (size_t)OSBL_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// discrete adjoint (central skew or LLF WENO5-Z, constant viscosity, periodic, RK3)
+// ---------------------------------------------------------------------------
+// the work buffers of one adjoint call
+struct AdjBufs {
+  double* Abuf;   // stage state with halos (nv padded fields)
+  double* Mbuf;   // adjoint weights with halos (nv padded fields)
+  double* pk;     // 3 packed arrays: lam_Q2, lam_Q1, lam_Qo
+  double* vpart;  // per-segment partial sums of the characteristic VJP
+  double* vw;     // viscous VJP fields (padded)
+  double* P;      // constituent relations over the padded grid
+  int vseg;       // segment length of the characteristic VJP sweeps
+};
+
+static int adjoint_check(const osbl_ctx* c, bool need_rk3) {
   if (c->cfg.conv != OSBL_CONV_CENTRAL_SKEW && c->cfg.conv != OSBL_CONV_WENO_Z)
     return fail(OSBL_E_ARG, "the adjoint covers the central skew-symmetric and WENO5-Z schemes "
                             "(TENO's cut-off is not differentiable)");
@@ -1215,6 +1241,19 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
   for (int d = 0; d < c->nd; ++d)
     if (c->cfg.bc[d][0] != OSBL_BC_PERIODIC) return fail(OSBL_E_ARG, "the adjoint needs periodic boundaries");
   if (need_rk3 && c->cfg.rk != OSBL_RK3) return fail(OSBL_E_ARG, "the adjoint step covers RungeKutta(3)");
+  return OSBL_OK;
+}
+
+// which buffers the context's scheme uses
+static bool adj_needs_halo_bufs(const osbl_ctx* c) {
+  return c->cfg.conv == OSBL_CONV_CENTRAL_SKEW || c->cfg.viscous == OSBL_VISC_CONST;
+}
+static bool adj_needs_prims(const osbl_ctx* c) { return adj_needs_halo_bufs(c); }
+
+// allocating set-up: the context's own buffers, made on first use
+static int adjoint_ready(osbl_ctx* c, bool need_rk3, AdjBufs* b) {
+  int rc = adjoint_check(c, need_rk3);
+  if (rc) return rc;
   if (!c->Abuf) {
     const size_t bundle = (size_t)c->g.fs * c->nv * sizeof(double);
     const size_t packed = (size_t)cells(c) * c->nv * sizeof(double);
@@ -1229,55 +1268,162 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3) {
     if (hipMalloc((void**)&c->vw, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint viscous scratch");
     HIPCHK(hipMemsetAsync(c->vw, 0, bytes, c->stream));
   }
+  if (adj_needs_prims(c) && !c->P) {
+    // 3-D characteristic blocks evaluate their viscous terms without a
+    // constituent-relation array; the viscous VJP needs one
+    const size_t bytes = (size_t)c->g.fs * (c->nd + 3) * sizeof(double);
+    if (hipMalloc((void**)&c->P, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint constituent relations");
+    HIPCHK(hipMemsetAsync(c->P, 0, bytes, c->stream));
+  }
   if (c->cfg.conv == OSBL_CONV_WENO_Z && !c->vpart) {
     c->vseg = adj_char_segment(c->g, c->vjp_seg);
     const size_t bytes = (size_t)adj_char_scratch(c->g, c->vseg) * sizeof(double);
     if (hipMalloc((void**)&c->vpart, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint scratch");
   }
+  *b = AdjBufs{c->Abuf, c->Mbuf, c->pk, c->vpart, c->vw, c->P, c->vseg};
   return OSBL_OK;
 }
 
 // out (packed) += (dR/dQ at q)^T mu
-static void vjp(osbl_ctx* c, const double* q, const double* mu, double* out) {
+static void vjp(osbl_ctx* c, const AdjBufs& b, const double* q, const double* mu, double* out) {
   const long long n = cells(c);
-  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, c->Abuf + m * c->g.fs, c->g, c->stream);
-  launch_halo_periodic(c->Abuf, c->nv, c->g, c->stream);
-  launch_prims(c->nd, c->Abuf, c->P, c->g, c->ph, c->stream);
-  for (int m = 0; m < c->nv; ++m) launch_unpack(mu + m * n, c->Mbuf + m * c->g.fs, c->g, c->stream);
-  launch_halo_periodic(c->Mbuf, c->nv, c->g, c->stream);
-  launch_rhs_vjp(c->nd, c->Abuf, c->P, c->Mbuf, out, c->g, c->ph, c->stream);
+  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, b.Abuf + m * c->g.fs, c->g, c->stream);
+  launch_halo_periodic(b.Abuf, c->nv, c->g, c->stream);
+  launch_prims(c->nd, b.Abuf, b.P, c->g, c->ph, c->stream);
+  for (int m = 0; m < c->nv; ++m) launch_unpack(mu + m * n, b.Mbuf + m * c->g.fs, c->g, c->stream);
+  launch_halo_periodic(b.Mbuf, c->nv, c->g, c->stream);
+  launch_rhs_vjp(c->nd, b.Abuf, b.P, b.Mbuf, out, c->g, c->ph, c->stream);
 }
 
 // out (packed) += (dV/dQ at q)^T (alpha A + beta B) for the viscous terms V
-static void visc_vjp(osbl_ctx* c, const double* q, const double* A, const double* B, double alpha, double beta,
-                     double* out) {
+static void visc_vjp(osbl_ctx* c, const AdjBufs& b, const double* q, const double* A, const double* B, double alpha,
+                     double beta, double* out) {
   const long long n = cells(c);
-  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, c->Abuf + m * c->g.fs, c->g, c->stream);
-  launch_halo_periodic(c->Abuf, c->nv, c->g, c->stream);
-  launch_prims(c->nd, c->Abuf, c->P, c->g, c->ph, c->stream);
-  launch_visc_vjp(c->nd, c->Abuf, c->P, A, B, alpha, beta, c->Mbuf, c->vw, out, c->g, c->ph, c->stream);
+  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, b.Abuf + m * c->g.fs, c->g, c->stream);
+  launch_halo_periodic(b.Abuf, c->nv, c->g, c->stream);
+  launch_prims(c->nd, b.Abuf, b.P, c->g, c->ph, c->stream);
+  launch_visc_vjp(c->nd, b.Abuf, b.P, A, B, alpha, beta, b.Mbuf, b.vw, out, c->g, c->ph, c->stream);
 }
 
 // out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B), out2 = A + B (optional)
-static void adj_stage(osbl_ctx* c, const double* Qs, const double* A, const double* B, double alpha, double beta,
-                      double cc, double* out1, double* out2) {
+static void adj_stage(osbl_ctx* c, const AdjBufs& b, const double* Qs, const double* A, const double* B, double alpha,
+                      double beta, double cc, double* out1, double* out2) {
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
-    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, c->vpart, c->g, c->ph,
-                          c->vseg, false, c->stream);
+    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, b.vpart, c->g, c->ph,
+                          b.vseg, false, c->stream);
   else
     launch_adj_stage(c->nd, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->stream);
-  if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, Qs, A, B, alpha, beta, out1);
+  if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, b, Qs, A, B, alpha, beta, out1);
 }
+
+// one RK3 step backwards: lam <- (dQ_{n+1}/dQ_n)^T lam
+static int adjoint_step(osbl_ctx* c, const AdjBufs& bufs, const double* stages_dev, double* lam) {
+  const long long n = cells(c) * c->nv;
+  const double* Qn = stages_dev;
+  const double* Q1 = stages_dev + n;
+  const double* Q2 = stages_dev + 2 * n;
+  double* l2 = bufs.pk;
+  double* l1 = bufs.pk + n;
+  double* lo = bufs.pk + 2 * n;
+  const double dt = c->ph.dt;
+  const double* a = c->A;  // rknew
+  const double* b = c->B;  // rkold
+  // Q3 = Qo2 + dt a2 R(Q2), Qo2 = lam's other path:  lam_Q2 = VJP(Q2; dt a2 lam)
+  adj_stage(c, bufs, Q2, lam, nullptr, dt * a[2], 0.0, 0.0, l2, nullptr);
+  // Q2 = Qo1 + dt a1 R(Q1), Qo2 = Qo1 + dt b1 R(Q1):
+  //   lam_Q1 = VJP(Q1; dt a1 lam_Q2 + dt b1 lam_Qo2),  lam_Qo1 = lam_Q2 + lam_Qo2
+  adj_stage(c, bufs, Q1, l2, lam, dt * a[1], dt * b[1], 0.0, l1, lo);
+  // Q1 = Qn + dt a0 R(Qn), Qo1 = Qn + dt b0 R(Qn):
+  //   lam_Qn = lam_Q1 + lam_Qo1 + VJP(Qn; dt a0 lam_Q1 + dt b0 lam_Qo1)
+  adj_stage(c, bufs, Qn, l1, lo, dt * a[0], dt * b[0], 1.0, lam, nullptr);
+  HIPCHK(hipGetLastError());
+  return OSBL_OK;
+}
+
+// per-device address of the scratch array, looked up once per device
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// the scratch array's device address on device dev (made current for the lookup)
+static int scratch_address(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(OSBL_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess) return fail(OSBL_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return OSBL_OK;
+}
+
+__global__ void __launch_bounds__(256) k_zero(double* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// the adjoint step's buffers carved from the scratch array (256-byte
+// aligned pieces); the pieces the allocating set-up zeroes are cleared here
+// by a fill kernel, since the array keeps the previous call's contents
+static int scratch_bufs(osbl_ctx* c, AdjBufs* b) {
+  double* base = nullptr;
+  int rc = scratch_address(c->device, &base);
+  if (rc) return rc;
+  const long long fs = c->g.fs, packed = cells(c) * c->nv;
+  const int vseg = adj_char_segment(c->g, c->vjp_seg);
+  long long off = 0;
+  auto take = [&](long long ndoubles) {
+    const long long at = off;
+    off += (ndoubles + 31) / 32 * 32;
+    return at;
+  };
+  const long long o_pk = take(3 * packed);
+  const bool halo = adj_needs_halo_bufs(c);
+  const long long o_A = halo ? take(fs * c->nv) : -1;
+  const long long o_M = halo ? take(fs * c->nv) : -1;
+  const bool weno = c->cfg.conv == OSBL_CONV_WENO_Z;
+  const long long o_part = weno ? take(adj_char_scratch(c->g, vseg)) : -1;
+  const bool visc = c->cfg.viscous == OSBL_VISC_CONST;
+  const long long o_vw = visc ? take(fs * visc_vjp_fields(c->nd)) : -1;
+  const long long o_P = adj_needs_prims(c) ? take(fs * (c->nd + 3)) : -1;
+  const size_t need = (size_t)off * sizeof(double);
+  if (need > kScratchBytes)
+    return fail(OSBL_E_ARG, "osbl_adjoint_step_dev_noalloc: this block needs %zu bytes of scratch, more than the "
+                            "%zu-byte fixed scratch (use osbl_adjoint_step_dev)", need, kScratchBytes);
+  auto at = [&](long long o) { return o < 0 ? nullptr : base + o; };
+  *b = AdjBufs{at(o_A), at(o_M), at(o_pk), at(o_part), at(o_vw), at(o_P), vseg};
+  // zero everything from the first halo buffer on (A, M, the partial sums,
+  // the viscous fields, P): the allocating path's buffers start zeroed
+  if (halo || weno || visc) {
+    const long long from = halo ? o_A : (weno ? o_part : o_vw);
+    const long long n = off - from;
+    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    k_zero<<<blocks, 256, 0, c->stream>>>(base + from, n);
+    HIPCHK(hipGetLastError());
+  }
+  return OSBL_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int osbl_rhs_vjp_dev(osbl_ctx* c, const double* q_dev, const double* mu_dev, double* out_dev) {
   if (!c || !q_dev || !mu_dev || !out_dev) return fail(OSBL_E_ARG, "null argument");
-  int rc = adjoint_ready(c, false);
+  AdjBufs b;
+  int rc = adjoint_ready(c, false, &b);
   if (rc) return rc;
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
-    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, c->vpart,
-                          c->g, c->ph, c->vseg, true, c->stream);
-  else vjp(c, q_dev, mu_dev, out_dev);
-  if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, q_dev, mu_dev, nullptr, 1.0, 0.0, out_dev);
+    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, b.vpart,
+                          c->g, c->ph, b.vseg, true, c->stream);
+  else vjp(c, b, q_dev, mu_dev, out_dev);
+  if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, b, q_dev, mu_dev, nullptr, 1.0, 0.0, out_dev);
   HIPCHK(hipGetLastError());
   return OSBL_OK;
 }
@@ -1299,28 +1445,19 @@ int osbl_step_stages_dev(osbl_ctx* c, double* stages_dev) {
 
 int osbl_adjoint_step_dev(osbl_ctx* c, const double* stages_dev, double* lam) {
   if (!c || !stages_dev || !lam) return fail(OSBL_E_ARG, "null argument");
-  int rc = adjoint_ready(c, true);
+  AdjBufs b;
+  int rc = adjoint_ready(c, true, &b);
   if (rc) return rc;
-  const long long n = cells(c) * c->nv;
-  const double* Qn = stages_dev;
-  const double* Q1 = stages_dev + n;
-  const double* Q2 = stages_dev + 2 * n;
-  double* l2 = c->pk;
-  double* l1 = c->pk + n;
-  double* lo = c->pk + 2 * n;
-  const double dt = c->ph.dt;
-  const double* a = c->A;  // rknew
-  const double* b = c->B;  // rkold
-  // Q3 = Qo2 + dt a2 R(Q2), Qo2 = lam's other path:  lam_Q2 = VJP(Q2; dt a2 lam)
-  adj_stage(c, Q2, lam, nullptr, dt * a[2], 0.0, 0.0, l2, nullptr);
-  // Q2 = Qo1 + dt a1 R(Q1), Qo2 = Qo1 + dt b1 R(Q1):
-  //   lam_Q1 = VJP(Q1; dt a1 lam_Q2 + dt b1 lam_Qo2),  lam_Qo1 = lam_Q2 + lam_Qo2
-  adj_stage(c, Q1, l2, lam, dt * a[1], dt * b[1], 0.0, l1, lo);
-  // Q1 = Qn + dt a0 R(Qn), Qo1 = Qn + dt b0 R(Qn):
-  //   lam_Qn = lam_Q1 + lam_Qo1 + VJP(Qn; dt a0 lam_Q1 + dt b0 lam_Qo1)
-  adj_stage(c, Qn, l1, lo, dt * a[0], dt * b[0], 1.0, lam, nullptr);
-  HIPCHK(hipGetLastError());
-  return OSBL_OK;
+  return adjoint_step(c, b, stages_dev, lam);
+}
+
+int osbl_adjoint_step_dev_noalloc(osbl_ctx* c, const double* stages_dev, double* lam) {
+  if (!c || !stages_dev || !lam) return fail(OSBL_E_ARG, "null argument");
+  int rc = adjoint_check(c, true);
+  if (rc) return rc;
+  AdjBufs b;
+  if ((rc = scratch_bufs(c, &b))) return rc;
+  return adjoint_step(c, b, stages_dev, lam);
 }
 
 }  // extern "C"

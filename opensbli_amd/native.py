@@ -28,7 +28,8 @@ EXPORTS = [
     "osbl_profile", "osbl_kernel_stats", "osbl_work_per_stage", "osbl_kernel_work", "osbl_num_stages",
     "osbl_batch_create", "osbl_batch_destroy", "osbl_batch_set_state", "osbl_batch_get_state", "osbl_batch_run",
     "osbl_batch_set_threads", "osbl_batch_set_graph_steps", "osbl_batch_save", "osbl_batch_restore",
-    "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_use_graphs",
+    "osbl_rhs_vjp_dev", "osbl_step_stages_dev", "osbl_adjoint_step_dev", "osbl_adjoint_step_dev_noalloc",
+    "osbl_use_graphs",
     "osbl_set_coordinates", "osbl_set_bc_plane", "osbl_set_filter_mask", "osbl_get_sfd_field",
     "osbl_set_sfd_field", "osbl_get_field_halo", "osbl_set_grid_coordinates", "osbl_get_metric_field",
     "osbl_get_sensor",
@@ -136,6 +137,7 @@ def load():
     L.osbl_rhs_vjp_dev.argtypes = [vp, vp, vp, vp]
     L.osbl_step_stages_dev.argtypes = [vp, vp]
     L.osbl_adjoint_step_dev.argtypes = [vp, vp, vp]
+    L.osbl_adjoint_step_dev_noalloc.argtypes = [vp, vp, vp]
     L.osbl_use_graphs.argtypes = [vp, ctypes.c_int]
     L.osbl_set_coordinates.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_set_bc_plane.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
@@ -406,6 +408,10 @@ class Block:
 
     def adjoint_step_dev(self, stages_ptr, lam_ptr):
         check(load().osbl_adjoint_step_dev(self.h, ctypes.c_void_p(stages_ptr), ctypes.c_void_p(lam_ptr)))
+
+    def adjoint_step_dev_noalloc(self, stages_ptr, lam_ptr):
+        """adjoint_step_dev with its work buffers in the library's fixed device scratch (no allocation)."""
+        check(load().osbl_adjoint_step_dev_noalloc(self.h, ctypes.c_void_p(stages_ptr), ctypes.c_void_p(lam_ptr)))
 
     def num_stages(self):
         return check(load().osbl_num_stages(self.h))
