@@ -122,8 +122,10 @@ def main():
     ap.add_argument("--conv", default="weno_z", choices=["weno_z", "weno_js", "teno6", "teno5", "central_skew"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--rk", default="rk3", choices=["rk3", "ls3", "ls3_ssp", "ls4"],
+                    help="tgv: RungeKutta(3) (3-register) or a low-storage scheme")
     ap.add_argument("--workload", default="tgv", choices=["tgv", "ensemble", "flat_plate", "adjoint",
-                                                           "curvilinear"])
+                                                           "curvilinear", "cpu"])
     ap.add_argument("--no-adjoint", action="store_true", help="tgv: skip the adjoint/forward ratio measurement")
     ap.add_argument("--no-sub", action="store_true", help="tgv: skip the ensemble and flat-plate sub-results")
     ap.add_argument("--plain-stride", type=int, default=16,
@@ -140,6 +142,8 @@ def main():
         return bench_flat_plate(args)
     if args.workload == "curvilinear":
         return bench_curvilinear(args)
+    if args.workload == "cpu":
+        return bench_cpu(args)
     if args.workload == "adjoint":
         out = adjoint_ratio()
         out["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
@@ -152,11 +156,11 @@ def main():
     from opensbli_amd.problems import taylor_green
 
     n = args.n
-    problem, q0 = taylor_green(n, conv=args.conv)
+    problem, q0 = taylor_green(n, conv=args.conv, rk=args.rk)
     blk = Block(problem)
     blk.set_state(q0)
     stages = blk.num_stages()
-    _log(f"bench: TGV {n}^3 {args.conv} RK3, warmup {args.warmup} steps")
+    _log(f"bench: TGV {n}^3 {args.conv} {args.rk}, warmup {args.warmup} steps")
     blk.run(args.warmup)
     blk.sync()
 
@@ -249,7 +253,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (analytic Taylor-Green initial state)",
-        "config": {"workload": f"taylor_green_vortex_{n}^3_{args.conv}_roe_viscous_rk3", "grid": [n, n, n],
+        "config": {"workload": f"taylor_green_vortex_{n}^3_{args.conv}_roe_viscous_{args.rk}", "grid": [n, n, n],
                    "scheme": args.conv, "rk_stages": stages, "Re": 1600.0, "Minf": 0.1},
         "roofline": roofline,
         "roofline_hbm": roofline_hbm,
@@ -436,6 +440,78 @@ def ensemble_result(plain_stride=16):
 def bench_ensemble(args):
     out = ensemble_result(args.plain_stride)
     out.update(n_gpus=1, higher_is_better=True, dtype="f64")
+    print(json.dumps(out), flush=True)
+
+
+def _oracle_instance(i, par):
+    """Run ensemble instance i on the oracle to its step count (process-pool worker)."""
+    from opensbli_amd.ensemble import instance_problem
+    from oracle.ref import RefSolver
+    problem, q0 = instance_problem(par, i)
+    r = RefSolver(problem)
+    r.set_state(q0)
+    r.run(int(par["nsteps"][i]))
+    return i
+
+
+def bench_cpu(args):
+    """The oracle on this host for the other configurations, each as one
+    warm-up run then the median of three timed runs: the 64^3 central TGV
+    (RungeKuttaLS(3), 20 steps), the 512^2 Kelvin-Helmholtz layer (central
+    skew, RK3, 20 steps), the 480x256x128 flat plate (1 step) and the
+    512-tube ensemble (every 8th instance to t = 0.2, one single-threaded
+    oracle process per core)."""
+    from opensbli_amd.ensemble import ensemble_parameters
+    from opensbli_amd.problems import flat_plate, kelvin_helmholtz, taylor_green
+    from oracle.ref import RefSolver
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    out = {"metric": "oracle Mcell-updates/s per RK stage on the host", "unit": "Mcell-updates/s",
+           "cores": threads, "kind": "port", "host": host_cpu(), "cases": {}}
+
+    def timed_case(name, problem, q0, steps):
+        _log(f"bench: cpu {name}")
+        s = RefSolver(problem)
+        s.set_state(q0)
+        dt, ts = _median_timed(s, steps)
+        cells = int(np.prod(problem["n"][: problem["ndim"]]))
+        stages = {"rk3": 3, "ls3": 3, "ls3_ssp": 3, "ls4": 4}[problem.get("rk", "rk3")]
+        out["cases"][name] = {"value": cells * stages * steps / dt / 1e6, "steps": steps,
+                              "runs_s": [round(t, 3) for t in ts]}
+
+    p, q = taylor_green(64, conv="central_skew", rk="ls3")
+    timed_case("C1 taylor_green_64^3_central_skew_ls3", p, q, 20)
+    amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
+    p, q = kelvin_helmholtz(512, amps, conv="central_skew")
+    timed_case("C4 kelvin_helmholtz_512^2_central_skew_rk3", p, q, 20)
+    p, q = flat_plate()
+    timed_case("C5 flat_plate_480x256x128_weno_z_sutherland_tanh5_ls3", p, q, 1)
+    del q
+    # the tubes are independent: one single-threaded oracle process per core,
+    # instances handed out in order of decreasing work
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    par = ensemble_parameters(512, seed=3)
+    idx = list(range(0, 512, 8))
+    work = par["n"].astype(float) * par["nsteps"]
+    idx.sort(key=lambda i: -work[i])
+    _log(f"bench: cpu C3 ensemble sample ({len(idx)} instances on {threads} single-threaded processes)")
+    omp = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        with ProcessPoolExecutor(max_workers=threads, mp_context=mp.get_context("spawn")) as ex:
+            list(ex.map(_oracle_instance, [0], [par]))  # start the workers
+            t0 = time.perf_counter()
+            list(ex.map(_oracle_instance, idx, [par] * len(idx)))
+            el = time.perf_counter() - t0
+    finally:
+        if omp is None:
+            del os.environ["OMP_NUM_THREADS"]
+        else:
+            os.environ["OMP_NUM_THREADS"] = omp
+    out["cases"]["C3 ensemble_512_shock_tubes_weno_z_rk3"] = {
+        "value": 3 * work[idx].sum() / el / 1e6, "sample": f"every 8th instance ({len(idx)} of 512, "
+        f"{work[idx].sum() / work.sum():.3f} of the cell updates) to t = 0.2 on {threads} single-threaded oracle "
+        f"processes in {el:.1f} s", "estimated_all_512_s": el * work.sum() / work[idx].sum()}
     print(json.dumps(out), flush=True)
 
 

@@ -17,6 +17,13 @@ void launch_update(int nd, int conv, bool visc, int mode, const double* Q, const
                    double* Qn, const Grid& g, const Phys& ph, double cA, double cB, int first, hipStream_t s);
 void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g, const Phys& ph,
                           double cA, double cB, int first, int sz, hipStream_t s);
+void launch_conv3d(int conv, int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g,
+                   const Phys& ph, double cA, double cB, int first, int sz, hipStream_t s);
+// z segments of the tiled viscous and central-convective kernels for a grid,
+// and the convective kernel's block count at segment sz
+int visc3d_segment(const Grid& g);
+int conv3d_segment(const Grid& g);
+long long conv3d_blocks(const Grid& g, int sz);
 void launch_update_gen(int nd, int conv, bool visc, int mode, const double* Q, const double* P, double* R,
                        double* Qo, double* Qn, const Grid& g, const Phys& ph, const Bnd& b, double cA, double cB,
                        int first, int suth, hipStream_t s);

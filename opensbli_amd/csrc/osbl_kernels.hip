@@ -650,7 +650,7 @@ __global__ void __launch_bounds__(256) k_update(const double* __restrict__ Q, co
 // plane z-1 vacated, and one barrier per plane separates those writes from
 // the next plane's reads.
 // ---------------------------------------------------------------------------
-constexpr int VTX = 64, VTY = 4, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
+constexpr int VTX = 64, VTY = 8, VPX = VTX + 4, VPY = VTY + 4, VPL = VPX * VPY;
 constexpr int VNT = VTX * VTY;                 // threads per block
 constexpr int VPP = (VPL + VNT - 1) / VNT;     // staged points per thread and plane
 
@@ -875,6 +875,267 @@ void launch_visc_update3d(int mode, const double* Q, double* R, double* Qo, doub
   else if (mode == 1) k_visc_update3d<1><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
   else if (mode == 2) k_visc_update3d<2><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
   else k_visc_update3d<3><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz);
+}
+
+// ---------------------------------------------------------------------------
+// 3-D central convective terms, tiled 2.5-D.  Skew-symmetric (Blaisdell) form
+// (parsing.py:75-111, scheme.py:187-252) or the conservative flux form
+// (parsing.py:189-216), with the formulas of central_convective.  A CTX x CTY
+// block of (x, y) columns marches along z.  The plane being evaluated is
+// staged in LDS (Q, u_x, u_y, p with a 2-point x/y halo, the x/y stencils);
+// each thread keeps a register queue of its own column's Q, u_z, p over the
+// five planes z-2..z+2 (the z stencils).  Software pipeline: the raw Q of the
+// next plane's staged points and of the column point three planes ahead are
+// loaded at the top of an iteration and consumed after its arithmetic; two
+// LDS slots alternate, one barrier per plane.
+//   MODE 0/1: RK update (no viscous terms); MODE 2: the terms are assigned to
+//   R (the viscous kernel then adds its terms and applies the update).
+// ---------------------------------------------------------------------------
+constexpr int CTX = 64, CTY = 4, CPX = CTX + 4, CPY = CTY + 4, CPL = CPX * CPY;
+constexpr int CNT = CTX * CTY;                 // threads per block
+constexpr int CPP = (CPL + CNT - 1) / CNT;     // staged points per thread and plane
+constexpr int CNF = 8;                         // staged fields: Q0..Q4, u_x, u_y, p
+
+template <int CONV, int MODE>
+__global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q, double* __restrict__ R,
+                                                   double* __restrict__ Qo, double* __restrict__ Qn, Grid g, Phys ph,
+                                                   double cA, double cB, int first, int sz) {
+  constexpr int NV = 5;
+  __shared__ double sp[2][CNF][CPL];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * CTX + tx;
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const int x0 = bx * CTX, y0 = by * CTY;
+  const int z0 = bz * sz;
+  const int z1 = min(z0 + sz, g.n[2]);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool inside = x < g.n[0] && y < g.n[1];
+
+  int sidx[CPP], soff[CPP];
+  bool sval[CPP];
+#pragma unroll
+  for (int k = 0; k < CPP; ++k) {
+    const int p = tid + k * CNT;
+    const int xx = p % CPX, yy = p / CPX;
+    const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
+    sidx[k] = p;
+    sval[k] = p < CPL && gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1];
+    soff[k] = sval[k] ? g.idx(gx, gy, 0) : 0;
+  }
+  double pq[CPP][NV];  // raw Q of the next plane's staged points
+  auto fetch = [&](int zp) {
+    const int zo = zp * g.plane;
+#pragma unroll
+    for (int k = 0; k < CPP; ++k)
+      if (sval[k]) {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) pq[k][m] = Q[m * g.fs + soff[k] + zo];
+      }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < CPP; ++k)
+      if (sval[k]) {
+        const double rho = pq[k][0];
+        const double ir = 1.0 / rho;
+        const double u0 = pq[k][1] * ir, u1 = pq[k][2] * ir, u2 = pq[k][3] * ir;
+        const double pr = ph.gm1 * (pq[k][4] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
+        const int p = sidx[k];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) sp[slot][m][p] = pq[k][m];
+        sp[slot][5][p] = u0;
+        sp[slot][6][p] = u1;
+        sp[slot][7][p] = pr;
+      }
+  };
+  // this thread's column: raw Q of one plane, and its queue entries Q0..Q4, u_z, p
+  const int col = inside ? g.idx(x, y, 0) : 0;
+  double oq[NV];
+  auto fetch_col = [&](int zp) {
+    if (inside) {
+      const int o = col + zp * g.plane;
+#pragma unroll
+      for (int m = 0; m < NV; ++m) oq[m] = Q[m * g.fs + o];
+    }
+  };
+  auto col_entry = [&](double* e) {
+    const double rho = oq[0];
+    const double ir = 1.0 / rho;
+    const double u0 = oq[1] * ir, u1 = oq[2] * ir, u2 = oq[3] * ir;
+#pragma unroll
+    for (int m = 0; m < NV; ++m) e[m] = oq[m];
+    e[5] = u2;
+    e[6] = ph.gm1 * (oq[4] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
+  };
+  const int c = (ty + 2) * CPX + tx + 2;  // this thread's point in a slot
+
+  double zq[5][7];  // planes z-2..z+2 of this column
+  // prologue: the queue for z0 (planes z0-2 .. z0+2), plane z0 staged
+  for (int k = 0; k < 5; ++k) {
+    fetch_col(z0 - 2 + k);
+    if (inside) col_entry(zq[k]);
+  }
+  fetch(z0);
+  store(z0 & 1);
+  if (z0 + 1 < z1) {  // in flight during the first plane
+    fetch(z0 + 1);
+    fetch_col(z0 + 3);
+  }
+  __syncthreads();
+
+  constexpr double W1A = 1.0 / 12.0, W1B = 2.0 / 3.0;
+  for (int z = z0; z < z1; ++z) {
+    const bool more = z + 1 < z1;
+    const int o = g.idx(x, y, z);
+    if (inside) {
+      // RK registers of the cell, in flight during the arithmetic
+      [[maybe_unused]] double qa[NV], qb[NV];
+#pragma unroll
+      for (int m = 0; m < NV; ++m) {
+        if constexpr (MODE == 0) qa[m] = first ? 0.0 : Qo[m * g.fs + o];
+        if constexpr (MODE == 1) qa[m] = cA == 0.0 ? 0.0 : Qo[m * g.fs + o];
+      }
+      const int sl = z & 1;
+      double r[NV];
+#pragma unroll
+      for (int m = 0; m < NV; ++m) r[m] = 0.0;
+      // x and y from LDS (stride 1 and CPX), then z from the queue
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double inv = ph.inv_delta[j];
+        const double wts[4] = {W1A, -W1B, W1B, -W1A};
+        auto qv = [&](int q, int m) -> double {  // Q_m at stencil point q (offsets -2,-1,+1,+2)
+          if (j == 2) return zq[q < 2 ? q : q + 1][m];
+          const int st = j == 0 ? 1 : CPX;
+          const int off = (q < 2 ? q - 2 : q - 1) * st;
+          return sp[sl][m][c + off];
+        };
+        auto uv = [&](int q) -> double {  // u_j at stencil point q
+          if (j == 2) return zq[q < 2 ? q : q + 1][5];
+          const int st = j == 0 ? 1 : CPX;
+          const int off = (q < 2 ? q - 2 : q - 1) * st;
+          return sp[sl][5 + j][c + off];
+        };
+        auto pv = [&](int q) -> double {
+          if (j == 2) return zq[q < 2 ? q : q + 1][6];
+          const int st = j == 0 ? 1 : CPX;
+          const int off = (q < 2 ? q - 2 : q - 1) * st;
+          return sp[sl][7][c + off];
+        };
+        if constexpr (CONV == OSBL_CONV_CENTRAL_SKEW) {
+          double d_ru[NV], d_q[NV], d_uj = 0.0, d_p = 0.0, d_puj = 0.0;
+#pragma unroll
+          for (int m = 0; m < NV; ++m) { d_ru[m] = 0.0; d_q[m] = 0.0; }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double w = wts[q];
+            const double uj = uv(q);
+            const double pp = pv(q);
+#pragma unroll
+            for (int m = 0; m < NV; ++m) {
+              const double qm = qv(q, m);
+              d_ru[m] += w * (qm * uj);
+              d_q[m] += w * qm;
+            }
+            d_uj += w * uj;
+            d_p += w * pp;
+            d_puj += w * (pp * uj);
+          }
+          const double uj = j == 2 ? zq[2][5] : sp[sl][5 + j][c];
+          const double duj = inv * d_uj;
+#pragma unroll
+          for (int m = 0; m < NV; ++m) {
+            const double qm = zq[2][m];
+            double t = -0.5 * (inv * d_ru[m] + uj * (inv * d_q[m]) + qm * duj);
+            if (m == 1 + j) t -= inv * d_p;
+            if (m == NV - 1) t -= inv * d_puj;
+            r[m] += t;
+          }
+        } else {
+          double dF[NV];
+#pragma unroll
+          for (int m = 0; m < NV; ++m) dF[m] = 0.0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double w = wts[q];
+            const double uj = uv(q);
+            const double pp = pv(q);
+            dF[0] += w * qv(q, 1 + j);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) dF[1 + m] += w * (qv(q, 1 + m) * uj + (m == j ? pp : 0.0));
+            dF[NV - 1] += w * ((pp + qv(q, NV - 1)) * uj);
+          }
+#pragma unroll
+          for (int m = 0; m < NV; ++m) r[m] -= inv * dF[m];
+        }
+      }
+      if constexpr (MODE == 2) {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) R[m * g.fs + o] = r[m];
+      } else if constexpr (MODE == 0) {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          const double qo = first ? zq[2][m] : qa[m];
+          Qn[m * g.fs + o] = qo + ph.dt * cA * r[m];
+          Qo[m * g.fs + o] = qo + ph.dt * cB * r[m];
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+          const double tmp = (cA == 0.0 ? 0.0 : cA * qa[m]) + ph.dt * r[m];
+          Qo[m * g.fs + o] = tmp;
+          Qn[m * g.fs + o] = zq[2][m] + cB * tmp;
+        }
+      }
+    }
+    if (more) {
+      // plane z+1 goes into the slot plane z-1 left (last read before the
+      // previous barrier); the queue moves on by one plane
+      store((z + 1) & 1);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int e = 0; e < 7; ++e) zq[k][e] = zq[k + 1][e];
+      if (inside) col_entry(zq[4]);
+      if (z + 2 < z1) {  // planes up to z1 + 1 are read
+        fetch(z + 2);
+        fetch_col(z + 4);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// z segment of a 2.5-D tiled kernel over tx x ty columns: 32 planes, halved
+// (down to 8) while the launch would have fewer than `want` blocks
+static int tile_segment(const Grid& g, int tx, int ty, long long want) {
+  const long long tiles = (long long)((g.n[0] + tx - 1) / tx) * ((g.n[1] + ty - 1) / ty);
+  int sz = 32;
+  while (sz > 8 && tiles * ((g.n[2] + sz - 1) / sz) < want) sz /= 2;
+  return sz;
+}
+int visc3d_segment(const Grid& g) { return tile_segment(g, VTX, VTY, 1024); }
+int conv3d_segment(const Grid& g) { return tile_segment(g, CTX, CTY, 2048); }
+long long conv3d_blocks(const Grid& g, int sz) {
+  return (long long)((g.n[0] + CTX - 1) / CTX) * ((g.n[1] + CTY - 1) / CTY) * ((g.n[2] + sz - 1) / sz);
+}
+
+void launch_conv3d(int conv, int mode, const double* Q, double* R, double* Qo, double* Qn, const Grid& g,
+                   const Phys& ph, double cA, double cB, int first, int sz, hipStream_t s) {
+  dim3 grid((g.n[0] + CTX - 1) / CTX, (g.n[1] + CTY - 1) / CTY, (g.n[2] + sz - 1) / sz);
+  dim3 block(CTX, CTY);
+#define OSBL_CONV3D(C, M) k_conv3d<C, M><<<grid, block, 0, s>>>(Q, R, Qo, Qn, g, ph, cA, cB, first, sz)
+  if (conv == OSBL_CONV_CENTRAL_SKEW) {
+    if (mode == 0) OSBL_CONV3D(OSBL_CONV_CENTRAL_SKEW, 0);
+    else if (mode == 1) OSBL_CONV3D(OSBL_CONV_CENTRAL_SKEW, 1);
+    else OSBL_CONV3D(OSBL_CONV_CENTRAL_SKEW, 2);
+  } else {
+    if (mode == 0) OSBL_CONV3D(OSBL_CONV_CENTRAL_CONS, 0);
+    else if (mode == 1) OSBL_CONV3D(OSBL_CONV_CENTRAL_CONS, 1);
+    else OSBL_CONV3D(OSBL_CONV_CENTRAL_CONS, 2);
+  }
+#undef OSBL_CONV3D
 }
 
 // interior <-> contiguous copies

@@ -47,9 +47,9 @@ namespace {
     if (e_ != hipSuccess) return fail(OSBL_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-enum Family { F_BC = 0, F_PRIMS, F_SWEEP_X, F_SWEEP_Y, F_SWEEP_Z, F_UPDATE, F_FILTER, F_COUNT };
+enum Family { F_BC = 0, F_PRIMS, F_SWEEP_X, F_SWEEP_Y, F_SWEEP_Z, F_UPDATE, F_FILTER, F_CENTRAL, F_COUNT };
 const char* kFamilyNames[F_COUNT] = {"boundary", "constituent_relations", "char_sweep_x", "char_sweep_y",
-                                     "char_sweep_z", "residual_rk_update", "filters"};
+                                     "char_sweep_z", "residual_rk_update", "filters", "central_convective"};
 
 int rk_coefficients(int rk, double* a, double* b) {
   switch (rk) {
@@ -195,10 +195,12 @@ struct osbl_ctx {
   double A[5]{}, B[5]{};
   int seg = 32;
   int vsz = 32;             // z segment of the tiled viscous kernels
+  int csz = 32;             // z segment of the tiled central convective kernel
   int vjp_seg = 8;          // preferred interfaces per thread of the characteristic VJP sweeps
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
   bool tiled_gen = false;   // the same on the general path (closures/metric/mu(T) in x and y only)
+  bool tiled_central = false;  // 3-D periodic central scheme: tiled convective kernel (+ tiled viscous kernel)
   bool general = false;     // closures, metric or Sutherland viscosity: the general update kernel
   bool suth = false;
   Bnd bnd{};
@@ -349,6 +351,20 @@ void stage(osbl_ctx* c, int mode, double cA, double cB, int first) {
         launch_update(c->nd, conv, false, mode, Q, c->P, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->stream);
       });
     }
+    return;
+  }
+  if (c->tiled_central) {
+    // 3-D central scheme: the tiled convective kernel applies the RK update,
+    // or, with viscous terms, assigns R and the tiled viscous kernel adds its
+    // terms and applies the update (mode 2: residual only, left in R)
+    const bool visc = c->cfg.viscous != OSBL_VISC_NONE;
+    timed(c, F_CENTRAL, [&] {
+      launch_conv3d(conv, visc ? 2 : mode, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->csz, c->stream);
+    });
+    if (visc)
+      timed(c, F_UPDATE, [&] {
+        launch_visc_update3d(mode, Q, c->R, c->Qo, Qn, c->g, c->ph, cA, cB, first, c->vsz, c->stream);
+      });
     return;
   }
   if (c->tiled_visc && mode != 2) {
@@ -600,7 +616,13 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   c->tiled_visc = !c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE;
   c->tiled_gen = c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE &&
                  !c->bnd.mod[2][0] && !c->bnd.mod[2][1] && !cfg->stretch[2];
-  c->need_prims = is_central(cfg->conv) || (cfg->viscous != OSBL_VISC_NONE && !c->tiled_visc);
+  // the tiled central kernels march columns along z: a grid with too few
+  // tiles to fill the GPU (64^3 and below) keeps the per-point kernels
+  c->vsz = visc3d_segment(c->g);
+  c->csz = conv3d_segment(c->g);
+  c->tiled_central = !c->general && c->nd == 3 && is_central(cfg->conv) && !cfg->curvilinear &&
+                     conv3d_blocks(c->g, c->csz) >= 512;
+  c->need_prims = !c->tiled_central && (is_central(cfg->conv) || (cfg->viscous != OSBL_VISC_NONE && !c->tiled_visc));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t bundle = (size_t)fs * c->nv * sizeof(double);
   auto alloc = [&](double** p, size_t bytes) -> int {
@@ -1182,7 +1204,17 @@ int osbl_kernel_work(osbl_ctx* c, int family, double* bytes, double* flops) {
       f = (interface_flops(c) + (c->tiled_visc || c->tiled_gen ? 6.0 * nv : 0.0)) * n;
       break;
     case F_FILTER: b = 16.0 * nv * n; break;
+    case F_CENTRAL:
+      // read Q; write R (viscous terms follow) or read the register and write Q and the register
+      b = (c->cfg.viscous != OSBL_VISC_NONE ? 16.0 : 32.0) * nv * n;
+      f = (c->nd * central_flops(c) + (c->cfg.viscous != OSBL_VISC_NONE ? 0.0 : 6.0 * nv)) * n;
+      break;
     case F_UPDATE: {
+      if (c->tiled_central) {  // viscous terms + RK update: read Q, R, the register; write Q and the register
+        b = 40.0 * nv * n;
+        f = (viscous_flops(c) + 6.0 * nv) * n;
+        break;
+      }
       if (c->tiled_visc || c->tiled_gen) {  // viscous terms only: read Q (or the CRs), RMW R
         b = 24.0 * nv * n;
         f = viscous_flops(c) * n;

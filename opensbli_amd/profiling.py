@@ -23,6 +23,7 @@ FAMILIES = [
     ("k_char_sweep<3, 0,", "char_sweep_x"),
     ("k_char_sweep<3, 1,", "char_sweep_y"),
     ("k_char_sweep<3, 2,", "char_sweep_z"),
+    ("k_conv3d", "central_convective"),
     ("k_visc_update3d", "residual_rk_update"),
     ("k_visc_gen3d", "residual_rk_update"),
     ("k_visc_shell", "residual_rk_update"),

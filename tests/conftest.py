@@ -71,13 +71,16 @@ def generic_state(n, nd, gama=1.4, seed=0):
     component are sums of six seeded Fourier modes (|k_d| <= 2) about 1,
     1/gama and 0, Mach number of order 0.3.  No term of the residual is a
     small difference of large ones (unlike a low-Mach, divergence-free
-    field such as the Taylor-Green vortex's initial state)."""
+    field such as the Taylor-Green vortex's initial state).  n: points per
+    direction, one number or one per direction (x first)."""
     rs = np.random.RandomState(seed)
-    x = np.arange(n) * (2.0 * np.pi / n)
-    grids = np.meshgrid(*([x] * nd), indexing="ij")[::-1]
+    ns = [n] * nd if np.isscalar(n) else list(n)
+    axes = [np.arange(m) * (2.0 * np.pi / m) for m in ns]
+    grids = np.meshgrid(*axes[::-1], indexing="ij")[::-1]  # grids[d] varies along direction d
+    shape = tuple(ns[::-1])
 
     def field(mean, amp):
-        f = np.full((n,) * nd, float(mean))
+        f = np.full(shape, float(mean))
         for _ in range(6):
             k = rs.randint(-2, 3, nd)
             f = f + amp * rs.uniform(0.3, 1.0) * np.cos(sum(k[d] * grids[d] for d in range(nd)) +
