@@ -8,11 +8,12 @@ RK3 iteration (three stages).  The state is resident in HBM before timing.
 
 Prints one JSON line:
   value = Mcell-updates/s per RK stage = cells * stages * steps / t / 1e6
-  roofline = the dominant kernel against its binding roofline, FP64 issue:
-             its FP64 operations per launch (counted from the kernel
-             formulas) / its mean launch time (HIP events on the library's
-             stream) vs 78.6 TFLOP/s; roofline_hbm gives the same kernel's
-             algorithmic bytes per launch against 8 TB/s
+  roofline = the dominant kernel against its binding roofline (FP64 issue
+             for the characteristic sweeps): its FP64 operations per launch
+             (counted from the kernel formulas) / its mean launch time (HIP
+             events on the library's stream) vs 78.6 TFLOP/s, or its
+             algorithmic bytes per launch vs 8 TB/s for an HBM-bound kernel;
+             roofline_fp64 and roofline_hbm give both views
   ensemble = the batched-ensemble speed-up (512 uneven 1-D WENO5-Z shock
              tubes: one persistent launch vs per-instance launches)
   flat_plate = the 480x256x128 flat plate on the tanh (factor 5) wall-normal
@@ -37,8 +38,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # measured HBM bytes per launch of each kernel family (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes over this benchmark, scaled by a plain copy's readings);
-# the commands and raw figures are in profiles/hbm_traffic_weno_z_tgv256.md
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_weno_z_tgv256.json")
+# the commands and raw figures are in profiles/hbm_traffic_<scheme>_tgv256.md
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f"hbm_traffic_{s}_tgv256.json") for s in ("weno_z", "central_skew")]
 FP64_PEAK_TFS = 78.6
 
 
@@ -49,15 +50,16 @@ def _log(msg):
 def measured_traffic(family, n, conv):
     """HBM bytes per launch of a kernel family from the committed measurement,
     when it covers this workload; None otherwise."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if t.get("grid") != n or t.get("scheme") != conv:
-        return None
-    fam = t.get("families", {}).get(family)
-    return None if fam is None else fam.get("bytes_per_launch")
+    for path in TRAFFIC_FILES:
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if t.get("grid") == n and t.get("scheme") == conv:
+            fam = t.get("families", {}).get(family)
+            return None if fam is None else fam.get("bytes_per_launch")
+    return None
 
 
 def host_cpu():
@@ -188,9 +190,11 @@ def main():
     avg_s = tot_ms / 1e3 / max(launches, 1)
     kb, kf = work[fam]
     traffic = measured_traffic(fam, n, args.conv)
-    # the characteristic sweeps do ~20 FP64 operations per algorithmic byte
-    # against a machine balance of 78.6e12 / 8e12 ~ 9.8: FP64 issue binds
-    roofline = {
+    # the binding roofline of the dominant kernel: FP64 issue when its FP64
+    # operations per algorithmic byte exceed the machine balance
+    # 78.6e12 / 8e12 ~ 9.8 (the characteristic sweeps do ~20), HBM otherwise
+    # (the central schemes' kernels do ~2-4)
+    roofline_fp64 = {
         "kernel": fam,
         "bound": "fp64",
         "achieved": kf / avg_s / 1e12,
@@ -212,6 +216,8 @@ def main():
         "traffic": traffic,
         "bytes_per_launch": kb,
     }
+    fp64_bound = kf / max(kb, 1.0) > FP64_PEAK_TFS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    roofline = roofline_fp64 if fp64_bound else roofline_hbm
     stage_roofline = {
         "hbm_frac": stage_bytes * stage_evals / elapsed / 1e9 / HBM_PEAK_GBS,
         "fp64_frac": stage_flops * stage_evals / elapsed / 1e12 / FP64_PEAK_TFS,
@@ -257,6 +263,7 @@ def main():
                    "scheme": args.conv, "rk_stages": stages, "Re": 1600.0, "Minf": 0.1},
         "roofline": roofline,
         "roofline_hbm": roofline_hbm,
+        "roofline_fp64": roofline_fp64,
         "stage_roofline": stage_roofline,
         "kernels": kernels,
         "ensemble": ens,

@@ -1108,15 +1108,15 @@ __global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q,
 }
 
 // z segment of a 2.5-D tiled kernel over tx x ty columns: 32 planes, halved
-// (down to 8) while the launch would have fewer than `want` blocks
+// (down to 2) while the launch would have fewer than `want` blocks
 static int tile_segment(const Grid& g, int tx, int ty, long long want) {
   const long long tiles = (long long)((g.n[0] + tx - 1) / tx) * ((g.n[1] + ty - 1) / ty);
   int sz = 32;
-  while (sz > 8 && tiles * ((g.n[2] + sz - 1) / sz) < want) sz /= 2;
+  while (sz > 2 && tiles * ((g.n[2] + sz - 1) / sz) < want) sz /= 2;
   return sz;
 }
 int visc3d_segment(const Grid& g) { return tile_segment(g, VTX, VTY, 1024); }
-int conv3d_segment(const Grid& g) { return tile_segment(g, CTX, CTY, 2048); }
+int conv3d_segment(const Grid& g) { return tile_segment(g, CTX, CTY, 1024); }
 long long conv3d_blocks(const Grid& g, int sz) {
   return (long long)((g.n[0] + CTX - 1) / CTX) * ((g.n[1] + CTY - 1) / CTY) * ((g.n[2] + sz - 1) / sz);
 }

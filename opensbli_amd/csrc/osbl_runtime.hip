@@ -616,12 +616,13 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
   c->tiled_visc = !c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE;
   c->tiled_gen = c->general && c->nd == 3 && !is_central(cfg->conv) && cfg->viscous != OSBL_VISC_NONE &&
                  !c->bnd.mod[2][0] && !c->bnd.mod[2][1] && !cfg->stretch[2];
-  // the tiled central kernels march columns along z: a grid with too few
-  // tiles to fill the GPU (64^3 and below) keeps the per-point kernels
+  // the tiled central kernels march columns along z in segments short
+  // enough to give the GPU ~1024 blocks; a grid with too few tiles even then
+  // (below ~32^3) keeps the per-point kernels
   c->vsz = visc3d_segment(c->g);
   c->csz = conv3d_segment(c->g);
   c->tiled_central = !c->general && c->nd == 3 && is_central(cfg->conv) && !cfg->curvilinear &&
-                     conv3d_blocks(c->g, c->csz) >= 512;
+                     conv3d_blocks(c->g, c->csz) >= 128;
   c->need_prims = !c->tiled_central && (is_central(cfg->conv) || (cfg->viscous != OSBL_VISC_NONE && !c->tiled_visc));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t bundle = (size_t)fs * c->nv * sizeof(double);
