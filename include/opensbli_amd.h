@@ -298,7 +298,7 @@ int osbl_adjoint_step_dev(osbl_ctx* ctx, const double* stages, double* lambda);
  * allocation (no counterpart in the reference).  For callers that cannot
  * allocate: code under HIP graph capture, tight backward loops, harnesses
  * that forbid hipMalloc.  Its work buffers (the three stage adjoints, the
- * characteristic VJP's partial sums, the viscous VJP's fields) come from a
+ * halo-padded state and weights, the viscous VJP's fields) come from a
  * fixed device array of OSBL_NOALLOC_SCRATCH_MIB MiB that each device running
  * the library holds until the process ends; a block needing more returns
  * OSBL_E_ARG.  Every call on one device shares that array (its contents are

@@ -9,7 +9,9 @@ Forward: the block is advanced with the GPU kernels; the state is kept every
 `every` steps.  Backward: for each segment, from last to first, the segment
 is recomputed from its checkpoint while the input states of every RK stage
 are stored, then the adjoint of each step (osbl_adjoint_step_dev) is applied
-from the last step back; the block is then left at the forward output.
+from the last step back; the recomputation of the next segment runs on a
+second context of the same problem, on its own stream, during the adjoint
+steps of the current one.  The block is then left at the forward output.
 Covered: the central skew-symmetric scheme and the
 LLF characteristic WENO5-Z scheme, with or without constant-viscosity terms,
 on periodic blocks with RungeKutta(3) (TENO's cut-off is not differentiable).
@@ -54,26 +56,53 @@ class _Integrate(torch.autograd.Function):
         lam = gN.detach().to(torch.float64).contiguous().clone()
         nseg = len(ctx.checkpoints) - 1
         ns = block.num_stages()
-        stages = None
-        for seg in range(nseg - 1, -1, -1):
-            k = min(ctx.every, ctx.nsteps - seg * ctx.every)
-            if stages is None or stages.shape[0] < k:
-                stages = torch.empty((k, ns) + tuple(lam.shape), dtype=torch.float64, device=lam.device)
-            _sync()
+
+        def steps_of(seg):
+            return min(ctx.every, ctx.nsteps - seg * ctx.every)
+
+        kmax = max(steps_of(seg) for seg in range(nseg))
+        bufs = [torch.empty((kmax, ns) + tuple(lam.shape), dtype=torch.float64, device=lam.device)
+                for _ in range(2)]
+        # a second context of the same problem recomputes the stage inputs of
+        # the next segment (backwards) on its own stream while this block runs
+        # the adjoint steps of the current one; the two share only read-only
+        # checkpoints and alternate between the two stage buffers
+        twin = _twin(block)
+
+        def recompute(seg, stages):
             q = ctx.checkpoints[seg]
-            block.set_state_dev([q[m].data_ptr() for m in range(nv)])
-            for s in range(k):
-                block.step_stages_dev(stages[s].data_ptr())
-            for s in range(k - 1, -1, -1):
-                block.adjoint_step_dev(stages[s].data_ptr(), lam.data_ptr())
+            twin.set_state_dev([q[m].data_ptr() for m in range(nv)])
+            for s in range(steps_of(seg)):
+                twin.step_stages_dev(stages[s].data_ptr())
+
+        _sync()
+        recompute(nseg - 1, bufs[0])
+        twin.sync()
+        cur = 0
+        for seg in range(nseg - 1, -1, -1):
+            if seg > 0:
+                recompute(seg - 1, bufs[1 - cur])
+            for s in range(steps_of(seg) - 1, -1, -1):
+                block.adjoint_step_dev(bufs[cur][s].data_ptr(), lam.data_ptr())
             block.sync()
-        # the recomputation moved the block's state: leave it at the forward
-        # output, where the forward pass left it, so a caller can go on
-        # integrating the same block
+            twin.sync()
+            cur = 1 - cur
+        # leave the block at the forward output, where the forward pass left
+        # it, so a caller can go on integrating the same block
         qN = ctx.checkpoints[-1]
         block.set_state_dev([qN[m].data_ptr() for m in range(nv)])
         block.sync()
         return lam, None, None, None
+
+
+def _twin(block):
+    """The second context used to recompute segments during the backward
+    pass (made once per block, kept with it)."""
+    twin = getattr(block, "_adjoint_twin", None)
+    if twin is None:
+        twin = native.Block(block.p)
+        block._adjoint_twin = twin
+    return twin
 
 
 def integrate(q0, block, nsteps, every=20):

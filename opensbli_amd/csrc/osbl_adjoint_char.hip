@@ -17,9 +17,11 @@
 //     hand-written reverses (dual numbers over the averaged state for the
 //     1-D eigensystem),
 //   * the constituent relations u_D, p, a of every stencil point.
-// Each interface is evaluated once: a thread owns a segment of interfaces,
-// accumulates their contributions to the cells in registers and writes
-// per-segment partial sums, which a second kernel adds per cell.
+// The lanes of a wave take 64 consecutive interfaces of a grid line: the
+// chunk's points sit in LDS with their constituent relations, every lane adds
+// its interface's contributions to its own LDS column, and each cell gathers
+// the six contributions that reach it from its neighbours' columns; no
+// partial sums leave the wave.
 #include <algorithm>
 
 #include "osbl_kernels.h"
@@ -151,8 +153,9 @@ __device__ __forceinline__ Dual<N> dsqrt(const Dual<N>& a) {
 __device__ __forceinline__ double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
 
 // Gradient of weno5z(x0..x4) (the one-division WENO-Z of osbl_device.h) with
-// respect to its five arguments, scaled by gbar.
-__device__ __forceinline__ void weno5z_grad(const double x[5], double gbar, double g[5]) {
+// respect to its five arguments, scaled by gbar; returns the reconstruction
+// itself (the same weights, the candidates' 1/6 applied to each candidate).
+__device__ __forceinline__ double weno5z_grad(const double x[5], double gbar, double g[5]) {
   const double fm2 = x[0], fm1 = x[1], f0 = x[2], f1 = x[3], f2 = x[4];
   const double t0 = f0 - 2.0 * f1 + f2, s0 = 3.0 * f0 - 4.0 * f1 + f2;
   const double t1 = fm1 - 2.0 * f0 + f1, s1 = fm1 - f1;
@@ -199,6 +202,7 @@ __device__ __forceinline__ void weno5z_grad(const double x[5], double gbar, doub
   g[3] = -2.0 * tb0 - 4.0 * sb0 + tb1 - sb1 + rb0 * (5.0 * S6) + rb1 * (2.0 * S6);
   // f2
   g[4] = tb0 + sb0 - rb0 * S6;
+  return W;
 }
 
 // Adjoints of the averaged state's symbols (AvgT's members).
@@ -400,13 +404,28 @@ __device__ __forceinline__ double lam_point(double ud, double a, double& dud, do
   return fabs(l);
 }
 
-// g[p][m] += (d F_{i+1/2} / d Q_p)^T nu for the window w[0..5] = Q_{i-2..i+3}
-template <int ND, int D, int AVG>
-__device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double nu[ND + 2], const Phys& ph,
-                                              double (&g)[6][ND + 2]) {
+// Accumulator of the VJP's contributions g[p][m] (window point p, variable m)
+// in a lane's own LDS column: an atomic add that returns nothing, so no
+// register waits for the old value (every lane owns its column: no contention)
+template <int ND>
+struct LdsAcc {
+  double* s;   // slot (0, 0) of this lane
+  int stride;  // doubles between slots
+  __device__ __forceinline__ void add(int p, int m, double v) const { atomicAdd(&s[(p * (ND + 2) + m) * stride], v); }
+};
+
+// g[p][m] += (d F_{i+1/2} / d Q_p)^T nu for the window w = Q_{i-2..i+3}
+// (points 0..5, read through the LdsWindow accessors)
+template <int ND, int D, int AVG, class W, class G>
+__device__ __forceinline__ void char_flux_vjp(const W& w, const double nu[ND + 2], const Phys& ph, const G& g) {
   constexpr int NV = ND + 2, NS = ND + 2;  // averaged symbols: rho, a, u_0..u_{ND-1}
   Avg<ND> v;
-  interface_state<ND, AVG>(v, w[2], w[3], ph);
+  {
+    Pt<ND> L, R;
+    w.point(2, L);
+    w.point(3, R);
+    interface_state<ND, AVG>(v, L, R, ph);
+  }
   // adjoints of the averaged state: by the hand-written reverse of the
   // eigenvector entries (ND >= 2) or by dual numbers (the 1-D eigensystem)
   AvgBar<ND> b;
@@ -425,7 +444,7 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
 #pragma unroll
   for (int p = 0; p < 6; ++p) { nud[p] = 0.0; npp[p] = 0.0; na[p] = 0.0; }
 
-  sfor<0, NV>([&](auto kc) {
+  auto field = [&](auto kc) {
     constexpr int K = decltype(kc)::value;
     constexpr unsigned LM = lmask<ND, D, K>();
     constexpr unsigned RM = rmask<ND, D, K>();
@@ -450,7 +469,7 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
       double du_, da_;
-      const double lp = lam_point<ND, K>(w[p].u[D], w[p].a, du_, da_);
+      const double lp = lam_point<ND, K>(w.u(p, D), w.a(p), du_, da_);
       if (lp > lam) {
         lam = lp;
         pst = p;
@@ -464,28 +483,28 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
       double c = 0.0;
 #pragma unroll
       for (int m = 0; m < NV; ++m)
-        if ((LM >> m) & 1u) c += l[m] * w[p].q[m];
+        if ((LM >> m) & 1u) c += l[m] * w.q(p, m);
       double lpp = 0.0;
       if constexpr ((LM >> (1 + D)) & 1u) lpp = l[1 + D];
-      if constexpr ((LM >> (NV - 1)) & 1u) lpp += w[p].u[D] * l[NV - 1];
-      const double cf = w[p].u[D] * c + w[p].p * lpp;
+      if constexpr ((LM >> (NV - 1)) & 1u) lpp += w.u(p, D) * l[NV - 1];
+      const double cf = w.u(p, D) * c + w.pr(p) * lpp;
       cs[p] = c;
       lpv[p] = lpp;
       fp[p] = 0.5 * (cf + lam * c);
       fm[p] = 0.5 * (cf - lam * c);
     }
-    const double rc = weno5z(fp[0], fp[1], fp[2], fp[3], fp[4]) + weno5z(fm[5], fm[4], fm[3], fm[2], fm[1]);
-    // F += r rc
+    // F += r rc: rc's adjoint, then the two reconstructions (values and
+    // gradients from one evaluation of their weights each)
     double nrc = 0.0;
+#pragma unroll
+    for (int m = 0; m < NV; ++m)
+      if ((RM >> m) & 1u) nrc += r[m] * nu[m];
+    double gp[5], gm[5];
+    const double fmr[5] = {fm[5], fm[4], fm[3], fm[2], fm[1]};
+    const double rc = weno5z_grad(fp, nrc, gp) + weno5z_grad(fmr, nrc, gm);
     double nr[NV];
 #pragma unroll
-    for (int m = 0; m < NV; ++m) {
-      nr[m] = 0.0;
-      if ((RM >> m) & 1u) {
-        nrc += r[m] * nu[m];
-        nr[m] = rc * nu[m];
-      }
-    }
+    for (int m = 0; m < NV; ++m) nr[m] = ((RM >> m) & 1u) ? rc * nu[m] : 0.0;
     if constexpr (ND == 1) {
 #pragma unroll
       for (int m = 0; m < NV; ++m)
@@ -496,11 +515,6 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
     } else {
       rcol_rev<ND, D, K>(v, ph, nr, b);
     }
-    // reconstructions
-    double gp[5], gm[5];
-    weno5z_grad(fp, nrc, gp);
-    const double fmr[5] = {fm[5], fm[4], fm[3], fm[2], fm[1]};
-    weno5z_grad(fmr, nrc, gm);
     double nl[NV];
 #pragma unroll
     for (int m = 0; m < NV; ++m) nl[m] = 0.0;
@@ -513,20 +527,20 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
       double ncs = 0.5 * lam * (a_ - b_);
       nlam += 0.5 * cs[p] * (a_ - b_);
       // cf = u_D cs + p lp, lp = l_{1+D} + u_D l_E
-      ncs += w[p].u[D] * ncf;
+      ncs += w.u(p, D) * ncf;
       nud[p] += cs[p] * ncf;
       npp[p] += lpv[p] * ncf;
-      if constexpr ((LM >> (1 + D)) & 1u) nl[1 + D] += w[p].p * ncf;
+      if constexpr ((LM >> (1 + D)) & 1u) nl[1 + D] += w.pr(p) * ncf;
       if constexpr ((LM >> (NV - 1)) & 1u) {
-        nl[NV - 1] += w[p].p * w[p].u[D] * ncf;
-        nud[p] += w[p].p * l[NV - 1] * ncf;
+        nl[NV - 1] += w.pr(p) * w.u(p, D) * ncf;
+        nud[p] += w.pr(p) * l[NV - 1] * ncf;
       }
       // cs = sum_m l_m q_m
 #pragma unroll
       for (int m = 0; m < NV; ++m)
         if ((LM >> m) & 1u) {
-          g[p][m] += l[m] * ncs;
-          nl[m] += w[p].q[m] * ncs;
+          g.add(p, m, l[m] * ncs);
+          nl[m] += w.q(p, m) * ncs;
         }
     }
     // lambda_max
@@ -546,7 +560,26 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
     } else {
       lrow_rev<ND, D, K>(v, ph, nl, b);
     }
-  });
+    // LDS window: the next field reloads its points (its work does not
+    // overlap this one's, which keeps the live registers to one field's)
+    if constexpr (W::STREAM) asm volatile("" ::: "memory");
+  };
+  // one characteristic field at a time (a loop the compiler keeps rolled,
+  // so no field's work is scheduled into another's registers)
+#pragma nounroll
+  for (int k = 0; k < NV; ++k) {
+    switch (k) {
+      case 0: field(std::integral_constant<int, 0>{}); break;
+      case 1: field(std::integral_constant<int, 1>{}); break;
+      case 2: field(std::integral_constant<int, 2>{}); break;
+      case 3:
+        if constexpr (NV > 3) field(std::integral_constant<int, (NV > 3 ? 3 : 0)>{});
+        break;
+      default:
+        if constexpr (NV > 4) field(std::integral_constant<int, (NV > 4 ? 4 : 0)>{});
+        break;
+    }
+  }
 
   // averaged state -> the two states
   if constexpr (ND == 1) {
@@ -556,44 +589,45 @@ __device__ __forceinline__ void char_flux_vjp(const Pt<ND> (&w)[6], const double
   } else {
     finish_avg_rev<ND>(v, ph, b);
   }
-  average_rev<ND, AVG>(w[2], w[3], ph, b, g[2], g[3]);
+  {
+    Pt<ND> L, R;
+    w.point(2, L);
+    w.point(3, R);
+    double gl[NV], gr[NV];
+#pragma unroll
+    for (int m = 0; m < NV; ++m) { gl[m] = 0.0; gr[m] = 0.0; }
+    average_rev<ND, AVG>(L, R, ph, b, gl, gr);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      g.add(2, m, gl[m]);
+      g.add(3, m, gr[m]);
+    }
+  }
   // u_D, p, a of every point -> its conservative variables
 #pragma unroll
   for (int p = 0; p < 6; ++p) {
-    const double rho = w[p].q[0], ir = 1.0 / rho;
-    const double a = w[p].a;
+    const double rho = w.q(p, 0), ir = 1.0 / rho;
+    const double a = w.a(p);
     double uu = 0.0;
 #pragma unroll
-    for (int m = 0; m < ND; ++m) uu += w[p].u[m] * w[p].u[m];
+    for (int m = 0; m < ND; ++m) uu += w.u(p, m) * w.u(p, m);
     // a = sqrt(gama p / rho)
     const double np_ = npp[p] + na[p] * ph.gama * 0.5 / (a * rho);
     double g0 = -na[p] * 0.5 * a * ir;
     // p = (gama-1)(E - |rho u|^2 / (2 rho))
     g0 += np_ * ph.gm1 * 0.5 * uu;
 #pragma unroll
-    for (int m = 0; m < ND; ++m) g[p][1 + m] += -np_ * ph.gm1 * w[p].u[m];
-    g[p][NV - 1] += np_ * ph.gm1;
+    for (int m = 0; m < ND; ++m) g.add(p, 1 + m, -np_ * ph.gm1 * w.u(p, m));
+    g.add(p, NV - 1, np_ * ph.gm1);
     // u_D = rhou_D / rho
-    g0 += -nud[p] * w[p].u[D] * ir;
-    g[p][1 + D] += nud[p] * ir;
-    g[p][0] += g0;
+    g0 += -nud[p] * w.u(p, D) * ir;
+    g.add(p, 1 + D, nud[p] * ir);
+    g.add(p, 0, g0);
   }
 }
 
 }  // namespace
 
-// One direction of the characteristic residual's VJP on packed periodic
-// arrays (n0 x n1 x n2, x fastest), in two passes.  mu = alpha A + beta B (B
-// optional) is formed on the fly.
-//
-// k_char_vjp_part: thread (line, segment s) evaluates the VJP of the
-// segment's own interfaces i = c0..c1-1 once each, sliding the window along
-// the line, and writes its partial sums for the cells c0-2 .. c1+2 the
-// interfaces touch (seg + 5 slots) to `part` (vjp_pidx).  A cell is written
-// as soon as no later interface of the segment reaches it.
-// k_char_vjp_gather: per cell, the sum of the (at most three) partials that
-// cover it, periodic images included, in a fixed order; ACC = false:
-// out1 = c (A + B) + V (and out2 = A + B when given); ACC = true: out1 += V.
 // origin and stride of grid line `line` along D (lines numbered with the
 // lower remaining direction fastest)
 template <int D>
@@ -612,135 +646,102 @@ __device__ __forceinline__ void vjp_line(long long line, int n0, int n1, long lo
   sd = D == 0 ? 1 : (D == 1 ? (long long)n0 : (long long)n0 * n1);
 }
 
-// Index of a partial sum (variable m, segment s, slot k, line).  Along y and z
-// the lanes of a wave are adjacent lines, along x adjacent segments of a line;
-// either way they write adjacent doubles.
-template <int D>
-__device__ __forceinline__ long long vjp_pidx(int m, int s, int k, long long line, int nseg, int SP,
-                                              long long lines) {
-  if constexpr (D == 0) return ((long long)m * SP + k) * (lines * nseg) + line * nseg + s;
-  else return (((long long)m * nseg + s) * SP + k) * lines + line;
-}
+// One direction of the characteristic residual's VJP with the lanes of a wave
+// along the grid line: a wave owns a chunk of 64 consecutive interfaces
+// I = a .. a+63 of one line.  The chunk's 69 points (a-2 .. a+66, periodic
+// images) are staged in LDS with their constituent relations, lane l
+// evaluates the VJP of interface a+l from LDS, and the contributions of the
+// six interfaces j-3 .. j+2 that reach cell j = a+l are gathered by
+// cross-lane shuffles; lanes 3..61 hold complete cells, so consecutive
+// chunks of a line start 59 cells apart (five interfaces are evaluated
+// twice) and no partial sums leave the wave.
+//   ACC = false: out1 = c (A + B) + V (and out2 = A + B when given);
+//   ACC = true:  out1 += V.
+constexpr int VJW = 2;         // waves per block
+constexpr int VJP = 64 + 5;    // points staged per chunk
+constexpr int VJSTEP = 59;     // cells completed per chunk
 
-template <int ND, int D, int AVG>
-__global__ void __launch_bounds__(128) k_char_vjp_part(const double* __restrict__ Qs, const double* __restrict__ A,
-                                                       const double* __restrict__ B, double alpha, double beta,
-                                                       double* __restrict__ part, int n0, int n1, int n2, Phys ph,
-                                                       int seg) {
+template <int ND, int D, int AVG, bool ACC>
+__global__ void __launch_bounds__(64 * VJW) k_char_vjp_lanes(const double* __restrict__ Qs,
+                                                             const double* __restrict__ A,
+                                                             const double* __restrict__ B, double alpha, double beta,
+                                                             double c, double* __restrict__ out1,
+                                                             double* __restrict__ out2, int n0, int n1, int n2,
+                                                             Phys ph) {
   constexpr int NV = ND + 2;
+  constexpr int NF = NV + ND + 2;  // q, u, p, a
+  __shared__ double sh[VJW][NF][VJP];
+  __shared__ double ga[VJW][6 * NV][64];  // each lane's contributions, slot (p, m) at row p * NV + m
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nn[3] = {n0, n1, n2};
   const long long cells = (long long)n0 * n1 * n2;
   const int N = nn[D];
-  const int nseg = (N + seg - 1) / seg;
-  const int SP = seg + 5;
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= nn[e];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= lines * nseg) return;
-  // x: lanes along a line's segments (their loads share cache lines); y, z: lanes along x
-  const long long line = D == 0 ? t / nseg : t % lines;
-  const int s = (int)(D == 0 ? t % nseg : t / lines);
+  const int nchunk = (N + VJSTEP - 1) / VJSTEP;
+  const long long wid = (long long)blockIdx.x * VJW + wv;
+  if (wid >= lines * nchunk) return;  // whole waves only: no block-wide barrier below
+  // adjacent waves take adjacent lines (for y and z: adjacent x, sharing cache lines)
+  const long long line = wid % lines;
+  const int chunk = (int)(wid / lines);
   long long base, sd;
   vjp_line<D>(line, n0, n1, base, sd);
-  const int c0 = s * seg, c1 = min(c0 + seg, N);
   auto at = [&](int i) { return base + (long long)(((i % N) + N) % N) * sd; };
-  auto load = [&](Pt<ND>& pt, int i) {
+  const int a0 = chunk * VJSTEP - 3;  // interface of lane 0; lane 3 completes cell chunk * VJSTEP
+  double (*S)[VJP] = sh[wv];
+  auto stage = [&](int slot, int i) {
+    Pt<ND> pt;
     const long long y = at(i);
 #pragma unroll
     for (int m = 0; m < NV; ++m) pt.q[m] = Qs[m * cells + y];
     finish_pt<ND>(pt, ph);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) S[m][slot] = pt.q[m];
+#pragma unroll
+    for (int m = 0; m < ND; ++m) S[NV + m][slot] = pt.u[m];
+    S[NV + ND][slot] = pt.p;
+    S[NV + ND + 1][slot] = pt.a;
   };
-  auto load_mu = [&](double* mu, int i) {
-    const long long y = at(i);
+  // slot s holds point a0 - 2 + s
+  stage(lane, a0 - 2 + lane);
+  if (lane < VJP - 64) stage(64 + lane, a0 + 62 + lane);
+  // the interface's weights nu = (mu_{I+1} - mu_I) / Delta_D, mu = alpha A + beta B
+  const int I = a0 + lane;
+  double nu[NV];
+  {
+    const long long y0 = at(I), y1 = at(I + 1);
+    const double inv = ph.inv_delta[D];
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
-      double v = alpha * A[m * cells + y];
-      if (B) v += beta * B[m * cells + y];
-      mu[m] = v;
+      double m0 = alpha * A[m * cells + y0], m1 = alpha * A[m * cells + y1];
+      if (B) {
+        m0 += beta * B[m * cells + y0];
+        m1 += beta * B[m * cells + y1];
+      }
+      nu[m] = inv * (m1 - m0);
     }
-  };
-  // partial of slot k (cell c0 - 2 + k), variable m
-  auto put = [&](int k, const double* v) {
-#pragma unroll
-    for (int m = 0; m < NV; ++m) part[vjp_pidx<D>(m, s, k, line, nseg, SP, lines)] = v[m];
-  };
-  const double inv = ph.inv_delta[D];
-  Pt<ND> w[6];
-  double mu2[NV], mu3[NV];  // weights of the interface's two cells
-  double acc[6][NV];
-#pragma unroll
-  for (int p = 0; p < 6; ++p)
-#pragma unroll
-    for (int m = 0; m < NV; ++m) acc[p][m] = 0.0;
-  // window for interface i holds points i-2..i+3; the first interface is c0
-#pragma unroll
-  for (int p = 0; p < 5; ++p) load(w[p], c0 - 2 + p);
-  load_mu(mu2, c0);
-  for (int i = c0; i < c1; ++i) {
-    load(w[5], i + 3);
-    load_mu(mu3, i + 1);
-    double nu[NV];
-#pragma unroll
-    for (int m = 0; m < NV; ++m) nu[m] = inv * (mu3[m] - mu2[m]);
-    char_flux_vjp<ND, D, AVG>(w, nu, ph, acc);
-    // cell i - 2 gets nothing from the segment's later interfaces
-    put(i - c0, acc[0]);
-#pragma unroll
-    for (int m = 0; m < NV; ++m) mu2[m] = mu3[m];
-#pragma unroll
-    for (int p = 0; p < 5; ++p) {
-      w[p] = w[p + 1];
-#pragma unroll
-      for (int m = 0; m < NV; ++m) acc[p][m] = acc[p + 1][m];
-    }
-#pragma unroll
-    for (int m = 0; m < NV; ++m) acc[5][m] = 0.0;
   }
-  // cells c1-2 .. c1+2
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double (*G)[64] = ga[wv];
 #pragma unroll
-  for (int p = 0; p < 5; ++p) put(c1 - c0 + p, acc[p]);
-}
-
-template <int ND, int D, bool ACC>
-__global__ void __launch_bounds__(256) k_char_vjp_gather(const double* __restrict__ A, const double* __restrict__ B,
-                                                         double c, double* __restrict__ out1,
-                                                         double* __restrict__ out2, const double* __restrict__ part,
-                                                         int n0, int n1, int n2, int seg) {
-  constexpr int NV = ND + 2;
-  const int nn[3] = {n0, n1, n2};
-  const long long cells = (long long)n0 * n1 * n2;
-  const int N = nn[D];
-  const int nseg = (N + seg - 1) / seg;
-  const int SP = seg + 5;
-  long long lines = 1;
-  for (int e = 0; e < 3; ++e)
-    if (e != D) lines *= nn[e];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= lines * N) return;
-  // lanes along x in every direction (coalesced state and output)
-  const long long line = D == 0 ? t / N : t % lines;
-  const int j = (int)(D == 0 ? t % N : t / lines);
-  long long base, sd;
-  vjp_line<D>(line, n0, n1, base, sd);
+  for (int r = 0; r < 6 * NV; ++r) G[r][lane] = 0.0;
+  char_flux_vjp<ND, D, AVG>(LdsWindow<ND>{&S[0][lane], VJP}, nu, ph, LdsAcc<ND>{&G[0][lane], 64});
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // cell j = a0 + lane receives slot k of interface j + 2 - k (lane + 2 - k)
+  const int j = a0 + lane;
+  if (lane < 3 || lane > 61 || j >= N) return;
   double v[NV];
 #pragma unroll
   for (int m = 0; m < NV; ++m) v[m] = 0.0;
-  // the cell at line position j, and its periodic images j - N, j + N, for
-  // every segment whose slot range [s seg - 2, min(s seg + seg, N) + 2] holds it
 #pragma unroll
-  for (int w = -1; w <= 1; ++w) {
-    const int jj = j + w * N;
-    if (jj < -2 || jj > N + 2) continue;
-    const int shi = min(nseg - 1, (jj + 2) / seg);
-    for (int s = max(0, (jj + 2) / seg - 2); s <= shi; ++s) {
-      const int c0 = s * seg, c1 = min(c0 + seg, N);
-      const int k = jj - (c0 - 2);
-      if (k < 0 || k > c1 - c0 + 4) continue;
+  for (int k = 0; k < 6; ++k)
 #pragma unroll
-      for (int m = 0; m < NV; ++m) v[m] += part[vjp_pidx<D>(m, s, k, line, nseg, SP, lines)];
-    }
-  }
+    for (int m = 0; m < NV; ++m) v[m] += G[k * NV + m][lane + 2 - k];
   const long long x = base + (long long)j * sd;
 #pragma unroll
   for (int m = 0; m < NV; ++m) {
@@ -754,79 +755,44 @@ __global__ void __launch_bounds__(256) k_char_vjp_gather(const double* __restric
   }
 }
 
-// doubles of the partial-sum scratch one direction needs
-template <int ND, int D>
-static long long vjp_part_size(const Grid& g, int seg) {
-  long long lines = 1;
-  for (int e = 0; e < 3; ++e)
-    if (e != D) lines *= g.n[e];
-  return (long long)(ND + 2) * lines * ((g.n[D] + seg - 1) / seg) * (seg + 5);
-}
-
 template <int ND, int D, int AVG>
 static void launch_cv_d(const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
-                        double* out1, double* out2, double* part, const Grid& g, const Phys& ph, int seg, bool acc,
-                        hipStream_t s) {
+                        double* out1, double* out2, const Grid& g, const Phys& ph, bool acc, hipStream_t s) {
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= g.n[e];
-  const long long total = lines * ((g.n[D] + seg - 1) / seg);
-  k_char_vjp_part<ND, D, AVG><<<(unsigned)((total + 127) / 128), 128, 0, s>>>(Qs, A, B, alpha, beta, part, g.n[0],
-                                                                            g.n[1], g.n[2], ph, seg);
-  const long long ncell = lines * g.n[D];
-  const unsigned gb = (unsigned)((ncell + 255) / 256);
+  const long long waves = lines * ((g.n[D] + VJSTEP - 1) / VJSTEP);
+  const unsigned blocks = (unsigned)((waves + VJW - 1) / VJW);
   if (acc)
-    k_char_vjp_gather<ND, D, true><<<gb, 256, 0, s>>>(A, B, c, out1, out2, part, g.n[0], g.n[1], g.n[2], seg);
+    k_char_vjp_lanes<ND, D, AVG, true><<<blocks, 64 * VJW, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0],
+                                                                     g.n[1], g.n[2], ph);
   else
-    k_char_vjp_gather<ND, D, false><<<gb, 256, 0, s>>>(A, B, c, out1, out2, part, g.n[0], g.n[1], g.n[2], seg);
+    k_char_vjp_lanes<ND, D, AVG, false><<<blocks, 64 * VJW, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, g.n[0],
+                                                                      g.n[1], g.n[2], ph);
 }
 
 template <int ND, int AVG>
 static void launch_cv_nd(const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
-                         double* out1, double* out2, double* part, const Grid& g, const Phys& ph, int seg, bool acc,
-                         hipStream_t s) {
-  launch_cv_d<ND, 0, AVG>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
-  if constexpr (ND >= 2) launch_cv_d<ND, 1, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, part, g, ph, seg, true, s);
-  if constexpr (ND >= 3) launch_cv_d<ND, 2, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, part, g, ph, seg, true, s);
-}
-
-template <int ND>
-static long long vjp_part_nd(const Grid& g, int seg) {
-  long long n = vjp_part_size<ND, 0>(g, seg);
-  if constexpr (ND >= 2) n = std::max(n, vjp_part_size<ND, 1>(g, seg));
-  if constexpr (ND >= 3) n = std::max(n, vjp_part_size<ND, 2>(g, seg));
-  return n;
-}
-
-// Segment length of the VJP sweeps: the preferred one, halved down to 4 while
-// a launch would have fewer than 2^17 threads (a launch along any direction
-// has about cells / seg threads).
-int adj_char_segment(const Grid& g, int seg) {
-  return fit_segment((long long)g.n[1] * g.n[2], g.n[0], seg, 4);
-}
-
-// doubles of the scratch launch_adj_stage_char needs for segment length seg
-long long adj_char_scratch(const Grid& g, int seg) {
-  if (g.nd == 3) return vjp_part_nd<3>(g, seg);
-  if (g.nd == 2) return vjp_part_nd<2>(g, seg);
-  return vjp_part_nd<1>(g, seg);
+                         double* out1, double* out2, const Grid& g, const Phys& ph, bool acc, hipStream_t s) {
+  launch_cv_d<ND, 0, AVG>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+  if constexpr (ND >= 2) launch_cv_d<ND, 1, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, g, ph, true, s);
+  if constexpr (ND >= 3) launch_cv_d<ND, 2, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, g, ph, true, s);
 }
 
 // out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B) for the WENO5-Z
 // characteristic residual, out2 = A + B (optional); with acc, out1 += the
-// product instead.  Packed periodic arrays; part holds adj_char_scratch(g, seg)
-// doubles.
+// product instead.  Packed periodic arrays; out1 aliases neither A nor B.
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
-                           double beta, double c, double* out1, double* out2, double* part, const Grid& g,
-                           const Phys& ph, int seg, bool acc, hipStream_t s) {
+                           double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, bool acc,
+                           hipStream_t s) {
   if (avg == OSBL_AVG_ROE) {
-    if (nd == 3) launch_cv_nd<3, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
-    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
-    else launch_cv_nd<1, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+    if (nd == 3) launch_cv_nd<3, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+    else launch_cv_nd<1, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
   } else {
-    if (nd == 3) launch_cv_nd<3, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
-    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
-    else launch_cv_nd<1, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, part, g, ph, seg, acc, s);
+    if (nd == 3) launch_cv_nd<3, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+    else launch_cv_nd<1, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
   }
 }
 

@@ -16,8 +16,9 @@
 // followed by the chain rule through u_m = rhou_m / rho and
 // T = gama Minf^2 p / rho, p = (gama-1)(rhoE - rho |u|^2 / 2).
 // Two passes over the padded arrays: the first forms w, 2 mu_E tau and
-// mu_E V at every cell, the second applies the transposed stencils (after a
-// periodic halo fill of w and 2 mu_E tau) and adds the result to `out`.
+// mu_E V at every cell and on two halo layers (from periodic images, so no
+// halo fill of its own), the second applies the transposed stencils and adds
+// the result to `out`.
 #include "osbl_kernels.h"
 
 namespace osbl {
@@ -45,9 +46,12 @@ template <int ND>
 __global__ void __launch_bounds__(128) k_visc_vjp_pre(const double* __restrict__ P, const double* __restrict__ M,
                                                       double* __restrict__ W, Grid g, Phys ph) {
   constexpr int NV = ND + 2, E = NV - 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n[0]) return;
-  const int x = g.idx(i, blockIdx.y, blockIdx.z);
+  // the interior and two halo layers (the second pass's stencil reach), from
+  // P and M over the padded grid (halo >= 4): no halo fill of W is needed
+  const int w2[3] = {2, ND >= 2 ? 2 : 0, ND >= 3 ? 2 : 0};
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - w2[0];
+  if (i >= g.n[0] + w2[0]) return;
+  const int x = g.idx(i, (int)blockIdx.y - w2[1], (int)blockIdx.z - w2[2]);
   const int st[3] = {1, g.pitch, g.plane};
   const long long fs = g.fs;
   const double* U[3] = {P, P + fs, P + 2 * fs};
@@ -81,7 +85,7 @@ __global__ void __launch_bounds__(128) k_visc_vjp_pre(const double* __restrict__
 // pass 2: the transposed stencils and the chain rule to the conservative
 // variables, added to out (packed, x fastest)
 template <int ND>
-__global__ void __launch_bounds__(128) k_visc_vjp(const double* __restrict__ Q, const double* __restrict__ P,
+__global__ void __launch_bounds__(128) k_visc_vjp(const double* __restrict__ q, const double* __restrict__ P,
                                                   const double* __restrict__ M, const double* __restrict__ W,
                                                   double* __restrict__ out, Grid g, Phys ph) {
   constexpr int NV = ND + 2, E = NV - 1;
@@ -109,7 +113,9 @@ __global__ void __launch_bounds__(128) k_visc_vjp(const double* __restrict__ Q, 
   for (int j = 0; j < ND; ++j) gT += ph.inv_delta[j] * ph.inv_delta[j] * vd2(M + E * fs, x, st[j]);
   gT *= ph.cq;
   // chain rule at this cell
-  const double rho = Q[x], ir = 1.0 / rho;
+  const long long cells = (long long)g.n[0] * g.n[1] * g.n[2];
+  const long long o = i + (long long)g.n[0] * (j0 + (long long)g.n[1] * k0);
+  const double rho = q[o], ir = 1.0 / rho;
   const double T = P[(ND + 1) * fs + x];
   double uu = 0.0, g0 = 0.0;
   double u[ND];
@@ -120,8 +126,6 @@ __global__ void __launch_bounds__(128) k_visc_vjp(const double* __restrict__ Q, 
   }
   const double gp = gT * ph.gama * ph.Minf * ph.Minf * ir;
   g0 += -gT * T * ir + gp * ph.gm1 * 0.5 * uu;
-  const long long cells = (long long)g.n[0] * g.n[1] * g.n[2];
-  const long long o = i + (long long)g.n[0] * (j0 + (long long)g.n[1] * k0);
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
     g0 += -gu[m] * u[m] * ir;
@@ -131,21 +135,81 @@ __global__ void __launch_bounds__(128) k_visc_vjp(const double* __restrict__ Q, 
   out[E * cells + o] += gp * ph.gm1;
 }
 
-// padded F[m] = alpha A[m] + beta B[m] (B optional) from packed arrays
-__global__ void __launch_bounds__(256) k_unpack_comb(const double* __restrict__ A, const double* __restrict__ B,
-                                                     double alpha, double beta, double* __restrict__ F, int nv,
-                                                     Grid g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n[0]) return;
-  const int j = blockIdx.y, k = blockIdx.z;
+// packed index of the periodic image of padded point (i, j, k)
+__device__ __forceinline__ long long wrap_index(const Grid& g, int i, int j, int k) {
+  const int a = (i % g.n[0] + g.n[0]) % g.n[0];
+  const int b = (j % g.n[1] + g.n[1]) % g.n[1];
+  const int c = (k % g.n[2] + g.n[2]) % g.n[2];
+  return a + (long long)g.n[0] * (b + (long long)g.n[1] * c);
+}
+
+// the constituent relations u_i, p, T of the packed state q (k_prims's
+// formulas) and M = alpha A + beta B (B optional) over the interior and every
+// halo layer, each halo point from its periodic image in the packed arrays
+template <int ND>
+__global__ void __launch_bounds__(256) k_adj_unpack(const double* __restrict__ q, const double* __restrict__ A,
+                                                    const double* __restrict__ B, double alpha, double beta,
+                                                    double* __restrict__ P, double* __restrict__ M, Grid g, Phys ph) {
+  constexpr int NV = ND + 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - g.h[0];
+  if (i >= g.n[0] + g.h[0]) return;
+  const int j = (int)blockIdx.y - g.h[1], k = (int)blockIdx.z - g.h[2];
   const long long cells = (long long)g.n[0] * g.n[1] * g.n[2];
-  const long long o = i + (long long)g.n[0] * (j + (long long)g.n[1] * k);
+  const long long o = wrap_index(g, i, j, k);
   const int x = g.idx(i, j, k);
-  for (int m = 0; m < nv; ++m) {
-    double v = alpha * A[m * cells + o];
-    if (B) v += beta * B[m * cells + o];
-    F[m * g.fs + x] = v;
+  double v[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    v[m] = q[m * cells + o];
+    double w = alpha * A[m * cells + o];
+    if (B) w += beta * B[m * cells + o];
+    M[m * g.fs + x] = w;
   }
+  const double rho = v[0];
+  const double ir = 1.0 / rho;
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    const double u = v[1 + m] * ir;
+    P[m * g.fs + x] = u;
+    ke += u * u;
+  }
+  const double p = ph.gm1 * (v[NV - 1] - 0.5 * rho * ke);
+  P[ND * g.fs + x] = p;
+  P[(ND + 1) * g.fs + x] = p * ph.gama * ph.Minf * ph.Minf * ir;
+}
+
+// the padded state and its constituent relations u_i, p, T (k_prims's
+// formulas) over the interior and every halo layer, each halo point from its
+// periodic image in the packed state
+template <int ND>
+__global__ void __launch_bounds__(256) k_unpack_prims(const double* __restrict__ q, double* __restrict__ Q,
+                                                      double* __restrict__ P, Grid g, Phys ph) {
+  constexpr int NV = ND + 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - g.h[0];
+  if (i >= g.n[0] + g.h[0]) return;
+  const int j = (int)blockIdx.y - g.h[1], k = (int)blockIdx.z - g.h[2];
+  const long long cells = (long long)g.n[0] * g.n[1] * g.n[2];
+  const long long o = wrap_index(g, i, j, k);
+  const int x = g.idx(i, j, k);
+  double v[NV];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) {
+    v[m] = q[m * cells + o];
+    Q[m * g.fs + x] = v[m];
+  }
+  const double rho = v[0];
+  const double ir = 1.0 / rho;
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    const double u = v[1 + m] * ir;
+    P[m * g.fs + x] = u;
+    ke += u * u;
+  }
+  const double p = ph.gm1 * (v[NV - 1] - 0.5 * rho * ke);
+  P[ND * g.fs + x] = p;
+  P[(ND + 1) * g.fs + x] = p * ph.gama * ph.Minf * ph.Minf * ir;
 }
 
 }  // namespace
@@ -153,26 +217,35 @@ __global__ void __launch_bounds__(256) k_unpack_comb(const double* __restrict__ 
 // fields of the scratch launch_visc_vjp needs (2 ND + ND^2 padded fields)
 int visc_vjp_fields(int nd) { return 2 * nd + nd * nd; }
 
-// out (packed) += (dV/dQ at Q)^T (alpha A + beta B) for the viscous terms V.
-// Qpad: the state on the padded grid with filled halos, P its constituent
-// relations over the padded grid; Mpad and W are scratch (nv and
-// visc_vjp_fields(nd) padded fields).
-void launch_visc_vjp(int nd, const double* Qpad, const double* P, const double* A, const double* B, double alpha,
-                     double beta, double* Mpad, double* W, double* out, const Grid& g, const Phys& ph,
-                     hipStream_t s) {
-  const int nv = nd + 2;
-  dim3 gu((g.n[0] + 255) / 256, g.n[1], g.n[2]);
-  k_unpack_comb<<<gu, 256, 0, s>>>(A, B, alpha, beta, Mpad, nv, g);
-  launch_halo_periodic(Mpad, nv, g, s);
+// the padded state (every halo layer filled with periodic images) and its
+// constituent relations from a packed state, in one pass
+void launch_unpack_prims(int nd, const double* q, double* Qpad, double* P, const Grid& g, const Phys& ph,
+                         hipStream_t s) {
+  dim3 gu((g.n[0] + 2 * g.h[0] + 255) / 256, g.n[1] + 2 * g.h[1], g.n[2] + 2 * g.h[2]);
+  if (nd == 3) k_unpack_prims<3><<<gu, 256, 0, s>>>(q, Qpad, P, g, ph);
+  else if (nd == 2) k_unpack_prims<2><<<gu, 256, 0, s>>>(q, Qpad, P, g, ph);
+  else k_unpack_prims<1><<<gu, 256, 0, s>>>(q, Qpad, P, g, ph);
+}
+
+// out (packed) += (dV/dQ at q)^T (alpha A + beta B) for the viscous terms V,
+// q the packed state.  P (nd + 2 padded fields), Mpad (nv) and W
+// (visc_vjp_fields(nd)) are scratch; the grid's halo is >= 4 (the first pass
+// reads P and M two layers beyond the two halo layers of W it forms).
+void launch_visc_vjp(int nd, const double* q, const double* A, const double* B, double alpha, double beta, double* P,
+                     double* Mpad, double* W, double* out, const Grid& g, const Phys& ph, hipStream_t s) {
+  dim3 gu((g.n[0] + 2 * g.h[0] + 255) / 256, g.n[1] + 2 * g.h[1], g.n[2] + 2 * g.h[2]);
+  if (nd == 3) k_adj_unpack<3><<<gu, 256, 0, s>>>(q, A, B, alpha, beta, P, Mpad, g, ph);
+  else if (nd == 2) k_adj_unpack<2><<<gu, 256, 0, s>>>(q, A, B, alpha, beta, P, Mpad, g, ph);
+  else k_adj_unpack<1><<<gu, 256, 0, s>>>(q, A, B, alpha, beta, P, Mpad, g, ph);
+  // w and 2 mu_E tau over the interior and two halo layers
+  dim3 gp((g.n[0] + 4 + 127) / 128, g.n[1] + (nd >= 2 ? 4 : 0), g.n[2] + (nd >= 3 ? 4 : 0));
+  if (nd == 3) k_visc_vjp_pre<3><<<gp, 128, 0, s>>>(P, Mpad, W, g, ph);
+  else if (nd == 2) k_visc_vjp_pre<2><<<gp, 128, 0, s>>>(P, Mpad, W, g, ph);
+  else k_visc_vjp_pre<1><<<gp, 128, 0, s>>>(P, Mpad, W, g, ph);
   dim3 grid((g.n[0] + 127) / 128, g.n[1], g.n[2]);
-  if (nd == 3) k_visc_vjp_pre<3><<<grid, 128, 0, s>>>(P, Mpad, W, g, ph);
-  else if (nd == 2) k_visc_vjp_pre<2><<<grid, 128, 0, s>>>(P, Mpad, W, g, ph);
-  else k_visc_vjp_pre<1><<<grid, 128, 0, s>>>(P, Mpad, W, g, ph);
-  // halos of w and 2 mu_E tau (the pointwise mu_E V needs none)
-  launch_halo_periodic(W, nd + nd * nd, g, s);
-  if (nd == 3) k_visc_vjp<3><<<grid, 128, 0, s>>>(Qpad, P, Mpad, W, out, g, ph);
-  else if (nd == 2) k_visc_vjp<2><<<grid, 128, 0, s>>>(Qpad, P, Mpad, W, out, g, ph);
-  else k_visc_vjp<1><<<grid, 128, 0, s>>>(Qpad, P, Mpad, W, out, g, ph);
+  if (nd == 3) k_visc_vjp<3><<<grid, 128, 0, s>>>(q, P, Mpad, W, out, g, ph);
+  else if (nd == 2) k_visc_vjp<2><<<grid, 128, 0, s>>>(q, P, Mpad, W, out, g, ph);
+  else k_visc_vjp<1><<<grid, 128, 0, s>>>(q, P, Mpad, W, out, g, ph);
 }
 
 }  // namespace osbl

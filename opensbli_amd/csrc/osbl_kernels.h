@@ -45,13 +45,13 @@ void launch_adj_stage(int nd, const double* Qs, const double* A, const double* B
                       double* out1, double* out2, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_pack_all(const double* F, double* out, const Grid& g, int nv, hipStream_t s);
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
-                           double beta, double c, double* out1, double* out2, double* part, const Grid& g,
-                           const Phys& ph, int seg, bool acc, hipStream_t s);
-int adj_char_segment(const Grid& g, int seg);
+                           double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, bool acc,
+                           hipStream_t s);
 int visc_vjp_fields(int nd);
-void launch_visc_vjp(int nd, const double* Qpad, const double* P, const double* A, const double* B, double alpha,
-                     double beta, double* Mpad, double* W, double* out, const Grid& g, const Phys& ph, hipStream_t s);
-long long adj_char_scratch(const Grid& g, int seg);
+void launch_unpack_prims(int nd, const double* q, double* Qpad, double* P, const Grid& g, const Phys& ph,
+                         hipStream_t s);
+void launch_visc_vjp(int nd, const double* q, const double* A, const double* B, double alpha, double beta, double* P,
+                     double* Mpad, double* W, double* out, const Grid& g, const Phys& ph, hipStream_t s);
 void launch_weno_filter_apply(int nd, double* Q, const double* R, double* theta, const Grid& g, const Phys& ph,
                               bool ducros, bool mach, double control, int walls, hipStream_t s);
 void launch_sfd(int nd, double* Q, double* F, const Grid& g, double c, double E, hipStream_t s);

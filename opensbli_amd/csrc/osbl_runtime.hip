@@ -196,7 +196,6 @@ struct osbl_ctx {
   int seg = 32;
   int vsz = 32;             // z segment of the tiled viscous kernels
   int csz = 32;             // z segment of the tiled central convective kernel
-  int vjp_seg = 8;          // preferred interfaces per thread of the characteristic VJP sweeps
   bool need_prims = false;
   bool tiled_visc = false;  // 3-D characteristic scheme with viscous terms
   bool tiled_gen = false;   // the same on the general path (closures/metric/mu(T) in x and y only)
@@ -227,9 +226,7 @@ struct osbl_ctx {
   double* Abuf = nullptr;  // stage state with halos
   double* Mbuf = nullptr;  // adjoint weights with halos
   double* pk = nullptr;    // 3 packed arrays: mu, l, lo
-  double* vpart = nullptr; // per-segment partial sums of the characteristic VJP
   double* vw = nullptr;    // viscous VJP fields (padded)
-  int vseg = 0;            // its segment length
 };
 
 namespace {
@@ -722,7 +719,6 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->Abuf);
   (void)hipFree(c->Mbuf);
   (void)hipFree(c->pk);
-  (void)hipFree(c->vpart);
   (void)hipFree(c->vw);
   (void)hipFree(c->cmet);
   (void)hipFree(c->cflux);
@@ -1257,10 +1253,8 @@ struct AdjBufs {
   double* Abuf;   // stage state with halos (nv padded fields)
   double* Mbuf;   // adjoint weights with halos (nv padded fields)
   double* pk;     // 3 packed arrays: lam_Q2, lam_Q1, lam_Qo
-  double* vpart;  // per-segment partial sums of the characteristic VJP
   double* vw;     // viscous VJP fields (padded)
   double* P;      // constituent relations over the padded grid
-  int vseg;       // segment length of the characteristic VJP sweeps
 };
 
 static int adjoint_check(const osbl_ctx* c, bool need_rk3) {
@@ -1308,21 +1302,14 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3, AdjBufs* b) {
     if (hipMalloc((void**)&c->P, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint constituent relations");
     HIPCHK(hipMemsetAsync(c->P, 0, bytes, c->stream));
   }
-  if (c->cfg.conv == OSBL_CONV_WENO_Z && !c->vpart) {
-    c->vseg = adj_char_segment(c->g, c->vjp_seg);
-    const size_t bytes = (size_t)adj_char_scratch(c->g, c->vseg) * sizeof(double);
-    if (hipMalloc((void**)&c->vpart, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint scratch");
-  }
-  *b = AdjBufs{c->Abuf, c->Mbuf, c->pk, c->vpart, c->vw, c->P, c->vseg};
+  *b = AdjBufs{c->Abuf, c->Mbuf, c->pk, c->vw, c->P};
   return OSBL_OK;
 }
 
 // out (packed) += (dR/dQ at q)^T mu
 static void vjp(osbl_ctx* c, const AdjBufs& b, const double* q, const double* mu, double* out) {
   const long long n = cells(c);
-  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, b.Abuf + m * c->g.fs, c->g, c->stream);
-  launch_halo_periodic(b.Abuf, c->nv, c->g, c->stream);
-  launch_prims(c->nd, b.Abuf, b.P, c->g, c->ph, c->stream);
+  launch_unpack_prims(c->nd, q, b.Abuf, b.P, c->g, c->ph, c->stream);
   for (int m = 0; m < c->nv; ++m) launch_unpack(mu + m * n, b.Mbuf + m * c->g.fs, c->g, c->stream);
   launch_halo_periodic(b.Mbuf, c->nv, c->g, c->stream);
   launch_rhs_vjp(c->nd, b.Abuf, b.P, b.Mbuf, out, c->g, c->ph, c->stream);
@@ -1331,19 +1318,15 @@ static void vjp(osbl_ctx* c, const AdjBufs& b, const double* q, const double* mu
 // out (packed) += (dV/dQ at q)^T (alpha A + beta B) for the viscous terms V
 static void visc_vjp(osbl_ctx* c, const AdjBufs& b, const double* q, const double* A, const double* B, double alpha,
                      double beta, double* out) {
-  const long long n = cells(c);
-  for (int m = 0; m < c->nv; ++m) launch_unpack(q + m * n, b.Abuf + m * c->g.fs, c->g, c->stream);
-  launch_halo_periodic(b.Abuf, c->nv, c->g, c->stream);
-  launch_prims(c->nd, b.Abuf, b.P, c->g, c->ph, c->stream);
-  launch_visc_vjp(c->nd, b.Abuf, b.P, A, B, alpha, beta, b.Mbuf, b.vw, out, c->g, c->ph, c->stream);
+  launch_visc_vjp(c->nd, q, A, B, alpha, beta, b.P, b.Mbuf, b.vw, out, c->g, c->ph, c->stream);
 }
 
 // out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B), out2 = A + B (optional)
 static void adj_stage(osbl_ctx* c, const AdjBufs& b, const double* Qs, const double* A, const double* B, double alpha,
                       double beta, double cc, double* out1, double* out2) {
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
-    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, b.vpart, c->g, c->ph,
-                          b.vseg, false, c->stream);
+    launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, false,
+                          c->stream);
   else
     launch_adj_stage(c->nd, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->stream);
   if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, b, Qs, A, B, alpha, beta, out1);
@@ -1409,7 +1392,6 @@ static int scratch_bufs(osbl_ctx* c, AdjBufs* b) {
   int rc = scratch_address(c->device, &base);
   if (rc) return rc;
   const long long fs = c->g.fs, packed = cells(c) * c->nv;
-  const int vseg = adj_char_segment(c->g, c->vjp_seg);
   long long off = 0;
   auto take = [&](long long ndoubles) {
     const long long at = off;
@@ -1420,8 +1402,6 @@ static int scratch_bufs(osbl_ctx* c, AdjBufs* b) {
   const bool halo = adj_needs_halo_bufs(c);
   const long long o_A = halo ? take(fs * c->nv) : -1;
   const long long o_M = halo ? take(fs * c->nv) : -1;
-  const bool weno = c->cfg.conv == OSBL_CONV_WENO_Z;
-  const long long o_part = weno ? take(adj_char_scratch(c->g, vseg)) : -1;
   const bool visc = c->cfg.viscous == OSBL_VISC_CONST;
   const long long o_vw = visc ? take(fs * visc_vjp_fields(c->nd)) : -1;
   const long long o_P = adj_needs_prims(c) ? take(fs * (c->nd + 3)) : -1;
@@ -1430,11 +1410,11 @@ static int scratch_bufs(osbl_ctx* c, AdjBufs* b) {
     return fail(OSBL_E_ARG, "osbl_adjoint_step_dev_noalloc: this block needs %zu bytes of scratch, more than the "
                             "%zu-byte fixed scratch (use osbl_adjoint_step_dev)", need, kScratchBytes);
   auto at = [&](long long o) { return o < 0 ? nullptr : base + o; };
-  *b = AdjBufs{at(o_A), at(o_M), at(o_pk), at(o_part), at(o_vw), at(o_P), vseg};
-  // zero everything from the first halo buffer on (A, M, the partial sums,
-  // the viscous fields, P): the allocating path's buffers start zeroed
-  if (halo || weno || visc) {
-    const long long from = halo ? o_A : (weno ? o_part : o_vw);
+  *b = AdjBufs{at(o_A), at(o_M), at(o_pk), at(o_vw), at(o_P)};
+  // zero everything from the first halo buffer on (A, M, the viscous
+  // fields, P): the allocating path's buffers start zeroed
+  if (halo || visc) {
+    const long long from = halo ? o_A : o_vw;
     const long long n = off - from;
     const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
     k_zero<<<blocks, 256, 0, c->stream>>>(base + from, n);
@@ -1453,8 +1433,8 @@ int osbl_rhs_vjp_dev(osbl_ctx* c, const double* q_dev, const double* mu_dev, dou
   int rc = adjoint_ready(c, false, &b);
   if (rc) return rc;
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
-    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, b.vpart,
-                          c->g, c->ph, b.vseg, true, c->stream);
+    launch_adj_stage_char(c->nd, c->cfg.averaging, q_dev, mu_dev, nullptr, 1.0, 0.0, 0.0, out_dev, nullptr, c->g,
+                          c->ph, true, c->stream);
   else vjp(c, b, q_dev, mu_dev, out_dev);
   if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, b, q_dev, mu_dev, nullptr, 1.0, 0.0, out_dev);
   HIPCHK(hipGetLastError());

@@ -297,6 +297,10 @@ class Block:
             check(load().osbl_set_sfd_field(self.h, m, _dptr(a)))
 
     def close(self):
+        twin = getattr(self, "_adjoint_twin", None)  # the adjoint's recomputation context
+        if twin is not None:
+            twin.close()
+            self._adjoint_twin = None
         if getattr(self, "h", None):
             load().osbl_destroy(self.h)
             self.h = None
