@@ -69,21 +69,28 @@ class _Integrate(torch.autograd.Function):
         # checkpoints and alternate between the two stage buffers
         twin = _twin(block)
 
-        def recompute(seg, stages):
+        def restart(seg):
             q = ctx.checkpoints[seg]
             twin.set_state_dev([q[m].data_ptr() for m in range(nv)])
-            for s in range(steps_of(seg)):
-                twin.step_stages_dev(stages[s].data_ptr())
 
         _sync()
-        recompute(nseg - 1, bufs[0])
+        restart(nseg - 1)
+        for s in range(steps_of(nseg - 1)):
+            twin.step_stages_dev(bufs[0][s].data_ptr())
         twin.sync()
         cur = 0
         for seg in range(nseg - 1, -1, -1):
-            if seg > 0:
-                recompute(seg - 1, bufs[1 - cur])
-            for s in range(steps_of(seg) - 1, -1, -1):
-                block.adjoint_step_dev(bufs[cur][s].data_ptr(), lam.data_ptr())
+            # the two streams' launches are issued alternately, so that
+            # neither waits for the host to finish queueing the other's
+            nrec = steps_of(seg - 1) if seg > 0 else 0
+            if nrec:
+                restart(seg - 1)
+            k = steps_of(seg)
+            for i in range(max(k, nrec)):
+                if i < nrec:
+                    twin.step_stages_dev(bufs[1 - cur][i].data_ptr())
+                if i < k:
+                    block.adjoint_step_dev(bufs[cur][k - 1 - i].data_ptr(), lam.data_ptr())
             block.sync()
             twin.sync()
             cur = 1 - cur
