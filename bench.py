@@ -148,8 +148,8 @@ def main():
         return bench_cpu(args)
     if args.workload == "adjoint":
         out = adjoint_ratio()
-        out["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
-        out["central_viscous"] = adjoint_ratio(viscous=True, reps=1)
+        out["weno_z"] = adjoint_ratio(conv="weno_z")
+        out["central_viscous"] = adjoint_ratio(viscous=True)
         print(json.dumps(out), flush=True)
         return
 
@@ -231,8 +231,8 @@ def main():
     if not args.no_adjoint:
         _log("bench: adjoint/forward ratio (2-D Kelvin-Helmholtz 512^2, 200 steps, checkpoints every 20)")
         adj = adjoint_ratio()
-        adj["weno_z"] = adjoint_ratio(conv="weno_z", reps=1)
-        adj["central_viscous"] = adjoint_ratio(viscous=True, reps=1)
+        adj["weno_z"] = adjoint_ratio(conv="weno_z")
+        adj["central_viscous"] = adjoint_ratio(viscous=True)
 
     ens = fp = None
     if not args.no_sub:
