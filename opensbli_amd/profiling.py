@@ -92,18 +92,50 @@ def summarise(fetch, write, copy_fetch, copy_write, copy_bytes):
     return out, {"fetch_factor": ff, "write_factor": fw, "copy_bytes": copy_bytes}
 
 
+SQ_COLUMNS = [("SQ_INSTS_VALU", "VALU instr"), ("SQ_INSTS_VALU_FMA_F64", "FMA f64"),
+              ("SQ_INSTS_VALU_MUL_F64", "MUL f64"), ("SQ_INSTS_VALU_ADD_F64", "ADD f64"),
+              ("SQ_INSTS_VALU_TRANS_F64", "TRANS f64"), ("SQ_INSTS_LDS", "LDS instr"),
+              ("SQ_INSTS_VMEM_RD", "VMEM rd"), ("SQ_INSTS_VMEM_WR", "VMEM wr")]
+
+
+def sq_table(paths, cells):
+    """Per-cell wave-instruction counts (x 64 lanes / cells) of every kernel
+    in one or more counter_collection.csv files, averaged over dispatches,
+    as a markdown table; FP64 flop counts an FMA as two."""
+    vals = collections.defaultdict(dict)
+    for path in paths:
+        for counter, _ in SQ_COLUMNS:
+            for name, v in read_counter(path, counter).items():
+                vals[name.split("(")[0]][counter] = sum(v) / len(v)
+    head = [c[1] for c in SQ_COLUMNS] + ["FP64 flop"]
+    lines = ["| Kernel | " + " | ".join(head) + " |", "|---" * (len(head) + 1) + "|"]
+    for name in sorted(vals):
+        d = vals[name]
+        if "SQ_INSTS_VALU" not in d or not family(name):
+            continue
+        per = {c: d.get(c, float("nan")) * 64.0 / cells for c, _ in SQ_COLUMNS}
+        flop = 2 * per["SQ_INSTS_VALU_FMA_F64"] + per["SQ_INSTS_VALU_MUL_F64"] + per["SQ_INSTS_VALU_ADD_F64"]
+        lines.append("| `%s` | " % name + " | ".join("%.0f" % per[c] for c, _ in SQ_COLUMNS) + " | %.0f |" % flop)
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-fetch", required=True)
-    ap.add_argument("--copy-write", required=True)
-    ap.add_argument("--copy-bytes", type=float, required=True, help="bytes the copy reads (and writes)")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
+    ap.add_argument("--copy-bytes", type=float, help="bytes the copy reads (and writes)")
     ap.add_argument("--grid", type=int, required=True)
-    ap.add_argument("--scheme", required=True)
+    ap.add_argument("--scheme")
     ap.add_argument("--json")
     ap.add_argument("--md")
+    ap.add_argument("--sq", nargs="+", help="counter_collection.csv files of SQ instruction counters: print "
+                                           "per-cell counts per kernel instead of the traffic summary")
     a = ap.parse_args()
+    if a.sq:
+        print(sq_table(a.sq, float(a.grid) ** 3))
+        return
     fams, cal = summarise(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes)
     doc = {"grid": a.grid, "scheme": a.scheme, "calibration": cal, "families": fams}
     if a.json:
