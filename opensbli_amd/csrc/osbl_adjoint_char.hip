@@ -680,9 +680,13 @@ __global__ void __launch_bounds__(64 * VJW) k_char_vjp_lanes(const double* __res
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= nn[e];
   const int nchunk = (N + VJSTEP - 1) / VJSTEP;
-  const long long wid = (long long)blockIdx.x * VJW + wv;
+  // adjacent waves take adjacent lines (for y and z: adjacent x, sharing cache
+  // lines); blocks go to the eight XCDs in turn, so consecutive blocks of one
+  // XCD take consecutive waves and the shared lines are fetched into one L2
+  const unsigned nb = gridDim.x, bl = blockIdx.x;
+  const unsigned lb = (nb % 8 == 0) ? (bl % 8) * (nb / 8) + bl / 8 : bl;
+  const long long wid = (long long)lb * VJW + wv;
   if (wid >= lines * nchunk) return;  // whole waves only: no block-wide barrier below
-  // adjacent waves take adjacent lines (for y and z: adjacent x, sharing cache lines)
   const long long line = wid % lines;
   const int chunk = (int)(wid / lines);
   long long base, sd;
