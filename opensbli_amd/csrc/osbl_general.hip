@@ -335,7 +335,8 @@ __device__ __forceinline__ void prims_at(const double* __restrict__ Q, long long
   }
   p = ph.gm1 * (Q[(ND + 1) * fs + x] - 0.5 * rho * ke);
   T = p * ph.gama * ph.Minf * ph.Minf * ir;
-  if constexpr (MU) mu = pow(T, 1.5) * (1.0 + ph.suth) / (T + ph.suth);
+  // T^(3/2) as T sqrt(T): two correctly rounded operations instead of pow
+  if constexpr (MU) mu = T * sqrt(T) * (1.0 + ph.suth) / (T + ph.suth);
 }
 
 constexpr int GTX = 64, GTY = 8, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
@@ -343,7 +344,9 @@ constexpr int GTX = 64, GTY = 8, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
 // closure rows of the first (Carpenter 4, ReducedAccess 2) and second (2) derivatives
 __host__ __device__ __forceinline__ int closure_rows(int mod) { return mod == 0 ? 0 : (mod == 1 ? 4 : 2); }
 
-template <bool SUTH>
+// MS: the stretched directions (bit 0: x, bit 1: y); the metric factors of
+// the others are 1 and 0, and their products are left out.
+template <bool SUTH, int MS>
 __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __restrict__ Q, double* __restrict__ R,
                                                             Grid g, Phys ph, Bnd b, int sz) {
   constexpr int NF = SUTH ? 5 : 4;  // u0, u1, u2, T (, mu)
@@ -359,9 +362,19 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
   const double ix = ph.inv_delta[0], iy = ph.inv_delta[1], iz = ph.inv_delta[2];
   const bool slow = x < closure_rows(b.mod[0][0]) || g.n[0] - 1 - x < closure_rows(b.mod[0][1]) ||
                     y < closure_rows(b.mod[1][0]) || g.n[1] - 1 - y < closure_rows(b.mod[1][1]);
-  const bool metric = b.dm[0] != nullptr || b.dm[1] != nullptr;
-  const double G0 = (b.dm[0] && inside) ? b.dm[0][x] : 1.0, S0 = (b.sd[0] && inside) ? b.sd[0][x] : 0.0;
-  const double G1 = (b.dm[1] && inside) ? b.dm[1][y] : 1.0, S1 = (b.sd[1] && inside) ? b.sd[1][y] : 0.0;
+  constexpr bool metric = MS != 0;
+  auto str = [](int j) { return ((MS >> j) & 1) != 0; };
+  const double G0 = ((MS & 1) && b.dm[0] && inside) ? b.dm[0][x] : 1.0;
+  const double S0 = ((MS & 1) && b.sd[0] && inside) ? b.sd[0][x] : 0.0;
+  const double G1 = ((MS & 2) && b.dm[1] && inside) ? b.dm[1][y] : 1.0;
+  const double S1 = ((MS & 2) && b.sd[1] && inside) ? b.sd[1][y] : 0.0;
+  // G_a G_b v, the factors of unstretched directions (1) left out
+  auto gg = [&](int a, int bb, double v, const double (&G)[3]) {
+    if (str(a) && str(bb)) return G[a] * G[bb] * v;
+    if (str(a)) return G[a] * v;
+    if (str(bb)) return G[bb] * v;
+    return v;
+  };
 
   // the constituent relations of every staged point, formed from Q
   auto stage = [&](int zp, int slot) {
@@ -439,8 +452,8 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
         dd[2] = qz2(m);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          DU[m][j] = metric ? G[j] * du[j] : du[j];
-          DD[m][j][j] = metric ? G[j] * G[j] * dd[j] + G[j] * S[j] * du[j] : dd[j];
+          DU[m][j] = str(j) ? G[j] * du[j] : du[j];
+          DD[m][j][j] = str(j) ? G[j] * G[j] * dd[j] + G[j] * S[j] * du[j] : dd[j];
         }
       }
       // mixed D_y(D_x u_m), m = 0, 1
@@ -449,23 +462,19 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
         const double* f = U[m];
         double v = iy * d1(ix * dx1(f, c - 2 * GPX), ix * dx1(f, c - GPX), ix * dx1(f, c + GPX),
                            ix * dx1(f, c + 2 * GPX));
-        if (metric) v = G[0] * G[1] * v;
+        v = gg(0, 1, v, G);
         DD[m][0][1] = v;
         DD[m][1][0] = v;
       }
       // mixed D_z(D_x u_m), m = 0, 2 and D_z(D_y u_m), m = 1, 2 from the queue
       {
-        double v = qz1(NF + 0);
-        if (metric) v = G[0] * G[2] * v;
+        double v = gg(0, 2, qz1(NF + 0), G);
         DD[0][0][2] = DD[0][2][0] = v;
-        v = qz1(NF + 1);
-        if (metric) v = G[0] * G[2] * v;
+        v = gg(0, 2, qz1(NF + 1), G);
         DD[2][0][2] = DD[2][2][0] = v;
-        v = qz1(NF + 2);
-        if (metric) v = G[1] * G[2] * v;
+        v = gg(1, 2, qz1(NF + 2), G);
         DD[1][1][2] = DD[1][2][1] = v;
-        v = qz1(NF + 3);
-        if (metric) v = G[1] * G[2] * v;
+        v = gg(1, 2, qz1(NF + 3), G);
         DD[2][1][2] = DD[2][2][1] = v;
       }
       double D2T[3], DT[3], DMU[3];
@@ -481,9 +490,9 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
         for (int j = 0; j < 3; ++j) {
           DT[j] = 0.0;
           DMU[j] = 0.0;
-          if (SUTH || metric) {
-            DT[j] = metric ? G[j] * t1[j] : t1[j];
-            D2T[j] = metric ? G[j] * G[j] * t2[j] + G[j] * S[j] * t1[j] : t2[j];
+          if constexpr (SUTH || metric) {
+            DT[j] = str(j) ? G[j] * t1[j] : t1[j];
+            D2T[j] = str(j) ? G[j] * G[j] * t2[j] + G[j] * S[j] * t1[j] : t2[j];
           } else {
             D2T[j] = t2[j];
           }
@@ -492,7 +501,7 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
           const double* MU = sp[sl][4];
           const double m1[3] = {ix * dx1(MU, c), iy * dy1(MU, c), qz1(4)};
 #pragma unroll
-          for (int j = 0; j < 3; ++j) DMU[j] = metric ? G[j] * m1[j] : m1[j];
+          for (int j = 0; j < 3; ++j) DMU[j] = str(j) ? G[j] * m1[j] : m1[j];
         }
       }
       const double um[3] = {zq[2][0], zq[2][1], zq[2][2]};
@@ -520,11 +529,12 @@ __global__ void __launch_bounds__(128) k_visc_shell(const double* __restrict__ P
     i = t;
     j = sr.ys[blockIdx.y];
   } else {
-    if (t >= g.n[1]) return;
+    // the columns of one y are adjacent threads (a face's columns share cache lines)
+    if (t >= g.n[1] * sr.nx) return;
+    j = t / sr.nx;
+    i = sr.xs[t % sr.nx];
     for (int q = 0; q < sr.ny; ++q)
-      if (sr.ys[q] == t) return;
-    i = sr.xs[blockIdx.y];
-    j = t;
+      if (sr.ys[q] == j) return;
   }
   const int x = g.idx(i, j, k);
   const int st[3] = {1, g.pitch, g.plane};
@@ -541,8 +551,19 @@ void launch_visc_gen3d(const double* Q, const double* P, double* R, const Grid& 
                        int suth, int sz, hipStream_t s) {
   dim3 grid((g.n[0] + GTX - 1) / GTX, (g.n[1] + GTY - 1) / GTY, (g.n[2] + sz - 1) / sz);
   dim3 block(GTX, GTY);
-  if (suth) k_visc_gen3d<true><<<grid, block, 0, s>>>(Q, R, g, ph, b, sz);
-  else k_visc_gen3d<false><<<grid, block, 0, s>>>(Q, R, g, ph, b, sz);
+  const int ms = ((b.dm[0] || b.sd[0]) ? 1 : 0) | ((b.dm[1] || b.sd[1]) ? 2 : 0);
+#define OSBL_VISC_GEN3D(SU, MS) k_visc_gen3d<SU, MS><<<grid, block, 0, s>>>(Q, R, g, ph, b, sz)
+  switch (ms + (suth ? 4 : 0)) {
+    case 0: OSBL_VISC_GEN3D(false, 0); break;
+    case 1: OSBL_VISC_GEN3D(false, 1); break;
+    case 2: OSBL_VISC_GEN3D(false, 2); break;
+    case 3: OSBL_VISC_GEN3D(false, 3); break;
+    case 4: OSBL_VISC_GEN3D(true, 0); break;
+    case 5: OSBL_VISC_GEN3D(true, 1); break;
+    case 6: OSBL_VISC_GEN3D(true, 2); break;
+    default: OSBL_VISC_GEN3D(true, 3); break;
+  }
+#undef OSBL_VISC_GEN3D
   // the closure rows and columns (distinct indices; a face's rows may overlap
   // the other face's on tiny blocks, hence the de-duplication)
   ShellRows sr{};
@@ -556,7 +577,7 @@ void launch_visc_gen3d(const double* Q, const double* P, double* R, const Grid& 
   for (int r = 0; r < closure_rows(b.mod[0][0]); ++r) add(sr.xs, sr.nx, r);
   for (int r = 0; r < closure_rows(b.mod[0][1]); ++r) add(sr.xs, sr.nx, g.n[0] - 1 - r);
   if (sr.ny) k_visc_shell<<<dim3((g.n[0] + 127) / 128, sr.ny, g.n[2]), 128, 0, s>>>(P, R, g, ph, b, suth, sr, 0);
-  if (sr.nx) k_visc_shell<<<dim3((g.n[1] + 127) / 128, sr.nx, g.n[2]), 128, 0, s>>>(P, R, g, ph, b, suth, sr, 1);
+  if (sr.nx) k_visc_shell<<<dim3((g.n[1] * sr.nx + 127) / 128, 1, g.n[2]), 128, 0, s>>>(P, R, g, ph, b, suth, sr, 1);
 }
 
 // Constituent relations with the Sutherland viscosity over the padded domain:

@@ -126,22 +126,25 @@ def main():
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=float, help="bytes the copy reads (and writes)")
-    ap.add_argument("--grid", type=int, required=True)
+    ap.add_argument("--grid", type=int, nargs="+", required=True,
+                    help="interior cells per direction (one value: a cube)")
     ap.add_argument("--scheme")
     ap.add_argument("--json")
     ap.add_argument("--md")
     ap.add_argument("--sq", nargs="+", help="counter_collection.csv files of SQ instruction counters: print "
                                            "per-cell counts per kernel instead of the traffic summary")
     a = ap.parse_args()
+    n = a.grid * 3 if len(a.grid) == 1 else a.grid
+    cells = float(n[0]) * n[1] * (n[2] if len(n) > 2 else 1)
     if a.sq:
-        print(sq_table(a.sq, float(a.grid) ** 3))
+        print(sq_table(a.sq, cells))
         return
     fams, cal = summarise(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes)
-    doc = {"grid": a.grid, "scheme": a.scheme, "calibration": cal, "families": fams}
+    doc = {"grid": a.grid[0] if len(a.grid) == 1 else a.grid, "scheme": a.scheme, "calibration": cal,
+           "families": fams}
     if a.json:
         with open(a.json, "w") as f:
             json.dump(doc, f, indent=1, sort_keys=True)
-    cells = float(a.grid) ** 3
     lines = ["| family | fetch B/launch | write B/launch | total B/cell | dispatches |", "|---|---|---|---|---|"]
     for fam in sorted(fams):
         d = fams[fam]
