@@ -31,6 +31,7 @@ the reference's apps) and refuses anything else with the reference's error
 types.
 The initial condition equations are evaluated on the host with SymPy.
 """
+import functools
 import math
 import re
 
@@ -739,6 +740,53 @@ def _norm(text):
     return t
 
 
+# operators of the equation language: kept as undefined functions, so that
+# only the algebra around and inside them is normalised
+_OPERATORS = ("Der", "Skew", "Conservative", "KD", "Dot", "Eq")
+
+
+@functools.lru_cache(maxsize=None)
+def _canon(text):
+    """Canonical form of an expression (or Eq) of the equation language:
+    sympy's ordering of sums and products with every product expanded, also
+    inside the operators' arguments, so that algebraically identical forms
+    (reordered factors or terms, a factor distributed over a sum) compare
+    equal.  Every name is a plain symbol or an undefined function (none takes
+    sympy's meaning, such as E or beta); the scheme keyword of Conservative
+    becomes a symbol argument.  Text that does not parse is compared as
+    written."""
+    t = _norm(text)
+    t = re.sub(r"@(\w+)", r",scheme_\1", t)
+    names = set(re.findall(r"[A-Za-z_]\w*", t))
+    local = {n: (sympy.Function(n) if n in _OPERATORS else sympy.Symbol(n)) for n in names}
+    try:
+        e = parse_expr(t, local_dict=local)
+    except Exception:  # noqa: BLE001 (anything sympy cannot read is compared as text)
+        return t
+    return sympy.srepr(sympy.expand(e, deep=True))
+
+
+def _canon_term(sign, text):
+    """(sign, canonical form) of a top-level term, a leading minus sign of the
+    term itself moved into `sign`."""
+    t = _norm(text)
+    t = re.sub(r"@(\w+)", r",scheme_\1", t)
+    names = set(re.findall(r"[A-Za-z_]\w*", t))
+    local = {n: (sympy.Function(n) if n in _OPERATORS else sympy.Symbol(n)) for n in names}
+    try:
+        e = sympy.expand(parse_expr(t, local_dict=local), deep=True)
+    except Exception:  # noqa: BLE001
+        return sign, _norm(text)
+    if e.could_extract_minus_sign():
+        sign, e = -sign, -e
+    return sign, sympy.srepr(e)
+
+
+def _terms(pairs):
+    """Canonical (sign, term) pairs of a set of (sign, text) pairs."""
+    return {_canon_term(sg, tx) for sg, tx in pairs}
+
+
 def _split_terms(rhs):
     """Top-level signed terms of an expression string."""
     terms, depth, cur, sign = [], 0, "", 1
@@ -835,8 +883,8 @@ def _recognise(block):
         m = re.fullmatch(r"Der\((\w+),t\)", lhs)
         if not m:
             raise NotImplementedError("left-hand side %r is not a time derivative" % lhs)
-        terms[m.group(1)] = set(_split_terms(rhs))
-        subs.update(_norm(s) for s in e.substitutions)
+        terms[m.group(1)] = _terms(_split_terms(rhs))
+        subs.update(_canon(s) for s in e.substitutions)
     if set(terms) != {"rho", "rhou_i", "rhoE"}:
         raise NotImplementedError("the HIPC target solves the mass, momentum and energy equations; got %s"
                                   % sorted(terms))
@@ -848,19 +896,20 @@ def _recognise(block):
         raise ValueError("Found more than one temporal scheme on the block" if temporal else
                          "no temporal scheme on the block")
     # a constant body force (apps/channel_flow): -KD(_i,_j)*c_j and -Dot(c_j, u_j)
-    fm, fe = (-1, "KD(_i,_j)*c_j"), (-1, "Dot(c_j,u_j)")
+    fm, fe = _canon_term(-1, "KD(_i,_j)*c_j"), _canon_term(-1, "Dot(c_j,u_j)")
     force = fm in terms["rhou_i"] and fe in terms["rhoE"]
     if force:
         terms["rhou_i"].discard(fm)
         terms["rhoE"].discard(fe)
     elif fm in terms["rhou_i"] or fe in terms["rhoE"]:
         raise NotImplementedError("a body force needs both -KD(_i,_j)*c_j and -Dot(c_j, u_j)")
-    viscous_terms = VISCOUS
+    viscous_terms = {k: _terms(v) for k, v in VISCOUS.items()}
+    heat_terms = {sg: _canon_term(sg, "Der(q_j,x_j)") for sg in (1, -1)}
     has_visc = any(t in terms[k] for k, v in viscous_terms.items() for t in v)
     heat = None
     if has_visc:
         for sign in (1, -1):
-            if (sign, "Der(q_j,x_j)") in terms["rhoE"]:
+            if heat_terms[sign] in terms["rhoE"]:
                 heat = sign
     conv = None
     averaging = "roe"
@@ -872,8 +921,8 @@ def _recognise(block):
             if has_visc:
                 got -= viscous_terms.get(var, set())
                 if var == "rhoE" and heat is not None:
-                    got.discard((heat, "Der(q_j,x_j)"))
-            if got != want[var]:
+                    got.discard(heat_terms[heat])
+            if got != _terms(want[var]):
                 ok = False
         if ok:
             conv = name
@@ -909,14 +958,14 @@ def _recognise(block):
             raise ValueError("viscous terms need a Central(4) scheme on the block")
     viscous = "none"
     if has_visc:
-        for var, want in VISCOUS.items():
+        for var, want in viscous_terms.items():
             if not want <= terms[var]:
                 raise NotImplementedError("partial viscous terms are not supported")
-        if _norm(STRESS) in subs:
-            heat_ok = (heat == 1 and _norm(HEAT_POS) in subs) or (heat == -1 and _norm(HEAT_NEG) in subs)
+        if _canon(STRESS) in subs:
+            heat_ok = (heat == 1 and _canon(HEAT_POS) in subs) or (heat == -1 and _canon(HEAT_NEG) in subs)
             viscous = "const"
-        elif _norm(STRESS_MU) in subs:
-            heat_ok = (heat == 1 and _norm(HEAT_MU_POS) in subs) or (heat == -1 and _norm(HEAT_MU_NEG) in subs)
+        elif _canon(STRESS_MU) in subs:
+            heat_ok = (heat == 1 and _canon(HEAT_MU_POS) in subs) or (heat == -1 and _canon(HEAT_MU_NEG) in subs)
             viscous = "sutherland"
         else:
             raise NotImplementedError("only the stress tensors (1.0/Re)*(...) and (mu/Re)*(...) are supported")
@@ -940,7 +989,7 @@ def _recognise(block):
         if k not in have:
             raise ValueError("Constituent relation is not found for %s" % k)
         want = SUTHERLAND if k == "mu" else CONSTITUENT[k]
-        if have[k] != _norm(want):
+        if _canon(have[k]) != _canon(want):
             raise NotImplementedError("constituent relation for %s not supported: %s" % (k, have[k]))
     sensor = 0
     tenos = [s for s in spatial if isinstance(s, LLFTeno)]

@@ -301,3 +301,52 @@ def test_channel_body_force_is_recognised():
         prog.problem()
     prog.set_values(["c0", "c1"], ["-1", "0"])
     assert prog.problem()["body_force"] == [-1.0, 0.0]
+
+
+def _euler_block(eqs, rels):
+    """A periodic 2-D block with the given Euler equations and constituent
+    relations (WENO5-Z, RK3), discretised."""
+    from opensbli_amd import api
+    ee = api.EinsteinEquation()
+    sim = api.SimulationEquations()
+    cr = api.ConstituentRelations()
+    for t in eqs:
+        sim.add_equations(ee.expand(t, 2, "x", [], ["gama", "Minf"]))
+    for t in rels:
+        cr.add_equations(ee.expand(t, 2, "x", [], ["gama", "Minf"]))
+    block = api.SimulationBlock(2)
+    block.set_block_boundaries([api.PeriodicBC(d, s) for d in range(2) for s in (0, 1)])
+    llf = api.LLFWeno(5, formulation="Z", averaging=api.RoeAverage([0, 1]))
+    rk = api.RungeKutta(3)
+    block.set_discretisation_schemes({llf.name: llf, rk.name: rk})
+    block.set_equations([cr, sim])
+    block.discretise()
+    return block
+
+
+def test_equations_are_recognised_up_to_algebra():
+    """Reordered factors and terms, and a factor distributed over a sum, are
+    the same equations; a different flux is refused."""
+    from opensbli_amd import api
+    sc = "**{'scheme':'Weno'}"
+    ref_eqs = ["Eq(Der(rho,t), - Conservative(rhou_j,x_j,%s))" % sc,
+               "Eq(Der(rhou_i,t) , -Conservative(rhou_i*u_j + KD(_i,_j)*p,x_j , %s)  )" % sc,
+               "Eq(Der(rhoE,t), - Conservative((p+rhoE)*u_j,x_j, %s))" % sc]
+    ref_rels = ["Eq(u_i, rhou_i/rho)", "Eq(p, (gama-1)*(rhoE - rho*(1/2)*(KD(_i,_j)*u_i*u_j)))",
+                "Eq(a, (gama*p/rho)**0.5)"]
+    alt_eqs = ["Eq(Der(rho,t), -Conservative(rhou_j, x_j, %s))" % sc,
+               "Eq(Der(rhou_i,t), -Conservative(p*KD(_i,_j) + u_j*rhou_i, x_j, %s))" % sc,
+               "Eq(Der(rhoE,t), -Conservative(u_j*rhoE + u_j*p, x_j, %s))" % sc]
+    alt_rels = ["Eq(u_i, rhou_i/rho)", "Eq(p, (rhoE - (1/2)*rho*(KD(_i,_j)*u_j*u_i))*(gama-1))",
+                "Eq(a, (p*gama/rho)**0.5)"]
+    want = api._recognise(_euler_block(ref_eqs, ref_rels))
+    assert want["conv"] == "weno_z"
+    assert api._recognise(_euler_block(alt_eqs, alt_rels)) == want
+    bad = list(ref_eqs)
+    bad[2] = "Eq(Der(rhoE,t), - Conservative((p+2*rhoE)*u_j,x_j, %s))" % sc
+    with pytest.raises(NotImplementedError, match="not recognised"):
+        api._recognise(_euler_block(bad, ref_rels))
+    bad_rels = list(ref_rels)
+    bad_rels[1] = "Eq(p, (gama-1)*(rhoE - rho*(KD(_i,_j)*u_i*u_j)))"
+    with pytest.raises(NotImplementedError, match="constituent relation for p"):
+        api._recognise(_euler_block(ref_eqs, bad_rels))
