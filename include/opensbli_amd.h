@@ -223,6 +223,19 @@ int osbl_get_metric_field(osbl_ctx* ctx, int idx, double* host);
    10^-floor(teno_a1 - teno_a2 (1 - (1 - theta)^4 (1 + 4 theta))) of it. */
 int osbl_get_sensor(osbl_ctx* ctx, double* host);
 
+/* Number of NaN values among the interior values of conservative variable
+   `var` of the current state, or of every variable when var < 0: the check
+   OPS's ops_NaNcheck(dat) makes for a simulation monitor
+   (core/diagnostics/simulation_monitors.py:112-113) or for
+   print_iteration_ops(NaN_check=...) (utilities/helperfunctions.py:172-190).
+   The caller decides what a non-zero count stops. */
+int osbl_nan_count(osbl_ctx* ctx, int var, int64_t* count);
+
+/* Value of conservative variable `var` at interior index `index` (ndim
+   entries, 0-based): the one-point reduction of a simulation monitor's probe
+   (core/diagnostics/simulation_monitors.py:73-106). */
+int osbl_get_point(osbl_ctx* ctx, int var, const int32_t* index, double* value);
+
 /* Per-face plane of conservative values, [nvar][n_e2][n_e1] over the interior
    extents of the two other directions (e1 < e2): the values a DirichletBC face
    sets (dirichlet.py:28-41), or the inflow state an InletPressureExtrapolateBC

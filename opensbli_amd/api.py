@@ -719,6 +719,50 @@ class SimulationBlock:
         self.discretised = True
 
 
+class SimulationMonitor:
+    """Probes of conservative variables at grid points, printed during the
+    run, with an optional NaN check (core/diagnostics/simulation_monitors.py:16-44).
+    After iteration 1 and every print_frequency-th iteration: the NaN check of
+    the first monitored array (NaNcheck=True; ops_NaNcheck stops the run, here
+    FloatingPointError), the header 'Iteration, Time, name(i, j, k), ...' at
+    the first print, then 'iteration, time, values...' with fp_precision
+    decimals, to stdout or appended to output_file (simulation_monitors.py:
+    112-160, the condition at algorithm.py:433-437)."""
+
+    def __init__(self, arrays, probe_locations, block, print_frequency=250, fp_precision=10, NaNcheck=True,
+                 output_file=None):
+        if len(arrays) != len(probe_locations):
+            raise ValueError("The number of arrays must equal the number of probe locations.")
+        self.arrays = [str(a) for a in arrays]
+        self.probe_locations = [tuple(int(i) for i in loc) if isinstance(loc, (tuple, list)) else (int(loc),)
+                                for loc in probe_locations]
+        self.block = block
+        self.ndim = block.ndim
+        self.frequency = int(print_frequency)
+        self.fp_precision = int(fp_precision)
+        self.NaNcheck = NaNcheck
+        self.output_file = output_file
+        names = _state_names(self.ndim)
+        for a, loc in zip(self.arrays, self.probe_locations):
+            if a not in names:
+                raise NotImplementedError("the HIPC target monitors the conservative arrays %s, not %s" % (names, a))
+            if len(loc) != self.ndim:
+                raise ValueError("probe location %s of %s has %d indices on a %d-D block"
+                                 % (loc, a, len(loc), self.ndim))
+
+    def header(self):
+        return ", ".join(["Iteration", "Time"] + ["%s%s" % (a, loc if len(loc) > 1 else "(%d)" % loc[0])
+                                                   for a, loc in zip(self.arrays, self.probe_locations)])
+
+    def line(self, iteration, time, values):
+        f = "%%.%df" % self.fp_precision
+        return ", ".join(["%d" % iteration, f % time] + [f % v for v in values])
+
+
+def _state_names(nd):
+    return ["rho"] + ["rhou%d" % d for d in range(nd)] + ["rhoE"]
+
+
 class TraditionalAlgorithmRK:
     """The RK time-loop algorithm of a block (algorithm.py:336)."""
 
@@ -1428,17 +1472,49 @@ class HIPC:
             if io.save_every:
                 io.file_name(int(io.save_every))
         n = int(self.values.get("niter", 0) if niter is None else niter)
+        names = _state_names(nd)
+        mon = getattr(self.alg, "simulation_monitor", None)
+        every, nan_name = getattr(self, "print_every", None), getattr(self, "nan_check", None)
+        if nan_name is not None and str(nan_name) not in names:
+            raise NotImplementedError("print_iteration_ops checks a conservative array %s, not %s" % (names, nan_name))
+        # states after k steps that something reads: the writes after step k;
+        # the monitor after step 1 and every frequency-th step; the iteration
+        # print of step k at the top of its loop body, i.e. after k-1 steps
+        saves = {k for io in writes if io.save_every for k in range(int(io.save_every), n + 1, int(io.save_every))}
+        monitored = ({k for k in range(mon.frequency, n + 1, mon.frequency)} | ({1} if n >= 1 else set())
+                     if mon is not None else set())
+        printed = {k - 1 for k in range(every, n + 1, every)} if every else set()
+        out = open(mon.output_file, "w") if mon is not None and mon.output_file else None
         blk = native.Block(problem)
+        dt = float(problem["dt"])
+
+        def nan_check(name, k):
+            if blk.nan_count(names.index(str(name))):
+                raise FloatingPointError("NaN detected in %s after iteration %d" % (name, k))
+
         try:
             blk.set_state(q0)
-            events = sorted({k for io in writes if io.save_every
-                             for k in range(int(io.save_every), n + 1, int(io.save_every))})
             done = 0
-            for k in events:
+            for k in sorted(saves | monitored | printed):
                 blk.run(k - done)
                 done = k
+                if k in printed:
+                    print("Iteration is %d" % (k + 1))
+                    if nan_name is not None:
+                        nan_check(nan_name, k)
+                if k in monitored:
+                    if mon.NaNcheck:
+                        nan_check(mon.arrays[0], k)
+                    text = ([mon.header()] if k == 1 else []) + [
+                        mon.line(k, k * dt, [blk.point(names.index(a), loc)
+                                             for a, loc in zip(mon.arrays, mon.probe_locations)])]
+                    for t in text:
+                        if out is not None:
+                            out.write(t + "\n")
+                        else:
+                            print(t)
                 for io in writes:
-                    if io.save_every and k % int(io.save_every) == 0:
+                    if io.save_every and k in saves and k % int(io.save_every) == 0:
                         self._write(io, io.file_name(k), blk, problem)
             blk.run(n - done)
             for io in writes:
@@ -1447,12 +1523,27 @@ class HIPC:
         except BaseException:
             blk.close()
             raise
-        names = ["rho"] + ["rhou%d" % d for d in range(problem["ndim"])] + ["rhoE"]
+        finally:
+            if out is not None:
+                out.close()
         result = {nm: q[i] for i, nm in enumerate(names)}
         if return_block:
             return result, blk
         blk.close()
         return result
+
+
+def print_iteration_ops(simulation_name="opensbli", every=250, NaN_check=None):
+    """Print 'Iteration is k' at the top of every iteration k that is a
+    multiple of `every`, with a NaN check of the conservative array named by
+    NaN_check there (helperfunctions.py:172-190 inserts the same into the
+    generated time loop; ops_NaNcheck stops the run, here FloatingPointError)."""
+    prog = _CURRENT.get(simulation_name)
+    if prog is None:
+        raise ValueError("no HIPC program named %r: call HIPC(alg) first" % simulation_name)
+    prog.print_every = int(every)
+    prog.nan_check = None if NaN_check is None else str(NaN_check)
+    return prog
 
 
 def substitute_simulation_parameters(constants, values, simulation_name="opensbli"):

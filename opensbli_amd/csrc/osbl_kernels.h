@@ -37,6 +37,7 @@ void launch_bc_face(int nd, double* Q, double* Q2, const Grid& g, const Phys& ph
 void launch_periodic_dir(double* Q, int nv, const Grid& g, int d, hipStream_t s);
 void launch_fill_face_halo(double* Q, int nv, const Grid& g, int d, int side, const double* plane, hipStream_t s);
 void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s);
+void launch_count_nan(const double* F, int v0, int v1, unsigned long long* count, const Grid& g, hipStream_t s);
 void launch_unpack(const double* in, double* F, const Grid& g, hipStream_t s);
 long long halo_points(const Grid& g);
 void launch_rhs_vjp(int nd, const double* Q, const double* P, const double* M, double* out, const Grid& g,

@@ -1152,6 +1152,24 @@ __global__ void __launch_bounds__(256) k_unpack(const double* __restrict__ in, d
   F[g.idx(i, j, k)] = in[(long long)i + (long long)g.n[0] * (j + (long long)g.n[1] * k)];
 }
 
+// NaN values among the interior values of fields v0 .. v1-1 (OPS's
+// ops_NaNcheck): counted per wave with a ballot, a wave adds to the counter
+// only when it found one, so a finite state costs no atomics
+__global__ void __launch_bounds__(256) k_count_nan(const double* __restrict__ F, Grid g, int v0, int v1,
+                                                  unsigned long long* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  unsigned long long found = 0;
+  if (i < g.n[0]) {
+    const int x = g.idx(i, j, k);
+    for (int v = v0; v < v1; ++v) found += __builtin_isnan(F[v * g.fs + x]) ? 1 : 0;
+  }
+  unsigned long long wave = 0;
+  for (int v = 0; v < v1 - v0; ++v) wave += __popcll(__ballot(found > (unsigned long long)v));
+  if ((threadIdx.x & 63) == 0 && wave) atomicAdd(count, wave);
+}
+__global__ void k_clear_count(unsigned long long* count) { *count = 0; }
+
 // ---------------------------------------------------------------------------
 // launch helpers (instantiation tables)
 // ---------------------------------------------------------------------------
@@ -1306,6 +1324,12 @@ void launch_extrap0(double* Q, int nv, const Grid& g, int d, int side, hipStream
   const int e1 = (d == 0) ? 1 : 0, e2 = (d == 2) ? 1 : 2;
   const int n = (g.n[e1] + 2 * g.h[e1]) * (g.n[e2] + 2 * g.h[e2]);
   k_extrap0<<<(n + 255) / 256, 256, 0, s>>>(Q, nv, g, d, side);
+}
+
+void launch_count_nan(const double* F, int v0, int v1, unsigned long long* count, const Grid& g, hipStream_t s) {
+  k_clear_count<<<1, 1, 0, s>>>(count);
+  dim3 grid((g.n[0] + 255) / 256, g.n[1], g.n[2]);
+  k_count_nan<<<grid, 256, 0, s>>>(F, g, v0, v1, count);
 }
 
 void launch_pack(const double* F, double* out, const Grid& g, hipStream_t s) {

@@ -1026,6 +1026,33 @@ int osbl_get_sensor(osbl_ctx* c, double* host) {
   return OSBL_OK;
 }
 
+int osbl_nan_count(osbl_ctx* c, int var, int64_t* count) {
+  if (!c || !count) return fail(OSBL_E_ARG, "null argument");
+  if (var >= c->nv) return fail(OSBL_E_ARG, "variable %d out of range (the block has %d)", var, c->nv);
+  auto* ctr = reinterpret_cast<unsigned long long*>(c->stage_buf);
+  launch_count_nan(Qcur(c), var < 0 ? 0 : var, var < 0 ? c->nv : var + 1, ctr, c->g, c->stream);
+  unsigned long long n = 0;
+  HIPCHK(hipMemcpyAsync(&n, ctr, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *count = (int64_t)n;
+  return OSBL_OK;
+}
+
+int osbl_get_point(osbl_ctx* c, int var, const int32_t* index, double* value) {
+  if (!c || !index || !value) return fail(OSBL_E_ARG, "null argument");
+  if (var < 0 || var >= c->nv) return fail(OSBL_E_ARG, "variable %d out of range (the block has %d)", var, c->nv);
+  int ijk[3] = {0, 0, 0};
+  for (int d = 0; d < c->g.nd; ++d) {
+    if (index[d] < 0 || index[d] >= c->g.n[d])
+      return fail(OSBL_E_ARG, "index %d of direction %d outside the interior 0..%d", index[d], d, c->g.n[d] - 1);
+    ijk[d] = index[d];
+  }
+  const double* src = Qcur(c) + var * c->g.fs + c->g.idx(ijk[0], ijk[1], ijk[2]);
+  HIPCHK(hipMemcpyAsync(value, src, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return OSBL_OK;
+}
+
 int osbl_set_filter_mask(osbl_ctx* c, const uint8_t* mask) {
   if (!c) return fail(OSBL_E_ARG, "null context");
   if (!(c->cfg.filters & OSBL_FILTER_BINOMIAL)) return fail(OSBL_E_STATE, "the block has no binomial filter");

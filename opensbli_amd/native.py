@@ -32,7 +32,7 @@ EXPORTS = [
     "osbl_use_graphs",
     "osbl_set_coordinates", "osbl_set_bc_plane", "osbl_set_filter_mask", "osbl_get_sfd_field",
     "osbl_set_sfd_field", "osbl_get_field_halo", "osbl_set_grid_coordinates", "osbl_get_metric_field",
-    "osbl_get_sensor",
+    "osbl_get_sensor", "osbl_nan_count", "osbl_get_point",
 ]
 
 
@@ -147,6 +147,8 @@ def load():
     L.osbl_set_grid_coordinates.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_get_metric_field.argtypes = [vp, ctypes.c_int, dp]
     L.osbl_get_sensor.argtypes = [vp, dp]
+    L.osbl_nan_count.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.osbl_get_point.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), dp]
     _lib = L
     return L
 
@@ -262,6 +264,23 @@ class Block:
         out = np.empty(self.shape)
         check(load().osbl_get_sensor(self.h, _dptr(out)))
         return out
+
+    def nan_count(self, var=-1):
+        """NaN values among the interior values of conservative variable `var`
+        of the current state (all variables when var < 0)."""
+        n = ctypes.c_int64(0)
+        check(load().osbl_nan_count(self.h, int(var), ctypes.byref(n)))
+        return n.value
+
+    def point(self, var, index):
+        """Value of conservative variable `var` at interior index (i, j, k)
+        (ndim entries, x first)."""
+        if len(index) != self.nd:
+            raise ValueError(f"index {tuple(index)} has {len(index)} entries, the block {self.nd} dimensions")
+        idx = (ctypes.c_int32 * self.nd)(*[int(i) for i in index])
+        v = ctypes.c_double(0.0)
+        check(load().osbl_get_point(self.h, int(var), idx, ctypes.byref(v)))
+        return v.value
 
     def metric_field(self, idx):
         """Interior values of D_ab (idx = a*ndim + b) or detJ (idx = ndim*ndim) of

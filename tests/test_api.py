@@ -350,3 +350,20 @@ def test_equations_are_recognised_up_to_algebra():
     bad_rels[1] = "Eq(p, (gama-1)*(rhoE - rho*(KD(_i,_j)*u_i*u_j)))"
     with pytest.raises(NotImplementedError, match="constituent relation for p"):
         api._recognise(_euler_block(ref_eqs, bad_rels))
+
+
+def test_simulation_monitor_checks_and_format():
+    """SimulationMonitor's checks (simulation_monitors.py:27-29) and its
+    header and line formats."""
+    from opensbli_amd import api
+    block = api.SimulationBlock(2)
+    with pytest.raises(ValueError, match="number of arrays"):
+        api.SimulationMonitor(["rho"], [(1, 2), (3, 4)], block)
+    with pytest.raises(NotImplementedError, match="conservative arrays"):
+        api.SimulationMonitor(["p"], [(1, 2)], block)
+    with pytest.raises(ValueError, match="indices"):
+        api.SimulationMonitor(["rho"], [(1, 2, 3)], block)
+    m = api.SimulationMonitor([api.DataObject("rho"), "rhou1"], [(1, 2), (3, 4)], block, print_frequency=10,
+                              fp_precision=3)
+    assert m.header() == "Iteration, Time, rho(1, 2), rhou1(3, 4)"
+    assert m.line(10, 0.01, [1.0, -0.5]) == "10, 0.010, 1.000, -0.500"
