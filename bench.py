@@ -274,7 +274,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def adjoint_ratio(n=512, steps=200, every=20, reps=3, conv="central_skew", viscous=False):
+def adjoint_ratio(n=512, steps=200, every=20, reps=5, conv="central_skew", viscous=False):
     """Backward/forward time of the differentiable 2-D Kelvin-Helmholtz run:
     forward = the checkpointed forward integration, backward = recomputation
     of every segment plus the hand-written adjoint steps (median of reps).
