@@ -339,6 +339,23 @@ __device__ __forceinline__ void prims_at(const double* __restrict__ Q, long long
   if constexpr (MU) mu = T * sqrt(T) * (1.0 + ph.suth) / (T + ph.suth);
 }
 
+// the same constituent relations from a point's conservative variables in registers
+template <int ND, bool MU>
+__device__ __forceinline__ void prims_of(const double (&q)[ND + 2], const Phys& ph, double (&u)[ND], double& p,
+                                         double& T, double& mu) {
+  const double rho = q[0];
+  const double ir = 1.0 / rho;
+  double ke = 0.0;
+#pragma unroll
+  for (int m = 0; m < ND; ++m) {
+    u[m] = q[1 + m] * ir;
+    ke += u[m] * u[m];
+  }
+  p = ph.gm1 * (q[ND + 1] - 0.5 * rho * ke);
+  T = p * ph.gama * ph.Minf * ph.Minf * ir;
+  if constexpr (MU) mu = T * sqrt(T) * (1.0 + ph.suth) / (T + ph.suth);
+}
+
 constexpr int GTX = 64, GTY = 8, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
 
 // closure rows of the first (Carpenter 4, ReducedAccess 2) and second (2) derivatives
@@ -351,7 +368,8 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
                                                             Grid g, Phys ph, Bnd b, int sz) {
   constexpr int NF = SUTH ? 5 : 4;  // u0, u1, u2, T (, mu)
   constexpr int NQ = NF + 4;        // + D_x u0, D_x u2, D_y u1, D_y u2
-  __shared__ double sp[3][NF][GPL];
+  constexpr int GNT = GTX * GTY, GPP = (GPL + GNT - 1) / GNT;  // staged points per thread and plane
+  __shared__ double sp[4][NF][GPL];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * GTX + tx;
   const int x0 = blockIdx.x * GTX, y0 = blockIdx.y * GTY;
@@ -376,22 +394,43 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
     return v;
   };
 
-  // the constituent relations of every staged point, formed from Q
-  auto stage = [&](int zp, int slot) {
-    for (int pt = tid; pt < GPL; pt += GTX * GTY) {
-      const int xx = pt % GPX, yy = pt / GPX;
-      const int gx = x0 - 2 + xx, gy = y0 - 2 + yy;
-      if (gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1]) {
+  // this thread's staged points (plane offset at z = 0) and the raw Q of the
+  // plane in flight; the constituent relations of every staged point are
+  // formed from Q when the plane goes into its LDS slot
+  int soff[GPP];
+  bool sval[GPP];
+#pragma unroll
+  for (int k = 0; k < GPP; ++k) {
+    const int pt = tid + k * GNT;
+    const int gx = x0 - 2 + pt % GPX, gy = y0 - 2 + pt / GPX;
+    sval[k] = pt < GPL && gx < g.n[0] + g.h[0] && gy < g.n[1] + g.h[1];
+    soff[k] = sval[k] ? g.idx(gx, gy, 0) : 0;
+  }
+  double pq[GPP][5];
+  auto fetch = [&](int zp) {
+    const int zo = zp * g.plane;
+#pragma unroll
+    for (int k = 0; k < GPP; ++k)
+      if (sval[k]) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) pq[k][m] = Q[m * g.fs + soff[k] + zo];
+      }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < GPP; ++k)
+      if (sval[k]) {
         double u[3], pr, T, mu = 0.0;
-        prims_at<3, SUTH>(Q, g.fs, g.idx(gx, gy, zp), ph, u, pr, T, mu);
+        prims_of<3, SUTH>(pq[k], ph, u, pr, T, mu);
+        const int pt = tid + k * GNT;
         sp[slot][0][pt] = u[0];
         sp[slot][1][pt] = u[1];
         sp[slot][2][pt] = u[2];
         sp[slot][3][pt] = T;
         if constexpr (SUTH) sp[slot][4][pt] = mu;
       }
-    }
   };
+  auto slot_of = [](int zp) { return (zp + 8) & 3; };
   const int c = (ty + 2) * GPX + tx + 2;
   auto d1 = [](double fm2, double fm1, double fp1, double fp2) {
     return (1.0 / 12.0) * (fm2 - fp2) + (2.0 / 3.0) * (fp1 - fm1);
@@ -414,27 +453,30 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
     cq[NF + 3] = iy * dy1(sp[slot][2], c);
   };
 
-  // one loop stages every plane zs = z0-2 .. z1+1 (slot (zs+3) % 3) and
-  // appends its column quantities to the queue, so every plane's are computed
-  // by the same code; plane z = zs - 2 is finished once z+2 is in
-  double zq[5][NQ] = {};
-  for (int zs = z0 - 2; zs < z1 + 2; ++zs) {
-    __syncthreads();
-    stage(zs, (zs + 3) % 3);
-    __syncthreads();
+  // software pipeline: planes z0-2 .. z0+2 are staged first; while plane z
+  // is evaluated, the raw Q of plane z+3 is in flight, and plane z+3 then
+  // takes the slot plane z-1 left, one barrier per plane
+  double zq[5][NQ];
+  for (int k = 0; k < 4; ++k) {
+    fetch(z0 - 2 + k);
+    store(slot_of(z0 - 2 + k));
+  }
+  __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int e = 0; e < NQ; ++e) zq[k][e] = zq[k + 1][e];
-    column((zs + 3) % 3, zq[4]);
-    const int z = zs - 2;
-    if (z < z0 || !inside || slow) continue;
+  for (int k = 0; k < 4; ++k) column(slot_of(z0 - 2 + k), zq[k]);
+  fetch(z0 + 2);
+  __syncthreads();
+  store(slot_of(z0 + 2));
+  __syncthreads();
+  column(slot_of(z0 + 2), zq[4]);
+  for (int z = z0; z < z1; ++z) {
+    const bool more = z + 1 < z1;  // plane z+3 feeds the next plane's queue entry
+    if (more) fetch(z + 3);
+    if (inside && !slow) {
     const int o = g.idx(x, y, z);
-    double r[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) r[m] = R[m * g.fs + o];
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     {
-      const int sl = (z + 3) % 3;
+      const int sl = slot_of(z);
       const double* U[3] = {sp[sl][0], sp[sl][1], sp[sl][2]};
       const double* T = sp[sl][3];
       const double G[3] = {G0, G1, 1.0}, S[3] = {S0, S1, 0.0};
@@ -507,8 +549,19 @@ __global__ void __launch_bounds__(GTX * GTY, 1) k_visc_gen3d(const double* __res
       const double um[3] = {zq[2][0], zq[2][1], zq[2][2]};
       viscous_assemble<3>(SUTH ? zq[2][4] : 1.0, um, DU, DD, D2T, DT, DMU, ph, SUTH, r);
     }
+    // the terms add to R's momentum and energy components (r[0] stays 0)
 #pragma unroll
-    for (int m = 0; m < 5; ++m) R[m * g.fs + o] = r[m];
+    for (int m = 1; m < 5; ++m) R[m * g.fs + o] += r[m];
+    }
+    if (more) {
+      store(slot_of(z + 3));
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int e = 0; e < NQ; ++e) zq[k][e] = zq[k + 1][e];
+      column(slot_of(z + 3), zq[4]);
+    }
   }
 }
 
