@@ -486,12 +486,12 @@ def bench_cpu(args):
                               "runs_s": [round(t, 3) for t in ts]}
 
     p, q = taylor_green(64, conv="central_skew", rk="ls3")
-    timed_case("C1 taylor_green_64^3_central_skew_ls3", p, q, 20)
+    timed_case("taylor_green_64^3_central_skew_ls3", p, q, 20)
     amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
     p, q = kelvin_helmholtz(512, amps, conv="central_skew")
-    timed_case("C4 kelvin_helmholtz_512^2_central_skew_rk3", p, q, 20)
+    timed_case("kelvin_helmholtz_512^2_central_skew_rk3", p, q, 20)
     p, q = flat_plate()
-    timed_case("C5 flat_plate_480x256x128_weno_z_sutherland_tanh5_ls3", p, q, 1)
+    timed_case("flat_plate_480x256x128_weno_z_sutherland_tanh5_ls3", p, q, 1)
     del q
     # the tubes are independent: one single-threaded oracle process per core,
     # instances handed out in order of decreasing work
@@ -501,7 +501,7 @@ def bench_cpu(args):
     idx = list(range(0, 512, 8))
     work = par["n"].astype(float) * par["nsteps"]
     idx.sort(key=lambda i: -work[i])
-    _log(f"bench: cpu C3 ensemble sample ({len(idx)} instances on {threads} single-threaded processes)")
+    _log(f"bench: cpu ensemble sample ({len(idx)} instances on {threads} single-threaded processes)")
     omp = os.environ.get("OMP_NUM_THREADS")
     os.environ["OMP_NUM_THREADS"] = "1"
     try:
@@ -515,7 +515,7 @@ def bench_cpu(args):
             del os.environ["OMP_NUM_THREADS"]
         else:
             os.environ["OMP_NUM_THREADS"] = omp
-    out["cases"]["C3 ensemble_512_shock_tubes_weno_z_rk3"] = {
+    out["cases"]["ensemble_512_shock_tubes_weno_z_rk3"] = {
         "value": 3 * work[idx].sum() / el / 1e6, "sample": f"every 8th instance ({len(idx)} of 512, "
         f"{work[idx].sum() / work.sum():.3f} of the cell updates) to t = 0.2 on {threads} single-threaded oracle "
         f"processes in {el:.1f} s", "estimated_all_512_s": el * work.sum() / work[idx].sum()}

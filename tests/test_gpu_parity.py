@@ -156,12 +156,12 @@ def test_c1_tgv64_central_100_steps(rk, oracle):
     """Configuration 1 at full size: 64^3, central skew, RK3 (the 3-register
     RungeKutta(3) and the low-storage RungeKuttaLS(3)), 100 steps."""
     from opensbli_amd.problems import taylor_green
-    _say(f"case C1 tgv64 central {rk} 100 steps")
+    _say(f"case tgv 64^3 central skew {rk} 100 steps")
     problem, q0 = taylor_green(64, conv="central_skew", rk=rk)
     g, r = _pair(problem, q0, oracle)
     g.run(100)
     r.run(100)
-    check_parity(g.get_state(), r.get_state(), f"C1 {rk}")
+    check_parity(g.get_state(), r.get_state(), f"tgv 64^3 central skew {rk}")
 
 
 @pytest.mark.parametrize("conv", ["weno_z", "teno6", "central_skew", "central_cons"])
