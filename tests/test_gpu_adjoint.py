@@ -129,11 +129,16 @@ def _generic_state(n, seed=0):
     return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
 
 
-def test_weno_vjp_dot_product():
+# The finite-difference identity is checked up to 64^2: at 128^2 the step
+# already crosses switches of the maximising point of lambda_max (the torch
+# restatement's own finite difference misses its autograd product by 1.7e-5
+# there), so the exact autograd comparison below covers 128^2.
+@pytest.mark.parametrize("n", [32, 64])
+def test_weno_vjp_dot_product(n):
     import torch
-    _say("case WENO-Z adjoint vjp dot product 32^2")
-    problem, _, _, b = _weno_block(32)
-    q0 = _generic_state(32)
+    _say(f"case WENO-Z adjoint vjp dot product {n}^2")
+    problem, _, _, b = _weno_block(n)
+    q0 = _generic_state(n)
     rs = np.random.RandomState(1)
     dq = rs.normal(size=q0.shape) * np.abs(q0).mean(axis=(1, 2))[:, None, None] * 1e-3
     mu = rs.normal(size=q0.shape)
@@ -156,12 +161,67 @@ def test_weno_vjp_dot_product():
     assert abs(lhs - rhs_) < 1e-7 * abs(lhs)
 
 
-def test_weno_vjp_vs_torch_autograd():
+@pytest.mark.parametrize("n,plane", [((64, 32, 8), "xy"), ((8, 32, 64), "yz")])
+def test_weno_vjp_3d_against_2d_reference(n, plane):
+    """3-D WENO5-Z VJP of a state and weights uniform along one direction
+    (its velocity component zero) equals the 2-D VJP of the torch
+    restatement on the other two: x and y lines (64 and 32 cells) with a
+    z-uniform field, y and z lines (32 and 64) with an x-uniform one."""
     import torch
     import torch_reference as tr
-    _say("case WENO-Z adjoint vjp vs torch autograd 32^2")
-    problem, _, _, b = _weno_block(32)
-    q0 = _generic_state(32)
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import taylor_green
+    _say(f"case WENO-Z adjoint vjp 3-D {n} uniform off the {plane} plane vs 2-D torch autograd")
+    a, b_ = (0, 1) if plane == "xy" else (1, 2)  # the in-plane directions, x' = a and y' = b_
+    q2 = _generic_state(64)[:, :n[b_], :n[a]]    # (4, n_y', n_x')
+    mu2 = np.random.RandomState(5).normal(size=q2.shape)
+    delta2 = [1.0 / n[a], 1.0 / n[b_]]
+    problem, _ = taylor_green(16, conv="weno_z", viscous="none")
+    delta3 = [0.1, 0.1, 0.1]
+    delta3[a], delta3[b_] = delta2[0], delta2[1]
+    problem.update(n=list(n), delta=delta3)
+    # 3-D arrays in (z, y, x) order; momentum of the uniform direction zero
+    shape3 = tuple(n[::-1])
+    q3 = np.zeros((5,) + shape3)
+    mu3 = np.zeros((5,) + shape3)
+    comp = [0, 1 + a, 1 + b_, 4]  # 3-D components of the 2-D (rho, rhou', rhov', rhoE)
+    lift = (lambda f: f[None, :, :]) if plane == "xy" else (lambda f: f[:, :, None])  # (y', x') -> (z, y, x)
+    for k2, k3 in enumerate(comp):
+        q3[k3] = lift(q2[k2])
+        mu3[k3] = lift(mu2[k2])
+    b = Block(problem)
+    gk = _gpu_vjp3(b, q3, mu3)
+    b.close()
+    qt = torch.tensor(q2, dtype=torch.float64, requires_grad=True)
+    r = tr.rhs_weno_z(qt, delta2, problem["gama"])
+    (gt,) = torch.autograd.grad((r * torch.tensor(mu2)).sum(), qt)
+    gt = gt.numpy()
+    err = 0.0
+    for k2, k3 in enumerate(comp):
+        err = max(err, np.abs(gk[k3] - lift(gt[k2])).max())
+    err /= np.abs(gt).max()
+    _say("max rel diff", err)
+    assert err < 1e-10
+
+
+def _gpu_vjp3(b, q, mu):
+    import torch
+    qd = torch.tensor(q, dtype=torch.float64, device="cuda").contiguous()
+    md = torch.tensor(mu, dtype=torch.float64, device="cuda").contiguous()
+    out = torch.zeros_like(qd)
+    torch.cuda.synchronize()
+    b.rhs_vjp_dev(qd.data_ptr(), md.data_ptr(), out.data_ptr())
+    b.sync()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_weno_vjp_vs_torch_autograd(n):
+    import torch
+    import torch_reference as tr
+    _say(f"case WENO-Z adjoint vjp vs torch autograd {n}^2")
+    problem, _, _, b = _weno_block(n)
+    q0 = _generic_state(n)
     mu = np.random.RandomState(2).normal(size=q0.shape)
     qd = torch.tensor(q0, dtype=torch.float64, device="cuda").contiguous()
     md = torch.tensor(mu, dtype=torch.float64, device="cuda").contiguous()
