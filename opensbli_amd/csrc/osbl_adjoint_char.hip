@@ -175,7 +175,7 @@ __device__ __forceinline__ double weno5z_grad(const double x[5], double gbar, do
   const double r0 = (2.0 * f0 + 5.0 * f1 - f2) * S6;
   const double r1 = (-fm1 + 5.0 * f0 + 2.0 * f1) * S6;
   const double r2 = (2.0 * fm2 - 7.0 * fm1 + 11.0 * f0) * S6;
-  const double iS = 1.0 / (A0 + A1 + A2);
+  const double iS = recip_pos(A0 + A1 + A2);
   const double W = (A0 * r0 + A1 * r1 + A2 * r2) * iS;
   // reverse
   const double rb0 = A0 * iS * gbar, rb1 = A1 * iS * gbar, rb2 = A2 * iS * gbar;
@@ -309,9 +309,10 @@ __device__ __forceinline__ void average_rev(const Pt<ND>& L, const Pt<ND>& R, co
   constexpr int NV = ND + 2;
   double pLb, pRb, rLb, rRb, uLb[ND], uRb[ND];
   if constexpr (AVG == OSBL_AVG_ROE) {
-    const double sL = sqrt(L.q[0]), sR = sqrt(R.q[0]);
-    const double inv = 1.0 / (sL + sR);
-    const double HL = (L.q[NV - 1] + L.p) / sL, HR = (R.q[NV - 1] + R.p) / sR;
+    const double sL = sqrt_pos(L.q[0]), sR = sqrt_pos(R.q[0]);
+    const double isL = recip_pos(sL), isR = recip_pos(sR);
+    const double inv = recip_pos(sL + sR);
+    const double HL = (L.q[NV - 1] + L.p) * isL, HR = (R.q[NV - 1] + R.p) * isR;
     double ke = 0.0, u[ND];
 #pragma unroll
     for (int m = 0; m < ND; ++m) {
@@ -319,9 +320,9 @@ __device__ __forceinline__ void average_rev(const Pt<ND>& L, const Pt<ND>& R, co
       ke += u[m] * u[m];
     }
     const double hh = (HL + HR) * inv;
-    const double a = sqrt(ph.gm1 * (hh - 0.5 * ke));
+    const double a = sqrt_pos(ph.gm1 * (hh - 0.5 * ke));
     // a = sqrt(gm1 (hh - ke/2))
-    const double tb = b.a * ph.gm1 / (2.0 * a);
+    const double tb = quot_pos(b.a * ph.gm1, 2.0 * a);
     double invb = tb * (HL + HR);
     const double Hb = tb * inv;
     double sLb = b.rho * sR, sRb = b.rho * sL;
@@ -335,15 +336,15 @@ __device__ __forceinline__ void average_rev(const Pt<ND>& L, const Pt<ND>& R, co
       invb += ub * (sL * L.u[m] + sR * R.u[m]);
     }
     const double sb = -inv * inv * invb;
-    sLb += sb - Hb * HL / sL;
-    sRb += sb - Hb * HR / sR;
+    sLb += sb - Hb * HL * isL;
+    sRb += sb - Hb * HR * isR;
     // H = (E + p) / s
-    pLb = Hb / sL;
-    pRb = Hb / sR;
-    gL[NV - 1] += Hb / sL;
-    gR[NV - 1] += Hb / sR;
-    rLb = sLb / (2.0 * sL);
-    rRb = sRb / (2.0 * sR);
+    pLb = Hb * isL;
+    pRb = Hb * isR;
+    gL[NV - 1] += Hb * isL;
+    gR[NV - 1] += Hb * isR;
+    rLb = 0.5 * sLb * isL;
+    rRb = 0.5 * sRb * isR;
   } else {
     // rho = (rho_L + rho_R)/2, u = (u_L + u_R)/2, a = (a_L + a_R)/2, a = sqrt(gama p / rho)
     rLb = 0.5 * b.rho;
@@ -354,10 +355,10 @@ __device__ __forceinline__ void average_rev(const Pt<ND>& L, const Pt<ND>& R, co
       uRb[m] = 0.5 * b.u[m];
     }
     const double aLb = 0.5 * b.a, aRb = 0.5 * b.a;
-    pLb = aLb * ph.gama / (2.0 * L.a * L.q[0]);
-    pRb = aRb * ph.gama / (2.0 * R.a * R.q[0]);
-    rLb += -aLb * L.a / (2.0 * L.q[0]);
-    rRb += -aRb * R.a / (2.0 * R.q[0]);
+    pLb = quot_pos(aLb * ph.gama, 2.0 * L.a * L.q[0]);
+    pRb = quot_pos(aRb * ph.gama, 2.0 * R.a * R.q[0]);
+    rLb += quot_pos(-aLb * L.a, 2.0 * L.q[0]);
+    rRb += quot_pos(-aRb * R.a, 2.0 * R.q[0]);
   }
   // p = gm1 (E - rho |u|^2 / 2), u_m = (rho u_m) / rho
   double keL = 0.0, keR = 0.0;
@@ -370,7 +371,7 @@ __device__ __forceinline__ void average_rev(const Pt<ND>& L, const Pt<ND>& R, co
   gR[NV - 1] += ph.gm1 * pRb;
   rLb += -0.5 * ph.gm1 * keL * pLb;
   rRb += -0.5 * ph.gm1 * keR * pRb;
-  const double irL = 1.0 / L.q[0], irR = 1.0 / R.q[0];
+  const double irL = recip_pos(L.q[0]), irR = recip_pos(R.q[0]);
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
     const double uLt = uLb[m] - ph.gm1 * L.q[0] * L.u[m] * pLb;
@@ -606,13 +607,13 @@ __device__ __forceinline__ void char_flux_vjp(const W& w, const double nu[ND + 2
   // u_D, p, a of every point -> its conservative variables
 #pragma unroll
   for (int p = 0; p < 6; ++p) {
-    const double rho = w.q(p, 0), ir = 1.0 / rho;
+    const double rho = w.q(p, 0), ir = recip_pos(rho);
     const double a = w.a(p);
     double uu = 0.0;
 #pragma unroll
     for (int m = 0; m < ND; ++m) uu += w.u(p, m) * w.u(p, m);
     // a = sqrt(gama p / rho)
-    const double np_ = npp[p] + na[p] * ph.gama * 0.5 / (a * rho);
+    const double np_ = npp[p] + quot_pos(na[p] * ph.gama * 0.5, a * rho);
     double g0 = -na[p] * 0.5 * a * ir;
     // p = (gama-1)(E - |rho u|^2 / (2 rho))
     g0 += np_ * ph.gm1 * 0.5 * uu;

@@ -127,13 +127,41 @@ struct Pt {
   double p, a;
 };
 
+// a/b (a of either sign) and 1/b for positive normal b far from the
+// overflow range:
+// the hardware reciprocal estimate refined by a Newton-Raphson step, the
+// quotient then corrected once by its residual (Markstein's method); within
+// an ulp of the correctly rounded result, in about half the instructions of
+// the IEEE division sequence, which must also handle scaling and special
+// operands.
+__device__ __forceinline__ double quot_pos(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = __fma_rn(__fma_rn(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return __fma_rn(__fma_rn(-b, q, a), r, q);
+}
+// sqrt(x) for positive normal x: the hardware reciprocal-square-root
+// estimate refined by a Newton-Raphson step, then y = x r corrected once by
+// its residual, y + r (x - y^2)/2; within an ulp of the correctly rounded root.
+__device__ __forceinline__ double sqrt_pos(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = r * __fma_rn(-0.5 * x * r, r, 1.5);
+  const double y = x * r;
+  return __fma_rn(0.5 * r, __fma_rn(-y, y, x), y);
+}
+__device__ __forceinline__ double recip_pos(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = __fma_rn(__fma_rn(-b, r, 1.0), r, r);
+  return __fma_rn(__fma_rn(-b, r, 1.0), r, r);
+}
+
 // Every operation is rounded on its own (no contraction into FMAs), so a
 // point's values are bit-identical wherever a kernel evaluates them -- in a
 // sweep's prologue or in its loop -- and a direction along which the state is
 // uniform keeps exactly zero flux differences.
 template <int ND>
 __device__ __forceinline__ void finish_pt(Pt<ND>& s, const Phys& ph) {
-  const double ir = 1.0 / s.q[0];
+  const double ir = recip_pos(s.q[0]);
   double ke = 0.0;
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
@@ -141,7 +169,7 @@ __device__ __forceinline__ void finish_pt(Pt<ND>& s, const Phys& ph) {
     ke = __dadd_rn(ke, __dmul_rn(s.u[m], s.u[m]));
   }
   s.p = __dmul_rn(ph.gm1, __dsub_rn(s.q[ND + 1], __dmul_rn(__dmul_rn(0.5, s.q[0]), ke)));
-  s.a = sqrt(__dmul_rn(__dmul_rn(ph.gama, s.p), ir));
+  s.a = sqrt_pos(__dmul_rn(__dmul_rn(ph.gama, s.p), ir));
 }
 
 // raw conservative variables of one point
@@ -181,7 +209,7 @@ __device__ __forceinline__ void finish_avg(AvgT<ND, T>& v, const Phys& ph) {
   for (int m = 1; m < ND; ++m) uu = uu + v.u[m] * v.u[m];
   if constexpr (std::is_same_v<T, double>) {
     // 1/a and 1/rho from one division
-    const double t = 1.0 / (v.a * v.rho);
+    const double t = recip_pos(v.a * v.rho);
     v.ia = v.rho * t;
     v.ir = v.a * t;
   } else {
@@ -204,14 +232,14 @@ struct RoeF {
 };
 template <int ND>
 __device__ __forceinline__ void roe_factors(const Pt<ND>& P, double& sq, double& h) {
-  sq = sqrt(P.q[0]);
-  h = (P.q[ND + 1] + P.p) * (sq / P.q[0]);
+  sq = sqrt_pos(P.q[0]);
+  h = (P.q[ND + 1] + P.p) * quot_pos(sq, P.q[0]);
 }
 
 template <int ND>
 __device__ __forceinline__ void roe_state(Avg<ND>& v, const Pt<ND>& L, const Pt<ND>& R, const RoeF& f,
                                           const Phys& ph) {
-  const double inv = 1.0 / (f.sR + f.sL);
+  const double inv = recip_pos(f.sR + f.sL);
   v.rho = f.sL * f.sR;  // sqrt(rho_L rho_R)
   double ke = 0.0;
 #pragma unroll
@@ -220,7 +248,7 @@ __device__ __forceinline__ void roe_state(Avg<ND>& v, const Pt<ND>& L, const Pt<
     ke += v.u[m] * v.u[m];
   }
   const double hh = (f.hL + f.hR) * inv;
-  v.a = sqrt(ph.gm1 * (hh - 0.5 * ke));
+  v.a = sqrt_pos(ph.gm1 * (hh - 0.5 * ke));
 }
 
 template <int ND, int AVG>
@@ -478,7 +506,7 @@ __device__ __forceinline__ double weno5z(double fm2, double fm1, double f0, doub
 // separately.
 __device__ __forceinline__ double sum_quotients(double n1, double d1, double n2, double d2) {
   const double dd = d1 * d2;
-  if (dd < 1.0e300) return (n1 * d2 + n2 * d1) / dd;
+  if (dd < 1.0e300) return quot_pos(n1 * d2 + n2 * d1, dd);
   return n1 / d1 + n2 / d2;
 }
 
@@ -600,7 +628,7 @@ __device__ __forceinline__ void teno_nd(double fm2, double fm1, double f0, doubl
     const double pr = p01 * p23;
     double a0, a1, a2, a3;
     if (fabs(pr) > 1.0e-300 && fabs(pr) < 1.0e300) {
-      const double tp = tau / pr;
+      const double tp = quot_pos(tau, pr);
       a0 = pow6(1.0 + (e1 * p23) * tp);
       a1 = pow6(1.0 + (e0 * p23) * tp);
       a2 = pow6(1.0 + (e3 * p01) * tp);
@@ -611,7 +639,7 @@ __device__ __forceinline__ void teno_nd(double fm2, double fm1, double f0, doubl
       a2 = pow6(1.0 + tau / e2);
       a3 = pow6(1.0 + tau / e3);
     }
-    const double ias = 1.0 / (a0 + a1 + a2 + a3);
+    const double ias = recip_pos(a0 + a1 + a2 + a3);
     const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
     const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
     const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
@@ -626,7 +654,7 @@ __device__ __forceinline__ void teno_nd(double fm2, double fm1, double f0, doubl
     const double pr = p01 * e2;
     double a0, a1, a2;
     if (fabs(pr) > 1.0e-300 && fabs(pr) < 1.0e300) {
-      const double tp = tau / pr;
+      const double tp = quot_pos(tau, pr);
       a0 = pow6(1.0 + (e1 * e2) * tp);
       a1 = pow6(1.0 + (e0 * e2) * tp);
       a2 = pow6(1.0 + p01 * tp);
@@ -635,7 +663,7 @@ __device__ __forceinline__ void teno_nd(double fm2, double fm1, double f0, doubl
       a1 = pow6(1.0 + tau / e1);
       a2 = pow6(1.0 + tau / e2);
     }
-    const double ias = 1.0 / (a0 + a1 + a2);
+    const double ias = recip_pos(a0 + a1 + a2);
     const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
     const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
     const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
@@ -647,7 +675,9 @@ __device__ __forceinline__ void teno_nd(double fm2, double fm1, double f0, doubl
 }
 
 // The same with one division per quotient (kernels that keep their stencil
-// in registers: the shared-division form needs more registers than they have).
+// in registers: the shared-division form needs more registers than they
+// have); the quotients are all of positive operands (eps + beta > 0, sums of
+// weights >= 1), formed by quot_pos / recip_pos.
 template <bool SIX, bool QUIRK>
 __device__ __forceinline__ double teno(double fm2, double fm1, double f0, double f1, double f2, double f3,
                                        double eps, double ct) {
@@ -679,31 +709,31 @@ __device__ __forceinline__ double teno(double fm2, double fm1, double f0, double
     else b3 += (781.0 / 720.0) * q * q;
     const double r3 = (3.0 * f0 + 13.0 * f1 - 5.0 * f2 + f3) * (1.0 / 12.0);
     const double tau = fabs(b3 - (b0 + b2 - 4.0 * b1) * S6);
-    const double a0 = pow6(1.0 + tau / (eps + b0));
-    const double a1 = pow6(1.0 + tau / (eps + b1));
-    const double a2 = pow6(1.0 + tau / (eps + b2));
-    const double a3 = pow6(1.0 + tau / (eps + b3));
-    const double ias = 1.0 / (a0 + a1 + a2 + a3);
+    const double a0 = pow6(1.0 + quot_pos(tau, eps + b0));
+    const double a1 = pow6(1.0 + quot_pos(tau, eps + b1));
+    const double a2 = pow6(1.0 + quot_pos(tau, eps + b2));
+    const double a3 = pow6(1.0 + quot_pos(tau, eps + b3));
+    const double ias = recip_pos(a0 + a1 + a2 + a3);
     const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
     const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
     const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
     const double d3 = (a3 * ias < ct) ? 0.0 : 1.0;
     constexpr double D0 = 231.0 / 500.0, D1 = 3.0 / 10.0, D2 = 27.0 / 500.0, D3 = 23.0 / 125.0;
     const double w0 = D0 * d0, w1 = D1 * d1, w2 = D2 * d2, w3 = D3 * d3;
-    const double iws = 1.0 / (w0 + w1 + w2 + w3);
+    const double iws = recip_pos(w0 + w1 + w2 + w3);
     return (w0 * r0 + w1 * r1 + w2 * r2 + w3 * r3) * iws;
   } else {
     const double tau = fabs(b0 - b2);
-    const double a0 = pow6(1.0 + tau / (eps + b0));
-    const double a1 = pow6(1.0 + tau / (eps + b1));
-    const double a2 = pow6(1.0 + tau / (eps + b2));
-    const double ias = 1.0 / (a0 + a1 + a2);
+    const double a0 = pow6(1.0 + quot_pos(tau, eps + b0));
+    const double a1 = pow6(1.0 + quot_pos(tau, eps + b1));
+    const double a2 = pow6(1.0 + quot_pos(tau, eps + b2));
+    const double ias = recip_pos(a0 + a1 + a2);
     const double d0 = (a0 * ias < ct) ? 0.0 : 1.0;
     const double d1 = (a1 * ias < ct) ? 0.0 : 1.0;
     const double d2 = (a2 * ias < ct) ? 0.0 : 1.0;
     constexpr double D0 = 11.0 / 20.0, D1 = 4.0 / 10.0, D2 = 1.0 / 20.0;
     const double w0 = D0 * d0, w1 = D1 * d1, w2 = D2 * d2;
-    const double iws = 1.0 / (w0 + w1 + w2);
+    const double iws = recip_pos(w0 + w1 + w2);
     return (w0 * r0 + w1 * r1 + w2 * r2) * iws;
   }
 }
