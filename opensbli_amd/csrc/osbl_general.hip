@@ -326,7 +326,7 @@ template <int ND, bool MU>
 __device__ __forceinline__ void prims_at(const double* __restrict__ Q, long long fs, int x, const Phys& ph,
                                          double (&u)[ND], double& p, double& T, double& mu) {
   const double rho = Q[x];
-  const double ir = 1.0 / rho;
+  const double ir = recip_pos(rho);
   double ke = 0.0;
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
@@ -335,8 +335,9 @@ __device__ __forceinline__ void prims_at(const double* __restrict__ Q, long long
   }
   p = ph.gm1 * (Q[(ND + 1) * fs + x] - 0.5 * rho * ke);
   T = p * ph.gama * ph.Minf * ph.Minf * ir;
-  // T^(3/2) as T sqrt(T): two correctly rounded operations instead of pow
-  if constexpr (MU) mu = T * sqrt(T) * (1.0 + ph.suth) / (T + ph.suth);
+  // T^(3/2) as T sqrt(T) (instead of pow); the reciprocals, quotient and
+  // root of these positive operands by the refined hardware estimates
+  if constexpr (MU) mu = T * sqrt_pos(T) * quot_pos(1.0 + ph.suth, T + ph.suth);
 }
 
 // the same constituent relations from a point's conservative variables in registers
@@ -344,7 +345,7 @@ template <int ND, bool MU>
 __device__ __forceinline__ void prims_of(const double (&q)[ND + 2], const Phys& ph, double (&u)[ND], double& p,
                                          double& T, double& mu) {
   const double rho = q[0];
-  const double ir = 1.0 / rho;
+  const double ir = recip_pos(rho);
   double ke = 0.0;
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
@@ -353,7 +354,7 @@ __device__ __forceinline__ void prims_of(const double (&q)[ND + 2], const Phys& 
   }
   p = ph.gm1 * (q[ND + 1] - 0.5 * rho * ke);
   T = p * ph.gama * ph.Minf * ph.Minf * ir;
-  if constexpr (MU) mu = T * sqrt(T) * (1.0 + ph.suth) / (T + ph.suth);
+  if constexpr (MU) mu = T * sqrt_pos(T) * quot_pos(1.0 + ph.suth, T + ph.suth);
 }
 
 constexpr int GTX = 64, GTY = 8, GPX = GTX + 4, GPY = GTY + 4, GPL = GPX * GPY;
