@@ -241,8 +241,10 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
 constexpr int XW = 4;             // waves (lines) per block
 constexpr int XP = 64 + 5;        // points staged per chunk
 
+// (at least three waves per SIMD: TENO6 needs ~178 VGPRs otherwise and
+// would run two; held to 168 it spills four and is 6 % faster)
 template <int ND, int CONV, int AVG, bool ACC>
-__global__ void __launch_bounds__(64 * XW) k_char_sweep_xl(const double* __restrict__ Q, double* __restrict__ R,
+__global__ void __launch_bounds__(64 * XW, 3) k_char_sweep_xl(const double* __restrict__ Q, double* __restrict__ R,
                                                           Grid g, Phys ph, int nrows, int periodic,
                                                           const double* __restrict__ met,
                                                           const double* __restrict__ theta) {
