@@ -6,9 +6,10 @@ recomputation.
     J = kinetic_energy(qN, problem); J.backward()       # q0.grad = dJ/dq0
 
 Forward: the block is advanced with the GPU kernels; the state is kept every
-`every` steps.  Backward: for each segment, from last to first, the segment
-is recomputed from its checkpoint while the input states of every RK stage
-are stored, then the adjoint of each step (osbl_adjoint_step_dev) is applied
+`every` steps, and the input states of every RK stage of the last segment
+are stored as it runs.  Backward: for each segment, from last to first, the
+segment is recomputed from its checkpoint (except the last) while the input
+states of every RK stage are stored, then the adjoint of each step (osbl_adjoint_step_dev) is applied
 from the last step back; the recomputation of the next segment runs on a
 second context of the same problem, on its own stream, during the adjoint
 steps of the current one.  The block is then left at the forward output.
@@ -34,10 +35,21 @@ class _Integrate(torch.autograd.Function):
         _sync()
         block.set_state_dev([q0c[m].data_ptr() for m in range(block.nvar)])
         checkpoints = [q0c.clone()]
+        # the last segment's stage inputs are stored as it runs (the backward
+        # pass starts there and would otherwise recompute them first, with
+        # nothing to overlap); the buffer has room for the longest segment and
+        # becomes one of the backward pass's two stage buffers
+        last = None
         done = 0
         while done < nsteps:
             k = min(every, nsteps - done)
-            block.run(k)
+            if done + k == nsteps and ctx.needs_input_grad[0]:
+                last = torch.empty((min(every, nsteps), block.num_stages()) + tuple(q0c.shape),
+                                   dtype=torch.float64, device=q0c.device)
+                for i in range(k):
+                    block.step_stages_dev(last[i].data_ptr())
+            else:
+                block.run(k)
             done += k
             snap = torch.empty_like(q0c)
             block.get_state_dev([snap[m].data_ptr() for m in range(block.nvar)])
@@ -47,6 +59,7 @@ class _Integrate(torch.autograd.Function):
         ctx.nsteps = nsteps
         ctx.every = every
         ctx.checkpoints = checkpoints
+        ctx.last = last
         return checkpoints[-1].clone()
 
     @staticmethod
@@ -61,8 +74,10 @@ class _Integrate(torch.autograd.Function):
             return min(ctx.every, ctx.nsteps - seg * ctx.every)
 
         kmax = max(steps_of(seg) for seg in range(nseg))
-        bufs = [torch.empty((kmax, ns) + tuple(lam.shape), dtype=torch.float64, device=lam.device)
-                for _ in range(2)]
+        last, ctx.last = ctx.last, None
+        bufs = [last if last is not None else
+                torch.empty((kmax, ns) + tuple(lam.shape), dtype=torch.float64, device=lam.device),
+                torch.empty((kmax, ns) + tuple(lam.shape), dtype=torch.float64, device=lam.device)]
         # a second context of the same problem recomputes the stage inputs of
         # the next segment (backwards) on its own stream while this block runs
         # the adjoint steps of the current one; the two share only read-only
@@ -74,10 +89,11 @@ class _Integrate(torch.autograd.Function):
             twin.set_state_dev([q[m].data_ptr() for m in range(nv)])
 
         _sync()
-        restart(nseg - 1)
-        for s in range(steps_of(nseg - 1)):
-            twin.step_stages_dev(bufs[0][s].data_ptr())
-        twin.sync()
+        if last is None:
+            restart(nseg - 1)
+            for s in range(steps_of(nseg - 1)):
+                twin.step_stages_dev(bufs[0][s].data_ptr())
+            twin.sync()
         cur = 0
         for seg in range(nseg - 1, -1, -1):
             # the two streams' launches are issued alternately, so that
