@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) k_prims(const double* __restrict__ Q, dou
   (void)PX; (void)PY; (void)PZ;
   const int x = g.idx(i, j, k);
   const double rho = Q[x];
-  const double ir = 1.0 / rho;
+  const double ir = recip_pos(rho);
   double ke = 0.0;
 #pragma unroll
   for (int m = 0; m < ND; ++m) {
@@ -699,7 +699,7 @@ __global__ void __launch_bounds__(VNT, 2) k_visc_update3d(const double* __restri
     for (int k = 0; k < VPP; ++k)
       if (sval[k]) {
         const double rho = pq[k][0];
-        const double ir = 1.0 / rho;
+        const double ir = recip_pos(rho);
         const double u0 = pq[k][1] * ir, u1 = pq[k][2] * ir, u2 = pq[k][3] * ir;
         const double pr = ph.gm1 * (pq[k][4] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
         const int p = sidx[k];
@@ -940,7 +940,7 @@ __global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q,
     for (int k = 0; k < CPP; ++k)
       if (sval[k]) {
         const double rho = pq[k][0];
-        const double ir = 1.0 / rho;
+        const double ir = recip_pos(rho);
         const double u0 = pq[k][1] * ir, u1 = pq[k][2] * ir, u2 = pq[k][3] * ir;
         const double pr = ph.gm1 * (pq[k][4] - 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2));
         const int p = sidx[k];
@@ -963,7 +963,7 @@ __global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q,
   };
   auto col_entry = [&](double* e) {
     const double rho = oq[0];
-    const double ir = 1.0 / rho;
+    const double ir = recip_pos(rho);
     const double u0 = oq[1] * ir, u1 = oq[2] * ir, u2 = oq[3] * ir;
 #pragma unroll
     for (int m = 0; m < NV; ++m) e[m] = oq[m];
