@@ -101,7 +101,7 @@ def cpu_baseline(n, conv, steps=1):
     from oracle.ref import RefSolver
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     problem, q0 = taylor_green(n, conv=conv)
-    s = RefSolver(problem)
+    s = RefSolver(problem, variant="fma")
     s.set_state(q0)
     dt, ts = _median_timed(s, steps)
     return {
@@ -110,7 +110,8 @@ def cpu_baseline(n, conv, steps=1):
         "cores": threads,
         "kind": "port",
         "sample": f"{steps} RK3 step(s) of the same {n}^3 TGV {conv} problem, median of 3 after 1 warm-up "
-                  f"({', '.join('%.2f' % t for t in ts)} s; OMP_NUM_THREADS={threads})",
+                  f"({', '.join('%.2f' % t for t in ts)} s; OMP_NUM_THREADS={threads}; oracle built with -O3 "
+                  f"-march=x86-64-v3 -ffp-contract=fast)",
         "host": host_cpu(),
     }
 
@@ -455,7 +456,7 @@ def _oracle_instance(i, par):
     from opensbli_amd.ensemble import instance_problem
     from oracle.ref import RefSolver
     problem, q0 = instance_problem(par, i)
-    r = RefSolver(problem)
+    r = RefSolver(problem, variant="fma")
     r.set_state(q0)
     r.run(int(par["nsteps"][i]))
     return i
@@ -473,11 +474,12 @@ def bench_cpu(args):
     from oracle.ref import RefSolver
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     out = {"metric": "oracle Mcell-updates/s per RK stage on the host", "unit": "Mcell-updates/s",
-           "cores": threads, "kind": "port", "host": host_cpu(), "cases": {}}
+           "cores": threads, "kind": "port", "host": host_cpu(),
+           "build": "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=fast", "cases": {}}
 
     def timed_case(name, problem, q0, steps):
         _log(f"bench: cpu {name}")
-        s = RefSolver(problem)
+        s = RefSolver(problem, variant="fma")
         s.set_state(q0)
         dt, ts = _median_timed(s, steps)
         cells = int(np.prod(problem["n"][: problem["ndim"]]))
