@@ -17,12 +17,19 @@ import collections
 import csv
 import json
 
-# kernel-name fragments -> family of osbl_kernel_stats (first match wins)
+# kernel-name fragments -> family of osbl_kernel_stats, plus "adjoint" for the
+# reverse-stage kernels (first match wins)
 FAMILIES = [
     ("k_char_sweep_xl", "char_sweep_x"),
     ("k_char_sweep<3, 0,", "char_sweep_x"),
     ("k_char_sweep<3, 1,", "char_sweep_y"),
     ("k_char_sweep<3, 2,", "char_sweep_z"),
+    ("k_char_sweep<2, 0,", "char_sweep_x"),
+    ("k_char_sweep<2, 1,", "char_sweep_y"),
+    ("k_char_sweep<1, 0,", "char_sweep_x"),
+    ("k_char_vjp", "adjoint"),
+    ("k_adj_", "adjoint"),
+    ("k_visc_vjp", "adjoint"),
     ("k_conv3d", "central_convective"),
     ("k_visc_update3d", "residual_rk_update"),
     ("k_visc_gen3d", "residual_rk_update"),
