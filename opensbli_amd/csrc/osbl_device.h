@@ -127,6 +127,13 @@ struct Pt {
   double p, a;
 };
 
+// Loads and stores of the arrays a stage kernel touches once per cell (the
+// residual and the RK registers) carry the non-temporal hint, so that they
+// stream through L2 without evicting the Q planes the stencils of
+// neighbouring tiles and rows still re-read from it.
+__device__ __forceinline__ double ld_once(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_once(double v, double* p) { __builtin_nontemporal_store(v, p); }
+
 // a/b (a of either sign) and 1/b for positive normal b far from the
 // overflow range:
 // the hardware reciprocal estimate refined by a Newton-Raphson step, the
