@@ -662,39 +662,29 @@ constexpr int VJW = 2;         // waves per block
 constexpr int VJP = 64 + 5;    // points staged per chunk
 constexpr int VJSTEP = 59;     // cells completed per chunk
 
-template <int ND, int D, int AVG, bool ACC>
-__global__ void __launch_bounds__(64 * VJW) k_char_vjp_lanes(const double* __restrict__ Qs,
-                                                             const double* __restrict__ A,
-                                                             const double* __restrict__ B, double alpha, double beta,
-                                                             double c, double* __restrict__ out1,
-                                                             double* __restrict__ out2, int n0, int n1, int n2,
-                                                             Phys ph) {
+// The work of wave `wid` of one direction's VJP (chunk wid / lines of line
+// wid % lines); S and G are the wave's LDS staging window and contribution
+// columns.  OUT 0: out1 = c (A + B) + V (and out2 = A + B when given);
+// OUT 1: out1 += V; OUT 2: out1 = V.
+template <int ND, int D, int AVG, int OUT>
+__device__ __forceinline__ void vjp_chunk(long long wid, int lane, double (*S)[VJP], double (*G)[64],
+                                          const double* __restrict__ Qs, const double* __restrict__ A,
+                                          const double* __restrict__ B, double alpha, double beta, double c,
+                                          double* __restrict__ out1, double* __restrict__ out2, int n0, int n1,
+                                          int n2, const Phys& ph) {
   constexpr int NV = ND + 2;
-  constexpr int NF = NV + ND + 2;  // q, u, p, a
-  __shared__ double sh[VJW][NF][VJP];
-  __shared__ double ga[VJW][6 * NV][64];  // each lane's contributions, slot (p, m) at row p * NV + m
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nn[3] = {n0, n1, n2};
   const long long cells = (long long)n0 * n1 * n2;
   const int N = nn[D];
   long long lines = 1;
   for (int e = 0; e < 3; ++e)
     if (e != D) lines *= nn[e];
-  const int nchunk = (N + VJSTEP - 1) / VJSTEP;
-  // adjacent waves take adjacent lines (for y and z: adjacent x, sharing cache
-  // lines); blocks go to the eight XCDs in turn, so consecutive blocks of one
-  // XCD take consecutive waves and the shared lines are fetched into one L2
-  const unsigned nb = gridDim.x, bl = blockIdx.x;
-  const unsigned lb = (nb % 8 == 0) ? (bl % 8) * (nb / 8) + bl / 8 : bl;
-  const long long wid = (long long)lb * VJW + wv;
-  if (wid >= lines * nchunk) return;  // whole waves only: no block-wide barrier below
   const long long line = wid % lines;
   const int chunk = (int)(wid / lines);
   long long base, sd;
   vjp_line<D>(line, n0, n1, base, sd);
   auto at = [&](int i) { return base + (long long)(((i % N) + N) % N) * sd; };
   const int a0 = chunk * VJSTEP - 3;  // interface of lane 0; lane 3 completes cell chunk * VJSTEP
-  double (*S)[VJP] = sh[wv];
   auto stage = [&](int slot, int i) {
     Pt<ND> pt;
     const long long y = at(i);
@@ -730,7 +720,6 @@ __global__ void __launch_bounds__(64 * VJW) k_char_vjp_lanes(const double* __res
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  double (*G)[64] = ga[wv];
 #pragma unroll
   for (int r = 0; r < 6 * NV; ++r) G[r][lane] = 0.0;
   char_flux_vjp<ND, D, AVG>(LdsWindow<ND>{&S[0][lane], VJP}, nu, ph, LdsAcc<ND>{&G[0][lane], 64});
@@ -750,14 +739,86 @@ __global__ void __launch_bounds__(64 * VJW) k_char_vjp_lanes(const double* __res
   const long long x = base + (long long)j * sd;
 #pragma unroll
   for (int m = 0; m < NV; ++m) {
-    if constexpr (ACC) {
+    if constexpr (OUT == 1) {
       out1[m * cells + x] += v[m];
+    } else if constexpr (OUT == 2) {
+      out1[m * cells + x] = v[m];
     } else {
       const double ab = A[m * cells + x] + (B ? B[m * cells + x] : 0.0);
       out1[m * cells + x] = c * ab + v[m];
       if (out2) out2[m * cells + x] = ab;
     }
   }
+}
+
+template <int ND, int D, int AVG, bool ACC>
+__global__ void __launch_bounds__(64 * VJW) k_char_vjp_lanes(const double* __restrict__ Qs,
+                                                             const double* __restrict__ A,
+                                                             const double* __restrict__ B, double alpha, double beta,
+                                                             double c, double* __restrict__ out1,
+                                                             double* __restrict__ out2, int n0, int n1, int n2,
+                                                             Phys ph) {
+  constexpr int NV = ND + 2;
+  constexpr int NF = NV + ND + 2;  // q, u, p, a
+  __shared__ double sh[VJW][NF][VJP];
+  __shared__ double ga[VJW][6 * NV][64];  // each lane's contributions, slot (p, m) at row p * NV + m
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nn[3] = {n0, n1, n2};
+  long long lines = 1;
+  for (int e = 0; e < 3; ++e)
+    if (e != D) lines *= nn[e];
+  const long long waves = lines * ((nn[D] + VJSTEP - 1) / VJSTEP);
+  // adjacent waves take adjacent lines (for y and z: adjacent x, sharing cache
+  // lines); blocks go to the eight XCDs in turn, so consecutive blocks of one
+  // XCD take consecutive waves and the shared lines are fetched into one L2
+  const unsigned nb = gridDim.x, bl = blockIdx.x;
+  const unsigned lb = (nb % 8 == 0) ? (bl % 8) * (nb / 8) + bl / 8 : bl;
+  const long long wid = (long long)lb * VJW + wv;
+  if (wid >= waves) return;  // whole waves only: no block-wide barrier in vjp_chunk
+  vjp_chunk<ND, D, AVG, ACC ? 1 : 0>(wid, lane, sh[wv], ga[wv], Qs, A, B, alpha, beta, c, out1, out2, n0, n1, n2,
+                                     ph);
+}
+
+// Both directions of a 2-D VJP in one launch, so that the y waves fill the
+// SIMDs the x waves leave idle in their last round (one wave per SIMD):
+// x waves as k_char_vjp_lanes<2, 0, AVG, false> (out1, out2), y waves write
+// their V to vy, which k_add_packed then adds to out1.  With both block
+// counts multiples of 8, each XCD takes an eighth of the x blocks and then an
+// eighth of the y blocks, in order; otherwise blocks run in launch order.
+template <int AVG>
+__global__ void __launch_bounds__(64 * VJW) k_char_vjp_xy(const double* __restrict__ Qs, const double* __restrict__ A,
+                                                          const double* __restrict__ B, double alpha, double beta,
+                                                          double c, double* __restrict__ out1,
+                                                          double* __restrict__ out2, double* __restrict__ vy, int n0,
+                                                          int n1, int bx, int by, Phys ph) {
+  constexpr int NV = 4, NF = 8;
+  __shared__ double sh[VJW][NF][VJP];
+  __shared__ double ga[VJW][6 * NV][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned bl = blockIdx.x;
+  int dir;
+  long long lb;
+  if (bx % 8 == 0 && by % 8 == 0) {
+    const int px = bx / 8, py = by / 8, slot = (int)(bl / 8), xcd = (int)(bl % 8);
+    dir = slot < px ? 0 : 1;
+    lb = dir == 0 ? (long long)xcd * px + slot : (long long)xcd * py + (slot - px);
+  } else {
+    dir = (int)bl < bx ? 0 : 1;
+    lb = dir == 0 ? (long long)bl : (long long)bl - bx;
+  }
+  const long long wid = lb * VJW + wv;
+  if (dir == 0) {
+    if (wid >= (long long)n1 * ((n0 + VJSTEP - 1) / VJSTEP)) return;
+    vjp_chunk<2, 0, AVG, 0>(wid, lane, sh[wv], ga[wv], Qs, A, B, alpha, beta, c, out1, out2, n0, n1, 1, ph);
+  } else {
+    if (wid >= (long long)n0 * ((n1 + VJSTEP - 1) / VJSTEP)) return;
+    vjp_chunk<2, 1, AVG, 2>(wid, lane, sh[wv], ga[wv], Qs, A, B, alpha, beta, 0.0, vy, nullptr, n0, n1, 1, ph);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_add_packed(double* __restrict__ out, const double* __restrict__ v, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] += v[i];
 }
 
 template <int ND, int D, int AVG>
@@ -778,7 +839,20 @@ static void launch_cv_d(const double* Qs, const double* A, const double* B, doub
 
 template <int ND, int AVG>
 static void launch_cv_nd(const double* Qs, const double* A, const double* B, double alpha, double beta, double c,
-                         double* out1, double* out2, const Grid& g, const Phys& ph, bool acc, hipStream_t s) {
+                         double* out1, double* out2, const Grid& g, const Phys& ph, bool acc, hipStream_t s,
+                         double* vy) {
+  if constexpr (ND == 2) {
+    if (!acc && vy) {
+      const long long wx = (long long)g.n[1] * ((g.n[0] + VJSTEP - 1) / VJSTEP);
+      const long long wy = (long long)g.n[0] * ((g.n[1] + VJSTEP - 1) / VJSTEP);
+      const int bx = (int)((wx + VJW - 1) / VJW), by = (int)((wy + VJW - 1) / VJW);
+      k_char_vjp_xy<AVG><<<bx + by, 64 * VJW, 0, s>>>(Qs, A, B, alpha, beta, c, out1, out2, vy, g.n[0], g.n[1], bx,
+                                                       by, ph);
+      const long long n = (long long)g.n[0] * g.n[1] * 4;
+      k_add_packed<<<(unsigned)std::min<long long>((n + 255) / 256, 4096), 256, 0, s>>>(out1, vy, n);
+      return;
+    }
+  }
   launch_cv_d<ND, 0, AVG>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
   if constexpr (ND >= 2) launch_cv_d<ND, 1, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, g, ph, true, s);
   if constexpr (ND >= 3) launch_cv_d<ND, 2, AVG>(Qs, A, B, alpha, beta, 0.0, out1, nullptr, g, ph, true, s);
@@ -787,17 +861,20 @@ static void launch_cv_nd(const double* Qs, const double* A, const double* B, dou
 // out1 = c (A + B) + (dR/dQ at Qs)^T (alpha A + beta B) for the WENO5-Z
 // characteristic residual, out2 = A + B (optional); with acc, out1 += the
 // product instead.  Packed periodic arrays; out1 aliases neither A nor B.
+// vy (optional, 2-D without acc: a packed array of the state's size) lets both
+// directions run in one launch, the y part staged in vy; the sums are the
+// same, in the same order, as with the two launches.
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
                            double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, bool acc,
-                           hipStream_t s) {
+                           hipStream_t s, double* vy) {
   if (avg == OSBL_AVG_ROE) {
-    if (nd == 3) launch_cv_nd<3, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
-    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
-    else launch_cv_nd<1, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+    if (nd == 3) launch_cv_nd<3, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s, vy);
+    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s, vy);
+    else launch_cv_nd<1, OSBL_AVG_ROE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s, vy);
   } else {
-    if (nd == 3) launch_cv_nd<3, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
-    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
-    else launch_cv_nd<1, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s);
+    if (nd == 3) launch_cv_nd<3, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s, vy);
+    else if (nd == 2) launch_cv_nd<2, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s, vy);
+    else launch_cv_nd<1, OSBL_AVG_SIMPLE>(Qs, A, B, alpha, beta, c, out1, out2, g, ph, acc, s, vy);
   }
 }
 

@@ -47,7 +47,7 @@ void launch_adj_stage(int nd, const double* Qs, const double* A, const double* B
 void launch_pack_all(const double* F, double* out, const Grid& g, int nv, hipStream_t s);
 void launch_adj_stage_char(int nd, int avg, const double* Qs, const double* A, const double* B, double alpha,
                            double beta, double c, double* out1, double* out2, const Grid& g, const Phys& ph, bool acc,
-                           hipStream_t s);
+                           hipStream_t s, double* vy = nullptr);
 int visc_vjp_fields(int nd);
 void launch_unpack_prims(int nd, const double* q, double* Qpad, double* P, const Grid& g, const Phys& ph,
                          hipStream_t s);

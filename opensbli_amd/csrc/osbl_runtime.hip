@@ -227,6 +227,7 @@ struct osbl_ctx {
   double* Mbuf = nullptr;  // adjoint weights with halos
   double* pk = nullptr;    // 3 packed arrays: mu, l, lo
   double* vw = nullptr;    // viscous VJP fields (padded)
+  double* vy = nullptr;    // y part of a 2-D WENO5-Z VJP (packed)
 };
 
 namespace {
@@ -720,6 +721,7 @@ void osbl_destroy(osbl_ctx* c) {
   (void)hipFree(c->Mbuf);
   (void)hipFree(c->pk);
   (void)hipFree(c->vw);
+  (void)hipFree(c->vy);
   (void)hipFree(c->cmet);
   (void)hipFree(c->cflux);
   (void)hipFree(c->cpt);
@@ -1282,6 +1284,7 @@ struct AdjBufs {
   double* pk;     // 3 packed arrays: lam_Q2, lam_Q1, lam_Qo
   double* vw;     // viscous VJP fields (padded)
   double* P;      // constituent relations over the padded grid
+  double* vy;     // y part of a 2-D WENO5-Z VJP (packed), or null: two launches
 };
 
 static int adjoint_check(const osbl_ctx* c, bool need_rk3) {
@@ -1329,7 +1332,11 @@ static int adjoint_ready(osbl_ctx* c, bool need_rk3, AdjBufs* b) {
     if (hipMalloc((void**)&c->P, bytes) != hipSuccess) return fail(OSBL_E_NOMEM, "adjoint constituent relations");
     HIPCHK(hipMemsetAsync(c->P, 0, bytes, c->stream));
   }
-  *b = AdjBufs{c->Abuf, c->Mbuf, c->pk, c->vw, c->P};
+  if (c->cfg.conv == OSBL_CONV_WENO_Z && c->nd == 2 && !c->vy) {
+    if (hipMalloc((void**)&c->vy, (size_t)cells(c) * c->nv * sizeof(double)) != hipSuccess)
+      return fail(OSBL_E_NOMEM, "adjoint VJP staging");
+  }
+  *b = AdjBufs{c->Abuf, c->Mbuf, c->pk, c->vw, c->P, c->vy};
   return OSBL_OK;
 }
 
@@ -1353,7 +1360,7 @@ static void adj_stage(osbl_ctx* c, const AdjBufs& b, const double* Qs, const dou
                       double beta, double cc, double* out1, double* out2) {
   if (c->cfg.conv == OSBL_CONV_WENO_Z)
     launch_adj_stage_char(c->nd, c->cfg.averaging, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, false,
-                          c->stream);
+                          c->stream, b.vy);
   else
     launch_adj_stage(c->nd, Qs, A, B, alpha, beta, cc, out1, out2, c->g, c->ph, c->stream);
   if (c->cfg.viscous == OSBL_VISC_CONST) visc_vjp(c, b, Qs, A, B, alpha, beta, out1);
@@ -1437,7 +1444,7 @@ static int scratch_bufs(osbl_ctx* c, AdjBufs* b) {
     return fail(OSBL_E_ARG, "osbl_adjoint_step_dev_noalloc: this block needs %zu bytes of scratch, more than the "
                             "%zu-byte fixed scratch (use osbl_adjoint_step_dev)", need, kScratchBytes);
   auto at = [&](long long o) { return o < 0 ? nullptr : base + o; };
-  *b = AdjBufs{at(o_A), at(o_M), at(o_pk), at(o_vw), at(o_P)};
+  *b = AdjBufs{at(o_A), at(o_M), at(o_pk), at(o_vw), at(o_P), nullptr};
   // zero everything from the first halo buffer on (A, M, the viscous
   // fields, P): the allocating path's buffers start zeroed
   if (halo || visc) {

@@ -286,3 +286,35 @@ def test_weno_gradient_vs_torch_autograd():
     _say("J", float(J), float(Jt), "max rel grad diff", np.abs(g - gt).max() / np.abs(gt).max())
     assert abs(float(J) - float(Jt)) < 1e-12 * abs(float(Jt))
     assert np.abs(g - gt).max() < 1e-8 * np.abs(gt).max()
+
+
+@pytest.mark.parametrize("n", [36, 48, 70, 128])
+def test_weno_adjoint_step_paired_launch_bitwise(n):
+    """The allocating 2-D WENO-Z adjoint step runs the x and y VJPs in one
+    launch (the y part staged in a buffer and added afterwards); the
+    allocation-free step runs them as two launches.  The sums are the same in
+    the same order, so the two agree bitwise.  n = 36 and 70 give block counts
+    that are not multiples of 8 (blocks in launch order), 48 and 128 ones that
+    are (each XCD an eighth of each direction)."""
+    import torch
+    from opensbli_amd.native import Block
+    from opensbli_amd.problems import kelvin_helmholtz
+    _say(f"case WENO-Z adjoint step, paired vs separate VJP launches, KH {n}^2")
+    amps = np.random.RandomState(4).normal(0.0, 1e-2, 16)
+    problem, q0 = kelvin_helmholtz(n, amps, conv="weno_z")
+    b = Block(problem)
+    b.set_state(q0)
+    stages = torch.empty((b.num_stages(),) + q0.shape, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    b.step_stages_dev(stages.data_ptr())
+    lam = torch.tensor(np.random.RandomState(n).normal(size=q0.shape), dtype=torch.float64, device="cuda")
+    la, lb = lam.clone(), lam.clone()
+    torch.cuda.synchronize()
+    b.adjoint_step_dev(stages.data_ptr(), la.data_ptr())
+    b.adjoint_step_dev_noalloc(stages.data_ptr(), lb.data_ptr())
+    b.sync()
+    a, c = la.cpu().numpy(), lb.cpu().numpy()
+    b.close()
+    _say("max |paired - separate|", np.abs(a - c).max(), "max |lambda|", np.abs(a).max())
+    assert np.all(np.isfinite(a)) and np.abs(a).max() > 0
+    assert np.array_equal(a, c)
