@@ -1109,16 +1109,20 @@ __global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q,
   }
 }
 
-// z segment of a 2.5-D tiled kernel over tx x ty columns: 32 planes, halved
-// (down to 2) while the launch would have fewer than `want` blocks
-static int tile_segment(const Grid& g, int tx, int ty, long long want) {
+// z segment of a 2.5-D tiled kernel over tx x ty columns: `start` planes,
+// halved (down to 2) while the launch would have fewer than `want` blocks.
+// The viscous kernels (one 512-thread block per CU, bound by their 104-130 KB
+// of LDS) take 64-plane segments from 512 blocks on: each segment stages four
+// planes beyond its own, and at 256^3 two rounds of 64 planes beat four of 32
+// (0.684 -> 0.673 ms for the central viscous + RK kernel).
+static int tile_segment(const Grid& g, int tx, int ty, long long want, int start) {
   const long long tiles = (long long)((g.n[0] + tx - 1) / tx) * ((g.n[1] + ty - 1) / ty);
-  int sz = 32;
+  int sz = start;
   while (sz > 2 && tiles * ((g.n[2] + sz - 1) / sz) < want) sz /= 2;
   return sz;
 }
-int visc3d_segment(const Grid& g) { return tile_segment(g, VTX, VTY, 1024); }
-int conv3d_segment(const Grid& g) { return tile_segment(g, CTX, CTY, 1024); }
+int visc3d_segment(const Grid& g) { return tile_segment(g, VTX, VTY, 512, 64); }
+int conv3d_segment(const Grid& g) { return tile_segment(g, CTX, CTY, 1024, 32); }
 long long conv3d_blocks(const Grid& g, int sz) {
   return (long long)((g.n[0] + CTX - 1) / CTX) * ((g.n[1] + CTY - 1) / CTY) * ((g.n[2] + sz - 1) / sz);
 }
