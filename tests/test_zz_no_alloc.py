@@ -1,6 +1,8 @@
 """The allocation-free adjoint step (osbl_adjoint_step_dev_noalloc) against
-the allocating one (osbl_adjoint_step_dev) through the C-ABI: the same
-kernels in the same order on the same inputs, so the results agree bitwise.
+the allocating one (osbl_adjoint_step_dev) through the C-ABI: the same sums
+in the same order on the same inputs (a 2-D WENO5-Z step runs its x and y
+VJPs in one launch in the allocating entry point and in two here, with the y
+part added last either way), so the results agree bitwise.
 Two calls in a row, the second on a smaller block (the fixed scratch then
 holds the first call's contents), then the other schemes, a 3-D viscous
 characteristic block, and a block too large for the fixed scratch, which the
