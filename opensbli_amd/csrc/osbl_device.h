@@ -70,6 +70,7 @@ struct Phys {
   double igm1;        // 1/(gama-1)
   double force[3];    // c_j of the constant body force (zero without one)
   int has_force;
+  int nt;             // non-temporal hint on the residual and RK registers
 };
 
 // 10^n for an integer n by repeated products (exact for |n| <= 22)
@@ -128,11 +129,19 @@ struct Pt {
 };
 
 // Loads and stores of the arrays a stage kernel touches once per cell (the
-// residual and the RK registers) carry the non-temporal hint, so that they
+// residual and the RK registers).  On blocks whose fields exceed the
+// last-level cache (Phys::nt) they carry the non-temporal hint, so that they
 // stream through L2 without evicting the Q planes the stencils of
-// neighbouring tiles and rows still re-read from it.
-__device__ __forceinline__ double ld_once(const double* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void st_once(double v, double* p) { __builtin_nontemporal_store(v, p); }
+// neighbouring tiles and rows still re-read from it; on smaller blocks the
+// whole working set stays cached between kernels and plain accesses keep it
+// there.
+__device__ __forceinline__ double ld_once(const double* p, int nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+__device__ __forceinline__ void st_once(double v, double* p, int nt) {
+  if (nt) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 
 // a/b (a of either sign) and 1/b for positive normal b far from the
 // overflow range:

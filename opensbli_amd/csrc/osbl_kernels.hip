@@ -182,8 +182,8 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
         const int x = off + 2 * st;
 #pragma unroll
         for (int m = 0; m < NV; ++m) {
-          rin[m] = ld_once(&R[m * g.fs + x]);
-          if constexpr (RKM >= 0) qoin[m] = (RKM == 0 && rk.first) ? 0.0 : ld_once(&rk.Qo[m * g.fs + x]);
+          rin[m] = ld_once(&R[m * g.fs + x], ph.nt);
+          if constexpr (RKM >= 0) qoin[m] = (RKM == 0 && rk.first) ? 0.0 : ld_once(&rk.Qo[m * g.fs + x], ph.nt);
         }
       }
     }
@@ -205,17 +205,17 @@ __global__ void __launch_bounds__(256) k_char_sweep(const double* __restrict__ Q
         if constexpr (RKM == 0) {
           const double r = (EARLY ? rin[m] : R[m * g.fs + x]) + v;
           const double qo = rk.first ? w[2].q[m] : (EARLY ? qoin[m] : rk.Qo[m * g.fs + x]);
-          st_once(qo + ph.dt * rk.cA * r, &rk.Qn[m * g.fs + x]);
-          st_once(qo + ph.dt * rk.cB * r, &rk.Qo[m * g.fs + x]);
+          st_once(qo + ph.dt * rk.cA * r, &rk.Qn[m * g.fs + x], ph.nt);
+          st_once(qo + ph.dt * rk.cB * r, &rk.Qo[m * g.fs + x], ph.nt);
         } else if constexpr (RKM == 1) {
           const double r = (EARLY ? rin[m] : R[m * g.fs + x]) + v;
           const double tmp = (rk.cA == 0.0 ? 0.0 : rk.cA * (EARLY ? qoin[m] : rk.Qo[m * g.fs + x])) + ph.dt * r;
-          st_once(tmp, &rk.Qo[m * g.fs + x]);
-          st_once(w[2].q[m] + rk.cB * tmp, &rk.Qn[m * g.fs + x]);
+          st_once(tmp, &rk.Qo[m * g.fs + x], ph.nt);
+          st_once(w[2].q[m] + rk.cB * tmp, &rk.Qn[m * g.fs + x], ph.nt);
         } else if constexpr (ACC) {
-          st_once((EARLY ? rin[m] : R[m * g.fs + x]) + v, &R[m * g.fs + x]);
+          st_once((EARLY ? rin[m] : R[m * g.fs + x]) + v, &R[m * g.fs + x], ph.nt);
         } else {
-          st_once(v, &R[m * g.fs + x]);
+          st_once(v, &R[m * g.fs + x], ph.nt);
         }
       }
     }
@@ -299,8 +299,8 @@ __global__ void __launch_bounds__(64 * XW, 3) k_char_sweep_xl(const double* __re
 #pragma unroll
     for (int m = 0; m < NV; ++m) {
       const double v = met ? -(gm * (r[m] * inv)) : -r[m] * inv;
-      if constexpr (ACC) st_once(ld_once(&R[m * g.fs + base + x]) + v, &R[m * g.fs + base + x]);
-      else st_once(v, &R[m * g.fs + base + x]);
+      if constexpr (ACC) st_once(ld_once(&R[m * g.fs + base + x], ph.nt) + v, &R[m * g.fs + base + x], ph.nt);
+      else st_once(v, &R[m * g.fs + base + x], ph.nt);
     }
   };
   if constexpr (CONV == OSBL_CONV_WENO_Z || CONV == OSBL_CONV_WENO_JS) {
@@ -763,11 +763,11 @@ __global__ void __launch_bounds__(VNT, 2) k_visc_update3d(const double* __restri
     if (inside) {
 #pragma unroll
       for (int m = 0; m < 5; ++m) {
-        if constexpr (MODE == 2) rin[m] = ld_once(&R[m * g.fs + o]);
-        if constexpr (MODE == 0 || MODE == 1) rin[m] = ld_once(&R[m * g.fs + o]);
-        if constexpr (MODE == 0) qa[m] = first ? Q[m * g.fs + o] : ld_once(&Qo[m * g.fs + o]);
+        if constexpr (MODE == 2) rin[m] = ld_once(&R[m * g.fs + o], ph.nt);
+        if constexpr (MODE == 0 || MODE == 1) rin[m] = ld_once(&R[m * g.fs + o], ph.nt);
+        if constexpr (MODE == 0) qa[m] = first ? Q[m * g.fs + o] : ld_once(&Qo[m * g.fs + o], ph.nt);
         if constexpr (MODE == 1) {
-          qa[m] = cA == 0.0 ? 0.0 : ld_once(&Qo[m * g.fs + o]);
+          qa[m] = cA == 0.0 ? 0.0 : ld_once(&Qo[m * g.fs + o], ph.nt);
           qb[m] = Q[m * g.fs + o];
         }
       }
@@ -840,19 +840,19 @@ __global__ void __launch_bounds__(VNT, 2) k_visc_update3d(const double* __restri
       }
       if constexpr (MODE == 2 || MODE == 3) {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) st_once(r[m], &R[m * g.fs + o]);
+        for (int m = 0; m < 5; ++m) st_once(r[m], &R[m * g.fs + o], ph.nt);
       } else if constexpr (MODE == 0) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-          st_once(qa[m] + ph.dt * cA * r[m], &Qn[m * g.fs + o]);
-          st_once(qa[m] + ph.dt * cB * r[m], &Qo[m * g.fs + o]);
+          st_once(qa[m] + ph.dt * cA * r[m], &Qn[m * g.fs + o], ph.nt);
+          st_once(qa[m] + ph.dt * cB * r[m], &Qo[m * g.fs + o], ph.nt);
         }
       } else {
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
           const double tmp = (cA == 0.0 ? 0.0 : cA * qa[m]) + ph.dt * r[m];
-          st_once(tmp, &Qo[m * g.fs + o]);
-          st_once(qb[m] + cB * tmp, &Qn[m * g.fs + o]);
+          st_once(tmp, &Qo[m * g.fs + o], ph.nt);
+          st_once(qb[m] + cB * tmp, &Qn[m * g.fs + o], ph.nt);
         }
       }
     }
@@ -995,8 +995,8 @@ __global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q,
       [[maybe_unused]] double qa[NV], qb[NV];
 #pragma unroll
       for (int m = 0; m < NV; ++m) {
-        if constexpr (MODE == 0) qa[m] = first ? 0.0 : ld_once(&Qo[m * g.fs + o]);
-        if constexpr (MODE == 1) qa[m] = cA == 0.0 ? 0.0 : ld_once(&Qo[m * g.fs + o]);
+        if constexpr (MODE == 0) qa[m] = first ? 0.0 : ld_once(&Qo[m * g.fs + o], ph.nt);
+        if constexpr (MODE == 1) qa[m] = cA == 0.0 ? 0.0 : ld_once(&Qo[m * g.fs + o], ph.nt);
       }
       const int sl = z & 1;
       double r[NV];
@@ -1074,20 +1074,20 @@ __global__ void __launch_bounds__(CNT, 2) k_conv3d(const double* __restrict__ Q,
       }
       if constexpr (MODE == 2) {
 #pragma unroll
-        for (int m = 0; m < NV; ++m) st_once(r[m], &R[m * g.fs + o]);
+        for (int m = 0; m < NV; ++m) st_once(r[m], &R[m * g.fs + o], ph.nt);
       } else if constexpr (MODE == 0) {
 #pragma unroll
         for (int m = 0; m < NV; ++m) {
           const double qo = first ? zq[2][m] : qa[m];
-          st_once(qo + ph.dt * cA * r[m], &Qn[m * g.fs + o]);
-          st_once(qo + ph.dt * cB * r[m], &Qo[m * g.fs + o]);
+          st_once(qo + ph.dt * cA * r[m], &Qn[m * g.fs + o], ph.nt);
+          st_once(qo + ph.dt * cB * r[m], &Qo[m * g.fs + o], ph.nt);
         }
       } else {
 #pragma unroll
         for (int m = 0; m < NV; ++m) {
           const double tmp = (cA == 0.0 ? 0.0 : cA * qa[m]) + ph.dt * r[m];
-          st_once(tmp, &Qo[m * g.fs + o]);
-          st_once(zq[2][m] + cB * tmp, &Qn[m * g.fs + o]);
+          st_once(tmp, &Qo[m * g.fs + o], ph.nt);
+          st_once(zq[2][m] + cB * tmp, &Qn[m * g.fs + o], ph.nt);
         }
       }
     }
