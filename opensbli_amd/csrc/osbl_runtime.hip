@@ -599,10 +599,13 @@ int osbl_create(const osbl_config* cfg, osbl_ctx** out) {
     ph.force[d] = d < c->nd ? cfg->body_force[d] : 0.0;
     ph.has_force |= ph.force[d] != 0.0;
   }
-  // the non-temporal hint on R and the RK registers pays once Q, the two
-  // registers and R exceed about half of the 256 MB last-level cache: at 64^3
-  // (64 MB) plain accesses are 4 % faster, at 128^3 and 256^3 the hint is
-  ph.nt = 4.0 * c->nv * (double)fs * sizeof(double) > 128.0 * (1 << 20) ? 1 : 0;
+  // the non-temporal hint on R and the RK registers: the central kernels gain
+  // from it once Q, the two registers and R exceed about half of the 256 MB
+  // last-level cache (at 64^3, 76 MB, plain accesses are 4 % faster; at 128^3
+  // and 256^3 the hint is), the characteristic sweeps, whose Q re-reads from
+  // L2 matter more than keeping R cached, at every measured size (2-D 512^2,
+  // 36 MB: 3 % faster forward steps)
+  ph.nt = (!is_central(cfg->conv) || 4.0 * c->nv * (double)fs * sizeof(double) > 128.0 * (1 << 20)) ? 1 : 0;
   c->nstages = rk_coefficients(cfg->rk, c->A, c->B);
   c->suth = cfg->viscous == OSBL_VISC_SUTHERLAND;
   bool closures = false, stretched = false;
